@@ -1,0 +1,127 @@
+/*
+ * mldgpu.h -- C ABI of the MI355X (gfx950) library for MILADY's fitting hot path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success or a
+ * positive MLD_E_* code; mld_last_error() gives the message.  The reference has
+ * no return codes (it calls stop / mld_mpi_abort); a Fortran shim maps a
+ * non-zero return to mld_mpi_abort(message).
+ *
+ * Layouts follow the Fortran callers: column-major arrays, 1-based kind_neigh.
+ *   pos   : (3, nat)   Angstrom            -- config_real%pos_cart
+ *   cell  : (3, 3)     columns = a1,a2,a3  -- config_real%cell (build_subdata.F90:2714-2727)
+ *   energy: (D, nat)                       -- config_desc%energy
+ *   force : (D, nat, 0:nn_max, 3)          -- config_desc%force after the sign flip
+ *                                              of descriptors.F90:298-302, i.e. dB/dr
+ * Functions whose name ends in _dev take device pointers; all others take host
+ * pointers and copy.
+ */
+#ifndef MLDGPU_H
+#define MLDGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MLD_OK = 0,
+    MLD_E_ARG = 1,        /* invalid argument / size */
+    MLD_E_DEVICE = 2,     /* HIP runtime error */
+    MLD_E_NEIGH = 3,      /* neighbour count above nn_max (neighbours.F90:352-358) */
+    MLD_E_SINGULAR = 4,   /* factorisation failed: matrix not positive definite */
+    MLD_E_UNSUPPORTED = 5,/* parameter combination not built */
+    MLD_E_STATE = 6       /* call order (e.g. solve before gram_begin) */
+};
+
+enum { MLD_PREC_FP64 = 0, MLD_PREC_FP32 = 1 };
+enum { MLD_DESC_BSO4 = 9 };
+enum { MLD_SOLVE_CHOL_IR = 0, MLD_SOLVE_CHOL = 1 };
+
+typedef struct mld_params {
+    int descriptor_type;        /* 9 = SO(4) bispectrum (ml_in_ndm_module descriptor_bispectrum_so4) */
+    double j_max;               /* &input_ml j_max; jj_max = int(2*j_max) (descriptors.F90:660) */
+    double r_cut;               /* &input_ml r_cut */
+    double inv_r0_input;        /* &input_ml inv_r0_input, default 1-0.02/pi (read_ml_file.F90:425) */
+    int lbso4_diag;             /* &input_ml lbso4_diag */
+    int nchannel;               /* 1, or 2 (weighted), or 3 (weighted_3ch) */
+    int nspecies;               /* number of species indices used by itype */
+    const double* weight;       /* [nspecies] weight_per_type (channel 2), may be NULL if nchannel == 1 */
+    const double* weight_3ch;   /* [nspecies] weight_per_type_3ch (channel 3) */
+    const double* r_cut_pair_in;/* [nspecies*nspecies] inner cutoff (row-major), NULL = none */
+    double r_cut_width_in;      /* &input_ml r_cut_width_in */
+    int precision;              /* MLD_PREC_FP64 or MLD_PREC_FP32 */
+    int device;                 /* HIP device ordinal */
+} mld_params;
+
+typedef struct mld_ctx mld_ctx;
+
+/* Context: builds the Clebsch-Gordan / recursion / contraction tables on the
+ * device (init_descriptors, descriptors.F90:658-671 with compute_cg_vector and
+ * gen_dimension_for_bispectrum_so4).  One context per host thread. */
+int mld_create(const mld_params* p, mld_ctx** out);
+int mld_destroy(mld_ctx* ctx);
+const char* mld_last_error(const mld_ctx* ctx);
+/* D = bisso4_dim * nchannel (dim_xdesc) */
+int mld_descriptor_dim(const mld_ctx* ctx, int* D);
+
+/* Largest per-atom neighbour count of one configuration (compute_max_n_neigh,
+ * first_neighbour_pass.F90:41-129): the caller sizes kind_neigh / force. */
+int mld_max_neighbours(mld_ctx* ctx, int nat, const double* pos, const double* cell, int* nn_max);
+
+/* Descriptors of one configuration: replaces compute_bispectrum_so4 plus the
+ * force sign flip at descriptors.F90:296-302.  force may be NULL when
+ * want_grad == 0.  n_neigh: (nat); kind_neigh: (nat, nn_max) 1-based. */
+int mld_bso4(mld_ctx* ctx, int nat, const double* pos, const double* cell, const int* itype,
+             int want_grad, double* energy, double* force, int* n_neigh, int* kind_neigh, int nn_max);
+
+/* ---- device-resident batch of configurations (one launch per phase) ----
+ * atom_off[nstruct+1] (host), pos_dev (3*natoms, [atom][xyz]), cell_dev
+ * (9*nstruct, column-major per structure), itype_dev (natoms).
+ * mld_batch_neighbours builds the pair list and reports the pair count P;
+ * the gradient buffer of mld_batch_bso4_dev holds (P + natoms) * 3 * D values
+ * in compact slot-major order: slot(a, k) = pair_off[a] + a + k, k = 0 the atom
+ * itself, k >= 1 its k-th neighbour; element [(slot*3 + x)*D + d].
+ * Precision of energy_dev / grad_dev: double (FP64 context) or float (FP32). */
+int mld_batch_neighbours(mld_ctx* ctx, int nstruct, const int* atom_off, const double* pos_dev,
+                         const double* cell_dev, const int* itype_dev, long long* npairs);
+int mld_batch_pairs_dev(mld_ctx* ctx, const int** pair_off_dev, const int** pair_idx_dev);
+int mld_batch_bso4_dev(mld_ctx* ctx, int want_grad, void* energy_dev, void* grad_dev);
+
+/* ---- streamed normal equations (gram_accumulate_config / gram_reduce_and_solve,
+ * module_online_fit.F90:138-598) ----
+ * The Gram G (D x D, FP64, upper triangle accumulated) and h live on the device.
+ * add_rows: exact replacement of gram_accumulate_config for caller-packed rows:
+ *   Ac (D, mc) column-major, y (mc), w (mc); columns are scaled by sqrt(w). */
+int mld_gram_begin(mld_ctx* ctx, int D);
+int mld_gram_add_rows(mld_ctx* ctx, int D, int mc, const double* Ac, const double* y, const double* w);
+int mld_gram_add_rows_dev(mld_ctx* ctx, int D, int mc, const double* Ac_dev, const double* y_dev,
+                          const double* w_dev);
+/* rows from the last batch descriptors: linear (order 1) or quadratic (order 2)
+ * energy + force rows (mld_energy.F90:91-266, mld_forces.F90:222-500) with
+ * per-structure targets E (nstruct, host) and forces F_dev (3*natoms, device),
+ * weights w_e, w_f.  mixed = 1 runs the Gram in split-BF16 MFMA with FP32
+ * accumulation (FP64 across tiles), 0 in FP64. */
+int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F_dev, double w_e,
+                       double w_f, int mixed);
+int mld_gram_export(mld_ctx* ctx, double* G, double* h);
+/* (G + lambda I) w = h by FP64 Cholesky with FP64 iterative refinement.
+ * diag_out (optional, 4 doubles): [iterations, final relative residual, min pivot, max pivot]. */
+int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w, double* diag_out);
+
+/* ---- kernel_po features (compute/compute_kernel.F90:91-150, 476-866) ----
+ * zref (D, nk) column-major: the standardised reference points (global_kernel).
+ * Features of the last batch: ek_dev (natoms, nk) row-major, fk_dev
+ * ((P+natoms)*3, nk) in the slot order above, or NULL for energies only. */
+int mld_kernel_po_set(mld_ctx* ctx, int nk, const double* zref, double sigma, double length, double power);
+int mld_kernel_po_dev(mld_ctx* ctx, void* ek_dev, void* fk_dev);
+
+/* Stream used by the library (hipStream_t as void*). */
+int mld_get_stream(mld_ctx* ctx, void** stream);
+int mld_synchronize(mld_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MLDGPU_H */

@@ -1,0 +1,224 @@
+"""mld_amd -- MI355X-native fitting hot path of MILADY (ai-atoms/MLD).
+
+Host-side mirror of the reference's descriptor / fit interface over the C ABI of
+``libmldgpu.so`` (include/mldgpu.h).  The shared library is built in-tree by
+``mld_amd.build``; every compute call goes through it, and importing a compute
+entry point without the library raises immediately (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libmldgpu.so")
+
+MLD_OK, MLD_E_ARG, MLD_E_DEVICE, MLD_E_NEIGH, MLD_E_SINGULAR, MLD_E_UNSUPPORTED, MLD_E_STATE = range(7)
+PREC_FP64, PREC_FP32 = 0, 1
+DESC_BSO4 = 9
+
+# Every symbol include/mldgpu.h declares.
+EXPORTS = [
+    "mld_create", "mld_destroy", "mld_last_error", "mld_descriptor_dim", "mld_max_neighbours", "mld_bso4",
+    "mld_batch_neighbours", "mld_batch_pairs_dev", "mld_batch_bso4_dev", "mld_gram_begin", "mld_gram_add_rows",
+    "mld_gram_add_rows_dev", "mld_gram_add_batch", "mld_gram_export", "mld_gram_solve", "mld_kernel_po_set",
+    "mld_kernel_po_dev", "mld_get_stream", "mld_synchronize",
+]
+
+
+class MldError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libmldgpu error {code}: {msg}")
+        self.code = code
+
+
+class mld_params(C.Structure):
+    _fields_ = [
+        ("descriptor_type", C.c_int),
+        ("j_max", C.c_double),
+        ("r_cut", C.c_double),
+        ("inv_r0_input", C.c_double),
+        ("lbso4_diag", C.c_int),
+        ("nchannel", C.c_int),
+        ("nspecies", C.c_int),
+        ("weight", C.c_void_p),
+        ("weight_3ch", C.c_void_p),
+        ("r_cut_pair_in", C.c_void_p),
+        ("r_cut_width_in", C.c_double),
+        ("precision", C.c_int),
+        ("device", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the in-tree HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise ImportError(f"{LIBPATH} is missing: run `python -m mld_amd.build` (hipcc, gfx950)")
+    L = C.CDLL(LIBPATH)
+    vp, ip, dp, i64p = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_longlong)
+    if hasattr(L, "mld_create"):
+        L.mld_create.argtypes = [C.POINTER(mld_params), C.POINTER(C.c_void_p)]
+    if hasattr(L, "mld_destroy"):
+        L.mld_destroy.argtypes = [vp]
+    if hasattr(L, "mld_last_error"):
+        L.mld_last_error.argtypes = [vp]
+    if hasattr(L, "mld_last_error"):
+        L.mld_last_error.restype = C.c_char_p
+    if hasattr(L, "mld_descriptor_dim"):
+        L.mld_descriptor_dim.argtypes = [vp, ip]
+    if hasattr(L, "mld_max_neighbours"):
+        L.mld_max_neighbours.argtypes = [vp, C.c_int, vp, vp, ip]
+    if hasattr(L, "mld_bso4"):
+        L.mld_bso4.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, vp, C.c_int]
+    if hasattr(L, "mld_batch_neighbours"):
+        L.mld_batch_neighbours.argtypes = [vp, C.c_int, vp, vp, vp, vp, i64p]
+    if hasattr(L, "mld_batch_pairs_dev"):
+        L.mld_batch_pairs_dev.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
+    if hasattr(L, "mld_batch_bso4_dev"):
+        L.mld_batch_bso4_dev.argtypes = [vp, C.c_int, vp, vp]
+    if hasattr(L, "mld_gram_begin"):
+        L.mld_gram_begin.argtypes = [vp, C.c_int]
+    if hasattr(L, "mld_gram_add_rows"):
+        L.mld_gram_add_rows.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+    if hasattr(L, "mld_gram_add_rows_dev"):
+        L.mld_gram_add_rows_dev.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+    if hasattr(L, "mld_gram_add_batch"):
+        L.mld_gram_add_batch.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int]
+    if hasattr(L, "mld_gram_export"):
+        L.mld_gram_export.argtypes = [vp, vp, vp]
+    if hasattr(L, "mld_gram_solve"):
+        L.mld_gram_solve.argtypes = [vp, C.c_double, C.c_int, vp, vp]
+    if hasattr(L, "mld_kernel_po_set"):
+        L.mld_kernel_po_set.argtypes = [vp, C.c_int, vp, C.c_double, C.c_double, C.c_double]
+    if hasattr(L, "mld_kernel_po_dev"):
+        L.mld_kernel_po_dev.argtypes = [vp, vp, vp]
+    if hasattr(L, "mld_get_stream"):
+        L.mld_get_stream.argtypes = [vp, C.POINTER(C.c_void_p)]
+    if hasattr(L, "mld_synchronize"):
+        L.mld_synchronize.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def default_inv_r0_input() -> float:
+    """read_ml_file.F90:425"""
+    return 1.0 - 0.02 / math.pi
+
+
+class Context:
+    """One descriptor context (mld_create); owns a HIP stream and the device tables.
+
+    Mirrors the &input_ml keys of the path: j_max, r_cut, inv_r0_input,
+    lbso4_diag, weighted / weighted_3ch (via ``nchannel`` and species weights),
+    the pair inner cutoff and the precision mode.
+    """
+
+    def __init__(self, j_max: float = 4.0, r_cut: float = 4.7, inv_r0_input: Optional[float] = None,
+                 lbso4_diag: bool = False, nspecies: int = 1, weight: Optional[Sequence[float]] = None,
+                 weight_3ch: Optional[Sequence[float]] = None, r_cut_pair_in: Optional[np.ndarray] = None,
+                 r_cut_width_in: float = 0.4, precision: int = PREC_FP64, device: int = 0):
+        L = load()
+        nch = 1 + (weight is not None) + (weight_3ch is not None)
+        self._keep = []
+        p = mld_params()
+        p.descriptor_type = DESC_BSO4
+        p.j_max = j_max
+        p.r_cut = r_cut
+        p.inv_r0_input = default_inv_r0_input() if inv_r0_input is None else inv_r0_input
+        p.lbso4_diag = int(lbso4_diag)
+        p.nchannel = nch
+        p.nspecies = nspecies
+        for name, arr in (("weight", weight), ("weight_3ch", weight_3ch), ("r_cut_pair_in", r_cut_pair_in)):
+            if arr is not None:
+                a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).reshape(-1))
+                self._keep.append(a)
+                setattr(p, name, a.ctypes.data)
+        p.r_cut_width_in = r_cut_width_in
+        p.precision = precision
+        p.device = device
+        h = C.c_void_p()
+        rc = L.mld_create(C.byref(p), C.byref(h))
+        if rc != 0:
+            raise MldError(rc, "mld_create failed (check j_max, r_cut, inv_r0_input, weights)")
+        self.h = h
+        self.L = L
+        self.precision = precision
+        self.nspecies = nspecies
+        d = C.c_int()
+        L.mld_descriptor_dim(h, C.byref(d))
+        self.D = d.value
+        self.r_cut = r_cut
+
+    def check(self, rc: int):
+        if rc != 0:
+            raise MldError(rc, self.L.mld_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mld_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------- drop-in single configuration ----------------
+    def max_neighbours(self, pos: np.ndarray, cell_rows: np.ndarray) -> int:
+        pos = np.ascontiguousarray(pos, np.float64)
+        cellc = np.ascontiguousarray(np.asarray(cell_rows, np.float64).reshape(3, 3))  # rows = columns of Fortran cell
+        nn = C.c_int()
+        self.check(self.L.mld_max_neighbours(self.h, pos.shape[0], _ptr(pos), _ptr(cellc), C.byref(nn)))
+        return nn.value
+
+    def bso4(self, pos: np.ndarray, cell_rows: np.ndarray, itype: np.ndarray, want_grad: bool = True,
+             nn_max: Optional[int] = None):
+        """compute_bispectrum_so4 + sign flip for one configuration.
+
+        Returns energy (D, nat) and force (D, nat, nn_max+1, 3) as Fortran-ordered
+        arrays (config_desc%energy / %force), n_neigh (nat) and kind_neigh
+        (nat, nn_max) with 1-based atom indices.
+        """
+        pos = np.ascontiguousarray(pos, np.float64)
+        nat = pos.shape[0]
+        cellc = np.ascontiguousarray(np.asarray(cell_rows, np.float64).reshape(3, 3))
+        itype = np.ascontiguousarray(itype, np.int32)
+        if nn_max is None:
+            nn_max = self.max_neighbours(pos, cell_rows)
+        energy = np.zeros((self.D, nat), order="F")
+        force = np.zeros((self.D, nat, nn_max + 1, 3), order="F") if want_grad else None
+        n_neigh = np.zeros(nat, np.int32)
+        kind = np.zeros((nat, nn_max), np.int32, order="F")
+        self.check(self.L.mld_bso4(self.h, nat, _ptr(pos), _ptr(cellc), _ptr(itype), int(want_grad),
+                                   energy.ctypes.data_as(C.c_void_p),
+                                   None if force is None else force.ctypes.data_as(C.c_void_p),
+                                   _ptr(n_neigh), kind.ctypes.data_as(C.c_void_p), nn_max))
+        return energy, force, n_neigh, kind
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        self.check(self.L.mld_get_stream(self.h, C.byref(s)))
+        return s.value or 0
+
+    def synchronize(self):
+        self.check(self.L.mld_synchronize(self.h))
+
+
+from .batch import Batch  # noqa: E402  (device-resident batch API)
+
+__all__ = ["Context", "Batch", "load", "MldError", "PREC_FP64", "PREC_FP32", "EXPORTS"]

@@ -1,0 +1,345 @@
+// C ABI entry points of libmldgpu (declared in include/mldgpu.h).
+#include <cmath>
+#include <cstring>
+
+#include "mld_internal.h"
+
+int mld_fail(mld_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+namespace {
+
+template <typename T>
+__global__ void to_reference_layout(int nat, int nn_max, int D, const int* __restrict__ pair_off,
+                                    const int* __restrict__ pair_idx, const T* __restrict__ grad,
+                                    double* __restrict__ force, int* __restrict__ kind, int* __restrict__ nneigh)
+{
+    // force (D, nat, 0:nn_max, 3) column-major; one block per (atom, slot)
+    int a = blockIdx.x;
+    int k = blockIdx.y;  // 0..nn_max
+    int n = pair_off[a + 1] - pair_off[a];
+    size_t base = (size_t)pair_off[a] + a;
+    for (int q = threadIdx.x; q < 3 * D; q += blockDim.x) {
+        int x = q / D, d = q - x * D;
+        double v = 0.0;
+        if (k <= n) v = (double)grad[((base + k) * 3 + x) * D + d];
+        force[(size_t)d + (size_t)D * ((size_t)a + (size_t)nat * ((size_t)k + (size_t)(nn_max + 1) * x))] = v;
+    }
+    if (threadIdx.x == 0) {
+        if (k >= 1) kind[(size_t)a + (size_t)nat * (k - 1)] = (k <= n) ? (pair_idx[pair_off[a] + k - 1] + 1) : 0;
+        if (k == 0) nneigh[a] = n;
+    }
+}
+
+template <typename T>
+__global__ void widen(size_t n, const T* __restrict__ in, double* __restrict__ out)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)in[i];
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* mld_last_error(const mld_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mld_create(const mld_params* p, mld_ctx** out)
+{
+    if (!p || !out) return MLD_E_ARG;
+    *out = nullptr;
+    mld_ctx* ctx = new mld_ctx();
+    ctx->p = *p;
+    if (p->descriptor_type != MLD_DESC_BSO4) {
+        delete ctx;
+        return MLD_E_UNSUPPORTED;
+    }
+    ctx->jj = (int)(2.0 * p->j_max);
+    if (ctx->jj < 1 || ctx->jj > mld::JJ_MAX || !(p->r_cut > 0.0) || !(p->inv_r0_input > 0.0 && p->inv_r0_input < 1.0) ||
+        p->nchannel < 1 || p->nchannel > mld::NCH_MAX || p->nspecies < 1 ||
+        (p->nchannel >= 2 && !p->weight) || (p->nchannel >= 3 && !p->weight_3ch)) {
+        delete ctx;
+        return MLD_E_ARG;
+    }
+    ctx->nch = p->nchannel;
+    ctx->nsp = p->nspecies;
+    // inv_r0 = pi * inv_r0_input (descriptors.F90:661), r0 = r_cut / inv_r0 (angular_functions.f90:79)
+    ctx->r0 = p->r_cut / (3.14159265358979323846 * p->inv_r0_input);
+    int rc = MLD_OK;
+    auto fail = [&](int code, const std::string& m) {
+        mld_fail(ctx, code, m);
+        rc = code;
+        return code;
+    };
+    if (hipSetDevice(p->device) != hipSuccess) { delete ctx; return MLD_E_DEVICE; }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return MLD_E_DEVICE; }
+    mld::build_tables(ctx->jj, p->lbso4_diag, ctx->ht);
+    if (ctx->ht.nb > mld::NB_MAX) { fail(MLD_E_UNSUPPORTED, "too many bispectrum components"); mld_destroy(ctx); return rc; }
+    ctx->nb = ctx->ht.nb;
+    ctx->D = ctx->nb * ctx->nch;
+    auto up = [&](void** dst, const void* src, size_t bytes) -> bool {
+        if (hipMalloc(dst, bytes ? bytes : 8) != hipSuccess) return false;
+        if (bytes && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return false;
+        return true;
+    };
+    bool ok = up(&ctx->d_rec, ctx->ht.rec.data(), ctx->ht.rec.size() * sizeof(mld::RecEntry)) &&
+              up(&ctx->d_rows, ctx->ht.rows.data(), ctx->ht.rows.size() * sizeof(mld::WRow)) &&
+              up(&ctx->d_cg1, ctx->ht.cg1.data(), ctx->ht.cg1.size() * sizeof(double));
+    std::vector<double> wf((size_t)ctx->nch * ctx->nsp, 1.0);
+    for (int s = 0; s < ctx->nsp; ++s) {
+        if (ctx->nch >= 2) wf[ctx->nsp + s] = p->weight[s];
+        if (ctx->nch >= 3) wf[2 * ctx->nsp + s] = p->weight_3ch[s];
+    }
+    ok = ok && up((void**)&ctx->d_wfac, wf.data(), wf.size() * sizeof(double));
+    if (ok && p->r_cut_pair_in)
+        ok = up((void**)&ctx->d_rin, p->r_cut_pair_in, (size_t)ctx->nsp * ctx->nsp * sizeof(double));
+    if (!ok) { fail(MLD_E_DEVICE, "device allocation of tables failed"); mld_destroy(ctx); return rc; }
+    mld::DevTables& dt = ctx->dt;
+    dt.jj = ctx->jj;
+    dt.nb = ctx->nb;
+    dt.nu = ctx->ht.nu;
+    dt.rec = (const mld::RecEntry*)ctx->d_rec;
+    dt.rows = (const mld::WRow*)ctx->d_rows;
+    dt.cg1 = (const double*)ctx->d_cg1;
+    for (int j = 0; j <= mld::JJ_MAX + 1; ++j) {
+        dt.rowbase[j] = (j <= ctx->jj + 1) ? ctx->ht.rowbase[j] : ctx->ht.rowbase[ctx->jj + 1];
+        dt.wofs[j] = (j <= ctx->jj + 1) ? ctx->ht.wofs[j] : ctx->ht.wofs[ctx->jj + 1];
+    }
+    for (int j = 0; j <= mld::JJ_MAX; ++j) dt.M[j] = (j <= ctx->jj) ? ctx->ht.M[j] : 0;
+    dt.wsize = ctx->ht.wsize;
+    for (int b = 0; b < ctx->nb; ++b) {
+        dt.l[b] = ctx->ht.l[b];
+        dt.l1[b] = ctx->ht.l1[b];
+        dt.l2[b] = ctx->ht.l2[b];
+    }
+    *out = ctx;
+    return MLD_OK;
+}
+
+int mld_destroy(mld_ctx* ctx)
+{
+    if (!ctx) return MLD_OK;
+    void* ptrs[] = {ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
+                    ctx->d_geo, ctx->d_count, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_scan_tmp,
+                    ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_zref, ctx->d_zz};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MLD_OK;
+}
+
+int mld_descriptor_dim(const mld_ctx* ctx, int* D)
+{
+    if (!ctx || !D) return MLD_E_ARG;
+    *D = ctx->D;
+    return MLD_OK;
+}
+
+int mld_get_stream(mld_ctx* ctx, void** stream)
+{
+    if (!ctx || !stream) return MLD_E_ARG;
+    *stream = (void*)ctx->stream;
+    return MLD_OK;
+}
+
+int mld_synchronize(mld_ctx* ctx)
+{
+    if (!ctx) return MLD_E_ARG;
+    MLD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MLD_OK;
+}
+
+int mld_batch_neighbours(mld_ctx* ctx, int nstruct, const int* atom_off, const double* pos_dev,
+                         const double* cell_dev, const int* itype_dev, long long* npairs)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (nstruct < 0 || !atom_off || (nstruct > 0 && (!pos_dev || !cell_dev || !itype_dev)))
+        return mld_fail(ctx, MLD_E_ARG, "mld_batch_neighbours: bad arguments");
+    int natoms = atom_off[nstruct];
+    if (atom_off[0] != 0) return mld_fail(ctx, MLD_E_ARG, "atom_off[0] must be 0");
+    for (int s = 0; s < nstruct; ++s)
+        if (atom_off[s + 1] < atom_off[s]) return mld_fail(ctx, MLD_E_ARG, "atom_off must be non-decreasing");
+    if (natoms > 0) {
+        // species indices select weight / inner-cutoff table rows: validate them once
+        std::vector<int> it(natoms);
+        MLD_HIP(ctx, hipMemcpy(it.data(), itype_dev, (size_t)natoms * sizeof(int), hipMemcpyDeviceToHost));
+        for (int a = 0; a < natoms; ++a)
+            if (it[a] < 0 || it[a] >= ctx->nsp) return mld_fail(ctx, MLD_E_ARG, "itype out of range");
+    }
+    // cells: fetch to host for the per-structure set-up (small/large box, reciprocal vectors)
+    std::vector<double> cells((size_t)9 * nstruct);
+    if (nstruct) MLD_HIP(ctx, hipMemcpy(cells.data(), cell_dev, cells.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<mld::StructGeo> geo(nstruct);
+    std::vector<int> astruct(natoms);
+    for (int s = 0; s < nstruct; ++s) {
+        mld::struct_geometry(&cells[(size_t)9 * s], ctx->p.r_cut, geo[s]);
+        geo[s].atom0 = atom_off[s];
+        geo[s].nat = atom_off[s + 1] - atom_off[s];
+        for (int a = atom_off[s]; a < atom_off[s + 1]; ++a) astruct[a] = s;
+    }
+    if ((size_t)natoms + 1 > ctx->cap_atoms) {
+        if (ctx->d_atom_struct) MLD_HIP(ctx, hipFree(ctx->d_atom_struct));
+        if (ctx->d_count) MLD_HIP(ctx, hipFree(ctx->d_count));
+        if (ctx->d_pair_off) MLD_HIP(ctx, hipFree(ctx->d_pair_off));
+        size_t cap = (size_t)natoms + 1 + natoms / 8 + 64;
+        MLD_HIP(ctx, hipMalloc(&ctx->d_atom_struct, cap * sizeof(int)));
+        MLD_HIP(ctx, hipMalloc(&ctx->d_count, cap * sizeof(int)));
+        MLD_HIP(ctx, hipMalloc(&ctx->d_pair_off, cap * sizeof(int)));
+        ctx->cap_atoms = cap;
+    }
+    if ((size_t)nstruct > ctx->cap_struct) {
+        if (ctx->d_geo) MLD_HIP(ctx, hipFree(ctx->d_geo));
+        size_t cap = (size_t)nstruct + 16;
+        MLD_HIP(ctx, hipMalloc(&ctx->d_geo, cap * sizeof(mld::StructGeo)));
+        ctx->cap_struct = cap;
+    }
+    if (nstruct) MLD_HIP(ctx, hipMemcpyAsync(ctx->d_geo, geo.data(), geo.size() * sizeof(mld::StructGeo), hipMemcpyHostToDevice, ctx->stream));
+    if (natoms) MLD_HIP(ctx, hipMemcpyAsync(ctx->d_atom_struct, astruct.data(), astruct.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    MLD_HIP(ctx, hipMemsetAsync(ctx->d_count + natoms, 0, sizeof(int), ctx->stream));
+    ctx->nstruct = nstruct;
+    ctx->natoms = natoms;
+    ctx->h_atom_off.assign(atom_off, atom_off + nstruct + 1);
+    ctx->d_itype = itype_dev;
+    int rc = launch_neighbours_pos(ctx, pos_dev);
+    if (rc) return rc;
+    if (npairs) *npairs = ctx->npairs;
+    return MLD_OK;
+}
+
+int mld_batch_pairs_dev(mld_ctx* ctx, const int** pair_off_dev, const int** pair_idx_dev)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (pair_off_dev) *pair_off_dev = ctx->d_pair_off;
+    if (pair_idx_dev) *pair_idx_dev = ctx->d_pair_idx;
+    return MLD_OK;
+}
+
+int mld_batch_bso4_dev(mld_ctx* ctx, int want_grad, void* energy_dev, void* grad_dev)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (!energy_dev || (want_grad && !grad_dev)) return mld_fail(ctx, MLD_E_ARG, "mld_batch_bso4_dev: null output");
+    if (ctx->natoms > 0 && !ctx->d_pair_off) return mld_fail(ctx, MLD_E_STATE, "mld_batch_neighbours first");
+    int rc = launch_bso4(ctx, want_grad, energy_dev, grad_dev);
+    if (rc) return rc;
+    ctx->last_energy = energy_dev;
+    ctx->last_grad = want_grad ? grad_dev : nullptr;
+    return MLD_OK;
+}
+
+int mld_max_neighbours(mld_ctx* ctx, int nat, const double* pos, const double* cell, int* nn_max)
+{
+    if (!ctx || nat < 0 || (nat > 0 && (!pos || !cell)) || !nn_max) return MLD_E_ARG;
+    DevBuf dpos, dcell, dtype;
+    MLD_HIP(ctx, hipMalloc(&dpos.p, (size_t)3 * nat * sizeof(double) + 8));
+    MLD_HIP(ctx, hipMalloc(&dcell.p, 9 * sizeof(double)));
+    MLD_HIP(ctx, hipMalloc(&dtype.p, (size_t)nat * sizeof(int) + 8));
+    MLD_HIP(ctx, hipMemcpy(dpos.p, pos, (size_t)3 * nat * sizeof(double), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMemcpy(dcell.p, cell, 9 * sizeof(double), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMemset(dtype.p, 0, (size_t)nat * sizeof(int) + 8));
+    int off[2] = {0, nat};
+    int rc = mld_batch_neighbours(ctx, 1, off, (const double*)dpos.p, (const double*)dcell.p, (const int*)dtype.p, nullptr);
+    if (rc) return rc;
+    std::vector<int> po(nat + 1);
+    MLD_HIP(ctx, hipMemcpy(po.data(), ctx->d_pair_off, (nat + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    int m = 0;
+    for (int a = 0; a < nat; ++a) m = std::max(m, po[a + 1] - po[a]);
+    *nn_max = m;
+    ctx->d_itype = nullptr;
+    return MLD_OK;
+}
+
+int mld_bso4(mld_ctx* ctx, int nat, const double* pos, const double* cell, const int* itype, int want_grad,
+             double* energy, double* force, int* n_neigh, int* kind_neigh, int nn_max)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (nat < 0 || (nat > 0 && (!pos || !cell || !itype || !energy || !n_neigh || !kind_neigh)) ||
+        (want_grad && !force) || nn_max < 0)
+        return mld_fail(ctx, MLD_E_ARG, "mld_bso4: bad arguments");
+    for (int a = 0; a < nat; ++a)
+        if (itype[a] < 0 || itype[a] >= ctx->nsp) return mld_fail(ctx, MLD_E_ARG, "itype out of range");
+    DevBuf dpos, dcell, dtype, den, dgr, dforce, dkind, dnn, dwide;
+    MLD_HIP(ctx, hipMalloc(&dpos.p, (size_t)3 * nat * sizeof(double) + 8));
+    MLD_HIP(ctx, hipMalloc(&dcell.p, 9 * sizeof(double)));
+    MLD_HIP(ctx, hipMalloc(&dtype.p, (size_t)nat * sizeof(int) + 8));
+    MLD_HIP(ctx, hipMemcpy(dpos.p, pos, (size_t)3 * nat * sizeof(double), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMemcpy(dcell.p, cell, 9 * sizeof(double), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMemcpy(dtype.p, itype, (size_t)nat * sizeof(int), hipMemcpyHostToDevice));
+    int off[2] = {0, nat};
+    long long P = 0;
+    int rc = mld_batch_neighbours(ctx, 1, off, (const double*)dpos.p, (const double*)dcell.p, (const int*)dtype.p, &P);
+    if (rc) return rc;
+    std::vector<int> po(nat + 1);
+    MLD_HIP(ctx, hipMemcpy(po.data(), ctx->d_pair_off, (nat + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    for (int a = 0; a < nat; ++a)
+        if (po[a + 1] - po[a] > nn_max) {
+            // build_local_neighbours_ja: ia_n > max_neigh stops (neighbours.F90:352-358)
+            return mld_fail(ctx, MLD_E_NEIGH, "atom " + std::to_string(a + 1) + " has " +
+                                                  std::to_string(po[a + 1] - po[a]) + " neighbours > nn_max=" +
+                                                  std::to_string(nn_max));
+        }
+    size_t esz = ctx->p.precision == MLD_PREC_FP32 ? sizeof(float) : sizeof(double);
+    size_t nslot = (size_t)P + nat;
+    MLD_HIP(ctx, hipMalloc(&den.p, (size_t)nat * ctx->D * esz + 8));
+    if (want_grad) MLD_HIP(ctx, hipMalloc(&dgr.p, nslot * 3 * ctx->D * esz + 8));
+    rc = mld_batch_bso4_dev(ctx, want_grad, den.p, want_grad ? dgr.p : nullptr);
+    if (rc) return rc;
+    size_t ne = (size_t)nat * ctx->D;
+    if (esz == sizeof(double)) {
+        MLD_HIP(ctx, hipMemcpyAsync(energy, den.p, ne * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+        MLD_HIP(ctx, hipMalloc(&dwide.p, ne * sizeof(double) + 8));
+        if (ne) hipLaunchKernelGGL(widen<float>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, ne, (const float*)den.p, (double*)dwide.p);
+        MLD_HIP(ctx, hipMemcpyAsync(energy, dwide.p, ne * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    size_t nf = (size_t)ctx->D * nat * (nn_max + 1) * 3;
+    size_t nk = (size_t)nat * nn_max;
+    MLD_HIP(ctx, hipMalloc(&dkind.p, nk * sizeof(int) + 8));
+    MLD_HIP(ctx, hipMalloc(&dnn.p, (size_t)nat * sizeof(int) + 8));
+    if (want_grad) MLD_HIP(ctx, hipMalloc(&dforce.p, nf * sizeof(double) + 8));
+    else MLD_HIP(ctx, hipMalloc(&dforce.p, (size_t)ctx->D * nat * 3 * sizeof(double) + 8));
+    if (nat) {
+        dim3 grid(nat, nn_max + 1);
+        if (!want_grad) grid.y = 1;
+        // without gradients only kind_neigh / n_neigh are produced: use a zero gradient source
+        if (want_grad) {
+            if (esz == sizeof(double))
+                hipLaunchKernelGGL(to_reference_layout<double>, grid, dim3(256), 0, ctx->stream, nat, nn_max, ctx->D,
+                                   ctx->d_pair_off, ctx->d_pair_idx, (const double*)dgr.p, (double*)dforce.p,
+                                   (int*)dkind.p, (int*)dnn.p);
+            else
+                hipLaunchKernelGGL(to_reference_layout<float>, grid, dim3(256), 0, ctx->stream, nat, nn_max, ctx->D,
+                                   ctx->d_pair_off, ctx->d_pair_idx, (const float*)dgr.p, (double*)dforce.p,
+                                   (int*)dkind.p, (int*)dnn.p);
+            MLD_HIP(ctx, hipGetLastError());
+        }
+    }
+    MLD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (want_grad) {
+        MLD_HIP(ctx, hipMemcpy(force, dforce.p, nf * sizeof(double), hipMemcpyDeviceToHost));
+        MLD_HIP(ctx, hipMemcpy(kind_neigh, dkind.p, nk * sizeof(int), hipMemcpyDeviceToHost));
+        MLD_HIP(ctx, hipMemcpy(n_neigh, dnn.p, (size_t)nat * sizeof(int), hipMemcpyDeviceToHost));
+    } else {
+        // neighbour tables from the pair list on the host
+        std::vector<int> pidx((size_t)P);
+        if (P) MLD_HIP(ctx, hipMemcpy(pidx.data(), ctx->d_pair_idx, (size_t)P * sizeof(int), hipMemcpyDeviceToHost));
+        std::memset(kind_neigh, 0, nk * sizeof(int));
+        for (int a = 0; a < nat; ++a) {
+            n_neigh[a] = po[a + 1] - po[a];
+            for (int k = 0; k < n_neigh[a]; ++k) kind_neigh[(size_t)a + (size_t)nat * k] = pidx[po[a] + k] + 1;
+        }
+    }
+    ctx->d_itype = nullptr;
+    ctx->last_energy = ctx->last_grad = nullptr;
+    return MLD_OK;
+}
+
+}  // extern "C"

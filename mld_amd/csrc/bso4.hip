@@ -1,0 +1,489 @@
+// SO(4) bispectrum descriptors and their position gradients on gfx950.
+//
+// One 256-thread workgroup (4 wavefronts) per atom.  Per channel:
+//   A. density c_j(a,b) = sum_nb fcut U_j(a,b) + delta_ab
+//      (compute_bispectrum_so4.F90:288-439).  A wavefront owns one neighbour at
+//      a time; lanes own the entries of a level, and the recursion of
+//      spherical_4d (angular_functions.f90:152-272) walks level by level
+//      through a wavefront-private LDS buffer.  c stays in registers per lane
+//      and is reduced over the 4 wavefronts once.
+//   B. Z contraction (pack_cmm_into_ZcmmABC, :1014-1143) straight into the
+//      merged gradient operand W_j (one LDS block per level j, layout [e][row]),
+//      and B_bi = Re sum conj(c_l) Z_A (:505-517) by a wavefront reduction.
+//   C. gradients: per neighbour the recursion is replayed with dU, the level-j
+//      vector V_j = conj(U) dfcut rhat + fcut conj(dU) goes to LDS and
+//      dB_bi/dr_k = Re sum_{j,e} W_j[bi][e] V_j[e]      (:505-652, sign flip of
+//      descriptors.F90:298-302 folded in), self slot = -sum_k (:692-695).
+// Precision T = double (plain FP64 form) or float (FP32 form; set-up geometry in FP64).
+#include "mld_internal.h"
+
+namespace {
+
+using mld::DevTables;
+using mld::RecEntry;
+using mld::WRow;
+using mld::ZTerm;
+
+template <typename T>
+struct cx {
+    T r, i;
+};
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b)
+{
+    return cx<T>{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return cx<T>{a.r + b.r, a.i + b.i}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return cx<T>{a.r - b.r, a.i - b.i}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(T s, cx<T> a) { return cx<T>{s * a.r, s * a.i}; }
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int NW = 4;
+constexpr int BS = 64 * NW;
+
+// upper bound of the merged W size (complex entries) over both enumerations
+constexpr int wsize_bound(int jj)
+{
+    int best = 0;
+    for (int diag = 0; diag < 2; ++diag) {
+        int tot = 0;
+        for (int l1 = 0; l1 <= jj; ++l1)
+            for (int l2 = diag ? l1 : 0; l2 <= l1; ++l2)
+                for (int l = (l1 > l2 ? l1 - l2 : l2 - l1); l <= (jj < l1 + l2 ? jj : l1 + l2); ++l) {
+                    if ((l1 + l2 + l) & 1) continue;
+                    if (!diag && l < l1) continue;
+                    tot += (l + 1) * (l + 1);
+                    if (l1 != l) tot += (l1 + 1) * (l1 + 1);
+                    if (l2 != l && l2 != l1) tot += (l2 + 1) * (l2 + 1);
+                }
+        if (tot > best) best = tot;
+    }
+    return best;
+}
+
+template <typename T>
+struct Geo {
+    cx<T> zp, zm, xp, xm;
+    cx<T> dzp[3], dzm[3], dxp[3], dxm[3];
+    T fc, dfc;
+    T drr[3];
+};
+
+// Per-neighbour scalars of spherical_4d (angular_functions.f90:79-115) and the
+// cutoff fcut = (1 - fcut_in) fcut_out (compute_bispectrum_so4.F90:349-377).
+// Evaluated in FP64 whatever T is.
+template <typename T>
+__device__ __forceinline__ Geo<T> make_geo(double4 d, double rcut, double r0, double rin, double win,
+                                           bool grad)
+{
+    const double pi = 3.14159265358979323846;
+    double rr = d.w, x = d.x, y = d.y, z = d.z;
+    double fco = 0.0, dfco = 0.0;
+    if (!(rr > rcut)) {
+        double xx = rr / rcut;
+        fco = 0.5 * (cos(pi * xx) + 1.0);
+        dfco = -0.5 * pi * sin(pi * xx) / rcut;
+    }
+    double fci = 0.0, dfci = 0.0;
+    if (rin > 0.0) {
+        double rc = rin + win;
+        if (!(rr > rc)) {
+            if (rr < rc - win) { fci = 1.0; dfci = 0.0; }
+            else {
+                double xx = pi * (rr - rc + win) / win;
+                fci = 0.5 * (cos(xx) + 1.0);
+                dfci = -0.5 * sin(xx) * pi / win;
+            }
+        }
+    }
+    double fc = (1.0 - fci) * fco;
+    double dfc = -dfci * fco + (1.0 - fci) * dfco;
+    double th0 = rr / r0;
+    double sn = sin(th0), cs = cos(th0);
+    double sin_i = 1.0 / sn, tan_i = 1.0 / tan(th0);
+    double l0 = rr * sin_i, l0i = 1.0 / l0, z0 = rr * tan_i;
+    Geo<T> g;
+    g.zp = cx<T>{(T)(z0 * l0i), (T)(z * l0i)};
+    g.zm = cx<T>{(T)(z0 * l0i), (T)(-z * l0i)};
+    g.xp = cx<T>{(T)(x * l0i), (T)(y * l0i)};
+    g.xm = cx<T>{(T)(x * l0i), (T)(-y * l0i)};
+    g.fc = (T)fc;
+    g.dfc = (T)dfc;
+    double drr[3] = {x / rr, y / rr, z / rr};
+    for (int k = 0; k < 3; ++k) g.drr[k] = (T)drr[k];
+    if (grad) {
+        double dz = tan_i - th0 * sin_i * sin_i;
+        double dil0 = (cs / r0 - l0i) / rr;
+        for (int k = 0; k < 3; ++k) {
+            double a = dz * l0i;
+            g.dzp[k] = cx<T>{(T)((z0 * dil0 + a) * drr[k]), (T)(z * dil0 * drr[k])};
+            g.dzm[k] = cx<T>{(T)((z0 * dil0 + a) * drr[k]), (T)(-z * dil0 * drr[k])};
+            g.dxp[k] = cx<T>{(T)(x * dil0 * drr[k]), (T)(y * dil0 * drr[k])};
+            g.dxm[k] = cx<T>{(T)(x * dil0 * drr[k]), (T)(-y * dil0 * drr[k])};
+        }
+        g.dzp[2].i += (T)l0i;
+        g.dzm[2].i -= (T)l0i;
+        g.dxp[0].r += (T)l0i;
+        g.dxp[1].i += (T)l0i;
+        g.dxm[0].r += (T)l0i;
+        g.dxm[1].i -= (T)l0i;
+    }
+    return g;
+}
+
+// One recursion step for entry (a, b) of level j from level j-1 (Up, dUp).
+template <typename T, bool GRAD>
+__device__ __forceinline__ void rec_step(const RecEntry& R, int b, const Geo<T>& g, const cx<T>* Up,
+                                         const cx<T>* dUp, cx<T>& u, cx<T>* du)
+{
+    bool k0 = (b == 0);
+    cx<T> zq = k0 ? g.zp : g.zm;
+    cx<T> xq = k0 ? g.xm : g.xp;
+    T c1 = (T)R.c1, c2 = (T)R.c2;
+    cx<T> U1 = Up[R.s1], U2 = Up[R.s2];
+    cx<T> t1 = cscale(c1, zq);
+    cx<T> t2 = cx<T>{-c2 * xq.i, c2 * xq.r};  // (i c2) xq
+    u = csub(cmul(t1, U1), cmul(t2, U2));
+    if (GRAD) {
+        cx<T> ic2 = cx<T>{(T)0, c2};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            cx<T> dzk = k0 ? g.dzp[k] : g.dzm[k];
+            cx<T> dxk = k0 ? g.dxm[k] : g.dxp[k];
+            cx<T> a = cadd(cmul(dzk, U1), cmul(zq, dUp[3 * R.s1 + k]));
+            cx<T> bq = cadd(cmul(dxk, U2), cmul(xq, dUp[3 * R.s2 + k]));
+            du[k] = csub(cscale(c1, a), cmul(ic2, bq));
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int JJ>
+constexpr int level_slots(int j)
+{
+    return ((j + 1) * (j + 1) + 63) / 64;
+}
+
+template <typename T, int JJ>
+struct Smem {
+    static constexpr int NU = mld::ucount(JJ);
+    static constexpr int NL = (JJ + 1) * (JJ + 1);
+    static constexpr int WMAX = wsize_bound(JJ);
+    cx<T> c[NU];
+    cx<T> W[WMAX];
+    T selfw[NW][mld::NB_MAX][3];
+    union {
+        cx<T> cpart[NW][NU];
+        struct {
+            cx<T> ub[NW][2][NL];
+            cx<T> dub[NW][2][NL][3];
+            cx<T> vb[NW][NL][3];
+            T outw[NW][mld::NB_MAX][3];
+        } g;
+    } u;
+};
+
+template <typename T, int JJ>
+__global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, const int* __restrict__ pair_off,
+                                                  const int* __restrict__ pair_idx,
+                                                  const double4* __restrict__ pair_d,
+                                                  const int* __restrict__ itype, const double* __restrict__ wfac,
+                                                  const double* __restrict__ rin_tab, int nsp, double win,
+                                                  double rcut, double r0, int nch, int want_grad,
+                                                  T* __restrict__ energy, T* __restrict__ grad)
+{
+    using S = Smem<T, JJ>;
+    constexpr int NU = S::NU;
+    constexpr int NL = S::NL;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    S& sm = *reinterpret_cast<S*>(smem_raw);
+
+    const int g_atom = blockIdx.x;
+    if (g_atom >= natoms) return;
+    const int tid = threadIdx.x;
+    const int w = tid >> 6;
+    const int lane = tid & 63;
+    const int p0 = pair_off[g_atom];
+    const int nn = pair_off[g_atom + 1] - p0;
+    const int nb = tb.nb;
+    const int D = nb * nch;
+    const size_t base = (size_t)p0 + g_atom;  // slot 0 of this atom
+    const int ti = itype[g_atom];
+
+    if (nn == 0) {
+        for (int d = tid; d < D; d += BS) energy[(size_t)g_atom * D + d] = (T)0;
+        if (want_grad)
+            for (int d = tid; d < 3 * D; d += BS) grad[base * 3 * D + d] = (T)0;
+        return;
+    }
+
+    for (int ch = 0; ch < nch; ++ch) {
+        // ---------------- A: density ----------------
+        cx<T> cacc[JJ + 1][2];
+#pragma unroll
+        for (int j = 0; j <= JJ; ++j) cacc[j][0] = cacc[j][1] = cx<T>{(T)0, (T)0};
+        for (int n = w; n < nn; n += NW) {
+            double4 d = pair_d[p0 + n];
+            int tj = itype[pair_idx[p0 + n]];
+            double rin = rin_tab ? rin_tab[ti * nsp + tj] : 0.0;
+            Geo<T> g = make_geo<T>(d, rcut, r0, rin, win, false);
+            T fw = g.fc * (T)wfac[ch * nsp + tj];
+            cx<T>* ub = sm.u.g.ub[w][0];
+            cx<T>* ubn = sm.u.g.ub[w][1];
+            (void)ubn;
+            if (lane == 0) {
+                sm.u.g.ub[w][0][0] = cx<T>{(T)1, (T)0};
+                cacc[0][0].r += fw;
+            }
+#pragma unroll
+            for (int j = 1; j <= JJ; ++j) {
+                wave_sync();
+                const cx<T>* Up = sm.u.g.ub[w][(j - 1) & 1];
+                cx<T>* Un = sm.u.g.ub[w][j & 1];
+#pragma unroll
+                for (int s = 0; s < level_slots<JJ>(j); ++s) {
+                    int e = lane + 64 * s;
+                    if (e < (j + 1) * (j + 1)) {
+                        int b = e % (j + 1);
+                        RecEntry R = tb.rec[mld::uoff(j) + e];
+                        cx<T> u;
+                        rec_step<T, false>(R, b, g, Up, nullptr, u, nullptr);
+                        Un[e] = u;
+                        cacc[j][s] = cadd(cacc[j][s], cscale(fw, u));
+                    }
+                }
+            }
+            (void)ub;
+            wave_sync();
+        }
+        __syncthreads();  // the cpart union member aliases the recursion buffers
+#pragma unroll
+        for (int j = 0; j <= JJ; ++j)
+#pragma unroll
+            for (int s = 0; s < level_slots<JJ>(j); ++s) {
+                int e = lane + 64 * s;
+                if (e < (j + 1) * (j + 1)) sm.u.cpart[w][mld::uoff(j) + e] = cacc[j][s];
+            }
+        __syncthreads();
+        for (int e = tid; e < NU; e += BS) {
+            cx<T> s = sm.u.cpart[0][e];
+#pragma unroll
+            for (int q = 1; q < NW; ++q) s = cadd(s, sm.u.cpart[q][e]);
+            // self term on the diagonal a == b (compute_bispectrum_so4.F90:431-439)
+            int j = 0;
+            while (mld::uoff(j + 1) <= e) ++j;
+            int le = e - mld::uoff(j);
+            if (le / (j + 1) == le % (j + 1)) s.r += (T)1;
+            sm.c[e] = s;
+        }
+        __syncthreads();
+
+        // ---------------- B: Z -> W and B ----------------
+        const int nrows = tb.rowbase[JJ + 1];
+        for (int rw = w; rw < nrows; rw += NW) {
+            int j = 0;
+            while (tb.rowbase[j + 1] <= rw) ++j;
+            const int r = rw - tb.rowbase[j];
+            const int M = tb.M[j];
+            const WRow row = tb.rows[rw];
+            const int K = (j + 1) * (j + 1);
+            T bpart = (T)0;
+            for (int e = lane; e < K; e += 64) {
+                const int A = e / (j + 1), B = e % (j + 1);
+                const int m1 = -j + 2 * A, m2 = -j + 2 * B;
+                cx<T> wv{(T)0, (T)0};
+                for (int q = 0; q < row.nz; ++q) {
+                    const ZTerm zt = row.z[q];
+                    const int J1 = zt.J1, J2 = zt.J2;
+                    const double* cg = tb.cg1 + zt.cgoff;
+                    const cx<T>* c1 = sm.c + mld::uoff(J1);
+                    const cx<T>* c2 = sm.c + mld::uoff(J2);
+                    int malo = max(-J1, m1 - J2), mahi = min(J1, m1 + J2);
+                    int mblo = max(-J1, m2 - J2), mbhi = min(J1, m2 + J2);
+                    cx<T> s{(T)0, (T)0};
+                    for (int ma = malo; ma <= mahi; ma += 2) {
+                        const int ia = (ma + J1) >> 1, ka = (m1 - ma + J2) >> 1;
+                        const T cga = (T)cg[ia * (j + 1) + A];
+                        for (int mb = mblo; mb <= mbhi; mb += 2) {
+                            const int ib = (mb + J1) >> 1, kb = (m2 - mb + J2) >> 1;
+                            const T cgab = cga * (T)cg[ib * (j + 1) + B];
+                            cx<T> p = cmul(c1[ia * (J1 + 1) + ib], c2[ka * (J2 + 1) + kb]);
+                            s.r += cgab * p.r;
+                            s.i += cgab * p.i;
+                        }
+                    }
+                    if (zt.isA) {
+                        cx<T> cl = sm.c[mld::uoff(j) + e];
+                        bpart += cl.r * s.r + cl.i * s.i;  // Re(conj(c_l) Z_A)
+                    }
+                    wv = cadd(wv, cscale((T)zt.scale, s));
+                }
+                sm.W[tb.wofs[j] + e * M + r] = wv;
+            }
+            bool hasA = false;
+            for (int q = 0; q < row.nz; ++q) hasA |= (row.z[q].isA != 0);
+            if (hasA) {
+                T bsum = wave_sum(bpart);
+                if (lane == 0) energy[(size_t)g_atom * D + ch * nb + row.bi] = bsum;
+            }
+        }
+        __syncthreads();
+        if (!want_grad) continue;
+
+        // ---------------- C: gradients ----------------
+        for (int q = tid; q < NW * mld::NB_MAX * 3; q += BS) (&sm.selfw[0][0][0])[q] = (T)0;
+        __syncthreads();
+        for (int n = w; n < nn; n += NW) {
+            double4 d = pair_d[p0 + n];
+            int tj = itype[pair_idx[p0 + n]];
+            double rin = rin_tab ? rin_tab[ti * nsp + tj] : 0.0;
+            Geo<T> g = make_geo<T>(d, rcut, r0, rin, win, true);
+            const T wn = (T)wfac[ch * nsp + tj];
+            T* outw = &sm.u.g.outw[w][0][0];
+            for (int q = lane; q < nb * 3; q += 64) outw[q] = (T)0;
+            cx<T>* vb = &sm.u.g.vb[w][0][0];
+            // level 0: U = 1, dU = 0, V = dfcut rhat
+            if (lane == 0) {
+                sm.u.g.ub[w][0][0] = cx<T>{(T)1, (T)0};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    sm.u.g.dub[w][0][0][k] = cx<T>{(T)0, (T)0};
+                    vb[k] = cx<T>{g.dfc * g.drr[k], (T)0};
+                }
+            }
+#pragma unroll
+            for (int j = 0; j <= JJ; ++j) {
+                if (j > 0) {
+                    wave_sync();
+                    const cx<T>* Up = sm.u.g.ub[w][(j - 1) & 1];
+                    const cx<T>* dUp = &sm.u.g.dub[w][(j - 1) & 1][0][0];
+                    cx<T>* Un = sm.u.g.ub[w][j & 1];
+                    cx<T>* dUn = &sm.u.g.dub[w][j & 1][0][0];
+                    cx<T> us[level_slots<JJ>(JJ)];
+                    cx<T> dus[level_slots<JJ>(JJ)][3];
+#pragma unroll
+                    for (int s = 0; s < level_slots<JJ>(j); ++s) {
+                        int e = lane + 64 * s;
+                        if (e < (j + 1) * (j + 1)) {
+                            int b = e % (j + 1);
+                            RecEntry R = tb.rec[mld::uoff(j) + e];
+                            rec_step<T, true>(R, b, g, Up, dUp, us[s], dus[s]);
+                        }
+                    }
+#pragma unroll
+                    for (int s = 0; s < level_slots<JJ>(j); ++s) {
+                        int e = lane + 64 * s;
+                        if (e < (j + 1) * (j + 1)) {
+                            Un[e] = us[s];
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                dUn[3 * e + k] = dus[s][k];
+                                // vecall = conj(U) dfcut rhat + fcut conj(dU)   (angular_functions.f90:255-258)
+                                T a = g.dfc * g.drr[k];
+                                vb[3 * e + k] = cx<T>{us[s].r * a + g.fc * dus[s][k].r,
+                                                      -us[s].i * a - g.fc * dus[s][k].i};
+                            }
+                        }
+                    }
+                }
+                wave_sync();
+                // contraction of level j: lane r < M_j owns row r, all three directions
+                const int M = tb.M[j];
+                const int K = (j + 1) * (j + 1);
+                const cx<T>* Wj = sm.W + tb.wofs[j];
+                for (int r = lane; r < M; r += 64) {
+                    T a0 = 0, a1 = 0, a2 = 0;
+                    for (int e = 0; e < K; ++e) {
+                        cx<T> wv = Wj[e * M + r];
+                        cx<T> v0 = vb[3 * e], v1 = vb[3 * e + 1], v2 = vb[3 * e + 2];
+                        a0 += wv.r * v0.r - wv.i * v0.i;
+                        a1 += wv.r * v1.r - wv.i * v1.i;
+                        a2 += wv.r * v2.r - wv.i * v2.i;
+                    }
+                    const int bi = tb.rows[tb.rowbase[j] + r].bi;
+                    outw[bi * 3 + 0] += a0;
+                    outw[bi * 3 + 1] += a1;
+                    outw[bi * 3 + 2] += a2;
+                }
+                wave_sync();
+            }
+            // flush neighbour n: grad slot (base + 1 + n), layout [slot][x][D]
+            T* gout = grad + (base + 1 + n) * 3 * (size_t)D + ch * nb;
+            for (int q = lane; q < nb * 3; q += 64) {
+                int x = q / nb, bi = q - x * nb;
+                T v = wn * outw[bi * 3 + x];
+                gout[(size_t)x * D + bi] = v;
+                sm.selfw[w][bi][x] -= v;
+            }
+            wave_sync();
+        }
+        __syncthreads();
+        T* gself = grad + base * 3 * (size_t)D + ch * nb;
+        for (int q = tid; q < nb * 3; q += BS) {
+            int x = q / nb, bi = q - x * nb;
+            T s = sm.selfw[0][bi][x];
+#pragma unroll
+            for (int qq = 1; qq < NW; ++qq) s += sm.selfw[qq][bi][x];
+            gself[(size_t)x * D + bi] = s;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int JJ>
+int launch_t(mld_ctx* ctx, int want_grad, void* energy, void* grad)
+{
+    size_t sh = sizeof(Smem<T, JJ>);
+    auto k = bso4_kernel<T, JJ>;
+    static bool attr_done = false;
+    if (!attr_done) {
+        MLD_HIP(ctx, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+        attr_done = true;
+    }
+    if (ctx->natoms == 0) return MLD_OK;
+    hipLaunchKernelGGL(k, dim3(ctx->natoms), dim3(BS), sh, ctx->stream, ctx->dt, ctx->natoms, ctx->d_pair_off,
+                       ctx->d_pair_idx, ctx->d_pair_d, ctx->d_itype, ctx->d_wfac, ctx->d_rin, ctx->nsp,
+                       ctx->p.r_cut_width_in, ctx->p.r_cut, ctx->r0, ctx->nch, want_grad, (T*)energy, (T*)grad);
+    MLD_HIP(ctx, hipGetLastError());
+    return MLD_OK;
+}
+
+template <typename T>
+int launch_prec(mld_ctx* ctx, int want_grad, void* energy, void* grad)
+{
+    switch (ctx->jj) {
+    case 1: return launch_t<T, 1>(ctx, want_grad, energy, grad);
+    case 2: return launch_t<T, 2>(ctx, want_grad, energy, grad);
+    case 3: return launch_t<T, 3>(ctx, want_grad, energy, grad);
+    case 4: return launch_t<T, 4>(ctx, want_grad, energy, grad);
+    case 5: return launch_t<T, 5>(ctx, want_grad, energy, grad);
+    case 6: return launch_t<T, 6>(ctx, want_grad, energy, grad);
+    case 7: return launch_t<T, 7>(ctx, want_grad, energy, grad);
+    case 8: return launch_t<T, 8>(ctx, want_grad, energy, grad);
+    default: return mld_fail(ctx, MLD_E_UNSUPPORTED, "jj_max = int(2*j_max) must be 1..8");
+    }
+}
+
+}  // namespace
+
+int launch_bso4(mld_ctx* ctx, int want_grad, void* energy, void* grad)
+{
+    if (ctx->p.precision == MLD_PREC_FP32) return launch_prec<float>(ctx, want_grad, energy, grad);
+    return launch_prec<double>(ctx, want_grad, energy, grad);
+}

@@ -1,0 +1,146 @@
+// Internal declarations shared by the HIP translation units of libmldgpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mldgpu.h"
+
+namespace mld {
+
+constexpr int JJ_MAX = 8;          // largest int(2*j_max) built
+constexpr int NB_MAX = 64;         // bispectrum components per channel
+constexpr int NCH_MAX = 3;
+
+// Flat index of U^j entries: level-major, within a level [a][b] where
+// a = (m2 + j)/2 (first index of Umm), b = (m1 + j)/2.
+__host__ __device__ constexpr int uoff(int j) { return j * (j + 1) * (2 * j + 1) / 6; }
+__host__ __device__ constexpr int ucount(int jj) { return uoff(jj + 1); }
+
+// Recursion coefficients of one U^j entry (spherical_4d, angular_functions.f90:152-272):
+//   kind b == 0 : u = c1 zp U'[s1] - (i c2) xm U'[s2]
+//   kind b >= 1 : u = c1 zm U'[s1] - (i c2) xp U'[s2]
+// s1, s2 index level j-1 (local index a'*j + b'); a missing term has c = 0.
+struct RecEntry {
+    double c1, c2;
+    int s1, s2;
+};
+
+// One Z contribution to a merged W row (pack_cmm_into_ZcmmABC,
+// compute_bispectrum_so4.F90:1014-1143): Z(J1,J2 -> J)(m1,m2) =
+//   sum_{ma,mb} CG1[ia][i1] CG1[ib][i2] c_J1[ma][mb] c_J2[m1-ma][m2-mb]
+// with CG1[ia][i1] = cg(J1,ma,J2,m1-ma | J,m1) stored at cgoff, (J1+1) x (J+1).
+struct ZTerm {
+    int J1, J2, cgoff, isA;  // isA: this is Z_A (enters B)
+    double scale;            // 1, (1+l)/(1+l1) or (1+l)/(1+l2)
+};
+
+struct WRow {
+    int bi;
+    int nz;
+    ZTerm z[3];
+};
+
+// Device-side table bundle (all pointers are device memory).
+struct DevTables {
+    int jj, nb, nu;
+    const RecEntry* rec;   // [nu]
+    const WRow* rows;      // [sum_j M_j], level j rows start at rowbase[j]
+    const double* cg1;     // CG1 tables
+    int rowbase[JJ_MAX + 2];
+    int wofs[JJ_MAX + 2];  // complex offset of the level-j W block (layout [e][row])
+    int M[JJ_MAX + 1];
+    int wsize;             // total complex entries of W
+    int l[NB_MAX], l1[NB_MAX], l2[NB_MAX];
+};
+
+struct HostTables {
+    int jj = 0, nb = 0, nu = 0;
+    std::vector<int> l, l1, l2;
+    std::vector<RecEntry> rec;
+    std::vector<WRow> rows;
+    std::vector<double> cg1;
+    int rowbase[JJ_MAX + 2] = {0};
+    int wofs[JJ_MAX + 2] = {0};
+    int M[JJ_MAX + 1] = {0};
+    int wsize = 0;
+};
+
+void build_tables(int jj, int diag, HostTables& t);
+double cg_racah(int j1, int m1, int j2, int m2, int j, int m);
+
+// Per-structure geometry for the neighbour search.
+struct StructGeo {
+    double cell[9];  // column-major, columns = lattice vectors
+    double bg[9];    // reciprocal vectors, bg[k + 3*i] = component k of b_i
+    int small;       // explicit replicas (neighbours_small) vs minimum image
+    int ns[3];       // replica half-range per direction
+    int atom0, nat;
+};
+void struct_geometry(const double* cell_colmajor, double rcut, StructGeo& g);
+
+}  // namespace mld
+
+struct mld_ctx {
+    mld_params p{};
+    int D = 0, nb = 0, jj = 0, nch = 1, nsp = 1;
+    double r0 = 0.0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    mld::HostTables ht;
+    mld::DevTables dt{};
+    // device copies
+    void* d_rec = nullptr;
+    void* d_rows = nullptr;
+    void* d_cg1 = nullptr;
+    double* d_wfac = nullptr;    // [nch * nsp] channel weights (channel 0 = 1)
+    double* d_rin = nullptr;     // [nsp * nsp] inner cutoff or null
+    // batch state
+    int nstruct = 0, natoms = 0;
+    long long npairs = 0;
+    std::vector<int> h_atom_off;
+    int* d_atom_struct = nullptr; size_t cap_atoms = 0;
+    mld::StructGeo* d_geo = nullptr; size_t cap_struct = 0;
+    int* d_count = nullptr;
+    int* d_pair_off = nullptr;   // [natoms + 1]
+    int* d_pair_idx = nullptr;   // [npairs] global atom index
+    double4* d_pair_d = nullptr; // [npairs] (dx, dy, dz, r)
+    size_t cap_pairs = 0;
+    const int* d_itype = nullptr;
+    void* d_scan_tmp = nullptr; size_t cap_scan = 0;
+    // last descriptor outputs (device, owned by caller)
+    const void* last_energy = nullptr;
+    const void* last_grad = nullptr;
+    // Gram state
+    int gD = 0;
+    double* d_G = nullptr;
+    double* d_h = nullptr;
+    void* d_work = nullptr; size_t cap_work = 0;
+    // kernel_po state
+    int nk = 0;
+    double k_sigma = 0, k_length = 1, k_power = 2;
+    double* d_zref = nullptr;    // [nk][D] row-major
+    double* d_zz = nullptr;      // [nk] |z|^2
+};
+
+int mld_fail(mld_ctx* ctx, int code, const std::string& msg);
+#define MLD_HIP(ctx, call)                                                                       \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return mld_fail((ctx), MLD_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// launchers implemented in the .hip units
+int launch_neighbours_pos(mld_ctx* ctx, const double* pos);
+int launch_bso4(mld_ctx* ctx, int want_grad, void* energy, void* grad);
+int launch_to_reference_layout(mld_ctx* ctx, int nat, int nn_max, const void* grad, double* force_dev,
+                               int* kind_dev, int* nneigh_dev);
+int launch_rows_gram(mld_ctx* ctx, int order, const double* E, const double* F_dev, double w_e, double w_f,
+                     int mixed);
+int launch_gram_rows(mld_ctx* ctx, int D, int mc, const double* Ac, const double* y, const double* w);
+int launch_solve(mld_ctx* ctx, double lambda, int method, double* w, double* diag);
+int launch_kernel_po(mld_ctx* ctx, void* ek, void* fk);

@@ -1,0 +1,175 @@
+"""File formats on the input side of the fitting path.
+
+- ``read_poscar``: MILADY's poscar variant (header ``EFS nspecies {El Z}... E E1 E2``,
+  scale, three lattice rows, optional VASP5 species line, counts, ``Cartesian`` or
+  ``Direct``, positions, then optionally a blank line + forces and a blank line +
+  stress).  Restates ``read_poscar_sasha`` (build_subdata.F90:2484-2954).
+- ``read_db_model``: ``db_model.in`` rows ``class klm n_total n_selected E F S
+  w_e w_f w_s ...`` (build_subdata.F90:1757-1790).
+- ``read_eml`` / ``write_eml``: the descriptor dump rows written by
+  ``write_energy_descriptors`` (read_write_descriptors.F90:125-253): atom index +
+  D components, each ``es30.18e3``.
+- ``ELEMENTS``: the periodic-table fields the path reads (Z, mass, covalent
+  radius), from ``periodic_table`` in ml_in_ndm_module.F90.  The reference
+  assigns the masses from default-REAL literals, so the stored double is the
+  single-precision value widened; ``element_mass`` reproduces that.
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+# symbol: (Z, mass literal, covalent radius in pm)  -- ml_in_ndm_module.F90:1863-2339, 2677-2729
+ELEMENTS: Dict[str, tuple] = {
+    "Ti": (22, 47.867, 147.0),
+    "V": (23, 50.942, 134.0),
+    "Cr": (24, 51.996, 130.0),
+    "Mn": (25, 54.938, 135.0),
+    "Fe": (26, 55.845, 126.0),
+    "Co": (27, 58.933, 125.0),
+    "Ni": (28, 58.693, 124.0),
+    "Nb": (41, 92.906, 146.0),
+    "Mo": (42, 95.940, 139.0),
+    "Ta": (73, 180.948, 149.0),
+    "W": (74, 183.840, 141.0),
+}
+
+
+def element_mass(sym: str) -> float:
+    """Mass as the reference stores it (REAL literal widened to double)."""
+    return float(np.float32(ELEMENTS[sym][1]))
+
+
+def element_z(sym: str) -> float:
+    return float(ELEMENTS[sym][0])
+
+
+def element_rcov(sym: str) -> float:
+    return float(ELEMENTS[sym][2])
+
+
+@dataclasses.dataclass
+class Config:
+    """One atomic configuration (``config_real`` fields the hot path reads)."""
+
+    pos: np.ndarray          # (nat, 3) Cartesian positions, Angstrom
+    cell: np.ndarray         # (3, 3) rows are lattice vectors a1, a2, a3
+    itype: np.ndarray        # (nat,) 0-based species index within ``species``
+    species: List[str]
+    energy: float = 0.0      # E_total as read
+    forces: Optional[np.ndarray] = None  # (nat, 3)
+    stress: Optional[np.ndarray] = None
+    has_energy: bool = True
+    has_force: bool = False
+    has_stress: bool = False
+    name: str = ""
+
+    @property
+    def nat(self) -> int:
+        return int(self.pos.shape[0])
+
+
+def _has_number(line: str) -> bool:
+    for tok in line.split():
+        try:
+            float(tok)
+            return True
+        except ValueError:
+            pass
+    return False
+
+
+def read_poscar(path: str, read_forces: bool = True) -> Config:
+    """Read a MILADY poscar (build_subdata.F90:2484-2954)."""
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    head = lines[0].split()
+    efs = head[0]
+    nsp = int(head[1])
+    species = [head[2 + 2 * i] for i in range(nsp)]
+    rest = head[2 + 2 * nsp:]
+    e_total = float(rest[0]) if rest else 0.0
+    alat = float(lines[1].split()[0])
+    box = np.array([[float(t) for t in lines[2 + i].split()[:3]] for i in range(3)]) * alat
+    li = 5
+    if not _has_number(lines[li]):
+        li += 1  # VASP5 species line
+    counts = [int(t) for t in lines[li].split()[:nsp]]
+    li += 1
+    mode = lines[li].strip()[:1].lower()
+    li += 1
+    nat = sum(counts)
+    xc = np.array([[float(t) for t in lines[li + i].split()[:3]] for i in range(nat)])
+    li += nat
+    if mode == "d":
+        pos = xc @ box
+    else:
+        pos = xc
+    itype = np.concatenate([np.full(c, i, dtype=np.int32) for i, c in enumerate(counts)])
+    forces = stress = None
+    if read_forces and li + 1 + nat <= len(lines):
+        try:
+            forces = np.array([[float(t) for t in lines[li + 1 + i].split()[:3]] for i in range(nat)])
+            li2 = li + 1 + nat
+            if li2 + 1 < len(lines):
+                stress = np.array([float(t) for t in lines[li2 + 1].split()[:6]])
+        except (ValueError, IndexError):
+            forces = None
+    return Config(pos=pos, cell=box, itype=itype, species=species, energy=e_total,
+                  forces=forces, stress=stress, has_energy=efs[0] == "1",
+                  has_force=len(efs) > 1 and efs[1] == "1" and forces is not None,
+                  has_stress=len(efs) > 2 and efs[2] == "1" and stress is not None,
+                  name=os.path.basename(path))
+
+
+@dataclasses.dataclass
+class DbClass:
+    cls: str
+    klm: str
+    n_total: int
+    n_selected: int
+    use_e: bool
+    use_f: bool
+    use_s: bool
+    weights: Sequence[float]
+
+
+def read_db_model(path: str) -> List[DbClass]:
+    """Read db_model.in (build_subdata.F90:1757-1790); '#' lines are comments."""
+    out = []
+    with open(path) as fh:
+        for line in fh:
+            s = line.strip()
+            if not s or s.startswith("#"):
+                continue
+            t = s.split()
+            flag = lambda v: v.upper().startswith("T")
+            out.append(DbClass(cls=t[0], klm=t[1], n_total=int(t[2]), n_selected=int(t[3]),
+                               use_e=flag(t[4]), use_f=flag(t[5]), use_s=flag(t[6]),
+                               weights=[float(x.replace("d", "e").replace("D", "e")) for x in t[7:]]))
+    return out
+
+
+def read_eml(path: str) -> np.ndarray:
+    """Rows of an .eml descriptor dump: column 0 = atom index, then D components."""
+    return np.loadtxt(path, ndmin=2)
+
+
+def write_eml(path: str, energy: np.ndarray) -> None:
+    """Write per-atom descriptors (nat, D) in the es30.18e3 row format."""
+    with open(path, "w") as fh:
+        for ia, row in enumerate(np.asarray(energy, dtype=np.float64)):
+            fh.write(_es30(float(ia + 1)) + " " + "".join(_es30(float(v)) for v in row) + "\n")
+
+
+def _es30(v: float) -> str:
+    # Fortran es30.18e3: d.dddE+eee, right-aligned in 30 characters
+    if v == 0.0:
+        s = "0.000000000000000000E+000"
+    else:
+        m, e = f"{v:.18E}".split("E")
+        s = f"{m}E{int(e):+04d}"
+    return s.rjust(30)
