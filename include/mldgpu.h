@@ -106,6 +106,10 @@ int mld_gram_add_rows_dev(mld_ctx* ctx, int D, int mc, const double* Ac_dev, con
 int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F_dev, double w_e,
                        double w_f, int mixed);
 int mld_gram_export(mld_ctx* ctx, double* G, double* h);
+/* keep = 1: mld_gram_add_batch keeps its FP64 rows resident in HBM so that the
+ * refinement residual of mld_gram_solve is r = A^T (y - A w) - Lambda w from the
+ * rows themselves (used with the mixed Gram; bytes = rows * padded columns * 8). */
+int mld_gram_set_keep_rows(mld_ctx* ctx, int keep);
 /* (G + lambda I) w = h by FP64 Cholesky with FP64 iterative refinement.
  * diag_out (optional, 4 doubles): [iterations, final relative residual, min pivot, max pivot]. */
 int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w, double* diag_out);

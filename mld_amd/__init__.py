@@ -25,7 +25,8 @@ DESC_BSO4 = 9
 EXPORTS = [
     "mld_create", "mld_destroy", "mld_last_error", "mld_descriptor_dim", "mld_max_neighbours", "mld_bso4",
     "mld_batch_neighbours", "mld_batch_pairs_dev", "mld_batch_bso4_dev", "mld_gram_begin", "mld_gram_add_rows",
-    "mld_gram_add_rows_dev", "mld_gram_add_batch", "mld_gram_export", "mld_gram_solve", "mld_kernel_po_set",
+    "mld_gram_add_rows_dev", "mld_gram_add_batch", "mld_gram_export", "mld_gram_set_keep_rows", "mld_gram_solve",
+    "mld_kernel_po_set",
     "mld_kernel_po_dev", "mld_get_stream", "mld_synchronize",
 ]
 
@@ -64,6 +65,12 @@ def load() -> C.CDLL:
         return _lib
     if not os.path.exists(LIBPATH):
         raise ImportError(f"{LIBPATH} is missing: run `python -m mld_amd.build` (hipcc, gfx950)")
+    try:
+        # torch ships its own libamdhip64.so.7 (same soname as the system one): load it
+        # first so the process holds a single HIP runtime shared by torch and the library
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIBPATH)
     vp, ip, dp, i64p = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_longlong)
     if hasattr(L, "mld_create"):
@@ -96,6 +103,8 @@ def load() -> C.CDLL:
         L.mld_gram_add_batch.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int]
     if hasattr(L, "mld_gram_export"):
         L.mld_gram_export.argtypes = [vp, vp, vp]
+    if hasattr(L, "mld_gram_set_keep_rows"):
+        L.mld_gram_set_keep_rows.argtypes = [vp, C.c_int]
     if hasattr(L, "mld_gram_solve"):
         L.mld_gram_solve.argtypes = [vp, C.c_double, C.c_int, vp, vp]
     if hasattr(L, "mld_kernel_po_set"):

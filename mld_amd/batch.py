@@ -68,9 +68,15 @@ class Batch:
         return energy, grad
 
     def bso4(self, energy, grad=None):
+        """Descriptors (+ gradients) of every structure into caller tensors.
+
+        The context keeps raw device pointers to these outputs for the row /
+        kernel phases that follow, so the batch holds references to them until
+        the next call."""
         ctx = self.ctx
         ctx.check(ctx.L.mld_batch_bso4_dev(ctx.h, int(grad is not None), C.c_void_p(energy.data_ptr()),
                                            None if grad is None else C.c_void_p(grad.data_ptr())))
+        self.energy, self.grad = energy, grad
 
 
 _hip = None

@@ -127,7 +127,9 @@ int mld_create(const mld_params* p, mld_ctx** out)
 int mld_destroy(mld_ctx* ctx)
 {
     if (!ctx) return MLD_OK;
-    void* ptrs[] = {ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
+    gram_scratch_free(ctx);
+    kept_rows_free(ctx);
+    void* ptrs[] = {ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
                     ctx->d_geo, ctx->d_count, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_scan_tmp,
                     ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_zref, ctx->d_zz};
     for (void* q : ptrs)
@@ -202,6 +204,13 @@ int mld_batch_neighbours(mld_ctx* ctx, int nstruct, const int* atom_off, const d
         MLD_HIP(ctx, hipMalloc(&ctx->d_geo, cap * sizeof(mld::StructGeo)));
         ctx->cap_struct = cap;
     }
+    if ((size_t)nstruct + 1 > ctx->cap_struct_off) {
+        if (ctx->d_atom_off) MLD_HIP(ctx, hipFree(ctx->d_atom_off));
+        size_t cap = (size_t)nstruct + 17;
+        MLD_HIP(ctx, hipMalloc(&ctx->d_atom_off, cap * sizeof(int)));
+        ctx->cap_struct_off = cap;
+    }
+    MLD_HIP(ctx, hipMemcpyAsync(ctx->d_atom_off, atom_off, (size_t)(nstruct + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     if (nstruct) MLD_HIP(ctx, hipMemcpyAsync(ctx->d_geo, geo.data(), geo.size() * sizeof(mld::StructGeo), hipMemcpyHostToDevice, ctx->stream));
     if (natoms) MLD_HIP(ctx, hipMemcpyAsync(ctx->d_atom_struct, astruct.data(), astruct.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     MLD_HIP(ctx, hipMemsetAsync(ctx->d_count + natoms, 0, sizeof(int), ctx->stream));

@@ -1,0 +1,636 @@
+// Design-matrix rows and the streamed normal equations G += A^T A, h += A^T y on gfx950.
+//
+// Rows (restating the packing of the reference, scaled by sqrt(w) like
+// gram_accumulate_config, module_online_fit.F90:210-215):
+//   energy row of structure s : [N_s, sum_a B_a, (order 2) sum_a B_a(p) B_a(q), p <= q]
+//        (mld_energy.F90:91-117, 246-258; module_online_fit.F90:272-317)
+//   force row of atom k, dir x: [0, -(dB_k^(0) + sum_{i: k in nb(i)} dB_i^(k)),
+//        (order 2) -sum (g_p B_i(q) + g_q B_i(p)), p <= q]
+//        (mld_forces.F90:222-263, 471-489; module_online_fit.F90:324-369)
+//   The force scatter runs through a CSR transpose of the pair list built with a
+//   stable radix sort, so every row is summed in a fixed order without atomics.
+//   Order 2 keeps the symmetric-unique columns (1 + D + D(D+1)/2); the full
+//   D^2 layout of the reference is recovered when the coefficients are written.
+// Gram (module_online_fit.F90:253-258, DSYRK 'U' + DGEMV):
+//   FP64 form  : v_mfma_f64_16x16x4_f64 tiles, FP64 accumulation;
+//   mixed form : every row element split into three bf16 terms (hi, mid, lo),
+//                products hi*hi + hi*mid + mid*hi + mid*mid + hi*lo + lo*hi on
+//                v_mfma_f32_16x16x32_bf16 with FP32 accumulation, flushed to FP64
+//                every KFLUSH rows.
+// Split-K over row slices writes per-slice partial tiles that a reduction kernel
+// adds in slice order: the result does not depend on scheduling.
+#include <hipcub/hipcub.hpp>
+
+#include "mld_internal.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int TILE = 64;
+
+__host__ __device__ inline int tri_cols(int D) { return D * (D + 1) / 2; }
+
+// ---------------------------------------------------------------- reverse pairs
+__global__ void pair_src_kernel(int natoms, const int* __restrict__ pair_off, int* __restrict__ pair_src)
+{
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms) return;
+    for (int p = pair_off[a]; p < pair_off[a + 1]; ++p) pair_src[p] = a;
+}
+
+__global__ void iota_kernel(int n, int* __restrict__ v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = i;
+}
+
+__global__ void rev_count_kernel(int natoms, const int* __restrict__ sorted_tgt, int npairs, int* __restrict__ rev_off)
+{
+    // rev_off[t] = first position of target t in the sorted key array (lower bound)
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > natoms) return;
+    int lo = 0, hi = npairs;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (sorted_tgt[mid] < t) lo = mid + 1;
+        else hi = mid;
+    }
+    rev_off[t] = lo;
+}
+
+// ---------------------------------------------------------------- rows
+template <typename T>
+__global__ void rows_energy_kernel(int order, int D, int ldA, const int* __restrict__ atom_off,
+                                   const T* __restrict__ energy, const double* __restrict__ E, double swe,
+                                   double* __restrict__ A, double* __restrict__ y)
+{
+    const int s = blockIdx.x;
+    const int a0 = atom_off[s], a1 = atom_off[s + 1];
+    const size_t row = (size_t)s + 3 * (size_t)a0;
+    double* Ar = A + row * ldA;
+    const int ncol = 1 + D + (order == 2 ? tri_cols(D) : 0);
+    for (int c = threadIdx.x; c < ldA; c += blockDim.x) {
+        double v = 0.0;
+        if (c == 0) v = (double)(a1 - a0);
+        else if (c <= D) {
+            for (int a = a0; a < a1; ++a) v += (double)energy[(size_t)a * D + c - 1];
+        } else if (c < ncol) {
+            int q = c - 1 - D, p = 0;
+            while (q >= D - p) { q -= D - p; ++p; }
+            q += p;
+            for (int a = a0; a < a1; ++a) v += (double)energy[(size_t)a * D + p] * (double)energy[(size_t)a * D + q];
+        }
+        Ar[c] = v * swe;
+    }
+    if (threadIdx.x == 0) y[row] = E[s] * swe;
+}
+
+// one block per (atom k, direction x)
+template <typename T>
+__global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int ldA, const int* __restrict__ atom_struct,
+                                                         const int* __restrict__ atom_off, const int* __restrict__ pair_off,
+                                                         const int* __restrict__ rev_off, const int* __restrict__ rev_pair,
+                                                         const int* __restrict__ pair_src, const T* __restrict__ energy,
+                                                         const T* __restrict__ grad, const double* __restrict__ F,
+                                                         double swf, double* __restrict__ A, double* __restrict__ y)
+{
+    const int k = blockIdx.x;
+    const int x = blockIdx.y;
+    const int s = atom_struct[k];
+    const size_t row = (size_t)s + 3 * (size_t)atom_off[s] + 1 + 3 * (size_t)(k - atom_off[s]) + x;
+    double* Ar = A + row * ldA;
+    const int r0 = rev_off[k], r1 = rev_off[k + 1];
+    const int nslot = 1 + (r1 - r0);
+    constexpr int CH = 16;
+    __shared__ double sg[CH][mld::NB_MAX * mld::NCH_MAX];
+    __shared__ double sb[CH][mld::NB_MAX * mld::NCH_MAX];
+    const int ncol = 1 + D + (order == 2 ? tri_cols(D) : 0);
+    // per-thread accumulators for up to 8 columns: c = threadIdx.x + 256*u
+    double acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+    for (int c0 = 0; c0 < nslot; c0 += CH) {
+        const int nc = min(CH, nslot - c0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < nc * D; q += blockDim.x) {
+            int sl = q / D, d = q - sl * D;
+            int idx = c0 + sl;
+            size_t slot;
+            int centre;
+            if (idx == 0) {
+                slot = (size_t)pair_off[k] + k;  // self slot of k
+                centre = k;
+            } else {
+                int p = rev_pair[r0 + idx - 1];
+                centre = pair_src[p];
+                slot = (size_t)p + centre + 1;
+            }
+            sg[sl][d] = (double)grad[(slot * 3 + x) * D + d];
+            sb[sl][d] = (double)energy[(size_t)centre * D + d];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int c = threadIdx.x + 256 * u;
+            if (c >= ncol || c == 0) continue;
+            if (c <= D) {
+                for (int sl = 0; sl < nc; ++sl) acc[u] -= sg[sl][c - 1];
+            } else {
+                int q = c - 1 - D, p = 0;
+                while (q >= D - p) { q -= D - p; ++p; }
+                q += p;
+                for (int sl = 0; sl < nc; ++sl) acc[u] -= sg[sl][p] * sb[sl][q] + sg[sl][q] * sb[sl][p];
+            }
+        }
+    }
+    for (int u = 0; u < 8; ++u) {
+        int c = threadIdx.x + 256 * u;
+        if (c < ldA) Ar[c] = acc[u] * swf;  // column 0 and padding stay zero
+    }
+    for (int c = threadIdx.x + 256 * 8; c < ldA; c += blockDim.x) Ar[c] = 0.0;
+    if (threadIdx.x == 0) y[row] = F[3 * (size_t)k + x] * swf;
+}
+
+__global__ void scale_rows_kernel(int M, int Dc, int ldA, const double* __restrict__ Ain, const double* __restrict__ yin,
+                                  const double* __restrict__ w, double* __restrict__ A, double* __restrict__ y)
+{
+    // caller-packed columns Ac (Dc, M) column-major -> rows scaled by sqrt(w)
+    int r = blockIdx.x;
+    double sw = sqrt(w[r]);
+    for (int c = threadIdx.x; c < ldA; c += blockDim.x) A[(size_t)r * ldA + c] = (c < Dc) ? Ain[(size_t)r * Dc + c] * sw : 0.0;
+    if (threadIdx.x == 0) y[r] = yin[r] * sw;
+}
+
+// ---------------------------------------------------------------- FP64 SYRK
+// grid.x = upper tiles (ti <= tj), grid.y = row slices; 256 threads; wave (wi, wj)
+// owns a 32x32 quarter = 2x2 MFMA 16x16 tiles.
+__global__ void __launch_bounds__(256) syrk_f64_kernel(int M, int ldA, int ntile, const double* __restrict__ A,
+                                                       int rows_per_slice, double* __restrict__ part)
+{
+    int t = blockIdx.x, ti = 0;
+    while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
+    const int tj = ti + t;
+    const int r0 = blockIdx.y * rows_per_slice;
+    const int r1 = min(M, r0 + rows_per_slice);
+    constexpr int KB = 16;
+    __shared__ double Ai[KB][TILE + 1];
+    __shared__ double Aj[KB][TILE + 1];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int wi = w >> 1, wj = w & 1;
+    d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = d4{0, 0, 0, 0};
+    for (int rb = r0; rb < r1; rb += KB) {
+        __syncthreads();
+        for (int q = tid; q < KB * TILE; q += 256) {
+            int rr = q / TILE, cc = q - rr * TILE;
+            int r = rb + rr;
+            Ai[rr][cc] = (r < r1) ? A[(size_t)r * ldA + ti * TILE + cc] : 0.0;
+            Aj[rr][cc] = (r < r1) ? A[(size_t)r * ldA + tj * TILE + cc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k0 = 0; k0 < KB; k0 += 4) {
+            double af[2], bf[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) af[a] = Ai[k0 + (lane >> 4)][wi * 32 + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bf[b] = Aj[k0 + (lane >> 4)][wj * 32 + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+        }
+    }
+    // f64 16x16x4 output: register v of lane l holds C[l/16 + 4v][l%16]
+    double* P = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * TILE * TILE;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int i = wi * 32 + a * 16 + (lane >> 4) + 4 * v;
+                int j = wj * 32 + b * 16 + (lane & 15);
+                P[i * TILE + j] = acc[a][b][v];
+            }
+}
+
+// ---------------------------------------------------------------- mixed SYRK
+__global__ void split_bf16_kernel(size_t n, const double* __restrict__ A, __bf16* __restrict__ hi,
+                                  __bf16* __restrict__ mid, __bf16* __restrict__ lo)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a = (float)A[i];
+    __bf16 h = (__bf16)a;
+    float r1 = a - (float)h;
+    __bf16 m = (__bf16)r1;
+    float r2 = r1 - (float)m;
+    hi[i] = h;
+    mid[i] = m;
+    lo[i] = (__bf16)r2;
+}
+
+template <int NT>
+__global__ void __launch_bounds__(256) syrk_bf16_kernel(int M, int ldA, int ntile, const __bf16* __restrict__ Ah,
+                                                        const __bf16* __restrict__ Am, const __bf16* __restrict__ Al,
+                                                        int rows_per_slice, int kflush, double* __restrict__ part)
+{
+    int t = blockIdx.x, ti = 0;
+    while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
+    const int tj = ti + t;
+    const int r0 = blockIdx.y * rows_per_slice;
+    const int r1 = min(M, r0 + rows_per_slice);
+    constexpr int KB = 32;
+    // transposed staging [col][row] so 8 consecutive rows of a column are one 16-byte read
+    __shared__ __attribute__((aligned(16))) __bf16 S[3][2][TILE][KB + 8];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int wi = w >> 1, wj = w & 1;
+    f4 acc[2][2];
+    double accd[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            acc[a][b] = f4{0, 0, 0, 0};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) accd[a][b][v] = 0.0;
+        }
+    const __bf16* src[3] = {Ah, Am, Al};
+    int since = 0;
+    for (int rb = r0; rb < r1; rb += KB) {
+        __syncthreads();
+        for (int q = tid; q < KB * TILE; q += 256) {
+            int rr = q / TILE, cc = q - rr * TILE;
+            int r = rb + rr;
+#pragma unroll
+            for (int part_ = 0; part_ < 3; ++part_) {
+                __bf16 vi = (__bf16)0.0f, vj = (__bf16)0.0f;
+                if (r < r1) {
+                    vi = src[part_][(size_t)r * ldA + ti * TILE + cc];
+                    vj = src[part_][(size_t)r * ldA + tj * TILE + cc];
+                }
+                S[part_][0][cc][rr] = vi;
+                S[part_][1][cc][rr] = vj;
+            }
+        }
+        __syncthreads();
+        bf8 af[3][2], bfr[3][2];
+#pragma unroll
+        for (int part_ = 0; part_ < 3; ++part_) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+                af[part_][a] = *reinterpret_cast<const bf8*>(&S[part_][0][wi * 32 + a * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                bfr[part_][b] = *reinterpret_cast<const bf8*>(&S[part_][1][wj * 32 + b * 16 + (lane & 15)][8 * (lane >> 4)]);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                // small terms first so they are not absorbed by the large ones
+                if (NT >= 6) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2][a], bfr[0][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[2][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][a], bfr[1][b], acc[a][b], 0, 0, 0);
+                }
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][a], bfr[0][b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[1][b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[0][b], acc[a][b], 0, 0, 0);
+            }
+        since += KB;
+        if (since >= kflush) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) accd[a][b][v] += (double)acc[a][b][v];
+                    acc[a][b] = f4{0, 0, 0, 0};
+                }
+            since = 0;
+        }
+    }
+    double* P = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * TILE * TILE;
+    // bf16 16x16x32 output: register v of lane l holds C[4*(l/16) + v][l%16]
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int i = wi * 32 + a * 16 + 4 * (lane >> 4) + v;
+                int j = wj * 32 + b * 16 + (lane & 15);
+                P[i * TILE + j] = accd[a][b][v] + (double)acc[a][b][v];
+            }
+}
+
+// G (ldG x ldG, row-major, upper tiles) += sum over slices of part
+__global__ void syrk_reduce_kernel(int ntile, int nslice, int ldG, const double* __restrict__ part, double* __restrict__ G)
+{
+    int t = blockIdx.x, ti = 0;
+    while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
+    const int tj = ti + t;
+    const int ntiles_up = gridDim.x;
+    for (int q = threadIdx.x; q < TILE * TILE; q += blockDim.x) {
+        double s = 0.0;
+        for (int sl = 0; sl < nslice; ++sl) s += part[((size_t)sl * ntiles_up + blockIdx.x) * TILE * TILE + q];
+        int i = q / TILE, j = q - i * TILE;
+        G[(size_t)(ti * TILE + i) * ldG + tj * TILE + j] += s;
+    }
+}
+
+// h += A^T y over row slices, partial[slice][c]
+__global__ void gemv_t_kernel(int M, int ldA, const double* __restrict__ A, const double* __restrict__ y,
+                              int rows_per_slice, double* __restrict__ part)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int r0 = blockIdx.y * rows_per_slice, r1 = min(M, r0 + rows_per_slice);
+    if (c >= ldA) return;
+    double s = 0.0;
+    for (int r = r0; r < r1; ++r) s += A[(size_t)r * ldA + c] * y[r];
+    part[(size_t)blockIdx.y * ldA + c] = s;
+}
+
+__global__ void gemv_reduce_kernel(int ldA, int nslice, const double* __restrict__ part, double* __restrict__ h)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ldA) return;
+    double s = 0.0;
+    for (int sl = 0; sl < nslice; ++sl) s += part[(size_t)sl * ldA + c];
+    h[c] += s;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host side
+static int ensure(mld_ctx* ctx, void** p, size_t* cap, size_t bytes)
+{
+    if (bytes <= *cap) return MLD_OK;
+    if (*p) MLD_HIP(ctx, hipFree(*p));
+    *p = nullptr;
+    MLD_HIP(ctx, hipMalloc(p, bytes));
+    *cap = bytes;
+    return MLD_OK;
+}
+
+struct GramScratch {
+    void* part = nullptr; size_t cap_part = 0;
+    void* gpart = nullptr; size_t cap_gpart = 0;
+    void* rows = nullptr; size_t cap_rows = 0;
+    void* y = nullptr; size_t cap_y = 0;
+    void* split = nullptr; size_t cap_split = 0;
+    void* rev = nullptr; size_t cap_rev = 0;
+};
+
+// Gram of row-major rows A (M x ldA, ldA multiple of 64) into ctx->d_G / d_h.
+int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, int mixed)
+{
+    if (M <= 0) return MLD_OK;
+    hipStream_t st = ctx->stream;
+    const int ldA = ctx->gld;
+    const int ntile = ldA / TILE;
+    const int nup = ntile * (ntile + 1) / 2;
+    int nslice = std::max(1, std::min(1024, (2048 + nup - 1) / nup));
+    int rps = (M + nslice - 1) / nslice;
+    rps = ((rps + 31) / 32) * 32;
+    nslice = (M + rps - 1) / rps;
+    GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
+    int rc = ensure(ctx, &gs.part, &gs.cap_part, (size_t)nslice * nup * TILE * TILE * sizeof(double));
+    if (rc) return rc;
+    if (!mixed) {
+        hipLaunchKernelGGL(syrk_f64_kernel, dim3(nup, nslice), dim3(256), 0, st, M, ldA, ntile, A, rps,
+                           (double*)gs.part);
+    } else {
+        size_t n = (size_t)M * ldA;
+        rc = ensure(ctx, &gs.split, &gs.cap_split, 3 * n * sizeof(__bf16));
+        if (rc) return rc;
+        __bf16* h = (__bf16*)gs.split;
+        hipLaunchKernelGGL(split_bf16_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, A, h, h + n, h + 2 * n);
+        hipLaunchKernelGGL(syrk_bf16_kernel<6>, dim3(nup, nslice), dim3(256), 0, st, M, ldA, ntile, h, h + n,
+                           h + 2 * n, rps, 512, (double*)gs.part);
+    }
+    MLD_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(syrk_reduce_kernel, dim3(nup), dim3(256), 0, st, ntile, nslice, ldA, (const double*)gs.part,
+                       ctx->d_G);
+    // h
+    int hs = std::max(1, std::min(512, M / 256));
+    int hrps = (M + hs - 1) / hs;
+    hs = (M + hrps - 1) / hrps;
+    rc = ensure(ctx, &gs.gpart, &gs.cap_gpart, (size_t)hs * ldA * sizeof(double));
+    if (rc) return rc;
+    hipLaunchKernelGGL(gemv_t_kernel, dim3((ldA + 255) / 256, hs), dim3(256), 0, st, M, ldA, A, y, hrps,
+                       (double*)gs.gpart);
+    hipLaunchKernelGGL(gemv_reduce_kernel, dim3((ldA + 255) / 256), dim3(256), 0, st, ldA, hs,
+                       (const double*)gs.gpart, ctx->d_h);
+    MLD_HIP(ctx, hipGetLastError());
+    ctx->gram_rows += M;
+    return MLD_OK;
+}
+
+// rows of the last descriptor batch (energies + forces), order 1 or 2
+int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_dev, double w_e, double w_f,
+                double** A_out, double** y_out, int* M_out)
+{
+    hipStream_t st = ctx->stream;
+    const int natoms = ctx->natoms, nstruct = ctx->nstruct;
+    const long long P = ctx->npairs;
+    const int D = ctx->D;
+    const int ldA = ctx->gld;
+    const int M = nstruct + 3 * natoms;
+    GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
+    // CSR transpose of the pair list: rev_pair sorted by (target, pair index)
+    auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
+    size_t tmp = 0;
+    MLD_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const int*)nullptr, (int*)nullptr, (const int*)nullptr,
+                                                    (int*)nullptr, (int)P, 0, 32, st));
+    size_t need = 4 * r256((size_t)P * sizeof(int)) + r256((size_t)(natoms + 2) * sizeof(int)) + r256(tmp) + 256;
+    int rc = ensure(ctx, &gs.rev, &gs.cap_rev, need);
+    if (rc) return rc;
+    char* base = (char*)gs.rev;
+    int* pair_src = (int*)base; base += ((P * sizeof(int) + 255) / 256) * 256;
+    int* iota = (int*)base; base += ((P * sizeof(int) + 255) / 256) * 256;
+    int* keys = (int*)base; base += ((P * sizeof(int) + 255) / 256) * 256;
+    int* rev_pair = (int*)base; base += ((P * sizeof(int) + 255) / 256) * 256;
+    int* rev_off = (int*)base; base += (((natoms + 2) * sizeof(int) + 255) / 256) * 256;
+    void* sort_tmp = base;
+    if (natoms) hipLaunchKernelGGL(pair_src_kernel, dim3((natoms + 255) / 256), dim3(256), 0, st, natoms, ctx->d_pair_off, pair_src);
+    if (P) {
+        hipLaunchKernelGGL(iota_kernel, dim3((P + 255) / 256), dim3(256), 0, st, (int)P, iota);
+        MLD_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(sort_tmp, tmp, ctx->d_pair_idx, keys, iota, rev_pair, (int)P, 0, 32, st));
+    }
+    hipLaunchKernelGGL(rev_count_kernel, dim3((natoms + 1 + 255) / 256), dim3(256), 0, st, natoms, keys, (int)P, rev_off);
+    MLD_HIP(ctx, hipGetLastError());
+    rc = ensure(ctx, &gs.rows, &gs.cap_rows, (size_t)M * ldA * sizeof(double) + 64);
+    if (rc) return rc;
+    rc = ensure(ctx, &gs.y, &gs.cap_y, (size_t)M * sizeof(double) + 64);
+    if (rc) return rc;
+    double* A = (double*)gs.rows;
+    double* y = (double*)gs.y;
+    double swe = std::sqrt(w_e), swf = std::sqrt(w_f);
+    bool f32 = ctx->p.precision == MLD_PREC_FP32;
+    if (nstruct) {
+        if (f32)
+            hipLaunchKernelGGL(rows_energy_kernel<float>, dim3(nstruct), dim3(256), 0, st, order, D, ldA, ctx->d_atom_off,
+                               (const float*)ctx->last_energy, E_dev, swe, A, y);
+        else
+            hipLaunchKernelGGL(rows_energy_kernel<double>, dim3(nstruct), dim3(256), 0, st, order, D, ldA, ctx->d_atom_off,
+                               (const double*)ctx->last_energy, E_dev, swe, A, y);
+    }
+    if (natoms) {
+        if (f32)
+            hipLaunchKernelGGL(rows_force_kernel<float>, dim3(natoms, 3), dim3(256), 0, st, order, D, ldA, ctx->d_atom_struct,
+                               ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src,
+                               (const float*)ctx->last_energy, (const float*)ctx->last_grad, F_dev, swf, A, y);
+        else
+            hipLaunchKernelGGL(rows_force_kernel<double>, dim3(natoms, 3), dim3(256), 0, st, order, D, ldA, ctx->d_atom_struct,
+                               ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src,
+                               (const double*)ctx->last_energy, (const double*)ctx->last_grad, F_dev, swf, A, y);
+    }
+    MLD_HIP(ctx, hipGetLastError());
+    *A_out = A;
+    *y_out = y;
+    *M_out = M;
+    return MLD_OK;
+}
+
+void gram_scratch_free(mld_ctx* ctx)
+{
+    if (!ctx->gram_scratch) return;
+    GramScratch* gs = reinterpret_cast<GramScratch*>(ctx->gram_scratch);
+    void* ps[] = {gs->part, gs->gpart, gs->rows, gs->y, gs->split, gs->rev};
+    for (void* q : ps)
+        if (q) (void)hipFree(q);
+    delete gs;
+    ctx->gram_scratch = nullptr;
+}
+
+extern "C" {
+
+int mld_gram_begin(mld_ctx* ctx, int D)
+{
+    if (!ctx || D <= 0) return ctx ? mld_fail(ctx, MLD_E_ARG, "mld_gram_begin: D must be positive") : MLD_E_ARG;
+    if (!ctx->gram_scratch) ctx->gram_scratch = new GramScratch();
+    int ld = ((D + TILE - 1) / TILE) * TILE;
+    if (ctx->d_G) MLD_HIP(ctx, hipFree(ctx->d_G));
+    if (ctx->d_h) MLD_HIP(ctx, hipFree(ctx->d_h));
+    ctx->d_G = ctx->d_h = nullptr;
+    MLD_HIP(ctx, hipMalloc(&ctx->d_G, (size_t)ld * ld * sizeof(double)));
+    MLD_HIP(ctx, hipMalloc(&ctx->d_h, (size_t)ld * sizeof(double)));
+    MLD_HIP(ctx, hipMemsetAsync(ctx->d_G, 0, (size_t)ld * ld * sizeof(double), ctx->stream));
+    MLD_HIP(ctx, hipMemsetAsync(ctx->d_h, 0, (size_t)ld * sizeof(double), ctx->stream));
+    ctx->gD = D;
+    ctx->gld = ld;
+    ctx->gram_rows = 0;
+    return MLD_OK;
+}
+
+int mld_gram_add_rows_dev(mld_ctx* ctx, int D, int mc, const double* Ac_dev, const double* y_dev, const double* w_dev)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
+    if (D != ctx->gD || mc < 0 || (mc > 0 && (!Ac_dev || !y_dev || !w_dev)))
+        return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_rows: D differs from mld_gram_begin or null input");
+    if (mc == 0) return MLD_OK;
+    GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
+    int rc = ensure(ctx, &gs.rows, &gs.cap_rows, (size_t)mc * ctx->gld * sizeof(double) + 64);
+    if (rc) return rc;
+    rc = ensure(ctx, &gs.y, &gs.cap_y, (size_t)mc * sizeof(double) + 64);
+    if (rc) return rc;
+    hipLaunchKernelGGL(scale_rows_kernel, dim3(mc), dim3(64), 0, ctx->stream, mc, D, ctx->gld, Ac_dev, y_dev, w_dev,
+                       (double*)gs.rows, (double*)gs.y);
+    MLD_HIP(ctx, hipGetLastError());
+    return gram_accumulate_rows(ctx, mc, (const double*)gs.rows, (const double*)gs.y, 0);
+}
+
+int mld_gram_add_rows(mld_ctx* ctx, int D, int mc, const double* Ac, const double* y, const double* w)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (mc < 0 || (mc > 0 && (!Ac || !y || !w))) return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_rows: null input");
+    if (mc == 0) return MLD_OK;
+    double *dA = nullptr, *dy = nullptr, *dw = nullptr;
+    MLD_HIP(ctx, hipMalloc(&dA, (size_t)D * mc * sizeof(double)));
+    MLD_HIP(ctx, hipMalloc(&dy, (size_t)mc * sizeof(double)));
+    MLD_HIP(ctx, hipMalloc(&dw, (size_t)mc * sizeof(double)));
+    MLD_HIP(ctx, hipMemcpy(dA, Ac, (size_t)D * mc * sizeof(double), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMemcpy(dy, y, (size_t)mc * sizeof(double), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMemcpy(dw, w, (size_t)mc * sizeof(double), hipMemcpyHostToDevice));
+    int rc = mld_gram_add_rows_dev(ctx, D, mc, dA, dy, dw);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dA);
+    (void)hipFree(dy);
+    (void)hipFree(dw);
+    if (rc) return rc;
+    if (e != hipSuccess) return mld_fail(ctx, MLD_E_DEVICE, hipGetErrorString(e));
+    return MLD_OK;
+}
+
+int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F_dev, double w_e, double w_f, int mixed)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
+    if (order != 1 && order != 2) return mld_fail(ctx, MLD_E_ARG, "order must be 1 or 2");
+    const int D = ctx->D;
+    int Dc = 1 + D + (order == 2 ? D * (D + 1) / 2 : 0);
+    if (Dc != ctx->gD) return mld_fail(ctx, MLD_E_ARG, "mld_gram_begin D must be 1 + D (+ D(D+1)/2 for order 2)");
+    if (Dc > 2048) return mld_fail(ctx, MLD_E_UNSUPPORTED, "row packing is built for at most 2048 columns");
+    ctx->gram_order = order;
+    if (!ctx->last_energy || !ctx->last_grad) return mld_fail(ctx, MLD_E_STATE, "mld_batch_bso4_dev with gradients first");
+    if (!E || (ctx->natoms && !F_dev)) return mld_fail(ctx, MLD_E_ARG, "targets missing");
+    if (!(w_e >= 0.0) || !(w_f >= 0.0)) return mld_fail(ctx, MLD_E_ARG, "weights must be non-negative");
+    // per-structure energies to the device
+    double* dE = nullptr;
+    MLD_HIP(ctx, hipMalloc(&dE, (size_t)(ctx->nstruct + 1) * sizeof(double)));
+    MLD_HIP(ctx, hipMemcpyAsync(dE, E, (size_t)ctx->nstruct * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    double *A = nullptr, *y = nullptr;
+    int M = 0;
+    int rc = launch_rows(ctx, order, dE, F_dev, w_e, w_f, &A, &y, &M);
+    if (!rc) rc = gram_accumulate_rows(ctx, M, A, y, mixed);
+    if (!rc && ctx->keep_rows) rc = keep_rows_copy(ctx, M, A, y);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dE);
+    if (rc) return rc;
+    if (e != hipSuccess) return mld_fail(ctx, MLD_E_DEVICE, hipGetErrorString(e));
+    return MLD_OK;
+}
+
+// Diagnostic (not part of the public header): copy the rows of the last
+// mld_gram_add_batch (sqrt(w)-scaled, padded to the Gram leading dimension).
+int mld_gram_debug_rows(mld_ctx* ctx, int* M, int* ld, double* A, double* y)
+{
+    if (!ctx || !ctx->gram_scratch) return MLD_E_ARG;
+    GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
+    int m = ctx->nstruct + 3 * ctx->natoms;
+    *M = m;
+    *ld = ctx->gld;
+    MLD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (A) MLD_HIP(ctx, hipMemcpy(A, gs.rows, (size_t)m * ctx->gld * sizeof(double), hipMemcpyDeviceToHost));
+    if (y) MLD_HIP(ctx, hipMemcpy(y, gs.y, (size_t)m * sizeof(double), hipMemcpyDeviceToHost));
+    return MLD_OK;
+}
+
+int mld_gram_export(mld_ctx* ctx, double* G, double* h)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
+    const int D = ctx->gD, ld = ctx->gld;
+    MLD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (G) {
+        // upper triangle accumulated; export the symmetrised D x D matrix (column-major = row-major)
+        std::vector<double> tmp((size_t)ld * ld);
+        MLD_HIP(ctx, hipMemcpy(tmp.data(), ctx->d_G, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) G[(size_t)i * D + j] = (i <= j) ? tmp[(size_t)i * ld + j] : tmp[(size_t)j * ld + i];
+    }
+    if (h) MLD_HIP(ctx, hipMemcpy(h, ctx->d_h, (size_t)D * sizeof(double), hipMemcpyDeviceToHost));
+    return MLD_OK;
+}
+
+}  // extern "C"

@@ -114,10 +114,17 @@ struct mld_ctx {
     // last descriptor outputs (device, owned by caller)
     const void* last_energy = nullptr;
     const void* last_grad = nullptr;
+    int* d_atom_off = nullptr;   // [nstruct + 1] device copy of atom_off
+    size_t cap_struct_off = 0;
     // Gram state
-    int gD = 0;
-    double* d_G = nullptr;
-    double* d_h = nullptr;
+    int gD = 0, gld = 0, gram_order = 1;
+    long long gram_rows = 0;
+    double* d_G = nullptr;       // gld x gld, row-major, upper tiles accumulated
+    double* d_h = nullptr;       // gld
+    void* gram_scratch = nullptr;
+    int keep_rows = 0;           // keep FP64 rows resident for the refinement residual
+    std::vector<double*> kept_A, kept_y;
+    std::vector<int> kept_M;
     void* d_work = nullptr; size_t cap_work = 0;
     // kernel_po state
     int nk = 0;
@@ -139,8 +146,7 @@ int launch_neighbours_pos(mld_ctx* ctx, const double* pos);
 int launch_bso4(mld_ctx* ctx, int want_grad, void* energy, void* grad);
 int launch_to_reference_layout(mld_ctx* ctx, int nat, int nn_max, const void* grad, double* force_dev,
                                int* kind_dev, int* nneigh_dev);
-int launch_rows_gram(mld_ctx* ctx, int order, const double* E, const double* F_dev, double w_e, double w_f,
-                     int mixed);
-int launch_gram_rows(mld_ctx* ctx, int D, int mc, const double* Ac, const double* y, const double* w);
-int launch_solve(mld_ctx* ctx, double lambda, int method, double* w, double* diag);
-int launch_kernel_po(mld_ctx* ctx, void* ek, void* fk);
+int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, int mixed);
+int keep_rows_copy(mld_ctx* ctx, int M, const double* A, const double* y);
+void gram_scratch_free(mld_ctx* ctx);
+void kept_rows_free(mld_ctx* ctx);

@@ -1,0 +1,416 @@
+// (G + Lambda) w = h on the device: FP64 blocked Cholesky with FP64 iterative refinement.
+//
+// Replaces the solve of gram_reduce_and_solve (module_online_fit.F90:550-590:
+// symmetrise, + lambda I, serial SVD) on one GPU.  The ridge matrix Lambda is
+// lambda on every column, except for the symmetric-unique quadratic columns
+// (p < q) where it is lambda/2: with the reference's full D^2 layout, where the
+// (p,q) and (q,p) columns are identical, the ridge solution has w_pq = w_qp and
+// ||w||^2 counts the pair twice, so the unique-column system with lambda/2 on
+// those columns is the same minimiser.
+// The matrix is Jacobi-scaled (S G S, S = diag(G)^-1/2) before the factorisation.
+// Refinement residuals r = h - (G + Lambda) w use the FP64 Gram (plain form) or,
+// when the FP64 rows were kept resident (mixed form), r = A^T (y - A w) - Lambda w
+// straight from the rows, so the refined w solves the FP64 normal equations even
+// when the factorised Gram came from split-BF16 products.
+#include <cmath>
+
+#include "mld_internal.h"
+
+namespace {
+
+constexpr int NB = 64;
+
+// L (n x n row-major, ld) lower, factor the diagonal block k in place
+__global__ void __launch_bounds__(256) chol_diag_kernel(int n, int ld, double* __restrict__ L, int k, int* __restrict__ info)
+{
+    __shared__ double a[NB][NB + 1];
+    const int k0 = k * NB, nb = min(NB, n - k0);
+    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
+        int i = q / NB, j = q % NB;
+        a[i][j] = (i < nb && j < nb) ? L[(size_t)(k0 + i) * ld + k0 + j] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        if (threadIdx.x == 0) {
+            double d = a[j][j];
+            if (!(d > 0.0)) {
+                if (*info == 0) *info = k0 + j + 1;
+                d = 1e-300;  // keep going without NaN; the caller reports the failure
+            }
+            a[j][j] = sqrt(d);
+        }
+        __syncthreads();
+        double djj = a[j][j];
+        for (int i = j + 1 + threadIdx.x; i < nb; i += blockDim.x) a[i][j] /= djj;
+        __syncthreads();
+        for (int q = threadIdx.x; q < (nb - j - 1) * (nb - j - 1); q += blockDim.x) {
+            int i = j + 1 + q / (nb - j - 1), c = j + 1 + q % (nb - j - 1);
+            if (c <= i) a[i][c] -= a[i][j] * a[c][j];
+        }
+        __syncthreads();
+    }
+    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
+        int i = q / NB, j = q % NB;
+        if (i < nb && j < nb && j <= i) L[(size_t)(k0 + i) * ld + k0 + j] = a[i][j];
+    }
+}
+
+// panel: rows below block k: X L_kk^T = A_ik  (one block of 64 rows per workgroup)
+__global__ void __launch_bounds__(256) chol_panel_kernel(int n, int ld, double* __restrict__ L, int k)
+{
+    __shared__ double lk[NB][NB + 1];
+    __shared__ double x[NB][NB + 1];
+    const int k0 = k * NB, nb = min(NB, n - k0);
+    const int i0 = (k + 1 + blockIdx.x) * NB, mb = min(NB, n - i0);
+    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
+        int i = q / NB, j = q % NB;
+        lk[i][j] = (i < nb && j < nb) ? L[(size_t)(k0 + i) * ld + k0 + j] : 0.0;
+        x[i][j] = (i < mb && j < nb) ? L[(size_t)(i0 + i) * ld + k0 + j] : 0.0;
+    }
+    __syncthreads();
+    // row r of X: x[r][j] = (a[r][j] - sum_{c<j} x[r][c] lk[j][c]) / lk[j][j]
+    for (int r = threadIdx.x; r < mb; r += blockDim.x)
+        for (int j = 0; j < nb; ++j) {
+            double s = x[r][j];
+            for (int c = 0; c < j; ++c) s -= x[r][c] * lk[j][c];
+            x[r][j] = s / lk[j][j];
+        }
+    __syncthreads();
+    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
+        int i = q / NB, j = q % NB;
+        if (i < mb && j < nb) L[(size_t)(i0 + i) * ld + k0 + j] = x[i][j];
+    }
+}
+
+// trailing update A_ij -= L_ik L_jk^T for k < j <= i (lower tiles)
+__global__ void __launch_bounds__(256) chol_update_kernel(int n, int ld, double* __restrict__ L, int k, int nrem)
+{
+    int t = blockIdx.x, bi = 0;
+    while (t > bi) { t -= bi + 1; ++bi; }
+    const int bj = t;  // 0 <= bj <= bi < nrem
+    const int k0 = k * NB, nb = min(NB, n - k0);
+    const int i0 = (k + 1 + bi) * NB, j0 = (k + 1 + bj) * NB;
+    const int mi = min(NB, n - i0), mj = min(NB, n - j0);
+    __shared__ double li[NB][NB + 1];
+    __shared__ double lj[NB][NB + 1];
+    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
+        int i = q / NB, c = q % NB;
+        li[i][c] = (i < mi && c < nb) ? L[(size_t)(i0 + i) * ld + k0 + c] : 0.0;
+        lj[i][c] = (i < mj && c < nb) ? L[(size_t)(j0 + i) * ld + k0 + c] : 0.0;
+    }
+    __syncthreads();
+    // each thread: 4x4 outputs
+    const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+    double acc[4][4] = {};
+    for (int c = 0; c < nb; ++c) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { a[u] = li[ty * 4 + u][c]; b[u] = lj[tx * 4 + u][c]; }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int i = ty * 4 + u, j = tx * 4 + v;
+            if (i < mi && j < mj && (bi != bj || j <= i)) L[(size_t)(i0 + i) * ld + j0 + j] -= acc[u][v];
+        }
+}
+
+// one workgroup: forward (trans = 0, L z = b) or backward (trans = 1, L^T x = z) substitution, in place
+__global__ void __launch_bounds__(256) trsv_kernel(int n, int ld, const double* __restrict__ L, double* __restrict__ b, int trans)
+{
+    __shared__ double red[256];
+    __shared__ double xv;
+    if (!trans) {
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int c = threadIdx.x; c < i; c += blockDim.x) s += L[(size_t)i * ld + c] * b[c];
+            red[threadIdx.x] = s;
+            __syncthreads();
+            for (int o = 128; o > 0; o >>= 1) {
+                if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) b[i] = (b[i] - red[0]) / L[(size_t)i * ld + i];
+            __syncthreads();
+        }
+    } else {
+        for (int i = n - 1; i >= 0; --i) {
+            if (threadIdx.x == 0) xv = b[i] / L[(size_t)i * ld + i];
+            __syncthreads();
+            double xi = xv;
+            for (int c = threadIdx.x; c < i; c += blockDim.x) b[c] -= L[(size_t)i * ld + c] * xi;
+            if (threadIdx.x == 0) b[i] = xi;
+            __syncthreads();
+        }
+    }
+}
+
+// S = diag^-1/2, Ls = lower(S (G+Lambda) S), from the upper-accumulated G
+__global__ void prep_kernel(int n, int ld, const double* __restrict__ G, const double* __restrict__ ridge,
+                            double* __restrict__ L, double* __restrict__ S)
+{
+    int i = blockIdx.x;
+    double gii = G[(size_t)i * ld + i] + ridge[i];
+    double si = gii > 0.0 ? 1.0 / sqrt(gii) : 1.0;
+    if (threadIdx.x == 0) S[i] = si;
+    for (int j = threadIdx.x; j <= i; j += blockDim.x) {
+        double gjj = G[(size_t)j * ld + j] + ridge[j];
+        double sj = gjj > 0.0 ? 1.0 / sqrt(gjj) : 1.0;
+        double v = G[(size_t)j * ld + i] + (i == j ? ridge[i] : 0.0);  // upper element (j <= i)
+        L[(size_t)i * ld + j] = si * v * sj;
+    }
+}
+
+// r = h - (G + Lambda) w   (symmetric G from the upper triangle)
+__global__ void resid_gram_kernel(int n, int ld, const double* __restrict__ G, const double* __restrict__ ridge,
+                                  const double* __restrict__ h, const double* __restrict__ w, double* __restrict__ r)
+{
+    int i = blockIdx.x;
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        double g = (j >= i) ? G[(size_t)i * ld + j] : G[(size_t)j * ld + i];
+        s += g * w[j];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) r[i] = h[i] - red[0] - ridge[i] * w[i];
+}
+
+// t_r = y_r - A_r . w  (one wavefront per row)
+__global__ void row_resid_kernel(int M, int ld, int n, const double* __restrict__ A, const double* __restrict__ y,
+                                 const double* __restrict__ w, double* __restrict__ t)
+{
+    int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (r >= M) return;
+    double s = 0.0;
+    for (int c = lane; c < n; c += 64) s += A[(size_t)r * ld + c] * w[c];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) t[r] = y[r] - s;
+}
+
+__global__ void at_t_kernel(int M, int ld, int n, const double* __restrict__ A, const double* __restrict__ t,
+                            int rps, double* __restrict__ part)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int r0 = blockIdx.y * rps, r1 = min(M, r0 + rps);
+    if (c >= n) return;
+    double s = 0.0;
+    for (int r = r0; r < r1; ++r) s += A[(size_t)r * ld + c] * t[r];
+    part[(size_t)blockIdx.y * n + c] = s;
+}
+
+__global__ void accumulate_part_kernel(int n, int ns, const double* __restrict__ part, double* __restrict__ r)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    double s = 0.0;
+    for (int q = 0; q < ns; ++q) s += part[(size_t)q * n + c];
+    r[c] += s;
+}
+
+__global__ void finish_resid_kernel(int n, const double* __restrict__ ridge, const double* __restrict__ w,
+                                    double* __restrict__ r)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) r[c] -= ridge[c] * w[c];
+}
+
+__global__ void scale_vec_kernel(int n, const double* __restrict__ S, double* __restrict__ v)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) v[c] *= S[c];
+}
+
+__global__ void axpy_kernel(int n, const double* __restrict__ d, double* __restrict__ w)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) w[c] += d[c];
+}
+
+}  // namespace
+
+int keep_rows_copy(mld_ctx* ctx, int M, const double* A, const double* y)
+{
+    double *dA = nullptr, *dy = nullptr;
+    MLD_HIP(ctx, hipMalloc(&dA, (size_t)M * ctx->gld * sizeof(double)));
+    MLD_HIP(ctx, hipMalloc(&dy, (size_t)M * sizeof(double)));
+    MLD_HIP(ctx, hipMemcpyAsync(dA, A, (size_t)M * ctx->gld * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    MLD_HIP(ctx, hipMemcpyAsync(dy, y, (size_t)M * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->kept_A.push_back(dA);
+    ctx->kept_y.push_back(dy);
+    ctx->kept_M.push_back(M);
+    return MLD_OK;
+}
+
+void kept_rows_free(mld_ctx* ctx)
+{
+    for (double* p : ctx->kept_A) (void)hipFree(p);
+    for (double* p : ctx->kept_y) (void)hipFree(p);
+    ctx->kept_A.clear();
+    ctx->kept_y.clear();
+    ctx->kept_M.clear();
+}
+
+static int residual(mld_ctx* ctx, const double* ridge, const double* w, double* r, double* tbuf, double* part,
+                    int use_rows)
+{
+    hipStream_t st = ctx->stream;
+    const int n = ctx->gD, ld = ctx->gld;
+    if (!use_rows) {
+        hipLaunchKernelGGL(resid_gram_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, ridge, ctx->d_h, w, r);
+        MLD_HIP(ctx, hipGetLastError());
+        return MLD_OK;
+    }
+    MLD_HIP(ctx, hipMemsetAsync(r, 0, (size_t)n * sizeof(double), st));
+    for (size_t q = 0; q < ctx->kept_A.size(); ++q) {
+        int M = ctx->kept_M[q];
+        hipLaunchKernelGGL(row_resid_kernel, dim3((M + 3) / 4), dim3(256), 0, st, M, ld, n, ctx->kept_A[q],
+                           ctx->kept_y[q], w, tbuf);
+        int ns = std::max(1, std::min(256, M / 1024));
+        int rps = (M + ns - 1) / ns;
+        ns = (M + rps - 1) / rps;
+        hipLaunchKernelGGL(at_t_kernel, dim3((n + 255) / 256, ns), dim3(256), 0, st, M, ld, n, ctx->kept_A[q], tbuf,
+                           rps, part);
+        hipLaunchKernelGGL(accumulate_part_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, ns, part, r);
+    }
+    hipLaunchKernelGGL(finish_resid_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, ridge, w, r);
+    MLD_HIP(ctx, hipGetLastError());
+    return MLD_OK;
+}
+
+extern "C" int mld_gram_set_keep_rows(mld_ctx* ctx, int keep)
+{
+    if (!ctx) return MLD_E_ARG;
+    ctx->keep_rows = keep ? 1 : 0;
+    if (!keep) kept_rows_free(ctx);
+    return MLD_OK;
+}
+
+extern "C" int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w_out, double* diag_out)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
+    if (!w_out) return mld_fail(ctx, MLD_E_ARG, "mld_gram_solve: null w");
+    if (method != MLD_SOLVE_CHOL_IR && method != MLD_SOLVE_CHOL) return mld_fail(ctx, MLD_E_ARG, "unknown method");
+    hipStream_t st = ctx->stream;
+    const int n = ctx->gD, ld = ctx->gld;
+    if (lambda < 0.0) lambda = 0.0;  // lambda_krr < 0 -> 0 (module_online_fit.F90:571)
+    // ridge per column
+    std::vector<double> ridge(n, lambda);
+    if (ctx->gram_order == 2) {
+        int D = ctx->D;
+        for (int c = 1 + D; c < n; ++c) {
+            int q = c - 1 - D, p = 0;
+            while (q >= D - p) { q -= D - p; ++p; }
+            q += p;
+            if (p != q) ridge[c] = 0.5 * lambda;
+        }
+    }
+    size_t nb = (size_t)((n + NB - 1) / NB);
+    size_t bytes_L = (size_t)ld * ld * sizeof(double);
+    size_t nvec = (size_t)ld;
+    size_t maxM = 1;
+    for (int M : ctx->kept_M) maxM = std::max(maxM, (size_t)M);
+    size_t need = bytes_L + 8 * nvec * sizeof(double) + maxM * sizeof(double) + 256 * nvec * sizeof(double) + 4096;
+    if (need > ctx->cap_work) {
+        if (ctx->d_work) MLD_HIP(ctx, hipFree(ctx->d_work));
+        ctx->d_work = nullptr;
+        MLD_HIP(ctx, hipMalloc(&ctx->d_work, need));
+        ctx->cap_work = need;
+    }
+    char* base = (char*)ctx->d_work;
+    double* L = (double*)base; base += bytes_L;
+    double* dridge = (double*)base; base += nvec * sizeof(double);
+    double* S = (double*)base; base += nvec * sizeof(double);
+    double* w = (double*)base; base += nvec * sizeof(double);
+    double* r = (double*)base; base += nvec * sizeof(double);
+    double* d = (double*)base; base += nvec * sizeof(double);
+    int* info = (int*)base; base += nvec * sizeof(double);
+    base += 2 * nvec * sizeof(double);
+    double* tbuf = (double*)base; base += maxM * sizeof(double);
+    double* part = (double*)base;
+    MLD_HIP(ctx, hipMemcpyAsync(dridge, ridge.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+    MLD_HIP(ctx, hipMemsetAsync(info, 0, sizeof(int), st));
+    MLD_HIP(ctx, hipMemsetAsync(L, 0, bytes_L, st));
+    hipLaunchKernelGGL(prep_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, dridge, L, S);
+    for (size_t k = 0; k < nb; ++k) {
+        hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, st, n, ld, L, (int)k, info);
+        int nrem = (int)(nb - k - 1);
+        if (nrem > 0) {
+            hipLaunchKernelGGL(chol_panel_kernel, dim3(nrem), dim3(256), 0, st, n, ld, L, (int)k);
+            hipLaunchKernelGGL(chol_update_kernel, dim3(nrem * (nrem + 1) / 2), dim3(256), 0, st, n, ld, L, (int)k, nrem);
+        }
+    }
+    MLD_HIP(ctx, hipGetLastError());
+    int hinfo = 0;
+    MLD_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
+    MLD_HIP(ctx, hipStreamSynchronize(st));
+    if (hinfo)
+        return mld_fail(ctx, MLD_E_SINGULAR, "Cholesky: leading minor " + std::to_string(hinfo) +
+                                                 " not positive definite (increase lambda)");
+    auto solve_scaled = [&](double* v) {  // v <- (G+Lambda)^-1 v using S L L^T S
+        hipLaunchKernelGGL(scale_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, S, v);
+        hipLaunchKernelGGL(trsv_kernel, dim3(1), dim3(256), 0, st, n, ld, L, v, 0);
+        hipLaunchKernelGGL(trsv_kernel, dim3(1), dim3(256), 0, st, n, ld, L, v, 1);
+        hipLaunchKernelGGL(scale_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, S, v);
+    };
+    int use_rows = ctx->keep_rows && !ctx->kept_A.empty();
+    MLD_HIP(ctx, hipMemsetAsync(w, 0, (size_t)n * sizeof(double), st));
+    // initial solve from the Gram right-hand side (or the row residual at w = 0)
+    int rc = residual(ctx, dridge, w, r, tbuf, part, use_rows);
+    if (rc) return rc;
+    std::vector<double> hr(n), hh(n);
+    MLD_HIP(ctx, hipMemcpyAsync(hh.data(), r, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    MLD_HIP(ctx, hipStreamSynchronize(st));
+    double hnorm = 0.0;
+    for (double v : hh) hnorm += v * v;
+    hnorm = std::sqrt(hnorm);
+    int iters = 0;
+    double relres = 1.0;
+    const int maxit = (method == MLD_SOLVE_CHOL_IR) ? 30 : 1;
+    double prev = 1e300;
+    for (int it = 0; it < maxit; ++it) {
+        MLD_HIP(ctx, hipMemcpyAsync(d, r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        solve_scaled(d);
+        hipLaunchKernelGGL(axpy_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, d, w);
+        ++iters;
+        rc = residual(ctx, dridge, w, r, tbuf, part, use_rows);
+        if (rc) return rc;
+        MLD_HIP(ctx, hipMemcpyAsync(hr.data(), r, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        MLD_HIP(ctx, hipStreamSynchronize(st));
+        double rn = 0.0;
+        for (double v : hr) rn += v * v;
+        relres = hnorm > 0 ? std::sqrt(rn) / hnorm : 0.0;
+        if (method != MLD_SOLVE_CHOL_IR) break;
+        if (relres < 1e-15 || relres > 0.5 * prev) break;
+        prev = relres;
+    }
+    MLD_HIP(ctx, hipMemcpy(w_out, w, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    if (diag_out) {
+        std::vector<double> hl((size_t)n);
+        double mn = 1e300, mx = 0.0;
+        std::vector<double> Lh((size_t)ld * ld);
+        MLD_HIP(ctx, hipMemcpy(Lh.data(), L, bytes_L, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            double p = Lh[(size_t)i * ld + i];
+            mn = std::min(mn, p);
+            mx = std::max(mx, p);
+        }
+        diag_out[0] = iters;
+        diag_out[1] = relres;
+        diag_out[2] = mn;
+        diag_out[3] = mx;
+    }
+    return MLD_OK;
+}

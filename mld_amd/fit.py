@@ -1,0 +1,112 @@
+"""Streamed normal-equation fit on the device (host mirror of main_train_online_fit,
+module_online_fit.F90:800-1096): per batch of configurations the descriptors,
+the energy / force rows and the Gram update run on the GPU; the solve is an FP64
+Cholesky with iterative refinement.
+
+Column layouts
+  order 1 : [N, B_1..B_D]                                  (1 + D columns)
+  order 2 : [N, B_1..B_D, (B_p B_q)_{p<=q}]                 (1 + D + D(D+1)/2)
+The reference stores the quadratic block as the full D^2 outer product
+(mld.F90:771-780); ``expand_quadratic`` maps a unique-column solution to that
+layout (w_pq = w_qp = u_pq / 2 off the diagonal).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Tuple
+
+import numpy as np
+
+SOLVE_CHOL_IR, SOLVE_CHOL = 0, 1
+
+
+def ncols(D: int, order: int) -> int:
+    return 1 + D + (D * (D + 1) // 2 if order == 2 else 0)
+
+
+def unique_pairs(D: int):
+    return [(p, q) for p in range(D) for q in range(p, D)]
+
+
+def expand_quadratic(w: np.ndarray, D: int) -> np.ndarray:
+    """Unique-column coefficients -> reference layout [N, D linear, D*D quadratic (id1-major)]."""
+    out = np.zeros(1 + D + D * D)
+    out[: 1 + D] = w[: 1 + D]
+    for c, (p, q) in enumerate(unique_pairs(D)):
+        u = w[1 + D + c]
+        if p == q:
+            out[1 + D + D * p + q] = u
+        else:
+            out[1 + D + D * p + q] = 0.5 * u
+            out[1 + D + D * q + p] = 0.5 * u
+    return out
+
+
+def full_to_unique_rows(rows_full: np.ndarray, D: int) -> np.ndarray:
+    """Reference-layout rows (1+D+D^2) -> unique columns (the (p,q) entry, p <= q)."""
+    out = np.zeros((rows_full.shape[0], ncols(D, 2)))
+    out[:, : 1 + D] = rows_full[:, : 1 + D]
+    for c, (p, q) in enumerate(unique_pairs(D)):
+        out[:, 1 + D + c] = rows_full[:, 1 + D + D * p + q]
+    return out
+
+
+class GramFit:
+    def __init__(self, ctx, order: int = 1, mixed: bool = False, keep_rows: bool = False):
+        self.ctx = ctx
+        self.order = order
+        self.mixed = bool(mixed)
+        self.D = ctx.D
+        self.ncol = ncols(ctx.D, order)
+        ctx.check(ctx.L.mld_gram_begin(ctx.h, self.ncol))
+        ctx.check(ctx.L.mld_gram_set_keep_rows(ctx.h, int(keep_rows)))
+
+    def add_batch(self, batch, E: np.ndarray, F_dev, w_e: float = 1.0, w_f: float = 1.0):
+        """Energy + force rows of the batch's last descriptors (mld_batch_bso4_dev with gradients)."""
+        E = np.ascontiguousarray(E, np.float64)
+        self.ctx.check(self.ctx.L.mld_gram_add_batch(self.ctx.h, self.order, E.ctypes.data_as(C.c_void_p),
+                                                     C.c_void_p(F_dev.data_ptr()), w_e, w_f, int(self.mixed)))
+
+    def add_rows(self, Ac: np.ndarray, y: np.ndarray, w: np.ndarray):
+        """gram_accumulate_config for caller-packed columns Ac (D, mc) (numpy: shape (mc, D) C-order)."""
+        A = np.ascontiguousarray(Ac, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        mc = A.shape[0]
+        self.ctx.check(self.ctx.L.mld_gram_add_rows(self.ctx.h, self.ncol, mc, A.ctypes.data_as(C.c_void_p),
+                                                    y.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p)))
+
+    def export(self) -> Tuple[np.ndarray, np.ndarray]:
+        G = np.zeros((self.ncol, self.ncol))
+        h = np.zeros(self.ncol)
+        self.ctx.check(self.ctx.L.mld_gram_export(self.ctx.h, G.ctypes.data_as(C.c_void_p),
+                                                  h.ctypes.data_as(C.c_void_p)))
+        return G, h
+
+    def solve(self, lam: float, method: int = SOLVE_CHOL_IR):
+        w = np.zeros(self.ncol)
+        diag = np.zeros(4)
+        self.ctx.check(self.ctx.L.mld_gram_solve(self.ctx.h, lam, method, w.ctypes.data_as(C.c_void_p),
+                                                 diag.ctypes.data_as(C.c_void_p)))
+        return w, {"iterations": int(diag[0]), "rel_residual": float(diag[1]), "min_pivot": float(diag[2]),
+                   "max_pivot": float(diag[3])}
+
+    def ridge_vector(self, lam: float) -> np.ndarray:
+        """The per-column ridge the solve applies (lambda, lambda/2 on off-diagonal quadratic columns)."""
+        r = np.full(self.ncol, max(lam, 0.0))
+        if self.order == 2:
+            for c, (p, q) in enumerate(unique_pairs(self.D)):
+                if p != q:
+                    r[1 + self.D + c] *= 0.5
+        return r
+
+
+def kernel_po_set(ctx, zref: np.ndarray, sigma: float, length: float, power: float):
+    """zref (nk, D): standardised reference points (global_kernel of init_kernel)."""
+    z = np.ascontiguousarray(zref, np.float64)
+    ctx.check(ctx.L.mld_kernel_po_set(ctx.h, z.shape[0], z.ctypes.data_as(C.c_void_p), sigma, length, power))
+
+
+def kernel_po(ctx, ek, fk=None):
+    ctx.check(ctx.L.mld_kernel_po_dev(ctx.h, C.c_void_p(ek.data_ptr()),
+                                      None if fk is None else C.c_void_p(fk.data_ptr())))

@@ -1,0 +1,105 @@
+"""End-to-end streamed linear / quadratic fits on the GPU against the FP64 oracle path
+(oracle descriptors + oracle rows + oracle Gram, solved with numpy)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from mld_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def workload(seed, ncell, n=4, sigma=0.05):
+    rng = np.random.default_rng(seed)
+    pos, cells = [], []
+    for _ in range(ncell):
+        p, c = synth.bcc_supercell(rng, 2.855, n, sigma)
+        pos.append(p)
+        cells.append(c)
+    types = [np.zeros(len(p), np.int32) for p in pos]
+    EF = [synth.morse(p, c) for p, c in zip(pos, cells)]
+    return pos, cells, types, np.array([e for e, _ in EF]), [f for _, f in EF]
+
+
+def oracle_fit(pos, cells, types, E, F, order, w_e, w_f, lam):
+    from mld_amd.fit import full_to_unique_rows, ncols
+
+    D = 55
+    nc = ncols(D, order)
+    G = np.zeros((nc, nc))
+    h = np.zeros(nc)
+    for s, (p, c, t) in enumerate(zip(pos, cells, types)):
+        e, g, nl = O.bso4(p, c, t, 8, 4.7)
+        R = O.rows(e, g, nl, order=order)
+        if order == 2:
+            R = full_to_unique_rows(R, D)
+        y = np.concatenate([[E[s]], F[s].reshape(-1)])
+        w = np.concatenate([[w_e], np.full(3 * len(p), w_f)])
+        O.gram_accumulate(G, h, np.ascontiguousarray(R), y, w)
+    Gs = np.triu(G) + np.triu(G, 1).T
+    return Gs, h
+
+
+def gpu_gram(ctx, pos, cells, types, E, F, order, w_e, w_f, mixed=False, keep_rows=False, chunk=4):
+    import torch
+    import mld_amd as M
+    from mld_amd.fit import GramFit
+
+    gf = GramFit(ctx, order=order, mixed=mixed, keep_rows=keep_rows)
+    for s0 in range(0, len(pos), chunk):
+        sl = slice(s0, s0 + chunk)
+        b = M.Batch(ctx, pos[sl], cells[sl], types[sl])
+        b.neighbours()
+        e, g = b.alloc_outputs(True)
+        b.bso4(e, g)
+        Fd = torch.from_numpy(np.concatenate(F[sl]).reshape(-1)).to("cuda:0")
+        gf.add_batch(b, E[sl], Fd, w_e, w_f)
+    return gf
+
+
+def test_linear_fit_matches_oracle():
+    import mld_amd as M
+    from mld_amd.fit import GramFit
+
+    pos, cells, types, E, F = workload(1, 6)
+    ctx = M.Context(j_max=4.0, r_cut=4.7)
+    gf = gpu_gram(ctx, pos, cells, types, E, F, 1, 1.0, 1.0)
+    G, h = gf.export()
+    Go, ho = oracle_fit(pos, cells, types, E, F, 1, 1.0, 1.0, 1e-8)
+    assert rel(G, Go) < 1e-12 and rel(h, ho) < 1e-12
+    lam = 1e-8
+    w, diag = gf.solve(lam)
+    wo = np.linalg.solve(Go + lam * np.eye(len(ho)), ho)
+    # predictions: same energies and forces to FP64 round-off of the normal equations
+    assert diag["rel_residual"] < 1e-10
+    Ep, Eo = Go @ w, Go @ wo
+    assert rel(Ep, Eo) < 1e-8
+    ctx.close()
+
+
+def test_quadratic_mixed_fit_refines_to_fp64():
+    """Quadratic (unique-column) fit: split-BF16 Gram + FP64 Cholesky + refinement against
+    the kept FP64 rows reaches the FP64 solution of the same rows."""
+    import mld_amd as M
+
+    pos, cells, types, E, F = workload(2, 16, n=3, sigma=0.1)   # 864 atoms, 2608 rows, 1596 columns
+    ctx = M.Context(j_max=4.0, r_cut=4.7)
+    g64 = gpu_gram(ctx, pos, cells, types, E, F, 2, 1.0, 0.1, keep_rows=False)
+    G, h = g64.export()
+    lam = 1e-6 * float(np.trace(G)) / G.shape[0]
+    w64, d64 = g64.solve(lam)
+    r = g64.ridge_vector(lam)
+    res64 = np.linalg.norm(G @ w64 + r * w64 - h) / np.linalg.norm(h)
+    assert res64 < 1e-10
+    # mixed Gram on the same FP64 descriptors, refined against the kept FP64 rows
+    gmx = gpu_gram(ctx, pos, cells, types, E, F, 2, 1.0, 0.1, mixed=True, keep_rows=True)
+    wmx, dmx = gmx.solve(lam)
+    assert dmx["iterations"] >= 1
+    resmx = np.linalg.norm(G @ wmx + r * wmx - h) / np.linalg.norm(h)
+    assert resmx < 1e-8
+    assert rel(G @ wmx, G @ w64) < 1e-8
+    ctx.close()
