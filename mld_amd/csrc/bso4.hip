@@ -70,6 +70,30 @@ constexpr int wsize_bound(int jj)
     return best;
 }
 
+// exact size of the CG1 tables (distinct (J1, J2, J) triples, (J1+1)(J+1) each)
+constexpr int cg1_size(int jj)
+{
+    int best = 0;
+    for (int diag = 0; diag < 2; ++diag) {
+        bool seen[mld::JJ_MAX + 1][mld::JJ_MAX + 1][mld::JJ_MAX + 1] = {};
+        int tot = 0;
+        for (int l1 = 0; l1 <= jj; ++l1)
+            for (int l2 = diag ? l1 : 0; l2 <= l1; ++l2)
+                for (int l = (l1 > l2 ? l1 - l2 : l2 - l1); l <= (jj < l1 + l2 ? jj : l1 + l2); ++l) {
+                    if ((l1 + l2 + l) & 1) continue;
+                    if (!diag && l < l1) continue;
+                    int tr[3][3] = {{l1, l2, l}, {l, l2, l1}, {l1, l, l2}};
+                    for (int q = 0; q < 3; ++q) {
+                        if (q == 1 && l1 != l) {}
+                        int a = tr[q][0], b = tr[q][1], c = tr[q][2];
+                        if (!seen[a][b][c]) { seen[a][b][c] = true; tot += (a + 1) * (c + 1); }
+                    }
+                }
+        if (tot > best) best = tot;
+    }
+    return best;
+}
+
 template <typename T>
 struct Geo {
     cx<T> zp, zm, xp, xm;
@@ -185,11 +209,17 @@ struct Smem {
     static constexpr int NU = mld::ucount(JJ);
     static constexpr int NL = (JJ + 1) * (JJ + 1);
     static constexpr int WMAX = wsize_bound(JJ);
+    static constexpr int NCG = cg1_size(JJ);
     cx<T> c[NU];
     cx<T> W[WMAX];
     T selfw[NW][mld::NB_MAX][3];
+    RecEntry rec[NU];
     union {
-        cx<T> cpart[NW][NU];
+        struct {
+            cx<T> cpart[NW][NU];
+            cx<T> ub[NW][2][NL];
+        } a;
+        T cg[NCG];
         struct {
             cx<T> ub[NW][2][NL];
             cx<T> dub[NW][2][NL][3];
@@ -233,58 +263,51 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
         return;
     }
 
+    for (int e = tid; e < NU; e += BS) sm.rec[e] = tb.rec[e];
+    __syncthreads();
+
     for (int ch = 0; ch < nch; ++ch) {
         // ---------------- A: density ----------------
-        cx<T> cacc[JJ + 1][2];
-#pragma unroll
-        for (int j = 0; j <= JJ; ++j) cacc[j][0] = cacc[j][1] = cx<T>{(T)0, (T)0};
+        // each lane owns fixed entries of every level, so it accumulates them in its
+        // wavefront's cpart row without conflicts
+        for (int e = lane; e < NU; e += 64) sm.u.a.cpart[w][e] = cx<T>{(T)0, (T)0};
         for (int n = w; n < nn; n += NW) {
             double4 d = pair_d[p0 + n];
             int tj = itype[pair_idx[p0 + n]];
             double rin = rin_tab ? rin_tab[ti * nsp + tj] : 0.0;
             Geo<T> g = make_geo<T>(d, rcut, r0, rin, win, false);
             T fw = g.fc * (T)wfac[ch * nsp + tj];
-            cx<T>* ub = sm.u.g.ub[w][0];
-            cx<T>* ubn = sm.u.g.ub[w][1];
-            (void)ubn;
+            cx<T>* cp = sm.u.a.cpart[w];
             if (lane == 0) {
-                sm.u.g.ub[w][0][0] = cx<T>{(T)1, (T)0};
-                cacc[0][0].r += fw;
+                sm.u.a.ub[w][0][0] = cx<T>{(T)1, (T)0};
+                cp[0].r += fw;
             }
-#pragma unroll
+#pragma unroll 1
             for (int j = 1; j <= JJ; ++j) {
                 wave_sync();
-                const cx<T>* Up = sm.u.g.ub[w][(j - 1) & 1];
-                cx<T>* Un = sm.u.g.ub[w][j & 1];
+                const cx<T>* Up = sm.u.a.ub[w][(j - 1) & 1];
+                cx<T>* Un = sm.u.a.ub[w][j & 1];
+                const int K = (j + 1) * (j + 1);
 #pragma unroll
-                for (int s = 0; s < level_slots<JJ>(j); ++s) {
-                    int e = lane + 64 * s;
-                    if (e < (j + 1) * (j + 1)) {
+                for (int s2 = 0; s2 < level_slots<JJ>(JJ); ++s2) {
+                    int e = lane + 64 * s2;
+                    if (e < K) {
                         int b = e % (j + 1);
-                        RecEntry R = tb.rec[mld::uoff(j) + e];
+                        RecEntry R = sm.rec[mld::uoff(j) + e];
                         cx<T> u;
                         rec_step<T, false>(R, b, g, Up, nullptr, u, nullptr);
                         Un[e] = u;
-                        cacc[j][s] = cadd(cacc[j][s], cscale(fw, u));
+                        cp[mld::uoff(j) + e] = cadd(cp[mld::uoff(j) + e], cscale(fw, u));
                     }
                 }
             }
-            (void)ub;
             wave_sync();
         }
-        __syncthreads();  // the cpart union member aliases the recursion buffers
-#pragma unroll
-        for (int j = 0; j <= JJ; ++j)
-#pragma unroll
-            for (int s = 0; s < level_slots<JJ>(j); ++s) {
-                int e = lane + 64 * s;
-                if (e < (j + 1) * (j + 1)) sm.u.cpart[w][mld::uoff(j) + e] = cacc[j][s];
-            }
         __syncthreads();
         for (int e = tid; e < NU; e += BS) {
-            cx<T> s = sm.u.cpart[0][e];
+            cx<T> s = sm.u.a.cpart[0][e];
 #pragma unroll
-            for (int q = 1; q < NW; ++q) s = cadd(s, sm.u.cpart[q][e]);
+            for (int q = 1; q < NW; ++q) s = cadd(s, sm.u.a.cpart[q][e]);
             // self term on the diagonal a == b (compute_bispectrum_so4.F90:431-439)
             int j = 0;
             while (mld::uoff(j + 1) <= e) ++j;
@@ -295,53 +318,54 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
         __syncthreads();
 
         // ---------------- B: Z -> W and B ----------------
-        const int nrows = tb.rowbase[JJ + 1];
-        for (int rw = w; rw < nrows; rw += NW) {
-            int j = 0;
-            while (tb.rowbase[j + 1] <= rw) ++j;
-            const int r = rw - tb.rowbase[j];
+        for (int q = tid; q < tb.ncg; q += BS) sm.u.cg[q] = (T)tb.cg1[q];
+        __syncthreads();
+        // balanced schedule: the k-th task of every thread costs about the same
+        for (int k = 0; k * BS + tid < tb.ntask; ++k) {
+            const int task = tb.tasks[k * BS + tid];
+            const int j = task >> 16, r = (task >> 8) & 255, e = task & 255;
             const int M = tb.M[j];
-            const WRow row = tb.rows[rw];
-            const int K = (j + 1) * (j + 1);
-            T bpart = (T)0;
-            for (int e = lane; e < K; e += 64) {
-                const int A = e / (j + 1), B = e % (j + 1);
-                const int m1 = -j + 2 * A, m2 = -j + 2 * B;
-                cx<T> wv{(T)0, (T)0};
-                for (int q = 0; q < row.nz; ++q) {
-                    const ZTerm zt = row.z[q];
-                    const int J1 = zt.J1, J2 = zt.J2;
-                    const double* cg = tb.cg1 + zt.cgoff;
-                    const cx<T>* c1 = sm.c + mld::uoff(J1);
-                    const cx<T>* c2 = sm.c + mld::uoff(J2);
-                    int malo = max(-J1, m1 - J2), mahi = min(J1, m1 + J2);
-                    int mblo = max(-J1, m2 - J2), mbhi = min(J1, m2 + J2);
-                    cx<T> s{(T)0, (T)0};
-                    for (int ma = malo; ma <= mahi; ma += 2) {
-                        const int ia = (ma + J1) >> 1, ka = (m1 - ma + J2) >> 1;
-                        const T cga = (T)cg[ia * (j + 1) + A];
-                        for (int mb = mblo; mb <= mbhi; mb += 2) {
-                            const int ib = (mb + J1) >> 1, kb = (m2 - mb + J2) >> 1;
-                            const T cgab = cga * (T)cg[ib * (j + 1) + B];
-                            cx<T> p = cmul(c1[ia * (J1 + 1) + ib], c2[ka * (J2 + 1) + kb]);
-                            s.r += cgab * p.r;
-                            s.i += cgab * p.i;
-                        }
-                    }
-                    if (zt.isA) {
-                        cx<T> cl = sm.c[mld::uoff(j) + e];
-                        bpart += cl.r * s.r + cl.i * s.i;  // Re(conj(c_l) Z_A)
-                    }
-                    wv = cadd(wv, cscale((T)zt.scale, s));
+            const WRow& row = tb.rows[tb.rowbase[j] + r];
+            const int A = e / (j + 1), B = e % (j + 1);
+            const int m1 = -j + 2 * A, m2 = -j + 2 * B;
+            cx<T> wv{(T)0, (T)0};
+            for (int q = 0; q < row.nz; ++q) {
+                const ZTerm zt = row.z[q];
+                const int J1 = zt.J1, J2 = zt.J2;
+                const T* cg = sm.u.cg + zt.cgoff;
+                const cx<T>* c1 = sm.c + mld::uoff(J1);
+                const cx<T>* c2 = sm.c + mld::uoff(J2);
+                const int ialo = (max(-J1, m1 - J2) + J1) >> 1, iahi = (min(J1, m1 + J2) + J1) >> 1;
+                const int iblo = (max(-J1, m2 - J2) + J1) >> 1, ibhi = (min(J1, m2 + J2) + J1) >> 1;
+                const int nbq = ibhi - iblo + 1;
+                const int nterm = (iahi - ialo + 1) * nbq;
+                cx<T> sacc{(T)0, (T)0};
+                for (int t = 0; t < nterm; ++t) {
+                    const int ia = ialo + t / nbq, ib = iblo + t % nbq;
+                    const int ka = (m1 + J1 - 2 * ia + J2) >> 1, kb = (m2 + J1 - 2 * ib + J2) >> 1;
+                    const T cgab = cg[ia * (j + 1) + A] * cg[ib * (j + 1) + B];
+                    cx<T> pr = cmul(c1[ia * (J1 + 1) + ib], c2[ka * (J2 + 1) + kb]);
+                    sacc.r += cgab * pr.r;
+                    sacc.i += cgab * pr.i;
                 }
-                sm.W[tb.wofs[j] + e * M + r] = wv;
+                wv = cadd(wv, cscale((T)zt.scale, sacc));
             }
-            bool hasA = false;
-            for (int q = 0; q < row.nz; ++q) hasA |= (row.z[q].isA != 0);
-            if (hasA) {
-                T bsum = wave_sum(bpart);
-                if (lane == 0) energy[(size_t)g_atom * D + ch * nb + row.bi] = bsum;
+            sm.W[tb.wofs[j] + e * M + r] = wv;
+        }
+        __syncthreads();
+        // B_bi = Re sum_e conj(c_l(e)) Z_A(e), and the level-l row of bi holds kappa * Z_A
+        for (int bi = w; bi < nb; bi += NW) {
+            const int l = tb.l[bi];
+            const int r = tb.arow[bi] - tb.rowbase[l];
+            const int M = tb.M[l];
+            T bpart = (T)0;
+            for (int e = lane; e < (l + 1) * (l + 1); e += 64) {
+                cx<T> cl = sm.c[mld::uoff(l) + e];
+                cx<T> wv = sm.W[tb.wofs[l] + e * M + r];
+                bpart += cl.r * wv.r + cl.i * wv.i;
             }
+            T bsum = wave_sum(bpart);
+            if (lane == 0) energy[(size_t)g_atom * D + ch * nb + bi] = bsum / (T)tb.kappa[bi];
         }
         __syncthreads();
         if (!want_grad) continue;
@@ -358,7 +382,8 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
             T* outw = &sm.u.g.outw[w][0][0];
             for (int q = lane; q < nb * 3; q += 64) outw[q] = (T)0;
             cx<T>* vb = &sm.u.g.vb[w][0][0];
-            // level 0: U = 1, dU = 0, V = dfcut rhat
+            // level 0: U = 1, dU = 0, V = dfcut rhat (the level loop below is not unrolled
+            // to keep the register footprint at two wavefronts per SIMD)
             if (lane == 0) {
                 sm.u.g.ub[w][0][0] = cx<T>{(T)1, (T)0};
 #pragma unroll
@@ -367,7 +392,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                     vb[k] = cx<T>{g.dfc * g.drr[k], (T)0};
                 }
             }
-#pragma unroll
+#pragma unroll 1
             for (int j = 0; j <= JJ; ++j) {
                 if (j > 0) {
                     wave_sync();
@@ -378,16 +403,16 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                     cx<T> us[level_slots<JJ>(JJ)];
                     cx<T> dus[level_slots<JJ>(JJ)][3];
 #pragma unroll
-                    for (int s = 0; s < level_slots<JJ>(j); ++s) {
+                    for (int s = 0; s < level_slots<JJ>(JJ); ++s) {
                         int e = lane + 64 * s;
                         if (e < (j + 1) * (j + 1)) {
                             int b = e % (j + 1);
-                            RecEntry R = tb.rec[mld::uoff(j) + e];
+                            RecEntry R = sm.rec[mld::uoff(j) + e];
                             rec_step<T, true>(R, b, g, Up, dUp, us[s], dus[s]);
                         }
                     }
 #pragma unroll
-                    for (int s = 0; s < level_slots<JJ>(j); ++s) {
+                    for (int s = 0; s < level_slots<JJ>(JJ); ++s) {
                         int e = lane + 64 * s;
                         if (e < (j + 1) * (j + 1)) {
                             Un[e] = us[s];
@@ -403,19 +428,52 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                     }
                 }
                 wave_sync();
-                // contraction of level j: lane r < M_j owns row r, all three directions
+                // contraction of level j: the K_j entries are cut into NS slices;
+                // lane r + M*s sums slice s of row r (two interleaved partial sums
+                // per direction), then the slices are added in slice order.
                 const int M = tb.M[j];
                 const int K = (j + 1) * (j + 1);
                 const cx<T>* Wj = sm.W + tb.wofs[j];
-                for (int r = lane; r < M; r += 64) {
-                    T a0 = 0, a1 = 0, a2 = 0;
-                    for (int e = 0; e < K; ++e) {
+                const int NS = (M > 0) ? max(1, min(4, 64 / M)) : 1;
+                const int slice = (K + NS - 1) / NS;
+                const int r = lane % max(M, 1), sidx = lane / max(M, 1);
+                T a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+                if (M > 0 && sidx < NS) {
+                    const int e0 = sidx * slice, e1 = min(K, e0 + slice);
+                    int e = e0;
+                    for (; e + 1 < e1; e += 2) {
+                        cx<T> wv = Wj[e * M + r], wu = Wj[(e + 1) * M + r];
+                        cx<T> v0 = vb[3 * e], v1 = vb[3 * e + 1], v2 = vb[3 * e + 2];
+                        cx<T> u0 = vb[3 * e + 3], u1 = vb[3 * e + 4], u2 = vb[3 * e + 5];
+                        a0 += wv.r * v0.r - wv.i * v0.i;
+                        a1 += wv.r * v1.r - wv.i * v1.i;
+                        a2 += wv.r * v2.r - wv.i * v2.i;
+                        b0 += wu.r * u0.r - wu.i * u0.i;
+                        b1 += wu.r * u1.r - wu.i * u1.i;
+                        b2 += wu.r * u2.r - wu.i * u2.i;
+                    }
+                    if (e < e1) {
                         cx<T> wv = Wj[e * M + r];
                         cx<T> v0 = vb[3 * e], v1 = vb[3 * e + 1], v2 = vb[3 * e + 2];
                         a0 += wv.r * v0.r - wv.i * v0.i;
                         a1 += wv.r * v1.r - wv.i * v1.i;
                         a2 += wv.r * v2.r - wv.i * v2.i;
                     }
+                    a0 += b0;
+                    a1 += b1;
+                    a2 += b2;
+                }
+                // gather the slices of row r into lane r (fixed order)
+                for (int q = 1; q < 4; ++q) {
+                    int src = lane + q * max(M, 1);
+                    T t0 = __shfl(a0, src & 63, 64), t1 = __shfl(a1, src & 63, 64), t2 = __shfl(a2, src & 63, 64);
+                    if (q < NS && sidx == 0) {
+                        a0 += t0;
+                        a1 += t1;
+                        a2 += t2;
+                    }
+                }
+                if (M > 0 && sidx == 0 && lane < M) {
                     const int bi = tb.rows[tb.rowbase[j] + r].bi;
                     outw[bi * 3 + 0] += a0;
                     outw[bi * 3 + 1] += a1;
@@ -481,6 +539,12 @@ int launch_prec(mld_ctx* ctx, int want_grad, void* energy, void* grad)
 }
 
 }  // namespace
+
+bool bso4_tables_fit(int jj, int ncg, int wsize)
+{
+    if (jj < 1 || jj > mld::JJ_MAX) return false;
+    return ncg <= cg1_size(jj) && wsize <= wsize_bound(jj);
+}
 
 int launch_bso4(mld_ctx* ctx, int want_grad, void* energy, void* grad)
 {

@@ -54,6 +54,11 @@ struct DevTables {
     int wofs[JJ_MAX + 2];  // complex offset of the level-j W block (layout [e][row])
     int M[JJ_MAX + 1];
     int wsize;             // total complex entries of W
+    int ncg;               // entries of the CG1 tables
+    const int* tasks;      // phase-B schedule: (j << 16) | (row-in-level << 8) | e, costliest first
+    int ntask;
+    int arow[NB_MAX];      // row of component bi at its own level l
+    double kappa[NB_MAX];  // W_l row of bi = kappa * Z_A
     int l[NB_MAX], l1[NB_MAX], l2[NB_MAX];
 };
 
@@ -63,6 +68,9 @@ struct HostTables {
     std::vector<RecEntry> rec;
     std::vector<WRow> rows;
     std::vector<double> cg1;
+    std::vector<int> tasks;
+    int arow[NB_MAX] = {0};
+    double kappa[NB_MAX] = {0};
     int rowbase[JJ_MAX + 2] = {0};
     int wofs[JJ_MAX + 2] = {0};
     int M[JJ_MAX + 1] = {0};
@@ -96,6 +104,7 @@ struct mld_ctx {
     void* d_rec = nullptr;
     void* d_rows = nullptr;
     void* d_cg1 = nullptr;
+    void* d_tasks = nullptr;
     double* d_wfac = nullptr;    // [nch * nsp] channel weights (channel 0 = 1)
     double* d_rin = nullptr;     // [nsp * nsp] inner cutoff or null
     // batch state
@@ -144,6 +153,7 @@ int mld_fail(mld_ctx* ctx, int code, const std::string& msg);
 // launchers implemented in the .hip units
 int launch_neighbours_pos(mld_ctx* ctx, const double* pos);
 int launch_bso4(mld_ctx* ctx, int want_grad, void* energy, void* grad);
+bool bso4_tables_fit(int jj, int ncg, int wsize);
 int launch_to_reference_layout(mld_ctx* ctx, int nat, int nn_max, const void* grad, double* force_dev,
                                int* kind_dev, int* nneigh_dev);
 int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, int mixed);

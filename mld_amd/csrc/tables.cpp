@@ -11,6 +11,7 @@
 //   build W_j = [j==l] Z_A + [j==l1] (l+1)/(l1+1) Z_B + [j==l2] (l+1)/(l2+1) Z_C
 //   (compute_bispectrum_so4.F90:505-652).
 #include <cmath>
+#include <algorithm>
 #include <map>
 #include <tuple>
 
@@ -112,9 +113,23 @@ void build_tables(int jj, int diag, HostTables& t)
             if (j != l && j != l1 && j != l2) continue;
             WRow row{};
             row.bi = bi;
-            if (j == l) row.z[row.nz++] = ZTerm{l1, l2, cgtab(l1, l2, l), 1, 1.0};
-            if (j == l1) row.z[row.nz++] = ZTerm{l, l2, cgtab(l, l2, l1), 0, (double)(1 + l) / (double)(1 + l1)};
-            if (j == l2) row.z[row.nz++] = ZTerm{l1, l, cgtab(l1, l, l2), 0, (double)(1 + l) / (double)(1 + l2)};
+            // terms with the same (J1, J2 -> j) are the same Z: merge them by adding scales
+            auto add = [&](int J1, int J2, int isA, double sc) {
+                for (int q = 0; q < row.nz; ++q)
+                    if (row.z[q].J1 == J1 && row.z[q].J2 == J2) {
+                        row.z[q].scale += sc;
+                        row.z[q].isA |= isA;
+                        return;
+                    }
+                row.z[row.nz++] = ZTerm{J1, J2, cgtab(J1, J2, j), isA, sc};
+            };
+            if (j == l) add(l1, l2, 1, 1.0);
+            if (j == l1) add(l, l2, 0, (double)(1 + l) / (double)(1 + l1));
+            if (j == l2) add(l1, l, 0, (double)(1 + l) / (double)(1 + l2));
+            if (j == l) {
+                t.arow[bi] = (int)t.rows.size();
+                t.kappa[bi] = row.z[0].scale;  // the merged Z_A term is always first
+            }
             t.rows.push_back(row);
         }
         t.M[j] = (int)t.rows.size() - t.rowbase[j];
@@ -123,6 +138,25 @@ void build_tables(int jj, int diag, HostTables& t)
     t.rowbase[jj + 1] = (int)t.rows.size();
     t.wofs[jj + 1] = woff;
     t.wsize = woff;
+    // phase-B schedule: one task per W entry, largest first, dealt round-robin to the
+    // threads of a workgroup so that the k-th tasks of all threads cost about the same
+    std::vector<std::pair<int, int>> cost;
+    for (int j = 0; j <= jj; ++j)
+        for (int rw = t.rowbase[j]; rw < t.rowbase[j + 1]; ++rw)
+            for (int e = 0; e < (j + 1) * (j + 1); ++e) {
+                int A = e / (j + 1), B = e % (j + 1), m1 = -j + 2 * A, m2 = -j + 2 * B, c = 0;
+                const WRow& row = t.rows[rw];
+                for (int q = 0; q < row.nz; ++q) {
+                    int J1 = row.z[q].J1, J2 = row.z[q].J2;
+                    int na = (std::min(J1, m1 + J2) - std::max(-J1, m1 - J2)) / 2 + 1;
+                    int nb2 = (std::min(J1, m2 + J2) - std::max(-J1, m2 - J2)) / 2 + 1;
+                    c += std::max(na, 0) * std::max(nb2, 0);
+                }
+                cost.push_back({-c, (j << 16) | ((rw - t.rowbase[j]) << 8) | e});
+            }
+    std::stable_sort(cost.begin(), cost.end());
+    t.tasks.clear();
+    for (auto& pr : cost) t.tasks.push_back(pr.second);
 }
 
 void struct_geometry(const double* cell, double rcut, StructGeo& g)
