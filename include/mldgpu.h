@@ -13,7 +13,10 @@
  *   force : (D, nat, 0:nn_max, 3)          -- config_desc%force after the sign flip
  *                                              of descriptors.F90:298-302, i.e. dB/dr
  * Functions whose name ends in _dev take device pointers; all others take host
- * pointers and copy.
+ * pointers and copy.  Device work runs on the context's own non-blocking stream
+ * (mld_get_stream): device inputs must be complete when a _dev call is made, and
+ * device outputs are complete after mld_synchronize (or for work the caller
+ * enqueues on that stream).
  */
 #ifndef MLDGPU_H
 #define MLDGPU_H
@@ -87,6 +90,8 @@ int mld_bso4(mld_ctx* ctx, int nat, const double* pos, const double* cell, const
 int mld_batch_neighbours(mld_ctx* ctx, int nstruct, const int* atom_off, const double* pos_dev,
                          const double* cell_dev, const int* itype_dev, long long* npairs);
 int mld_batch_pairs_dev(mld_ctx* ctx, const int** pair_off_dev, const int** pair_idx_dev);
+/* pair geometry of the last neighbour build: P x (dx, dy, dz, r), dx = r_j - r_i (Angstrom) */
+int mld_batch_pair_geometry_dev(mld_ctx* ctx, const double** pair_d_dev);
 int mld_batch_bso4_dev(mld_ctx* ctx, int want_grad, void* energy_dev, void* grad_dev);
 
 /* ---- streamed normal equations (gram_accumulate_config / gram_reduce_and_solve,
@@ -108,9 +113,14 @@ int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F
 int mld_gram_export(mld_ctx* ctx, double* G, double* h);
 /* keep = 1: mld_gram_add_batch keeps its FP64 rows resident in HBM so that the
  * refinement residual of mld_gram_solve is r = A^T (y - A w) - Lambda w from the
- * rows themselves (used with the mixed Gram; bytes = rows * padded columns * 8). */
+ * rows themselves (used with the mixed Gram; bytes = rows * padded columns * 8).
+ * mld_gram_begin releases the rows kept by the previous accumulation. */
 int mld_gram_set_keep_rows(mld_ctx* ctx, int keep);
-/* (G + lambda I) w = h by FP64 Cholesky with FP64 iterative refinement.
+/* (G + lambda I) w = h (lambda/2 on off-diagonal quadratic columns) by FP64 Cholesky.
+ * MLD_SOLVE_CHOL: one solve with the factor.  MLD_SOLVE_CHOL_IR: the factor then
+ * preconditions FP64 conjugate-gradient refinement whose operator comes from the kept
+ * FP64 rows (or the FP64 Gram); a Gram accumulated with mixed = 1 is factorised with a
+ * diagonal shift of 1e-6 of its diagonal (raised tenfold while a pivot fails).
  * diag_out (optional, 4 doubles): [iterations, final relative residual, min pivot, max pivot]. */
 int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w, double* diag_out);
 
@@ -120,6 +130,22 @@ int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w, double* d
  * ((P+natoms)*3, nk) in the slot order above, or NULL for energies only. */
 int mld_kernel_po_set(mld_ctx* ctx, int nk, const double* zref, double sigma, double length, double power);
 int mld_kernel_po_dev(mld_ctx* ctx, void* ek_dev, void* fk_dev);
+
+/* ---- prediction with a fitted model resident on the device ----
+ * mld_model_set: w_params (product_w_params_Amat, mld.F90:40-86): order 1 has
+ * 1 + D coefficients [N, B]; order 2 either the unique layout 1 + D + D(D+1)/2
+ * or the reference's full layout 1 + D + D^2 (mld.F90:771-780).  ref_energy
+ * (nspecies, may be NULL) is ref_energy_per_element, added per atom
+ * (md_mld_compute_energy, mld.F90:1610-1660).
+ * mld_mahalanobis_set: mean (D) and Sigma^-1 (D, D) column-major of
+ * compute_mcd_dist_for_one_config (main_descriptors_analysis.F90:1315-1405);
+ * NULL, NULL clears them.
+ * mld_predict_dev on the last batch (mld_batch_bso4_dev, gradients needed for
+ * forces): E_dev (nstruct), F_dev (3*natoms, [atom][xyz], md_mld_compute_force
+ * mld.F90:1673-1772), d2_dev (natoms); any may be NULL.  Double precision. */
+int mld_model_set(mld_ctx* ctx, int order, int ncol, const double* w, const double* ref_energy);
+int mld_mahalanobis_set(mld_ctx* ctx, const double* mean, const double* sigma_inv);
+int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev);
 
 /* Stream used by the library (hipStream_t as void*). */
 int mld_get_stream(mld_ctx* ctx, void** stream);

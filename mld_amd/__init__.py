@@ -24,10 +24,10 @@ DESC_BSO4 = 9
 # Every symbol include/mldgpu.h declares.
 EXPORTS = [
     "mld_create", "mld_destroy", "mld_last_error", "mld_descriptor_dim", "mld_max_neighbours", "mld_bso4",
-    "mld_batch_neighbours", "mld_batch_pairs_dev", "mld_batch_bso4_dev", "mld_gram_begin", "mld_gram_add_rows",
+    "mld_batch_neighbours", "mld_batch_pairs_dev", "mld_batch_pair_geometry_dev", "mld_batch_bso4_dev", "mld_gram_begin", "mld_gram_add_rows",
     "mld_gram_add_rows_dev", "mld_gram_add_batch", "mld_gram_export", "mld_gram_set_keep_rows", "mld_gram_solve",
-    "mld_kernel_po_set",
-    "mld_kernel_po_dev", "mld_get_stream", "mld_synchronize",
+    "mld_kernel_po_set", "mld_kernel_po_dev", "mld_model_set", "mld_mahalanobis_set", "mld_predict_dev",
+    "mld_get_stream", "mld_synchronize",
 ]
 
 
@@ -72,49 +72,41 @@ def load() -> C.CDLL:
     except ImportError:
         pass
     L = C.CDLL(LIBPATH)
-    vp, ip, dp, i64p = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_longlong)
-    if hasattr(L, "mld_create"):
-        L.mld_create.argtypes = [C.POINTER(mld_params), C.POINTER(C.c_void_p)]
-    if hasattr(L, "mld_destroy"):
-        L.mld_destroy.argtypes = [vp]
-    if hasattr(L, "mld_last_error"):
-        L.mld_last_error.argtypes = [vp]
-    if hasattr(L, "mld_last_error"):
-        L.mld_last_error.restype = C.c_char_p
-    if hasattr(L, "mld_descriptor_dim"):
-        L.mld_descriptor_dim.argtypes = [vp, ip]
-    if hasattr(L, "mld_max_neighbours"):
-        L.mld_max_neighbours.argtypes = [vp, C.c_int, vp, vp, ip]
-    if hasattr(L, "mld_bso4"):
-        L.mld_bso4.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, vp, C.c_int]
-    if hasattr(L, "mld_batch_neighbours"):
-        L.mld_batch_neighbours.argtypes = [vp, C.c_int, vp, vp, vp, vp, i64p]
-    if hasattr(L, "mld_batch_pairs_dev"):
-        L.mld_batch_pairs_dev.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
-    if hasattr(L, "mld_batch_bso4_dev"):
-        L.mld_batch_bso4_dev.argtypes = [vp, C.c_int, vp, vp]
-    if hasattr(L, "mld_gram_begin"):
-        L.mld_gram_begin.argtypes = [vp, C.c_int]
-    if hasattr(L, "mld_gram_add_rows"):
-        L.mld_gram_add_rows.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
-    if hasattr(L, "mld_gram_add_rows_dev"):
-        L.mld_gram_add_rows_dev.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
-    if hasattr(L, "mld_gram_add_batch"):
-        L.mld_gram_add_batch.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int]
-    if hasattr(L, "mld_gram_export"):
-        L.mld_gram_export.argtypes = [vp, vp, vp]
-    if hasattr(L, "mld_gram_set_keep_rows"):
-        L.mld_gram_set_keep_rows.argtypes = [vp, C.c_int]
-    if hasattr(L, "mld_gram_solve"):
-        L.mld_gram_solve.argtypes = [vp, C.c_double, C.c_int, vp, vp]
-    if hasattr(L, "mld_kernel_po_set"):
-        L.mld_kernel_po_set.argtypes = [vp, C.c_int, vp, C.c_double, C.c_double, C.c_double]
-    if hasattr(L, "mld_kernel_po_dev"):
-        L.mld_kernel_po_dev.argtypes = [vp, vp, vp]
-    if hasattr(L, "mld_get_stream"):
-        L.mld_get_stream.argtypes = [vp, C.POINTER(C.c_void_p)]
-    if hasattr(L, "mld_synchronize"):
-        L.mld_synchronize.argtypes = [vp]
+    vp, ip, i64p, d = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.c_double
+    vpp, i = C.POINTER(C.c_void_p), C.c_int
+    sigs = {
+        "mld_create": [C.POINTER(mld_params), vpp],
+        "mld_destroy": [vp],
+        "mld_last_error": [vp],
+        "mld_descriptor_dim": [vp, ip],
+        "mld_max_neighbours": [vp, i, vp, vp, ip],
+        "mld_bso4": [vp, i, vp, vp, vp, i, vp, vp, vp, vp, i],
+        "mld_batch_neighbours": [vp, i, vp, vp, vp, vp, i64p],
+        "mld_batch_pairs_dev": [vp, vpp, vpp],
+        "mld_batch_pair_geometry_dev": [vp, vpp],
+        "mld_batch_bso4_dev": [vp, i, vp, vp],
+        "mld_gram_begin": [vp, i],
+        "mld_gram_add_rows": [vp, i, i, vp, vp, vp],
+        "mld_gram_add_rows_dev": [vp, i, i, vp, vp, vp],
+        "mld_gram_add_batch": [vp, i, vp, vp, d, d, i],
+        "mld_gram_export": [vp, vp, vp],
+        "mld_gram_set_keep_rows": [vp, i],
+        "mld_gram_solve": [vp, d, i, vp, vp],
+        "mld_kernel_po_set": [vp, i, vp, d, d, d],
+        "mld_kernel_po_dev": [vp, vp, vp],
+        "mld_model_set": [vp, i, i, vp, vp],
+        "mld_mahalanobis_set": [vp, vp, vp],
+        "mld_predict_dev": [vp, vp, vp, vp],
+        "mld_get_stream": [vp, vpp],
+        "mld_synchronize": [vp],
+    }
+    assert set(sigs) == set(EXPORTS), "signature table out of step with EXPORTS"
+    missing = [n for n in EXPORTS if not hasattr(L, n)]
+    if missing:
+        raise ImportError(f"{LIBPATH} lacks {missing}: rebuild with `python -m mld_amd.build`")
+    for name, argt in sigs.items():
+        getattr(L, name).argtypes = argt
+    L.mld_last_error.restype = C.c_char_p
     _lib = L
     return L
 

@@ -140,6 +140,7 @@ int mld_destroy(mld_ctx* ctx)
     if (!ctx) return MLD_OK;
     gram_scratch_free(ctx);
     kept_rows_free(ctx);
+    predict_free(ctx);
     void* ptrs[] = {ctx->d_tasks, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
                     ctx->d_geo, ctx->d_count, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_scan_tmp,
                     ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_zref, ctx->d_zz};
@@ -240,6 +241,13 @@ int mld_batch_pairs_dev(mld_ctx* ctx, const int** pair_off_dev, const int** pair
     if (!ctx) return MLD_E_ARG;
     if (pair_off_dev) *pair_off_dev = ctx->d_pair_off;
     if (pair_idx_dev) *pair_idx_dev = ctx->d_pair_idx;
+    return MLD_OK;
+}
+
+int mld_batch_pair_geometry_dev(mld_ctx* ctx, const double** pair_d_dev)
+{
+    if (!ctx || !pair_d_dev) return MLD_E_ARG;
+    *pair_d_dev = (const double*)ctx->d_pair_d;
     return MLD_OK;
 }
 

@@ -371,7 +371,7 @@ __global__ void gemv_reduce_kernel(int ldA, int nslice, const double* __restrict
 }  // namespace
 
 // ------------------------------------------------------------------ host side
-static int ensure(mld_ctx* ctx, void** p, size_t* cap, size_t bytes)
+int device_ensure(mld_ctx* ctx, void** p, size_t* cap, size_t bytes)
 {
     if (bytes <= *cap) return MLD_OK;
     if (*p) MLD_HIP(ctx, hipFree(*p));
@@ -403,14 +403,14 @@ int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, 
     rps = ((rps + 31) / 32) * 32;
     nslice = (M + rps - 1) / rps;
     GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
-    int rc = ensure(ctx, &gs.part, &gs.cap_part, (size_t)nslice * nup * TILE * TILE * sizeof(double));
+    int rc = device_ensure(ctx, &gs.part, &gs.cap_part, (size_t)nslice * nup * TILE * TILE * sizeof(double));
     if (rc) return rc;
     if (!mixed) {
         hipLaunchKernelGGL(syrk_f64_kernel, dim3(nup, nslice), dim3(256), 0, st, M, ldA, ntile, A, rps,
                            (double*)gs.part);
     } else {
         size_t n = (size_t)M * ldA;
-        rc = ensure(ctx, &gs.split, &gs.cap_split, 3 * n * sizeof(__bf16));
+        rc = device_ensure(ctx, &gs.split, &gs.cap_split, 3 * n * sizeof(__bf16));
         if (rc) return rc;
         __bf16* h = (__bf16*)gs.split;
         hipLaunchKernelGGL(split_bf16_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, A, h, h + n, h + 2 * n);
@@ -424,7 +424,7 @@ int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, 
     int hs = std::max(1, std::min(512, M / 256));
     int hrps = (M + hs - 1) / hs;
     hs = (M + hrps - 1) / hrps;
-    rc = ensure(ctx, &gs.gpart, &gs.cap_gpart, (size_t)hs * ldA * sizeof(double));
+    rc = device_ensure(ctx, &gs.gpart, &gs.cap_gpart, (size_t)hs * ldA * sizeof(double));
     if (rc) return rc;
     hipLaunchKernelGGL(gemv_t_kernel, dim3((ldA + 255) / 256, hs), dim3(256), 0, st, M, ldA, A, y, hrps,
                        (double*)gs.gpart);
@@ -435,31 +435,29 @@ int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, 
     return MLD_OK;
 }
 
-// rows of the last descriptor batch (energies + forces), order 1 or 2
-int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_dev, double w_e, double w_f,
-                double** A_out, double** y_out, int* M_out)
+// CSR transpose of the pair list of the last neighbour build: for target atom t,
+// rev_pair[rev_off[t] .. rev_off[t+1]) are the pair indices p with pair_idx[p] == t
+// in increasing p (stable radix sort), pair_src[p] is the centre atom of pair p.
+int pair_transpose(mld_ctx* ctx, const int** rev_off_out, const int** rev_pair_out, const int** pair_src_out)
 {
     hipStream_t st = ctx->stream;
-    const int natoms = ctx->natoms, nstruct = ctx->nstruct;
+    const int natoms = ctx->natoms;
     const long long P = ctx->npairs;
-    const int D = ctx->D;
-    const int ldA = ctx->gld;
-    const int M = nstruct + 3 * natoms;
+    if (!ctx->gram_scratch) ctx->gram_scratch = new GramScratch();
     GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
-    // CSR transpose of the pair list: rev_pair sorted by (target, pair index)
     auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
     size_t tmp = 0;
     MLD_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const int*)nullptr, (int*)nullptr, (const int*)nullptr,
                                                     (int*)nullptr, (int)P, 0, 32, st));
     size_t need = 4 * r256((size_t)P * sizeof(int)) + r256((size_t)(natoms + 2) * sizeof(int)) + r256(tmp) + 256;
-    int rc = ensure(ctx, &gs.rev, &gs.cap_rev, need);
+    int rc = device_ensure(ctx, &gs.rev, &gs.cap_rev, need);
     if (rc) return rc;
     char* base = (char*)gs.rev;
-    int* pair_src = (int*)base; base += ((P * sizeof(int) + 255) / 256) * 256;
-    int* iota = (int*)base; base += ((P * sizeof(int) + 255) / 256) * 256;
-    int* keys = (int*)base; base += ((P * sizeof(int) + 255) / 256) * 256;
-    int* rev_pair = (int*)base; base += ((P * sizeof(int) + 255) / 256) * 256;
-    int* rev_off = (int*)base; base += (((natoms + 2) * sizeof(int) + 255) / 256) * 256;
+    int* pair_src = (int*)base; base += r256((size_t)P * sizeof(int));
+    int* iota = (int*)base; base += r256((size_t)P * sizeof(int));
+    int* keys = (int*)base; base += r256((size_t)P * sizeof(int));
+    int* rev_pair = (int*)base; base += r256((size_t)P * sizeof(int));
+    int* rev_off = (int*)base; base += r256((size_t)(natoms + 2) * sizeof(int));
     void* sort_tmp = base;
     if (natoms) hipLaunchKernelGGL(pair_src_kernel, dim3((natoms + 255) / 256), dim3(256), 0, st, natoms, ctx->d_pair_off, pair_src);
     if (P) {
@@ -468,9 +466,28 @@ int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_de
     }
     hipLaunchKernelGGL(rev_count_kernel, dim3((natoms + 1 + 255) / 256), dim3(256), 0, st, natoms, keys, (int)P, rev_off);
     MLD_HIP(ctx, hipGetLastError());
-    rc = ensure(ctx, &gs.rows, &gs.cap_rows, (size_t)M * ldA * sizeof(double) + 64);
+    *rev_off_out = rev_off;
+    *rev_pair_out = rev_pair;
+    *pair_src_out = pair_src;
+    return MLD_OK;
+}
+
+// rows of the last descriptor batch (energies + forces), order 1 or 2
+int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_dev, double w_e, double w_f,
+                double** A_out, double** y_out, int* M_out)
+{
+    hipStream_t st = ctx->stream;
+    const int natoms = ctx->natoms, nstruct = ctx->nstruct;
+    const int D = ctx->D;
+    const int ldA = ctx->gld;
+    const int M = nstruct + 3 * natoms;
+    const int *rev_off, *rev_pair, *pair_src;
+    int rc = pair_transpose(ctx, &rev_off, &rev_pair, &pair_src);
     if (rc) return rc;
-    rc = ensure(ctx, &gs.y, &gs.cap_y, (size_t)M * sizeof(double) + 64);
+    GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
+    rc = device_ensure(ctx, &gs.rows, &gs.cap_rows, (size_t)M * ldA * sizeof(double) + 64);
+    if (rc) return rc;
+    rc = device_ensure(ctx, &gs.y, &gs.cap_y, (size_t)M * sizeof(double) + 64);
     if (rc) return rc;
     double* A = (double*)gs.rows;
     double* y = (double*)gs.y;
@@ -529,6 +546,8 @@ int mld_gram_begin(mld_ctx* ctx, int D)
     ctx->gD = D;
     ctx->gld = ld;
     ctx->gram_rows = 0;
+    ctx->gram_mixed = 0;
+    kept_rows_free(ctx);  // rows kept for the refinement belong to the previous accumulation
     return MLD_OK;
 }
 
@@ -540,9 +559,9 @@ int mld_gram_add_rows_dev(mld_ctx* ctx, int D, int mc, const double* Ac_dev, con
         return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_rows: D differs from mld_gram_begin or null input");
     if (mc == 0) return MLD_OK;
     GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
-    int rc = ensure(ctx, &gs.rows, &gs.cap_rows, (size_t)mc * ctx->gld * sizeof(double) + 64);
+    int rc = device_ensure(ctx, &gs.rows, &gs.cap_rows, (size_t)mc * ctx->gld * sizeof(double) + 64);
     if (rc) return rc;
-    rc = ensure(ctx, &gs.y, &gs.cap_y, (size_t)mc * sizeof(double) + 64);
+    rc = device_ensure(ctx, &gs.y, &gs.cap_y, (size_t)mc * sizeof(double) + 64);
     if (rc) return rc;
     hipLaunchKernelGGL(scale_rows_kernel, dim3(mc), dim3(64), 0, ctx->stream, mc, D, ctx->gld, Ac_dev, y_dev, w_dev,
                        (double*)gs.rows, (double*)gs.y);
@@ -592,6 +611,7 @@ int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F
     double *A = nullptr, *y = nullptr;
     int M = 0;
     int rc = launch_rows(ctx, order, dE, F_dev, w_e, w_f, &A, &y, &M);
+    if (mixed) ctx->gram_mixed = 1;
     if (!rc) rc = gram_accumulate_rows(ctx, M, A, y, mixed);
     if (!rc && ctx->keep_rows) rc = keep_rows_copy(ctx, M, A, y);
     hipError_t e = hipStreamSynchronize(ctx->stream);

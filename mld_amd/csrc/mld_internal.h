@@ -127,6 +127,7 @@ struct mld_ctx {
     size_t cap_struct_off = 0;
     // Gram state
     int gD = 0, gld = 0, gram_order = 1;
+    int gram_mixed = 0;          // some rows went through the split-BF16 Gram
     long long gram_rows = 0;
     double* d_G = nullptr;       // gld x gld, row-major, upper tiles accumulated
     double* d_h = nullptr;       // gld
@@ -140,6 +141,8 @@ struct mld_ctx {
     double k_sigma = 0, k_length = 1, k_power = 2;
     double* d_zref = nullptr;    // [nk][D] row-major
     double* d_zz = nullptr;      // [nk] |z|^2
+    // fitted model and selection-score state (predict.hip)
+    void* predict_state = nullptr;
 };
 
 int mld_fail(mld_ctx* ctx, int code, const std::string& msg);
@@ -159,4 +162,7 @@ int launch_to_reference_layout(mld_ctx* ctx, int nat, int nn_max, const void* gr
 int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, int mixed);
 int keep_rows_copy(mld_ctx* ctx, int M, const double* A, const double* y);
 void gram_scratch_free(mld_ctx* ctx);
+int device_ensure(mld_ctx* ctx, void** p, size_t* cap, size_t bytes);
+int pair_transpose(mld_ctx* ctx, const int** rev_off, const int** rev_pair, const int** pair_src);
+void predict_free(mld_ctx* ctx);
 void kept_rows_free(mld_ctx* ctx);

@@ -1,0 +1,291 @@
+// Energies, forces and Mahalanobis selection scores of a fitted model on the
+// last descriptor batch, with the coefficients resident in HBM.
+//
+//   E_s    = w . row_E(s) + sum_a ref_energy(itype(a))       md_mld_compute_energy, mld.F90:1579-1669
+//   F_k,x  = w . row_F(k, x)                                  md_mld_compute_force,  mld.F90:1673-1772
+//   d2_a   = (B_a - mu)^T Sigma^-1 (B_a - mu)                 compute_mcd_dist_for_one_config,
+//                                                             main_descriptors_analysis.F90:1315-1405
+// with the rows of the fit (mld_energy.F90:91-266, mld_forces.F90:222-500).  By
+// linearity w . row_F never needs the rows: with e_a = w_lin . B_a + 1/2 B_a^T U B_a
+// (U the symmetric matrix of the quadratic coefficients, U_pp = 2 u_pp) and
+// g_a = dE/dB_a = w_lin + U B_a,
+//   F_k,x = -( g_k . dB_k^(0)_x + sum_{i: k in nb(i)} g_i . dB_i^(k)_x ).
+// Every sum runs in a fixed order that depends only on the structure itself, so
+// a structure's results are bitwise the same whatever batch it is evaluated in.
+#include "mld_internal.h"
+
+#include <cmath>
+#include <vector>
+
+namespace {
+
+constexpr int DMAX = mld::NB_MAX * mld::NCH_MAX;  // 192 = 3 values per lane
+
+struct PredictState {
+    int order = 0, ncol = 0, D = 0;
+    double w0 = 0.0;
+    double* d_w = nullptr;     // [D] linear coefficients
+    double* d_U = nullptr;     // [D*D] symmetric quadratic matrix, or null (order 1)
+    double* d_ref = nullptr;   // [nsp] reference energy per species, or null
+    double* d_mu = nullptr;    // [D]
+    double* d_sinv = nullptr;  // [D*D] Sigma^-1, column-major (Sigma_inv(:, d2) contiguous)
+    void* work = nullptr; size_t cap_work = 0;
+};
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// one wave per atom: g_a, e_a (+ reference energy) and d2_a
+template <typename T>
+__global__ void __launch_bounds__(256) atom_kernel(int natoms, int D, const T* __restrict__ energy,
+                                                   const int* __restrict__ itype, const double* __restrict__ w,
+                                                   const double* __restrict__ U, const double* __restrict__ ref,
+                                                   const double* __restrict__ mu, const double* __restrict__ sinv,
+                                                   double* __restrict__ g, double* __restrict__ e_atom,
+                                                   double* __restrict__ d2)
+{
+    __shared__ double sB[4][DMAX];
+    __shared__ double sV[4][DMAX];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int a = blockIdx.x * 4 + wv;
+    if (a >= natoms) return;  // whole wave leaves together; no block barrier below
+    for (int d = lane; d < D; d += 64) {
+        double b = (double)energy[(size_t)a * D + d];
+        sB[wv][d] = b;
+        if (mu) sV[wv][d] = b - mu[d];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    double epart = 0.0, dpart = 0.0;
+    for (int d = lane; d < D; d += 64) {
+        double hd = 0.0;
+        if (U) {
+            const double* Ur = U + (size_t)d * D;
+            for (int q = 0; q < D; ++q) hd += Ur[q] * sB[wv][q];
+        }
+        const double gd = w[d] + hd;
+        g[(size_t)a * D + d] = gd;
+        epart += sB[wv][d] * (w[d] + 0.5 * hd);
+        if (mu) {
+            // column d of Sigma^-1 dotted with v, times v_d (the reference's loop over d2)
+            const double* Sc = sinv + (size_t)d * D;
+            double t = 0.0;
+            for (int q = 0; q < D; ++q) t += sV[wv][q] * Sc[q];
+            dpart += t * sV[wv][d];
+        }
+    }
+    epart = wave_sum(epart);
+    if (mu) dpart = wave_sum(dpart);
+    if (lane == 0) {
+        e_atom[a] = epart + (ref ? ref[itype[a]] : 0.0);
+        if (d2) d2[a] = dpart;
+    }
+}
+
+// one wave per centre atom: s[slot*3 + x] = g_a . dB_a^(slot)_x
+template <typename T>
+__global__ void __launch_bounds__(256) slot_dot_kernel(int natoms, int D, const int* __restrict__ pair_off,
+                                                       const double* __restrict__ g, const T* __restrict__ grad,
+                                                       double* __restrict__ s)
+{
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int a = blockIdx.x * 4 + wv;
+    if (a >= natoms) return;
+    double gr[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        int d = lane + 64 * u;
+        gr[u] = d < D ? g[(size_t)a * D + d] : 0.0;
+    }
+    const size_t s0 = (size_t)pair_off[a] + a;
+    const int nslot = pair_off[a + 1] - pair_off[a] + 1;
+    for (int r = 0; r < 3 * nslot; ++r) {
+        const size_t row = s0 * 3 + r;
+        const T* src = grad + row * D;
+        double v = 0.0;
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            int d = lane + 64 * u;
+            if (d < D) v += gr[u] * (double)src[d];
+        }
+        v = wave_sum(v);
+        if (lane == 0) s[row] = v;
+    }
+}
+
+// one block per structure: E_s = w0 N_s + sum_a e_a (fixed tree order)
+__global__ void __launch_bounds__(256) energy_kernel(const int* __restrict__ atom_off, double w0,
+                                                     const double* __restrict__ e_atom, double* __restrict__ E)
+{
+    __shared__ double red[256];
+    const int s = blockIdx.x;
+    const int a0 = atom_off[s], a1 = atom_off[s + 1];
+    double v = 0.0;
+    for (int a = a0 + threadIdx.x; a < a1; a += 256) v += e_atom[a];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) E[s] = w0 * (double)(a1 - a0) + red[0];
+}
+
+// one thread per (atom, direction): F = -(self slot + slots of k in its neighbours' lists)
+__global__ void force_kernel(int natoms, const int* __restrict__ pair_off, const int* __restrict__ rev_off,
+                             const int* __restrict__ rev_pair, const int* __restrict__ pair_src,
+                             const double* __restrict__ s, double* __restrict__ F)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * natoms) return;
+    const int k = t / 3, x = t - 3 * k;
+    double v = s[((size_t)pair_off[k] + k) * 3 + x];
+    for (int r = rev_off[k]; r < rev_off[k + 1]; ++r) {
+        const int p = rev_pair[r];
+        v += s[((size_t)p + pair_src[p] + 1) * 3 + x];
+    }
+    F[t] = -v;
+}
+
+PredictState* state(mld_ctx* ctx)
+{
+    if (!ctx->predict_state) ctx->predict_state = new PredictState();
+    return reinterpret_cast<PredictState*>(ctx->predict_state);
+}
+
+int upload(mld_ctx* ctx, double** dst, const std::vector<double>& v)
+{
+    if (*dst) MLD_HIP(ctx, hipFree(*dst));
+    *dst = nullptr;
+    if (v.empty()) return MLD_OK;
+    MLD_HIP(ctx, hipMalloc((void**)dst, v.size() * sizeof(double)));
+    MLD_HIP(ctx, hipMemcpy(*dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    return MLD_OK;
+}
+
+}  // namespace
+
+void predict_free(mld_ctx* ctx)
+{
+    if (!ctx->predict_state) return;
+    PredictState* ps = reinterpret_cast<PredictState*>(ctx->predict_state);
+    void* q[] = {ps->d_w, ps->d_U, ps->d_ref, ps->d_mu, ps->d_sinv, ps->work};
+    for (void* p : q)
+        if (p) (void)hipFree(p);
+    delete ps;
+    ctx->predict_state = nullptr;
+}
+
+extern "C" {
+
+int mld_model_set(mld_ctx* ctx, int order, int ncol, const double* w, const double* ref_energy)
+{
+    if (!ctx || !w) return ctx ? mld_fail(ctx, MLD_E_ARG, "mld_model_set: null coefficients") : MLD_E_ARG;
+    const int D = ctx->D;
+    const int n_lin = 1 + D, n_uni = 1 + D + D * (D + 1) / 2, n_full = 1 + D + D * D;
+    if (order == 1 && ncol != n_lin)
+        return mld_fail(ctx, MLD_E_ARG, "mld_model_set: order 1 needs 1 + D coefficients");
+    if (order == 2 && ncol != n_uni && ncol != n_full)
+        return mld_fail(ctx, MLD_E_ARG, "mld_model_set: order 2 needs 1 + D + D(D+1)/2 or 1 + D + D^2 coefficients");
+    if (order != 1 && order != 2) return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_model_set: order must be 1 or 2");
+    PredictState* ps = state(ctx);
+    ps->order = order;
+    ps->ncol = ncol;
+    ps->D = D;
+    ps->w0 = w[0];
+    std::vector<double> lin(w + 1, w + 1 + D), U;
+    if (order == 2) {
+        U.assign((size_t)D * D, 0.0);
+        if (ncol == n_uni) {
+            int c = 1 + D;
+            for (int p = 0; p < D; ++p)
+                for (int q = p; q < D; ++q, ++c) {
+                    if (p == q) U[(size_t)p * D + p] = 2.0 * w[c];
+                    else U[(size_t)p * D + q] = U[(size_t)q * D + p] = w[c];
+                }
+        } else {
+            // reference layout: quadratic block w(1 + D + D*p + q) multiplies B_p B_q (mld.F90:771-780)
+            for (int p = 0; p < D; ++p)
+                for (int q = 0; q < D; ++q) {
+                    double v = w[1 + D + (size_t)D * p + q];
+                    U[(size_t)p * D + q] += v;
+                    U[(size_t)q * D + p] += v;
+                }
+        }
+    }
+    int rc = upload(ctx, &ps->d_w, lin);
+    if (!rc) rc = upload(ctx, &ps->d_U, U);
+    if (!rc) rc = upload(ctx, &ps->d_ref, ref_energy ? std::vector<double>(ref_energy, ref_energy + ctx->nsp)
+                                                      : std::vector<double>());
+    return rc;
+}
+
+int mld_mahalanobis_set(mld_ctx* ctx, const double* mean, const double* sigma_inv)
+{
+    if (!ctx) return MLD_E_ARG;
+    PredictState* ps = state(ctx);
+    const int D = ctx->D;
+    if (!mean || !sigma_inv) {
+        int rc = upload(ctx, &ps->d_mu, {});
+        return rc ? rc : upload(ctx, &ps->d_sinv, {});
+    }
+    int rc = upload(ctx, &ps->d_mu, std::vector<double>(mean, mean + D));
+    return rc ? rc : upload(ctx, &ps->d_sinv, std::vector<double>(sigma_inv, sigma_inv + (size_t)D * D));
+}
+
+int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev)
+{
+    if (!ctx) return MLD_E_ARG;
+    PredictState* ps = reinterpret_cast<PredictState*>(ctx->predict_state);
+    if (!ps || !ps->d_w) return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: mld_model_set first");
+    if (!ctx->last_energy) return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: no descriptors (mld_batch_bso4_dev first)");
+    if (F_dev && !ctx->last_grad)
+        return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: forces need descriptors with gradients");
+    if (d2_dev && !ps->d_mu) return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: mld_mahalanobis_set first");
+    if (ps->D != ctx->D) return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: model dimension changed");
+    const int natoms = ctx->natoms, nstruct = ctx->nstruct, D = ctx->D;
+    const long long P = ctx->npairs;
+    hipStream_t st = ctx->stream;
+    auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
+    const size_t bg = r256((size_t)natoms * D * sizeof(double)), be = r256((size_t)natoms * sizeof(double));
+    const size_t bs = r256((size_t)(P + natoms) * 3 * sizeof(double));
+    int rc = device_ensure(ctx, &ps->work, &ps->cap_work, bg + be + bs + 256);
+    if (rc) return rc;
+    double* g = (double*)ps->work;
+    double* e_atom = (double*)((char*)ps->work + bg);
+    double* s = (double*)((char*)ps->work + bg + be);
+    const bool f32 = ctx->p.precision == MLD_PREC_FP32;
+    const double* mu = d2_dev ? ps->d_mu : nullptr;
+    const int nblk = (natoms + 3) / 4;
+    if (natoms) {
+        if (f32)
+            hipLaunchKernelGGL(atom_kernel<float>, dim3(nblk), dim3(256), 0, st, natoms, D, (const float*)ctx->last_energy,
+                               ctx->d_itype, ps->d_w, ps->d_U, ps->d_ref, mu, ps->d_sinv, g, e_atom, d2_dev);
+        else
+            hipLaunchKernelGGL(atom_kernel<double>, dim3(nblk), dim3(256), 0, st, natoms, D, (const double*)ctx->last_energy,
+                               ctx->d_itype, ps->d_w, ps->d_U, ps->d_ref, mu, ps->d_sinv, g, e_atom, d2_dev);
+    }
+    if (E_dev && nstruct)
+        hipLaunchKernelGGL(energy_kernel, dim3(nstruct), dim3(256), 0, st, ctx->d_atom_off, ps->w0, e_atom, E_dev);
+    if (F_dev && natoms) {
+        if (f32)
+            hipLaunchKernelGGL(slot_dot_kernel<float>, dim3(nblk), dim3(256), 0, st, natoms, D, ctx->d_pair_off, g,
+                               (const float*)ctx->last_grad, s);
+        else
+            hipLaunchKernelGGL(slot_dot_kernel<double>, dim3(nblk), dim3(256), 0, st, natoms, D, ctx->d_pair_off, g,
+                               (const double*)ctx->last_grad, s);
+        const int *rev_off, *rev_pair, *pair_src;
+        rc = pair_transpose(ctx, &rev_off, &rev_pair, &pair_src);
+        if (rc) return rc;
+        hipLaunchKernelGGL(force_kernel, dim3((3 * natoms + 255) / 256), dim3(256), 0, st, natoms, ctx->d_pair_off,
+                           rev_off, rev_pair, pair_src, s, F_dev);
+    }
+    MLD_HIP(ctx, hipGetLastError());
+    return MLD_OK;
+}
+
+}  // extern "C"

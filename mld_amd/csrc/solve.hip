@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(256) trsv_kernel(int n, int ld, const double* 
 
 // S = diag^-1/2, Ls = lower(S (G+Lambda) S), from the upper-accumulated G
 __global__ void prep_kernel(int n, int ld, const double* __restrict__ G, const double* __restrict__ ridge,
-                            double* __restrict__ L, double* __restrict__ S)
+                            double shift, double* __restrict__ L, double* __restrict__ S)
 {
     int i = blockIdx.x;
     double gii = G[(size_t)i * ld + i] + ridge[i];
@@ -162,7 +162,7 @@ __global__ void prep_kernel(int n, int ld, const double* __restrict__ G, const d
         double gjj = G[(size_t)j * ld + j] + ridge[j];
         double sj = gjj > 0.0 ? 1.0 / sqrt(gjj) : 1.0;
         double v = G[(size_t)j * ld + i] + (i == j ? ridge[i] : 0.0);  // upper element (j <= i)
-        L[(size_t)i * ld + j] = si * v * sj;
+        L[(size_t)i * ld + j] = si * v * sj + (i == j ? shift : 0.0);
     }
 }
 
@@ -183,7 +183,7 @@ __global__ void resid_gram_kernel(int n, int ld, const double* __restrict__ G, c
         if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) r[i] = h[i] - red[0] - ridge[i] * w[i];
+    if (threadIdx.x == 0) r[i] = (h ? h[i] : 0.0) - red[0] - ridge[i] * w[i];
 }
 
 // t_r = y_r - A_r . w  (one wavefront per row)
@@ -196,7 +196,7 @@ __global__ void row_resid_kernel(int M, int ld, int n, const double* __restrict_
     double s = 0.0;
     for (int c = lane; c < n; c += 64) s += A[(size_t)r * ld + c] * w[c];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) t[r] = y[r] - s;
+    if (lane == 0) t[r] = (y ? y[r] : 0.0) - s;
 }
 
 __global__ void at_t_kernel(int M, int ld, int n, const double* __restrict__ A, const double* __restrict__ t,
@@ -262,13 +262,16 @@ void kept_rows_free(mld_ctx* ctx)
     ctx->kept_M.clear();
 }
 
+// r = h - (G + Lambda) w, or -(G + Lambda) w when with_rhs == 0; from the kept FP64
+// rows (r = A^T (y - A w) - Lambda w) when use_rows, else from the accumulated Gram
 static int residual(mld_ctx* ctx, const double* ridge, const double* w, double* r, double* tbuf, double* part,
-                    int use_rows)
+                    int use_rows, int with_rhs = 1)
 {
     hipStream_t st = ctx->stream;
     const int n = ctx->gD, ld = ctx->gld;
     if (!use_rows) {
-        hipLaunchKernelGGL(resid_gram_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, ridge, ctx->d_h, w, r);
+        hipLaunchKernelGGL(resid_gram_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, ridge,
+                           with_rhs ? ctx->d_h : nullptr, w, r);
         MLD_HIP(ctx, hipGetLastError());
         return MLD_OK;
     }
@@ -276,7 +279,7 @@ static int residual(mld_ctx* ctx, const double* ridge, const double* w, double* 
     for (size_t q = 0; q < ctx->kept_A.size(); ++q) {
         int M = ctx->kept_M[q];
         hipLaunchKernelGGL(row_resid_kernel, dim3((M + 3) / 4), dim3(256), 0, st, M, ld, n, ctx->kept_A[q],
-                           ctx->kept_y[q], w, tbuf);
+                           with_rhs ? ctx->kept_y[q] : nullptr, w, tbuf);
         int ns = std::max(1, std::min(256, M / 1024));
         int rps = (M + ns - 1) / ns;
         ns = (M + rps - 1) / rps;
@@ -341,61 +344,130 @@ extern "C" int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w
     double* tbuf = (double*)base; base += maxM * sizeof(double);
     double* part = (double*)base;
     MLD_HIP(ctx, hipMemcpyAsync(dridge, ridge.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
-    MLD_HIP(ctx, hipMemsetAsync(info, 0, sizeof(int), st));
-    MLD_HIP(ctx, hipMemsetAsync(L, 0, bytes_L, st));
-    hipLaunchKernelGGL(prep_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, dridge, L, S);
-    for (size_t k = 0; k < nb; ++k) {
-        hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, st, n, ld, L, (int)k, info);
-        int nrem = (int)(nb - k - 1);
-        if (nrem > 0) {
-            hipLaunchKernelGGL(chol_panel_kernel, dim3(nrem), dim3(256), 0, st, n, ld, L, (int)k);
-            hipLaunchKernelGGL(chol_update_kernel, dim3(nrem * (nrem + 1) / 2), dim3(256), 0, st, n, ld, L, (int)k, nrem);
+    // Factorise M = S^-1 (S (G + Lambda) S + shift I) S^-1.  The FP64 Gram is factorised
+    // as is (shift 0); a split-BF16 Gram carries a relative error of about 1e-6 of its
+    // diagonal, so its factor gets a diagonal shift of that size, raised tenfold while
+    // the factorisation meets a non-positive pivot.  M then only preconditions the
+    // FP64 iteration below, which solves (G + Lambda) w = h itself.
+    double shift = ctx->gram_mixed ? 1e-6 : 0.0;
+    for (;;) {
+        MLD_HIP(ctx, hipMemsetAsync(info, 0, sizeof(int), st));
+        MLD_HIP(ctx, hipMemsetAsync(L, 0, bytes_L, st));
+        hipLaunchKernelGGL(prep_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, dridge, shift, L, S);
+        for (size_t k = 0; k < nb; ++k) {
+            hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, st, n, ld, L, (int)k, info);
+            int nrem = (int)(nb - k - 1);
+            if (nrem > 0) {
+                hipLaunchKernelGGL(chol_panel_kernel, dim3(nrem), dim3(256), 0, st, n, ld, L, (int)k);
+                hipLaunchKernelGGL(chol_update_kernel, dim3(nrem * (nrem + 1) / 2), dim3(256), 0, st, n, ld, L, (int)k, nrem);
+            }
         }
+        MLD_HIP(ctx, hipGetLastError());
+        int hinfo = 0;
+        MLD_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
+        MLD_HIP(ctx, hipStreamSynchronize(st));
+        if (!hinfo) break;
+        if (!ctx->gram_mixed || shift >= 1e-2)
+            return mld_fail(ctx, MLD_E_SINGULAR, "Cholesky: leading minor " + std::to_string(hinfo) +
+                                                     " not positive definite (increase lambda)");
+        shift *= 10.0;
     }
-    MLD_HIP(ctx, hipGetLastError());
-    int hinfo = 0;
-    MLD_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
-    MLD_HIP(ctx, hipStreamSynchronize(st));
-    if (hinfo)
-        return mld_fail(ctx, MLD_E_SINGULAR, "Cholesky: leading minor " + std::to_string(hinfo) +
-                                                 " not positive definite (increase lambda)");
-    auto solve_scaled = [&](double* v) {  // v <- (G+Lambda)^-1 v using S L L^T S
+    auto precond = [&](double* v) {  // v <- M^-1 v using S L L^T S
         hipLaunchKernelGGL(scale_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, S, v);
         hipLaunchKernelGGL(trsv_kernel, dim3(1), dim3(256), 0, st, n, ld, L, v, 0);
         hipLaunchKernelGGL(trsv_kernel, dim3(1), dim3(256), 0, st, n, ld, L, v, 1);
         hipLaunchKernelGGL(scale_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, S, v);
     };
-    int use_rows = ctx->keep_rows && !ctx->kept_A.empty();
+    auto to_host = [&](std::vector<double>& hv, const double* dv) -> int {
+        MLD_HIP(ctx, hipMemcpyAsync(hv.data(), dv, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        MLD_HIP(ctx, hipStreamSynchronize(st));
+        return MLD_OK;
+    };
+    auto to_dev = [&](double* dv, const std::vector<double>& hv) -> int {
+        MLD_HIP(ctx, hipMemcpyAsync(dv, hv.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+        return MLD_OK;
+    };
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
+        long double s = 0.0L;
+        for (int i = 0; i < n; ++i) s += (long double)a[i] * b[i];
+        return (double)s;
+    };
+    const int use_rows = ctx->keep_rows && !ctx->kept_A.empty();
+    // b = h (from the rows when kept), w0 = M^-1 b
     MLD_HIP(ctx, hipMemsetAsync(w, 0, (size_t)n * sizeof(double), st));
-    // initial solve from the Gram right-hand side (or the row residual at w = 0)
     int rc = residual(ctx, dridge, w, r, tbuf, part, use_rows);
     if (rc) return rc;
-    std::vector<double> hr(n), hh(n);
-    MLD_HIP(ctx, hipMemcpyAsync(hh.data(), r, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
-    MLD_HIP(ctx, hipStreamSynchronize(st));
-    double hnorm = 0.0;
-    for (double v : hh) hnorm += v * v;
-    hnorm = std::sqrt(hnorm);
-    int iters = 0;
-    double relres = 1.0;
-    const int maxit = (method == MLD_SOLVE_CHOL_IR) ? 30 : 1;
-    double prev = 1e300;
-    for (int it = 0; it < maxit; ++it) {
+    std::vector<double> hb(n), hx(n), hr(n), hz(n), hp(n), hq(n), best(n);
+    if ((rc = to_host(hb, r))) return rc;
+    const double bnorm = std::sqrt(dot(hb, hb));
+    MLD_HIP(ctx, hipMemcpyAsync(d, r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    precond(d);
+    MLD_HIP(ctx, hipMemcpyAsync(w, d, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if ((rc = to_host(hx, w))) return rc;
+    int iters = 1;
+    rc = residual(ctx, dridge, w, r, tbuf, part, use_rows);
+    if (rc) return rc;
+    if ((rc = to_host(hr, r))) return rc;
+    double relres = bnorm > 0 ? std::sqrt(dot(hr, hr)) / bnorm : 0.0;
+    double best_res = relres;
+    best = hx;
+    if (method == MLD_SOLVE_CHOL_IR && relres > 1e-15) {
+        // preconditioned conjugate gradients on (G + Lambda) w = h in FP64: the operator
+        // is applied from the FP64 rows (or the FP64 Gram), the preconditioner is M
+        const int maxit = 200;
         MLD_HIP(ctx, hipMemcpyAsync(d, r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
-        solve_scaled(d);
-        hipLaunchKernelGGL(axpy_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, d, w);
-        ++iters;
+        precond(d);
+        if ((rc = to_host(hz, d))) return rc;
+        hp = hz;
+        double rz = dot(hr, hz);
+        int since_best = 0;
+        for (int it = 0; it < maxit; ++it) {
+            if ((rc = to_dev(d, hp))) return rc;
+            rc = residual(ctx, dridge, d, r, tbuf, part, use_rows, 0);  // r = -(G + Lambda) p
+            if (rc) return rc;
+            if ((rc = to_host(hq, r))) return rc;
+            for (double& v : hq) v = -v;
+            const double pq = dot(hp, hq);
+            if (!(pq > 0.0)) break;
+            const double alpha = rz / pq;
+            for (int i = 0; i < n; ++i) {
+                hx[i] += alpha * hp[i];
+                hr[i] -= alpha * hq[i];
+            }
+            ++iters;
+            // true residual every 8 steps (and at the end) keeps the recurrence honest
+            if ((it & 7) == 7) {
+                if ((rc = to_dev(w, hx))) return rc;
+                rc = residual(ctx, dridge, w, r, tbuf, part, use_rows);
+                if (rc) return rc;
+                if ((rc = to_host(hr, r))) return rc;
+            }
+            relres = bnorm > 0 ? std::sqrt(dot(hr, hr)) / bnorm : 0.0;
+            if (relres < best_res) {
+                best_res = relres;
+                best = hx;
+                since_best = 0;
+            } else if (++since_best > 16) {
+                break;
+            }
+            if (relres < 1e-14) break;
+            if ((rc = to_dev(d, hr))) return rc;
+            precond(d);
+            if ((rc = to_host(hz, d))) return rc;
+            const double rz_new = dot(hr, hz);
+            const double beta = rz_new / rz;
+            rz = rz_new;
+            for (int i = 0; i < n; ++i) hp[i] = hz[i] + beta * hp[i];
+        }
+        // report the true residual of the returned vector
+        if ((rc = to_dev(w, best))) return rc;
         rc = residual(ctx, dridge, w, r, tbuf, part, use_rows);
         if (rc) return rc;
-        MLD_HIP(ctx, hipMemcpyAsync(hr.data(), r, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
-        MLD_HIP(ctx, hipStreamSynchronize(st));
-        double rn = 0.0;
-        for (double v : hr) rn += v * v;
-        relres = hnorm > 0 ? std::sqrt(rn) / hnorm : 0.0;
-        if (method != MLD_SOLVE_CHOL_IR) break;
-        if (relres < 1e-15 || relres > 0.5 * prev) break;
-        prev = relres;
+        if ((rc = to_host(hr, r))) return rc;
+        relres = bnorm > 0 ? std::sqrt(dot(hr, hr)) / bnorm : 0.0;
     }
+    MLD_HIP(ctx, hipMemcpyAsync(w, best.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+    MLD_HIP(ctx, hipStreamSynchronize(st));
     MLD_HIP(ctx, hipMemcpy(w_out, w, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     if (diag_out) {
         std::vector<double> hl((size_t)n);

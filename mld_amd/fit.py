@@ -63,7 +63,10 @@ class GramFit:
 
     def add_batch(self, batch, E: np.ndarray, F_dev, w_e: float = 1.0, w_f: float = 1.0):
         """Energy + force rows of the batch's last descriptors (mld_batch_bso4_dev with gradients)."""
+        from .batch import torch_ready
+
         E = np.ascontiguousarray(E, np.float64)
+        torch_ready(F_dev.device)
         self.ctx.check(self.ctx.L.mld_gram_add_batch(self.ctx.h, self.order, E.ctypes.data_as(C.c_void_p),
                                                      C.c_void_p(F_dev.data_ptr()), w_e, w_f, int(self.mixed)))
 
@@ -108,5 +111,46 @@ def kernel_po_set(ctx, zref: np.ndarray, sigma: float, length: float, power: flo
 
 
 def kernel_po(ctx, ek, fk=None):
+    from .batch import torch_ready
+
+    torch_ready(ek.device)
     ctx.check(ctx.L.mld_kernel_po_dev(ctx.h, C.c_void_p(ek.data_ptr()),
                                       None if fk is None else C.c_void_p(fk.data_ptr())))
+    ctx.synchronize()
+
+
+def model_set(ctx, order: int, w: np.ndarray, ref_energy: Optional[np.ndarray] = None):
+    """Keep a fitted model on the device: w in the unique (1 + D + D(D+1)/2) or the
+    reference's full (1 + D + D^2) quadratic layout, or 1 + D for order 1."""
+    w = np.ascontiguousarray(w, np.float64)
+    ref = None if ref_energy is None else np.ascontiguousarray(ref_energy, np.float64)
+    ctx.check(ctx.L.mld_model_set(ctx.h, order, w.shape[0], w.ctypes.data_as(C.c_void_p),
+                                  None if ref is None else ref.ctypes.data_as(C.c_void_p)))
+    ctx._model_ref = (w, ref)
+
+
+def mahalanobis_set(ctx, mean: Optional[np.ndarray], sigma_inv: Optional[np.ndarray]):
+    """mean (D), sigma_inv (D, D) (symmetric; column-major = row-major)."""
+    if mean is None:
+        ctx.check(ctx.L.mld_mahalanobis_set(ctx.h, None, None))
+        return
+    mu = np.ascontiguousarray(mean, np.float64)
+    S = np.asfortranarray(sigma_inv, np.float64)
+    ctx.check(ctx.L.mld_mahalanobis_set(ctx.h, mu.ctypes.data_as(C.c_void_p), S.ctypes.data_as(C.c_void_p)))
+
+
+def predict(ctx, batch, energies: bool = True, forces: bool = True, d2: bool = False):
+    """E (nstruct), F (natoms, 3), d2 (natoms) of the batch's last descriptors, as device tensors."""
+    import torch
+
+    dev = batch.device
+    E = torch.empty(batch.nstruct, dtype=torch.float64, device=dev) if energies else None
+    F = torch.empty((batch.natoms, 3), dtype=torch.float64, device=dev) if forces else None
+    D2 = torch.empty(batch.natoms, dtype=torch.float64, device=dev) if d2 else None
+    ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+    from .batch import torch_ready
+
+    torch_ready(dev)
+    ctx.check(ctx.L.mld_predict_dev(ctx.h, ptr(E), ptr(F), ptr(D2)))
+    ctx.synchronize()
+    return E, F, D2

@@ -124,3 +124,43 @@ def hea_bcc(rng: np.random.Generator, n: int, a: float = 3.1, nspecies: int = 5,
     pos = pos + rng.normal(0.0, sigma, pos.shape)
     itype = rng.integers(0, nspecies, pos.shape[0]).astype(np.int32)
     return pos, cell, itype
+
+
+def bcc_vacancy_structures(rng: np.random.Generator, count: int, nmin: int = 128, nmax: int = 256,
+                           a: float = 2.855, sigma: float = 0.05):
+    """bcc supercells with N ~ U{nmin..nmax} atoms (random vacancies on the smallest
+    near-cubic supercell that holds N sites), Gaussian displacements sigma."""
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        ncell = _cells_for(n, 2)
+        pos, cell = lattice(BCC, a, ncell)
+        if pos.shape[0] > n:
+            keep = np.sort(rng.choice(pos.shape[0], n, replace=False))
+            pos = pos[keep]
+        out.append((pos + rng.normal(0.0, sigma, pos.shape), cell))
+    return out
+
+
+def morse_from_pairs(pair_off, pair_idx, pair_d, atom_off, D: float = 0.4, alpha: float = 1.4, r0: float = 2.5,
+                     rc: float = 4.7):
+    """Morse energies (per structure) and forces (per atom) from a device pair list (torch tensors);
+    pairs are directed (i -> j, d = r_j - r_i) and all have r < rc."""
+    import torch
+
+    natoms = pair_off.shape[0] - 1
+    counts = (pair_off[1:] - pair_off[:-1]).long()
+    src = torch.repeat_interleave(torch.arange(natoms, device=pair_d.device), counts)
+    r = pair_d[:, 3]
+    m = (r < rc).double()
+    ex = torch.exp(-alpha * (r - r0))
+    phi = D * (ex * ex - 2.0 * ex) * m
+    dphi = D * (-2.0 * alpha * ex * ex + 2.0 * alpha * ex) * m
+    e_atom = torch.zeros(natoms, dtype=torch.float64, device=pair_d.device).index_add_(0, src, 0.5 * phi)
+    F = torch.zeros((natoms, 3), dtype=torch.float64, device=pair_d.device)
+    F.index_add_(0, src, (dphi / r)[:, None] * pair_d[:, :3])
+    ao = torch.as_tensor(atom_off, device=pair_d.device).long()
+    nstruct = ao.shape[0] - 1
+    sid = torch.repeat_interleave(torch.arange(nstruct, device=pair_d.device), ao[1:] - ao[:-1])
+    E = torch.zeros(nstruct, dtype=torch.float64, device=pair_d.device).index_add_(0, sid, e_atom)
+    return E, F
