@@ -88,7 +88,22 @@ __global__ void rows_energy_kernel(int order, int D, int ldA, const int* __restr
     if (threadIdx.x == 0) y[row] = E[s] * swe;
 }
 
-// one block per (atom k, direction x)
+// One block per atom k: its three force rows (x = 0, 1, 2) are consecutive in A.
+//   linear columns    : A[1 + d]          = -sum_s g_s,x(d)
+//   quadratic columns : A[1 + D + c(p,q)] = -sum_s (g_s,x(p) b_s(q) + g_s,x(q) b_s(p)),  p <= q
+// over the slots s of k (its own slot, then the slots that name k as a neighbour, in
+// pair order), g = dB/dr of the slot and b = B of the slot's centre atom.  The
+// quadratic part is register-blocked in 4 x 4 (p, q) tiles; the three rows are staged
+// in LDS and written as one contiguous stream.  Dynamic LDS:
+//   sg[CH][3][DP] + sb[CH][DP] + out[3][ldA] doubles, DP = D rounded up to 4.
+constexpr int ROWS_CH = 16;
+
+__host__ __device__ inline size_t rows_force_smem(int D, int ldA)
+{
+    const int DP = (D + 3) & ~3;
+    return sizeof(double) * ((size_t)ROWS_CH * 3 * DP + (size_t)ROWS_CH * DP + 3 * (size_t)ldA);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int ldA, const int* __restrict__ atom_struct,
                                                          const int* __restrict__ atom_off, const int* __restrict__ pair_off,
@@ -97,61 +112,127 @@ __global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int l
                                                          const T* __restrict__ grad, const double* __restrict__ F,
                                                          double swf, double* __restrict__ A, double* __restrict__ y)
 {
+    extern __shared__ __align__(16) double rsm[];
+    const int DP = (D + 3) & ~3;
+    double* sg = rsm;                                  // [CH][3][DP]
+    double* sb = sg + ROWS_CH * 3 * DP;                // [CH][DP]
+    double* out = sb + ROWS_CH * DP;                   // [3][ldA]
     const int k = blockIdx.x;
-    const int x = blockIdx.y;
+    const int tid = threadIdx.x;
     const int s = atom_struct[k];
-    const size_t row = (size_t)s + 3 * (size_t)atom_off[s] + 1 + 3 * (size_t)(k - atom_off[s]) + x;
-    double* Ar = A + row * ldA;
+    const size_t row0 = (size_t)s + 3 * (size_t)atom_off[s] + 1 + 3 * (size_t)(k - atom_off[s]);
     const int r0 = rev_off[k], r1 = rev_off[k + 1];
     const int nslot = 1 + (r1 - r0);
-    constexpr int CH = 16;
-    __shared__ double sg[CH][mld::NB_MAX * mld::NCH_MAX];
-    __shared__ double sb[CH][mld::NB_MAX * mld::NCH_MAX];
-    const int ncol = 1 + D + (order == 2 ? tri_cols(D) : 0);
-    // per-thread accumulators for up to 8 columns: c = threadIdx.x + 256*u
-    double acc[8];
+    const int nb4 = DP >> 2;
+    const int ntri = nb4 * (nb4 + 1) / 2;
+    const int nquad = (order == 2) ? 3 * ntri : 0;
+    // quadratic items owned by this thread: it = tid, tid + 256 (x, bp, bq)
+    int qx[2], qp[2], qq[2];
+    bool qv[2];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
-    for (int c0 = 0; c0 < nslot; c0 += CH) {
-        const int nc = min(CH, nslot - c0);
+    for (int u = 0; u < 2; ++u) {
+        const int it = tid + 256 * u;
+        qv[u] = it < nquad;
+        int x = 0, bp = 0, bq = 0;
+        if (qv[u]) {
+            x = it / ntri;
+            int t = it - x * ntri;
+            while (t >= nb4 - bp) { t -= nb4 - bp; ++bp; }
+            bq = bp + t;
+        }
+        qx[u] = x; qp[u] = bp; qq[u] = bq;
+    }
+    double acc[2][4][4];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[u][i][j] = 0.0;
+    // linear items: (x, d) = tid + 256 v over 3 * D
+    double lin[3] = {0.0, 0.0, 0.0};
+    for (int c0 = 0; c0 < nslot; c0 += ROWS_CH) {
+        const int nc = min(ROWS_CH, nslot - c0);
         __syncthreads();
-        for (int q = threadIdx.x; q < nc * D; q += blockDim.x) {
-            int sl = q / D, d = q - sl * D;
-            int idx = c0 + sl;
+        for (int q = tid; q < nc * DP; q += 256) {
+            const int sl = q / DP, d = q - sl * DP;
+            const int idx = c0 + sl;
             size_t slot;
             int centre;
             if (idx == 0) {
                 slot = (size_t)pair_off[k] + k;  // self slot of k
                 centre = k;
             } else {
-                int p = rev_pair[r0 + idx - 1];
+                const int p = rev_pair[r0 + idx - 1];
                 centre = pair_src[p];
                 slot = (size_t)p + centre + 1;
             }
-            sg[sl][d] = (double)grad[(slot * 3 + x) * D + d];
-            sb[sl][d] = (double)energy[(size_t)centre * D + d];
+            const bool in = d < D;
+#pragma unroll
+            for (int x = 0; x < 3; ++x) sg[(sl * 3 + x) * DP + d] = in ? (double)grad[(slot * 3 + x) * D + d] : 0.0;
+            sb[sl * DP + d] = in ? (double)energy[(size_t)centre * D + d] : 0.0;
         }
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            int c = threadIdx.x + 256 * u;
-            if (c >= ncol || c == 0) continue;
-            if (c <= D) {
-                for (int sl = 0; sl < nc; ++sl) acc[u] -= sg[sl][c - 1];
-            } else {
-                int q = c - 1 - D, p = 0;
-                while (q >= D - p) { q -= D - p; ++p; }
-                q += p;
-                for (int sl = 0; sl < nc; ++sl) acc[u] -= sg[sl][p] * sb[sl][q] + sg[sl][q] * sb[sl][p];
+        for (int v = 0; v < 3; ++v) {
+            const int it = tid + 256 * v;
+            if (it < 3 * D) {
+                const int x = it / D, d = it - x * D;
+                for (int sl = 0; sl < nc; ++sl) lin[v] += sg[(sl * 3 + x) * DP + d];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (!qv[u]) continue;
+            const int p0 = 4 * qp[u], q0 = 4 * qq[u];
+            for (int sl = 0; sl < nc; ++sl) {
+                const double* g = sg + (sl * 3 + qx[u]) * DP;
+                const double* bb = sb + sl * DP;
+                double gp[4], gq[4], bp[4], bq[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    gp[i] = g[p0 + i];
+                    gq[i] = g[q0 + i];
+                    bp[i] = bb[p0 + i];
+                    bq[i] = bb[q0 + i];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[u][i][j] += gp[i] * bq[j] + gq[j] * bp[i];
             }
         }
     }
-    for (int u = 0; u < 8; ++u) {
-        int c = threadIdx.x + 256 * u;
-        if (c < ldA) Ar[c] = acc[u] * swf;  // column 0 and padding stay zero
+    // stage the three rows: zero, then linear and quadratic columns
+    for (int q = tid; q < 3 * ldA; q += 256) out[q] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const int it = tid + 256 * v;
+        if (it < 3 * D) {
+            const int x = it / D, d = it - x * D;
+            out[x * ldA + 1 + d] = -lin[v] * swf;
+        }
     }
-    for (int c = threadIdx.x + 256 * 8; c < ldA; c += blockDim.x) Ar[c] = 0.0;
-    if (threadIdx.x == 0) y[row] = F[3 * (size_t)k + x] * swf;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        if (!qv[u]) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int p = 4 * qp[u] + i;
+            if (p >= D) continue;
+            const int offp = 1 + D + p * D - p * (p - 1) / 2 - p;  // column of (p, q) = offp + q
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int q = 4 * qq[u] + j;
+                if (q < D && q >= p) out[qx[u] * ldA + offp + q] = -acc[u][i][j] * swf;
+            }
+        }
+    }
+    __syncthreads();
+    double* Ar = A + row0 * ldA;
+    for (int q = tid; q < 3 * ldA; q += 256) Ar[q] = out[q];
+    if (tid < 3) y[row0 + tid] = F[3 * (size_t)k + tid] * swf;
 }
 
 __global__ void scale_rows_kernel(int M, int Dc, int ldA, const double* __restrict__ Ain, const double* __restrict__ yin,
@@ -502,12 +583,21 @@ int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_de
                                (const double*)ctx->last_energy, E_dev, swe, A, y);
     }
     if (natoms) {
+        const size_t fsm = rows_force_smem(D, ldA);
+        static bool attr_done = false;  // the largest layout (D = 192 linear, D = 62 quadratic) is < 110 KB
+        if (!attr_done) {
+            MLD_HIP(ctx, hipFuncSetAttribute((const void*)rows_force_kernel<float>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            MLD_HIP(ctx, hipFuncSetAttribute((const void*)rows_force_kernel<double>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr_done = true;
+        }
         if (f32)
-            hipLaunchKernelGGL(rows_force_kernel<float>, dim3(natoms, 3), dim3(256), 0, st, order, D, ldA, ctx->d_atom_struct,
+            hipLaunchKernelGGL(rows_force_kernel<float>, dim3(natoms), dim3(256), fsm, st, order, D, ldA, ctx->d_atom_struct,
                                ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src,
                                (const float*)ctx->last_energy, (const float*)ctx->last_grad, F_dev, swf, A, y);
         else
-            hipLaunchKernelGGL(rows_force_kernel<double>, dim3(natoms, 3), dim3(256), 0, st, order, D, ldA, ctx->d_atom_struct,
+            hipLaunchKernelGGL(rows_force_kernel<double>, dim3(natoms), dim3(256), fsm, st, order, D, ldA, ctx->d_atom_struct,
                                ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src,
                                (const double*)ctx->last_energy, (const double*)ctx->last_grad, F_dev, swf, A, y);
     }

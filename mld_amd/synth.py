@@ -126,14 +126,27 @@ def hea_bcc(rng: np.random.Generator, n: int, a: float = 3.1, nspecies: int = 5,
     return pos, cell, itype
 
 
+def _near_cubic(nsites_min: int, nbasis: int, max_spread: int = 2) -> Tuple[int, int, int]:
+    """Smallest n1 <= n2 <= n3 supercell with n3 - n1 <= max_spread holding nsites_min sites."""
+    best = None
+    for n1 in range(1, 16):
+        for n2 in range(n1, n1 + max_spread + 1):
+            for n3 in range(n2, n1 + max_spread + 1):
+                s = nbasis * n1 * n2 * n3
+                if s >= nsites_min and (best is None or (s, n3 - n1) < best[0]):
+                    best = ((s, n3 - n1), (n1, n2, n3))
+    return best[1]
+
+
 def bcc_vacancy_structures(rng: np.random.Generator, count: int, nmin: int = 128, nmax: int = 256,
                            a: float = 2.855, sigma: float = 0.05):
     """bcc supercells with N ~ U{nmin..nmax} atoms (random vacancies on the smallest
-    near-cubic supercell that holds N sites), Gaussian displacements sigma."""
+    near-cubic supercell, edges differing by at most two cells, that holds N sites),
+    Gaussian displacements sigma."""
     out = []
     for _ in range(count):
         n = int(rng.integers(nmin, nmax + 1))
-        ncell = _cells_for(n, 2)
+        ncell = _near_cubic(n, 2)
         pos, cell = lattice(BCC, a, ncell)
         if pos.shape[0] > n:
             keep = np.sort(rng.choice(pos.shape[0], n, replace=False))
