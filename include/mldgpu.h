@@ -147,6 +147,30 @@ int mld_model_set(mld_ctx* ctx, int order, int ncol, const double* w, const doub
 int mld_mahalanobis_set(mld_ctx* ctx, const double* mean, const double* sigma_inv);
 int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev);
 
+/* ---- serving: continuous batching of prediction requests ----
+ * No counterpart in the reference (which evaluates one configuration at a time,
+ * mld.F90:1579-1772).  mld_serve_create takes the model and Mahalanobis state of
+ * ctx as they are at the call (mld_model_set first) and starts a worker thread with
+ * two private contexts on ctx's device.  Waiting requests are admitted, oldest
+ * first, into whichever batch slot is free, up to max_batch_structs structures,
+ * max_batch_atoms atoms and the HBM budget not held by the batch in flight; the two
+ * slots' batches overlap on the device.  A request is one structure: pos (3, nat)
+ * column-major, cell (3, 3) columns = lattice vectors, itype (nat) 0-based.
+ * mld_serve_poll (wait = 1 blocks) returns E, F (3, nat), d2 (nat) (pointers may be
+ * NULL) and releases the request; done = 0 while it is queued or running.
+ * mld_serve_stats fills 10 doubles over the completed requests: [count, p50 ms,
+ * p99 ms, mean ms, requests/s, atoms/s, largest batch, mean batch, peak admitted
+ * bytes, max ms]; latency = completion seen by the worker - submission. */
+typedef struct mld_server mld_server;
+int mld_serve_create(mld_ctx* ctx, size_t hbm_budget_bytes, int max_batch_structs, int max_batch_atoms,
+                     mld_server** out);
+int mld_serve_submit(mld_server* sv, int nat, const double* pos, const double* cell, const int* itype,
+                     int want_forces, int want_d2, long long* id);
+int mld_serve_poll(mld_server* sv, long long id, int wait, double* E, double* F, double* d2, double* latency_ms,
+                   int* done);
+int mld_serve_stats(mld_server* sv, double* stats);
+int mld_serve_destroy(mld_server* sv);
+
 /* Stream used by the library (hipStream_t as void*). */
 int mld_get_stream(mld_ctx* ctx, void** stream);
 int mld_synchronize(mld_ctx* ctx);

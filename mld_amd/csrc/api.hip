@@ -71,6 +71,13 @@ int mld_create(const mld_params* p, mld_ctx** out)
     }
     ctx->nch = p->nchannel;
     ctx->nsp = p->nspecies;
+    // keep private copies of the arrays the parameters point to
+    if (p->nchannel >= 2) ctx->h_weight.assign(p->weight, p->weight + p->nspecies);
+    if (p->nchannel >= 3) ctx->h_weight3.assign(p->weight_3ch, p->weight_3ch + p->nspecies);
+    if (p->r_cut_pair_in) ctx->h_rin.assign(p->r_cut_pair_in, p->r_cut_pair_in + (size_t)p->nspecies * p->nspecies);
+    ctx->p.weight = ctx->h_weight.empty() ? nullptr : ctx->h_weight.data();
+    ctx->p.weight_3ch = ctx->h_weight3.empty() ? nullptr : ctx->h_weight3.data();
+    ctx->p.r_cut_pair_in = ctx->h_rin.empty() ? nullptr : ctx->h_rin.data();
     // inv_r0 = pi * inv_r0_input (descriptors.F90:661), r0 = r_cut / inv_r0 (angular_functions.f90:79)
     ctx->r0 = p->r_cut / (3.14159265358979323846 * p->inv_r0_input);
     int rc = MLD_OK;
@@ -184,14 +191,19 @@ int mld_batch_neighbours(mld_ctx* ctx, int nstruct, const int* atom_off, const d
         if (atom_off[s + 1] < atom_off[s]) return mld_fail(ctx, MLD_E_ARG, "atom_off must be non-decreasing");
     if (natoms > 0) {
         // species indices select weight / inner-cutoff table rows: validate them once
+        // (read on the library stream, after any upload the caller queued there)
         std::vector<int> it(natoms);
-        MLD_HIP(ctx, hipMemcpy(it.data(), itype_dev, (size_t)natoms * sizeof(int), hipMemcpyDeviceToHost));
+        MLD_HIP(ctx, hipMemcpyAsync(it.data(), itype_dev, (size_t)natoms * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        MLD_HIP(ctx, hipStreamSynchronize(ctx->stream));
         for (int a = 0; a < natoms; ++a)
             if (it[a] < 0 || it[a] >= ctx->nsp) return mld_fail(ctx, MLD_E_ARG, "itype out of range");
     }
     // cells: fetch to host for the per-structure set-up (small/large box, reciprocal vectors)
     std::vector<double> cells((size_t)9 * nstruct);
-    if (nstruct) MLD_HIP(ctx, hipMemcpy(cells.data(), cell_dev, cells.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (nstruct) {
+        MLD_HIP(ctx, hipMemcpyAsync(cells.data(), cell_dev, cells.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        MLD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     std::vector<mld::StructGeo> geo(nstruct);
     std::vector<int> astruct(natoms);
     for (int s = 0; s < nstruct; ++s) {

@@ -143,6 +143,8 @@ struct mld_ctx {
     double* d_zz = nullptr;      // [nk] |z|^2
     // fitted model and selection-score state (predict.hip)
     void* predict_state = nullptr;
+    // copies of the caller arrays named by p (species weights, inner cutoffs)
+    std::vector<double> h_weight, h_weight3, h_rin;
 };
 
 int mld_fail(mld_ctx* ctx, int code, const std::string& msg);
@@ -165,4 +167,5 @@ void gram_scratch_free(mld_ctx* ctx);
 int device_ensure(mld_ctx* ctx, void** p, size_t* cap, size_t bytes);
 int pair_transpose(mld_ctx* ctx, const int** rev_off, const int** rev_pair, const int** pair_src);
 void predict_free(mld_ctx* ctx);
+int predict_clone(mld_ctx* dst, const mld_ctx* src);
 void kept_rows_free(mld_ctx* ctx);

@@ -180,6 +180,33 @@ void predict_free(mld_ctx* ctx)
     ctx->predict_state = nullptr;
 }
 
+// Copy the model and Mahalanobis state of src to dst (same D, same device).
+int predict_clone(mld_ctx* dst, const mld_ctx* src)
+{
+    const PredictState* ps = reinterpret_cast<const PredictState*>(src->predict_state);
+    if (!ps) return MLD_OK;
+    PredictState* pd = state(dst);
+    pd->order = ps->order;
+    pd->ncol = ps->ncol;
+    pd->D = ps->D;
+    pd->w0 = ps->w0;
+    auto cp = [&](double** d, const double* s, size_t n) -> int {
+        if (*d) MLD_HIP(dst, hipFree(*d));
+        *d = nullptr;
+        if (!s) return MLD_OK;
+        MLD_HIP(dst, hipMalloc((void**)d, n * sizeof(double)));
+        MLD_HIP(dst, hipMemcpy(*d, s, n * sizeof(double), hipMemcpyDeviceToDevice));
+        return MLD_OK;
+    };
+    const size_t D = (size_t)ps->D;
+    int rc = cp(&pd->d_w, ps->d_w, D);
+    if (!rc) rc = cp(&pd->d_U, ps->d_U, D * D);
+    if (!rc) rc = cp(&pd->d_ref, ps->d_ref, (size_t)src->nsp);
+    if (!rc) rc = cp(&pd->d_mu, ps->d_mu, D);
+    if (!rc) rc = cp(&pd->d_sinv, ps->d_sinv, D * D);
+    return rc;
+}
+
 extern "C" {
 
 int mld_model_set(mld_ctx* ctx, int order, int ncol, const double* w, const double* ref_energy)
