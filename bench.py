@@ -50,15 +50,20 @@ sys.path.insert(0, HERE)
 
 SEED = 20261203
 METRIC = "atom-envs/s (desc+grads+Gram) mixed vs FP64 at stated err; serve p50/p99 ms"
-# Reference-counted work per atom (complex multiply-adds x 8 flops): U/dU recursion
-# 4*285 per neighbour, three CG packings of 34,946 terms, and the zaxpy force
-# contraction 3 * 5,907 per neighbour (2j_max = 8, 55 components).
-NU, NCG, NZAXPY = 285, 34946, 5907
+# Reference-counted work (complex multiply-adds, 8 flops each) at 2j_max = 8, 55
+# components, as the oracle's counter tallies the reference's arithmetic sites:
+#   per neighbour pair : U and dU recursion, 14 per entry x 284 entries = 3,976
+#                        (spherical_4d, angular_functions.f90:152-272), and the
+#                        zaxpy force contraction 3 x 5,907 = 17,721
+#                        (compute_bispectrum_so4.F90:524-640)
+#   per atom           : the three CG packings, 3 x 34,946 = 104,838
+#                        (pack_cmm_into_ZcmmABC, :1014-1143)
+CFMA_PAIR_REC, CFMA_PAIR_GRAD, CFMA_ATOM = 3976, 17721, 104838
 PEAK_FP32, PEAK_FP64 = 157.3, 78.6   # TFLOP/s, MI355X dense
 
 
 def algorithmic_flops(natoms: int, npairs: int) -> float:
-    return 8.0 * (4 * NU * npairs + 3 * NCG * natoms + 3 * NZAXPY * npairs)
+    return 8.0 * ((CFMA_PAIR_REC + CFMA_PAIR_GRAD) * npairs + CFMA_ATOM * natoms)
 
 
 def algorithmic_bytes(natoms: int, npairs: int, D: int, esz: int) -> float:
@@ -77,7 +82,7 @@ def desc_workload(nstruct: int):
     from mld_amd import synth
 
     rng = np.random.default_rng(SEED)
-    return [(p, c) for p, c in synth.mixed_structures(rng, nstruct)]
+    return [(s[0], s[1]) for s in synth.mixed_structures(rng, nstruct)]
 
 
 def time_descriptors(prec: int, structs, steps: int, warmup: int):
@@ -455,11 +460,22 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the fit workload's structures")
     ap.add_argument("--nstruct", type=int, default=2000, help="structures of the desc workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device-to-device 3 times (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; the sharded design (per-GPU Gram "
                                    "partials + one reduction) is not run here. Use --gpus 1"}), flush=True)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.workload == "desc":
         main_desc(args)
     elif args.workload == "serve":

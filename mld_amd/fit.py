@@ -154,3 +154,21 @@ def predict(ctx, batch, energies: bool = True, forces: bool = True, d2: bool = F
     ctx.check(ctx.L.mld_predict_dev(ctx.h, ptr(E), ptr(F), ptr(D2)))
     ctx.synchronize()
     return E, F, D2
+
+
+def write_coefficients(directory: str, w: np.ndarray, D: int, order: int, elements, j_max: float, r_cut: float,
+                       lbso4_diag: bool = False, weights=None, weights_3ch=None, ref_energy=None):
+    """Write <bso4>_<snap1|qnml1>_params.pot and lammps_... in the reference formats; a
+    quadratic solution in the unique column layout is expanded to the full D^2 block first."""
+    import os
+
+    from . import io
+
+    w = np.asarray(w, dtype=np.float64)
+    if order == 2 and w.shape[0] == ncols(D, 2):
+        w = expand_quadratic(w, D)
+    pot, lmp = io.params_file_names("bso4", order)
+    io.write_params_pot(os.path.join(directory, pot), w)
+    io.write_lammps_params_bso4(os.path.join(directory, lmp), w, elements, j_max, r_cut, order=order,
+                                lbso4_diag=lbso4_diag, weights=weights, weights_3ch=weights_3ch, ref_energy=ref_energy)
+    return os.path.join(directory, pot), os.path.join(directory, lmp)

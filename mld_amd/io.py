@@ -173,3 +173,86 @@ def _es30(v: float) -> str:
         m, e = f"{v:.18E}".split("E")
         s = f"{m}E{int(e):+04d}"
     return s.rjust(30)
+
+
+# ----------------------------------------------------------------------------- coefficient files
+def _es(v: float, w: int = 25, d: int = 15) -> str:
+    """Fortran ESw.dE3: d.ddd...E+eee, right-aligned in w characters."""
+    if v == 0.0:
+        s = "0." + "0" * d + "E+000"
+    else:
+        m, e = f"{v:.{d}E}".split("E")
+        s = f"{m}E{int(e):+04d}"
+    return s.rjust(w)
+
+
+def _fe(v: float, w: int = 20, d: int = 10) -> str:
+    """Fortran Ew.d: 0.ddd...E+ee (d significant digits), right-aligned in w characters."""
+    if v == 0.0:
+        s = "0." + "0" * d + "E+00"
+    else:
+        m, e = f"{abs(v):.{d - 1}E}".split("E")
+        digits = m.replace(".", "")
+        s = ("-" if v < 0 else "") + "0." + digits + f"E{int(e) + 1:+03d}"
+    return s.rjust(w)
+
+
+def params_file_names(descriptor: str, order: int):
+    """<desc>_<snap1|qnml1>_params.pot and its lammps_ twin (build_name_of_parameters_file,
+    module_write_parameters.F90:764-781)."""
+    tag = {1: "snap1", 2: "qnml1"}[order]
+    return f"{descriptor}_{tag}_params.pot", f"lammps_{descriptor}_{tag}_params.pot"
+
+
+def write_params_pot(path: str, w: np.ndarray) -> None:
+    """The MILADY coefficient file: the count (list-directed integer) then every
+    coefficient in ES25.15E3 (write_snap_parameters, module_write_parameters.F90:1190-1205, 1470-1477).
+    w is the reference layout: [N, B_1..B_D] or, quadratic, followed by the D*D block."""
+    w = np.asarray(w, dtype=np.float64).reshape(-1)
+    with open(path, "w") as fh:
+        fh.write(f"{w.shape[0]:12d}\n")
+        for v in w:
+            fh.write(_es(float(v)) + "\n")
+
+
+def write_lammps_params_bso4(path: str, w: np.ndarray, elements, j_max: float, r_cut: float, order: int = 1,
+                             lbso4_diag: bool = False, weights=None, weights_3ch=None, ref_energy=None) -> None:
+    """lammps_bso4_<snap1|qnml1>_params.pot read by milady_lammps: element line, descriptor
+    line, masses, details, r_cut, then the coefficients (module_write_parameters.F90:876-937,
+    1065-1085, 1186-1188)."""
+    w = np.asarray(w, dtype=np.float64).reshape(-1)
+    ne = len(elements)
+    nch = 1 + (weights is not None) + (weights_3ch is not None)
+    w1 = [1.0] * ne if weights is None else list(weights)
+    w2 = [1.0] * ne if weights_3ch is None else list(weights_3ch)
+    er = [0.0] * ne if ref_energy is None else list(ref_energy)
+    lines = [
+        "# nchannels nelements element_1 weight_1_for_el_1 weight_2_for_el_1 element_2 weight_1_for_el_2 "
+        "weight_2_for_el_2 ... etc ",
+        "# descriptor type, nparameters_descriptor, ndetails",
+        "# mass_1 mass_2 ... mass_nelements",
+        "# detail_1 is the mld_order, ..., detail_ndetails (e.g. mld_order no_of_radial_channels "
+        "no_of_angular_channels)",
+        "# r_cut only one for all descriptors",
+        "# w_i  with i=1, dim_xdesc, dim_xdesc +1 ...  (nchannel-1)*dim_xdesc ... nchannel*dim_xdesc",
+    ]
+    head = f"{nch:7d}{ne:7d}"
+    for i, el in enumerate(elements):
+        second = w2[i] if nch == 3 else 1.0
+        head += el.ljust(2).rjust(4) + _fe(w1[i]) + _fe(second) + _fe(er[i])
+    lines.append(head)
+    lines.append(f"{9:4d}{w.shape[0]:12d}{4:4d}")
+    lines.append("".join(f"{element_mass(el):20.4f}" for el in elements))
+    lines.append(f"{order:5d}{j_max:5.1f}{int(lbso4_diag):4d}{0:4d}")
+    lines.append(f"{r_cut:20.4f}")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+        for v in w:
+            fh.write(_es(float(v)) + "\n")
+
+
+def read_params_pot(path: str) -> np.ndarray:
+    with open(path) as fh:
+        n = int(fh.readline())
+        w = np.array([float(fh.readline()) for _ in range(n)])
+    return w

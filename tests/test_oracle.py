@@ -183,3 +183,16 @@ def test_rows_and_gram_against_numpy():
     Gr = (rows1 * w[:, None]).T @ rows1
     assert np.abs(np.triu(G) - np.triu(Gr)).max() < 1e-10 * np.abs(Gr).max()
     assert np.abs(h - rows1.T @ (w * y)).max() < 1e-10 * np.abs(h).max()
+
+
+def test_bench_flop_model_matches_oracle_counter():
+    """bench.py's algorithmic flops = 8 x the oracle's complex-FMA counter (B + dB/dr)."""
+    import bench
+    from mld_amd import synth
+
+    rng = np.random.default_rng(0)
+    for n in (2, 3):
+        p, c = synth.bcc_supercell(rng, 2.855, n, 0.05)
+        O.cfma_reset()
+        e, g, nl = O.bso4(p, c, np.zeros(len(p), np.int32), 8, 4.7, want_grad=True)
+        assert 8 * O.cfma_count() == bench.algorithmic_flops(len(p), int(nl[0][-1]))
