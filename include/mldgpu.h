@@ -9,6 +9,7 @@
  * Layouts follow the Fortran callers: column-major arrays, 1-based kind_neigh.
  *   pos   : (3, nat)   Angstrom            -- config_real%pos_cart
  *   cell  : (3, 3)     columns = a1,a2,a3  -- config_real%cell (build_subdata.F90:2714-2727)
+ *   itype : (nat)      0-based species index (config_real%itype - 1)
  *   energy: (D, nat)                       -- config_desc%energy
  *   force : (D, nat, 0:nn_max, 3)          -- config_desc%force after the sign flip
  *                                              of descriptors.F90:298-302, i.e. dB/dr

@@ -196,3 +196,33 @@ def test_bench_flop_model_matches_oracle_counter():
         O.cfma_reset()
         e, g, nl = O.bso4(p, c, np.zeros(len(p), np.int32), 8, 4.7, want_grad=True)
         assert 8 * O.cfma_count() == bench.algorithmic_flops(len(p), int(nl[0][-1]))
+
+
+def test_kernel_po_normalisation_and_fd_gradient():
+    """kernel_po (compute_kernel.F90:91-150): k(x, x) = 1 for the normalised kernel, and the
+    stored gradient of slot k is d k(x_a, z) / d r_(neighbour k), checked by central differences
+    of the oracle's own B on a large-box structure (each neighbour appears once)."""
+    from mld_amd import synth
+
+    rng = np.random.default_rng(4)
+    p, c = synth.bcc_supercell(rng, 2.855, 4, 0.08)
+    t = np.zeros(len(p), np.int32)
+    e, g, nl = O.bso4(p, c, t, 8, 4.7)
+    off, nbr = nl[0], nl[1]
+    a = 5
+    zref = np.stack([e[a], e[17] * 0.9 + 0.05, e[40]])
+    sigma, length, power = 1.3, 0.8 * np.sqrt(np.mean(e[a] ** 2) * 55), 2.0
+    ek, fk = O.kernel_po(e, g, nl, zref, sigma, length, power)
+    assert abs(ek[a, 0] - 1.0) < 1e-14
+    h = 1e-5
+    js = [(a, 0)] + [(int(nbr[off[a] + k - 1]), k) for k in (1, 7)]
+    for j, slot in js:
+        for x in range(3):
+            pp, pm = p.copy(), p.copy()
+            pp[j, x] += h
+            pm[j, x] -= h
+            ep = O.kernel_po(O.bso4(pp, c, t, 8, 4.7)[0], None, nl, zref, sigma, length, power, want_grad=False)[0]
+            em = O.kernel_po(O.bso4(pm, c, t, 8, 4.7)[0], None, nl, zref, sigma, length, power, want_grad=False)[0]
+            fd = (ep[a] - em[a]) / (2 * h)
+            an = fk[off[a] + a + slot, x]
+            assert np.abs(fd - an).max() < 1e-6 * max(1.0, np.abs(an).max()), (j, slot, x, fd, an)
