@@ -61,9 +61,9 @@ constexpr int wsize_bound(int jj)
                 for (int l = (l1 > l2 ? l1 - l2 : l2 - l1); l <= (jj < l1 + l2 ? jj : l1 + l2); ++l) {
                     if ((l1 + l2 + l) & 1) continue;
                     if (!diag && l < l1) continue;
-                    tot += (l + 1) * (l + 1);
-                    if (l1 != l) tot += (l1 + 1) * (l1 + 1);
-                    if (l2 != l && l2 != l1) tot += (l2 + 1) * (l2 + 1);
+                    tot += mld::hcount(l);
+                    if (l1 != l) tot += mld::hcount(l1);
+                    if (l2 != l && l2 != l1) tot += mld::hcount(l2);
                 }
         if (tot > best) best = tot;
     }
@@ -173,7 +173,13 @@ __device__ __forceinline__ void rec_step(const RecEntry& R, int b, const Geo<T>&
     cx<T> zq = k0 ? g.zp : g.zm;
     cx<T> xq = k0 ? g.xm : g.xp;
     T c1 = (T)R.c1, c2 = (T)R.c2;
-    cx<T> U1 = Up[R.s1], U2 = Up[R.s2];
+    // sources through the symmetry: (index << 2) | (conjugate << 1) | negate
+    const int i1 = R.s1 >> 2, i2 = R.s2 >> 2;
+    const T cj1 = (R.s1 & 2) ? (T)-1 : (T)1, ng1 = (R.s1 & 1) ? (T)-1 : (T)1;
+    const T cj2 = (R.s2 & 2) ? (T)-1 : (T)1, ng2 = (R.s2 & 1) ? (T)-1 : (T)1;
+    cx<T> U1 = Up[i1], U2 = Up[i2];
+    U1 = cx<T>{ng1 * U1.r, ng1 * cj1 * U1.i};
+    U2 = cx<T>{ng2 * U2.r, ng2 * cj2 * U2.i};
     cx<T> t1 = cscale(c1, zq);
     cx<T> t2 = cx<T>{-c2 * xq.i, c2 * xq.r};  // (i c2) xq
     u = csub(cmul(t1, U1), cmul(t2, U2));
@@ -183,8 +189,11 @@ __device__ __forceinline__ void rec_step(const RecEntry& R, int b, const Geo<T>&
         for (int k = 0; k < 3; ++k) {
             cx<T> dzk = k0 ? g.dzp[k] : g.dzm[k];
             cx<T> dxk = k0 ? g.dxm[k] : g.dxp[k];
-            cx<T> a = cadd(cmul(dzk, U1), cmul(zq, dUp[3 * R.s1 + k]));
-            cx<T> bq = cadd(cmul(dxk, U2), cmul(xq, dUp[3 * R.s2 + k]));
+            cx<T> dU1 = dUp[3 * i1 + k], dU2 = dUp[3 * i2 + k];
+            dU1 = cx<T>{ng1 * dU1.r, ng1 * cj1 * dU1.i};
+            dU2 = cx<T>{ng2 * dU2.r, ng2 * cj2 * dU2.i};
+            cx<T> a = cadd(cmul(dzk, U1), cmul(zq, dU1));
+            cx<T> bq = cadd(cmul(dxk, U2), cmul(xq, dU2));
             du[k] = csub(cscale(c1, a), cmul(ic2, bq));
         }
     }
@@ -201,13 +210,13 @@ __device__ __forceinline__ T wave_sum(T v)
 template <int JJ>
 constexpr int level_slots(int j)
 {
-    return ((j + 1) * (j + 1) + 63) / 64;
+    return (mld::hcount(j) + 63) / 64;
 }
 
 template <typename T, int JJ>
 struct Smem {
     static constexpr int NU = mld::ucount(JJ);
-    static constexpr int NL = (JJ + 1) * (JJ + 1);
+    static constexpr int NL = mld::hcount(JJ);  // canonical half of the largest level
     static constexpr int WMAX = wsize_bound(JJ);
     static constexpr int NCG = cg1_size(JJ);
     cx<T> c[NU];
@@ -287,7 +296,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                 wave_sync();
                 const cx<T>* Up = sm.u.a.ub[w][(j - 1) & 1];
                 cx<T>* Un = sm.u.a.ub[w][j & 1];
-                const int K = (j + 1) * (j + 1);
+                const int K = mld::hcount(j);
 #pragma unroll
                 for (int s2 = 0; s2 < level_slots<JJ>(JJ); ++s2) {
                     int e = lane + 64 * s2;
@@ -305,15 +314,23 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
         }
         __syncthreads();
         for (int e = tid; e < NU; e += BS) {
+            int j = 0;
+            while (mld::uoff(j + 1) <= e) ++j;
+            const int le = e - mld::uoff(j);
+            if (le >= mld::hcount(j)) continue;
             cx<T> s = sm.u.a.cpart[0][e];
 #pragma unroll
             for (int q = 1; q < NW; ++q) s = cadd(s, sm.u.a.cpart[q][e]);
             // self term on the diagonal a == b (compute_bispectrum_so4.F90:431-439)
-            int j = 0;
-            while (mld::uoff(j + 1) <= e) ++j;
-            int le = e - mld::uoff(j);
-            if (le / (j + 1) == le % (j + 1)) s.r += (T)1;
+            const int a = le / (j + 1), b = le % (j + 1);
+            if (a == b) s.r += (T)1;
             sm.c[e] = s;
+            // the partner entry: (-1)^(a-b) conj
+            const int lp = (j + 1) * (j + 1) - 1 - le;
+            if (lp != le) {
+                const T sg = ((a - b) & 1) ? (T)-1 : (T)1;
+                sm.c[mld::uoff(j) + lp] = cx<T>{sg * s.r, -sg * s.i};
+            }
         }
         __syncthreads();
 
@@ -350,7 +367,9 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                 }
                 wv = cadd(wv, cscale((T)zt.scale, sacc));
             }
-            sm.W[tb.wofs[j] + e * M + r] = wv;
+            // a canonical entry stands for itself and its partner (same Re(W V))
+            const T pw = (2 * e + 1 == (j + 1) * (j + 1)) ? (T)1 : (T)2;
+            sm.W[tb.wofs[j] + e * M + r] = cscale(pw, wv);
         }
         __syncthreads();
         // B_bi = Re sum_e conj(c_l(e)) Z_A(e), and the level-l row of bi holds kappa * Z_A
@@ -359,7 +378,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
             const int r = tb.arow[bi] - tb.rowbase[l];
             const int M = tb.M[l];
             T bpart = (T)0;
-            for (int e = lane; e < (l + 1) * (l + 1); e += 64) {
+            for (int e = lane; e < mld::hcount(l); e += 64) {
                 cx<T> cl = sm.c[mld::uoff(l) + e];
                 cx<T> wv = sm.W[tb.wofs[l] + e * M + r];
                 bpart += cl.r * wv.r + cl.i * wv.i;
@@ -405,7 +424,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
 #pragma unroll
                     for (int s = 0; s < level_slots<JJ>(JJ); ++s) {
                         int e = lane + 64 * s;
-                        if (e < (j + 1) * (j + 1)) {
+                        if (e < mld::hcount(j)) {
                             int b = e % (j + 1);
                             RecEntry R = sm.rec[mld::uoff(j) + e];
                             rec_step<T, true>(R, b, g, Up, dUp, us[s], dus[s]);
@@ -414,7 +433,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
 #pragma unroll
                     for (int s = 0; s < level_slots<JJ>(JJ); ++s) {
                         int e = lane + 64 * s;
-                        if (e < (j + 1) * (j + 1)) {
+                        if (e < mld::hcount(j)) {
                             Un[e] = us[s];
 #pragma unroll
                             for (int k = 0; k < 3; ++k) {
@@ -432,7 +451,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                 // lane r + M*s sums slice s of row r (two interleaved partial sums
                 // per direction), then the slices are added in slice order.
                 const int M = tb.M[j];
-                const int K = (j + 1) * (j + 1);
+                const int K = mld::hcount(j);
                 const cx<T>* Wj = sm.W + tb.wofs[j];
                 const int NS = (M > 0) ? max(1, min(4, 64 / M)) : 1;
                 const int slice = (K + NS - 1) / NS;

@@ -19,11 +19,25 @@ constexpr int NCH_MAX = 3;
 // a = (m2 + j)/2 (first index of Umm), b = (m1 + j)/2.
 __host__ __device__ constexpr int uoff(int j) { return j * (j + 1) * (2 * j + 1) / 6; }
 __host__ __device__ constexpr int ucount(int jj) { return uoff(jj + 1); }
+// U^j_{-m2,-m1} = (-1)^{a-b} conj(U^j_{m2,m1}) (a, b = row, column of the level):
+// the entry e = a (j+1) + b and its partner (j+1)^2 - 1 - e carry the same
+// information, so only the first hcount(j) entries of a level (its "canonical half",
+// which includes the self-paired centre of an even level) are computed.  The
+// density c, Z, W and dU inherit the same symmetry, and Re(W V) of a pair is equal
+// for both partners.
+__host__ __device__ constexpr int hcount(int j) { return ((j + 1) * (j + 1) + 1) / 2; }
+// recursion source code: (index in the canonical half << 2) | (conjugate << 1) | (negate)
+__host__ __device__ constexpr int rec_code(int s, int jprev)
+{
+    return (s < hcount(jprev)) ? (s << 2)
+                               : (((jprev + 1) * (jprev + 1) - 1 - s) << 2) | 2 | (((s / (jprev + 1)) - (s % (jprev + 1))) & 1);
+}
 
 // Recursion coefficients of one U^j entry (spherical_4d, angular_functions.f90:152-272):
 //   kind b == 0 : u = c1 zp U'[s1] - (i c2) xm U'[s2]
 //   kind b >= 1 : u = c1 zm U'[s1] - (i c2) xp U'[s2]
-// s1, s2 index level j-1 (local index a'*j + b'); a missing term has c = 0.
+// s1, s2 name level-j-1 entries (local index a'*j + b') through rec_code, i.e. as a
+// canonical-half entry, possibly conjugated and negated; a missing term has c = 0.
 struct RecEntry {
     double c1, c2;
     int s1, s2;

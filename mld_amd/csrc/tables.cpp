@@ -86,6 +86,8 @@ void build_tables(int jj, int diag, HostTables& t)
                     if (a > 0) { r.c1 = std::sqrt((double)(2 * a) / (double)(2 * b)); r.s1 = (a - 1) * j + (b - 1); }
                     if (a < j) { r.c2 = std::sqrt((double)(2 * (j - a)) / (double)(2 * b)); r.s2 = a * j + (b - 1); }
                 }
+                r.s1 = rec_code(r.s1, j - 1);
+                r.s2 = rec_code(r.s2, j - 1);
                 t.rec[uoff(j) + a * (j + 1) + b] = r;
             }
     // CG1 tables keyed by (J1, J2, J)
@@ -133,7 +135,7 @@ void build_tables(int jj, int diag, HostTables& t)
             t.rows.push_back(row);
         }
         t.M[j] = (int)t.rows.size() - t.rowbase[j];
-        woff += t.M[j] * (j + 1) * (j + 1);
+        woff += t.M[j] * hcount(j);  // W keeps the canonical half of each level
     }
     t.rowbase[jj + 1] = (int)t.rows.size();
     t.wofs[jj + 1] = woff;
@@ -143,7 +145,7 @@ void build_tables(int jj, int diag, HostTables& t)
     std::vector<std::pair<int, int>> cost;
     for (int j = 0; j <= jj; ++j)
         for (int rw = t.rowbase[j]; rw < t.rowbase[j + 1]; ++rw)
-            for (int e = 0; e < (j + 1) * (j + 1); ++e) {
+            for (int e = 0; e < hcount(j); ++e) {
                 int A = e / (j + 1), B = e % (j + 1), m1 = -j + 2 * A, m2 = -j + 2 * B, c = 0;
                 const WRow& row = t.rows[rw];
                 for (int q = 0; q < row.nz; ++q) {
