@@ -4,6 +4,8 @@
 
 #include "mld_internal.h"
 
+#include <algorithm>
+
 int mld_fail(mld_ctx* ctx, int code, const std::string& msg)
 {
     if (ctx) ctx->err = msg;
@@ -90,7 +92,9 @@ int mld_create(const mld_params* p, mld_ctx** out)
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return MLD_E_DEVICE; }
     mld::build_tables(ctx->jj, p->lbso4_diag, ctx->ht);
     if (ctx->ht.nb > mld::NB_MAX) { fail(MLD_E_UNSUPPORTED, "too many bispectrum components"); mld_destroy(ctx); return rc; }
-    if (!bso4_tables_fit(ctx->jj, (int)ctx->ht.cg1.size(), ctx->ht.wsize)) {
+    int ncg_level = 0;
+    for (int j = 0; j <= ctx->jj; ++j) ncg_level = std::max(ncg_level, ctx->ht.cgbase[j + 1] - ctx->ht.cgbase[j]);
+    if (!bso4_tables_fit(ctx->jj, ncg_level, (int)ctx->ht.cg1.size(), ctx->ht.wsize)) {
         fail(MLD_E_UNSUPPORTED, "table sizes exceed the kernel's LDS layout");
         mld_destroy(ctx);
         return rc;
@@ -105,7 +109,8 @@ int mld_create(const mld_params* p, mld_ctx** out)
     bool ok = up(&ctx->d_rec, ctx->ht.rec.data(), ctx->ht.rec.size() * sizeof(mld::RecEntry)) &&
               up(&ctx->d_rows, ctx->ht.rows.data(), ctx->ht.rows.size() * sizeof(mld::WRow)) &&
               up(&ctx->d_cg1, ctx->ht.cg1.data(), ctx->ht.cg1.size() * sizeof(double)) &&
-              up(&ctx->d_tasks, ctx->ht.tasks.data(), ctx->ht.tasks.size() * sizeof(int));
+              up(&ctx->d_tasks, ctx->ht.tasks.data(), ctx->ht.tasks.size() * sizeof(int)) &&
+              up(&ctx->d_tasks_all, ctx->ht.tasks_all.data(), ctx->ht.tasks_all.size() * sizeof(int));
     std::vector<double> wf((size_t)ctx->nch * ctx->nsp, 1.0);
     for (int s = 0; s < ctx->nsp; ++s) {
         if (ctx->nch >= 2) wf[ctx->nsp + s] = p->weight[s];
@@ -130,7 +135,12 @@ int mld_create(const mld_params* p, mld_ctx** out)
     dt.wsize = ctx->ht.wsize;
     dt.ncg = (int)ctx->ht.cg1.size();
     dt.tasks = (const int*)ctx->d_tasks;
+    dt.tasks_all = (const int*)ctx->d_tasks_all;
     dt.ntask = (int)ctx->ht.tasks.size();
+    for (int j = 0; j <= mld::JJ_MAX + 1; ++j) {
+        dt.tbase[j] = ctx->ht.tbase[j];
+        dt.cgbase[j] = ctx->ht.cgbase[j];
+    }
     for (int b = 0; b < ctx->nb; ++b) {
         dt.l[b] = ctx->ht.l[b];
         dt.l1[b] = ctx->ht.l1[b];
@@ -148,7 +158,7 @@ int mld_destroy(mld_ctx* ctx)
     gram_scratch_free(ctx);
     kept_rows_free(ctx);
     predict_free(ctx);
-    void* ptrs[] = {ctx->d_tasks, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
+    void* ptrs[] = {ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
                     ctx->d_geo, ctx->d_count, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_scan_tmp,
                     ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_zref, ctx->d_zz};
     for (void* q : ptrs)

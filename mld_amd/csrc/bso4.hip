@@ -84,12 +84,35 @@ constexpr int cg1_size(int jj)
                     if (!diag && l < l1) continue;
                     int tr[3][3] = {{l1, l2, l}, {l, l2, l1}, {l1, l, l2}};
                     for (int q = 0; q < 3; ++q) {
-                        if (q == 1 && l1 != l) {}
                         int a = tr[q][0], b = tr[q][1], c = tr[q][2];
                         if (!seen[a][b][c]) { seen[a][b][c] = true; tot += (a + 1) * (c + 1); }
                     }
                 }
         if (tot > best) best = tot;
+    }
+    return best;
+}
+
+// largest CG1 range of one output level (the tables staged together in phase B)
+constexpr int cg_level_max(int jj)
+{
+    int best = 0;
+    for (int diag = 0; diag < 2; ++diag) {
+        bool seen[mld::JJ_MAX + 1][mld::JJ_MAX + 1][mld::JJ_MAX + 1] = {};
+        int tot[mld::JJ_MAX + 1] = {};
+        for (int l1 = 0; l1 <= jj; ++l1)
+            for (int l2 = diag ? l1 : 0; l2 <= l1; ++l2)
+                for (int l = (l1 > l2 ? l1 - l2 : l2 - l1); l <= (jj < l1 + l2 ? jj : l1 + l2); ++l) {
+                    if ((l1 + l2 + l) & 1) continue;
+                    if (!diag && l < l1) continue;
+                    int tr[3][3] = {{l1, l2, l}, {l, l2, l1}, {l1, l, l2}};
+                    for (int q = 0; q < 3; ++q) {
+                        int a = tr[q][0], b = tr[q][1], c = tr[q][2];
+                        if (!seen[a][b][c]) { seen[a][b][c] = true; tot[c] += (a + 1) * (c + 1); }
+                    }
+                }
+        for (int c = 0; c <= jj; ++c)
+            if (tot[c] > best) best = tot[c];
     }
     return best;
 }
@@ -218,11 +241,18 @@ struct Smem {
     static constexpr int NU = mld::ucount(JJ);
     static constexpr int NL = mld::hcount(JJ);  // canonical half of the largest level
     static constexpr int WMAX = wsize_bound(JJ);
-    static constexpr int NCG = cg1_size(JJ);
+    // FP32: every CG1 table is staged at once (fits); FP64: one output level at a time,
+    // which keeps the workgroup under 80 KB of LDS (two workgroups per CU)
+    static constexpr bool CG_BY_LEVEL = sizeof(T) == 8;
+    static constexpr int NCG = CG_BY_LEVEL ? cg_level_max(JJ) : cg1_size(JJ);
+    static constexpr int HU = mld::hoff(JJ + 1);
+    // FP32 accumulates the self slot per wavefront in LDS (registers are the scarcer
+    // resource there); FP64 keeps it in registers (LDS is)
+    static constexpr int NSELF = CG_BY_LEVEL ? 1 : NW * mld::NB_MAX * 3;
+    T selfw[NSELF];
     cx<T> c[NU];
     cx<T> W[WMAX];
-    T selfw[NW][mld::NB_MAX][3];
-    RecEntry rec[NU];
+    RecEntry rec[HU];
     union {
         struct {
             cx<T> cpart[NW][NU];
@@ -272,7 +302,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
         return;
     }
 
-    for (int e = tid; e < NU; e += BS) sm.rec[e] = tb.rec[e];
+    for (int e = tid; e < S::HU; e += BS) sm.rec[e] = tb.rec[e];
     __syncthreads();
 
     for (int ch = 0; ch < nch; ++ch) {
@@ -302,7 +332,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                     int e = lane + 64 * s2;
                     if (e < K) {
                         int b = e % (j + 1);
-                        RecEntry R = sm.rec[mld::uoff(j) + e];
+                        RecEntry R = sm.rec[mld::hoff(j) + e];
                         cx<T> u;
                         rec_step<T, false>(R, b, g, Up, nullptr, u, nullptr);
                         Un[e] = u;
@@ -335,11 +365,9 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
         __syncthreads();
 
         // ---------------- B: Z -> W and B ----------------
-        for (int q = tid; q < tb.ncg; q += BS) sm.u.cg[q] = (T)tb.cg1[q];
-        __syncthreads();
-        // balanced schedule: the k-th task of every thread costs about the same
-        for (int k = 0; k * BS + tid < tb.ntask; ++k) {
-            const int task = tb.tasks[k * BS + tid];
+        // Z -> W for one task (j, row r, canonical entry e); cg points at the staged
+        // copy of cg1[cgs ...]
+        auto w_task = [&](int task, int cgs) {
             const int j = task >> 16, r = (task >> 8) & 255, e = task & 255;
             const int M = tb.M[j];
             const WRow& row = tb.rows[tb.rowbase[j] + r];
@@ -349,7 +377,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
             for (int q = 0; q < row.nz; ++q) {
                 const ZTerm zt = row.z[q];
                 const int J1 = zt.J1, J2 = zt.J2;
-                const T* cg = sm.u.cg + zt.cgoff;
+                const T* cg = sm.u.cg + (zt.cgoff - cgs);
                 const cx<T>* c1 = sm.c + mld::uoff(J1);
                 const cx<T>* c2 = sm.c + mld::uoff(J2);
                 const int ialo = (max(-J1, m1 - J2) + J1) >> 1, iahi = (min(J1, m1 + J2) + J1) >> 1;
@@ -370,8 +398,25 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
             // a canonical entry stands for itself and its partner (same Re(W V))
             const T pw = (2 * e + 1 == (j + 1) * (j + 1)) ? (T)1 : (T)2;
             sm.W[tb.wofs[j] + e * M + r] = cscale(pw, wv);
+        };
+        if constexpr (S::CG_BY_LEVEL) {
+            // level by level: stage the level's CG tables, then its W entries, the k-th
+            // task of every thread costing about the same
+#pragma unroll 1
+            for (int j = 0; j <= JJ; ++j) {
+                const int cg0 = tb.cgbase[j], ncg = tb.cgbase[j + 1] - cg0;
+                for (int q = tid; q < ncg; q += BS) sm.u.cg[q] = (T)tb.cg1[cg0 + q];
+                __syncthreads();
+                const int t0 = tb.tbase[j], nt = tb.tbase[j + 1] - t0;
+                for (int k = 0; k * BS + tid < nt; ++k) w_task(tb.tasks[t0 + k * BS + tid], cg0);
+                __syncthreads();
+            }
+        } else {
+            for (int q = tid; q < tb.cgbase[JJ + 1]; q += BS) sm.u.cg[q] = (T)tb.cg1[q];
+            __syncthreads();
+            for (int k = 0; k * BS + tid < tb.ntask; ++k) w_task(tb.tasks_all[k * BS + tid], 0);
+            __syncthreads();
         }
-        __syncthreads();
         // B_bi = Re sum_e conj(c_l(e)) Z_A(e), and the level-l row of bi holds kappa * Z_A
         for (int bi = w; bi < nb; bi += NW) {
             const int l = tb.l[bi];
@@ -390,8 +435,10 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
         if (!want_grad) continue;
 
         // ---------------- C: gradients ----------------
-        for (int q = tid; q < NW * mld::NB_MAX * 3; q += BS) (&sm.selfw[0][0][0])[q] = (T)0;
-        __syncthreads();
+        T selfacc[3] = {(T)0, (T)0, (T)0};  // -sum over this wavefront's neighbours, q = lane + 64 u
+        T* selfw = &sm.selfw[S::CG_BY_LEVEL ? 0 : w * mld::NB_MAX * 3];
+        if constexpr (!S::CG_BY_LEVEL)
+            for (int q = lane; q < nb * 3; q += 64) selfw[q] = (T)0;
         for (int n = w; n < nn; n += NW) {
             double4 d = pair_d[p0 + n];
             int tj = itype[pair_idx[p0 + n]];
@@ -426,7 +473,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                         int e = lane + 64 * s;
                         if (e < mld::hcount(j)) {
                             int b = e % (j + 1);
-                            RecEntry R = sm.rec[mld::uoff(j) + e];
+                            RecEntry R = sm.rec[mld::hoff(j) + e];
                             rec_step<T, true>(R, b, g, Up, dUp, us[s], dus[s]);
                         }
                     }
@@ -502,21 +549,43 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
             }
             // flush neighbour n: grad slot (base + 1 + n), layout [slot][x][D]
             T* gout = grad + (base + 1 + n) * 3 * (size_t)D + ch * nb;
-            for (int q = lane; q < nb * 3; q += 64) {
-                int x = q / nb, bi = q - x * nb;
-                T v = wn * outw[bi * 3 + x];
-                gout[(size_t)x * D + bi] = v;
-                sm.selfw[w][bi][x] -= v;
+            if constexpr (S::CG_BY_LEVEL) {
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    const int q = lane + 64 * u;
+                    if (q < nb * 3) {
+                        const int x = q / nb, bi = q - x * nb;
+                        const T v = wn * outw[bi * 3 + x];
+                        gout[(size_t)x * D + bi] = v;
+                        selfacc[u] -= v;
+                    }
+                }
+            } else {
+                for (int q = lane; q < nb * 3; q += 64) {
+                    const int x = q / nb, bi = q - x * nb;
+                    const T v = wn * outw[bi * 3 + x];
+                    gout[(size_t)x * D + bi] = v;
+                    selfw[q] -= v;
+                }
             }
             wave_sync();
         }
+        // wavefront partials of the self slot through this wavefront's own outw block
+        {
+            T* outw = &sm.u.g.outw[w][0][0];
+#pragma unroll
+            for (int u = 0; u < 3; ++u)
+                if (lane + 64 * u < nb * 3) outw[lane + 64 * u] = S::CG_BY_LEVEL ? selfacc[u] : selfw[lane + 64 * u];
+        }
         __syncthreads();
+        // self slot = -sum of the neighbour slots (compute_bispectrum_so4.F90:692-695), the
+        // wavefront partials added in wavefront order
         T* gself = grad + base * 3 * (size_t)D + ch * nb;
         for (int q = tid; q < nb * 3; q += BS) {
-            int x = q / nb, bi = q - x * nb;
-            T s = sm.selfw[0][bi][x];
+            const int x = q / nb, bi = q - x * nb;
+            T s = sm.u.g.outw[0][0][q];
 #pragma unroll
-            for (int qq = 1; qq < NW; ++qq) s += sm.selfw[qq][bi][x];
+            for (int qq = 1; qq < NW; ++qq) s += sm.u.g.outw[qq][0][q];
             gself[(size_t)x * D + bi] = s;
         }
         __syncthreads();
@@ -559,10 +628,10 @@ int launch_prec(mld_ctx* ctx, int want_grad, void* energy, void* grad)
 
 }  // namespace
 
-bool bso4_tables_fit(int jj, int ncg, int wsize)
+bool bso4_tables_fit(int jj, int ncg_level, int ncg, int wsize)
 {
     if (jj < 1 || jj > mld::JJ_MAX) return false;
-    return ncg <= cg1_size(jj) && wsize <= wsize_bound(jj);
+    return ncg_level <= cg_level_max(jj) && ncg <= cg1_size(jj) && wsize <= wsize_bound(jj);
 }
 
 int launch_bso4(mld_ctx* ctx, int want_grad, void* energy, void* grad)

@@ -26,6 +26,9 @@ __host__ __device__ constexpr int ucount(int jj) { return uoff(jj + 1); }
 // density c, Z, W and dU inherit the same symmetry, and Re(W V) of a pair is equal
 // for both partners.
 __host__ __device__ constexpr int hcount(int j) { return ((j + 1) * (j + 1) + 1) / 2; }
+// offset of level j in a table that keeps the canonical halves only
+// = sum_{k=1..j} ceil(k^2 / 2) = (sum k^2 + number of odd k) / 2
+__host__ __device__ constexpr int hoff(int j) { return (j * (j + 1) * (2 * j + 1) / 6 + (j + 1) / 2) / 2; }
 // recursion source code: (index in the canonical half << 2) | (conjugate << 1) | (negate)
 __host__ __device__ constexpr int rec_code(int s, int jprev)
 {
@@ -61,7 +64,7 @@ struct WRow {
 // Device-side table bundle (all pointers are device memory).
 struct DevTables {
     int jj, nb, nu;
-    const RecEntry* rec;   // [nu]
+    const RecEntry* rec;   // [hoff(jj + 1)] canonical entries, level j at hoff(j)
     const WRow* rows;      // [sum_j M_j], level j rows start at rowbase[j]
     const double* cg1;     // CG1 tables
     int rowbase[JJ_MAX + 2];
@@ -69,8 +72,11 @@ struct DevTables {
     int M[JJ_MAX + 1];
     int wsize;             // total complex entries of W
     int ncg;               // entries of the CG1 tables
-    const int* tasks;      // phase-B schedule: (j << 16) | (row-in-level << 8) | e, costliest first
+    const int* tasks;      // phase-B schedule: (j << 16) | (row-in-level << 8) | e, by level, costliest first
     int ntask;
+    int tbase[JJ_MAX + 2];  // tasks of level j: [tbase[j], tbase[j+1])
+    const int* tasks_all;  // the same tasks in one list, costliest first (all CG tables staged)
+    int cgbase[JJ_MAX + 2]; // CG1 tables of level j (output J = j) are cg1[cgbase[j] .. cgbase[j+1])
     int arow[NB_MAX];      // row of component bi at its own level l
     double kappa[NB_MAX];  // W_l row of bi = kappa * Z_A
     int l[NB_MAX], l1[NB_MAX], l2[NB_MAX];
@@ -82,7 +88,9 @@ struct HostTables {
     std::vector<RecEntry> rec;
     std::vector<WRow> rows;
     std::vector<double> cg1;
-    std::vector<int> tasks;
+    std::vector<int> tasks, tasks_all;
+    int tbase[JJ_MAX + 2] = {0};
+    int cgbase[JJ_MAX + 2] = {0};
     int arow[NB_MAX] = {0};
     double kappa[NB_MAX] = {0};
     int rowbase[JJ_MAX + 2] = {0};
@@ -119,6 +127,7 @@ struct mld_ctx {
     void* d_rows = nullptr;
     void* d_cg1 = nullptr;
     void* d_tasks = nullptr;
+    void* d_tasks_all = nullptr;
     double* d_wfac = nullptr;    // [nch * nsp] channel weights (channel 0 = 1)
     double* d_rin = nullptr;     // [nsp * nsp] inner cutoff or null
     // batch state
@@ -172,7 +181,8 @@ int mld_fail(mld_ctx* ctx, int code, const std::string& msg);
 // launchers implemented in the .hip units
 int launch_neighbours_pos(mld_ctx* ctx, const double* pos);
 int launch_bso4(mld_ctx* ctx, int want_grad, void* energy, void* grad);
-bool bso4_tables_fit(int jj, int ncg, int wsize);
+// ncg_level: largest CG1 range of one output level (cgbase[j+1] - cgbase[j]); ncg: all tables
+bool bso4_tables_fit(int jj, int ncg_level, int ncg, int wsize);
 int launch_to_reference_layout(mld_ctx* ctx, int nat, int nn_max, const void* grad, double* force_dev,
                                int* kind_dev, int* nneigh_dev);
 int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, int mixed);

@@ -72,11 +72,12 @@ void build_tables(int jj, int diag, HostTables& t)
             }
     t.nb = (int)t.l.size();
     // recursion
-    t.rec.resize(t.nu);
-    t.rec[0] = RecEntry{0.0, 0.0, 0, 0};
+    // canonical halves only (level j at hoff(j), entry e = a (j+1) + b < hcount(j))
+    t.rec.assign(hoff(jj + 1), RecEntry{0.0, 0.0, 0, 0});
     for (int j = 1; j <= jj; ++j)
         for (int a = 0; a <= j; ++a)
             for (int b = 0; b <= j; ++b) {
+                if (a * (j + 1) + b >= hcount(j)) continue;
                 RecEntry r{0.0, 0.0, 0, 0};
                 if (b == 0) {
                     // m1 = -j: from column b' = 0 of level j-1
@@ -88,7 +89,7 @@ void build_tables(int jj, int diag, HostTables& t)
                 }
                 r.s1 = rec_code(r.s1, j - 1);
                 r.s2 = rec_code(r.s2, j - 1);
-                t.rec[uoff(j) + a * (j + 1) + b] = r;
+                t.rec[hoff(j) + a * (j + 1) + b] = r;
             }
     // CG1 tables keyed by (J1, J2, J)
     std::map<std::tuple<int, int, int>, int> cgmap;
@@ -110,6 +111,7 @@ void build_tables(int jj, int diag, HostTables& t)
     for (int j = 0; j <= jj; ++j) {
         t.rowbase[j] = (int)t.rows.size();
         t.wofs[j] = woff;
+        t.cgbase[j] = (int)t.cg1.size();  // tables with output J = j are created in this pass only
         for (int bi = 0; bi < t.nb; ++bi) {
             int l = t.l[bi], l1 = t.l1[bi], l2 = t.l2[bi];
             if (j != l && j != l1 && j != l2) continue;
@@ -139,11 +141,17 @@ void build_tables(int jj, int diag, HostTables& t)
     }
     t.rowbase[jj + 1] = (int)t.rows.size();
     t.wofs[jj + 1] = woff;
+    t.cgbase[jj + 1] = (int)t.cg1.size();
+    for (int j = jj + 2; j <= JJ_MAX + 1; ++j) t.cgbase[j] = t.cgbase[jj + 1];
     t.wsize = woff;
-    // phase-B schedule: one task per W entry, largest first, dealt round-robin to the
+    // phase-B schedule: one task per W entry, level by level (the CG tables of one level
+    // are staged at a time), largest first within a level, dealt round-robin to the
     // threads of a workgroup so that the k-th tasks of all threads cost about the same
-    std::vector<std::pair<int, int>> cost;
-    for (int j = 0; j <= jj; ++j)
+    t.tasks.clear();
+    std::vector<std::pair<int, int>> cost_all;
+    for (int j = 0; j <= jj; ++j) {
+        std::vector<std::pair<int, int>> cost;
+        t.tbase[j] = (int)t.tasks.size();
         for (int rw = t.rowbase[j]; rw < t.rowbase[j + 1]; ++rw)
             for (int e = 0; e < hcount(j); ++e) {
                 int A = e / (j + 1), B = e % (j + 1), m1 = -j + 2 * A, m2 = -j + 2 * B, c = 0;
@@ -156,9 +164,14 @@ void build_tables(int jj, int diag, HostTables& t)
                 }
                 cost.push_back({-c, (j << 16) | ((rw - t.rowbase[j]) << 8) | e});
             }
-    std::stable_sort(cost.begin(), cost.end());
-    t.tasks.clear();
-    for (auto& pr : cost) t.tasks.push_back(pr.second);
+        cost_all.insert(cost_all.end(), cost.begin(), cost.end());
+        std::stable_sort(cost.begin(), cost.end());
+        for (auto& pr : cost) t.tasks.push_back(pr.second);
+    }
+    std::stable_sort(cost_all.begin(), cost_all.end());
+    t.tasks_all.clear();
+    for (auto& pr : cost_all) t.tasks_all.push_back(pr.second);
+    for (int j = jj + 1; j <= JJ_MAX + 1; ++j) t.tbase[j] = (int)t.tasks.size();
 }
 
 void struct_geometry(const double* cell, double rcut, StructGeo& g)
