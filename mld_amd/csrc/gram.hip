@@ -266,15 +266,31 @@ __global__ void __launch_bounds__(256) syrk_f64_kernel(int M, int ldA, int ntile
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = d4{0, 0, 0, 0};
+    // register prefetch: the next KB rows are loaded while the current ones are multiplied
+    constexpr int PER = KB * TILE / 256;  // 4 elements per thread and panel
+    double pi_[PER], pj_[PER];
+    auto fetch = [&](int rb) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            const int rr = q / TILE, cc = q - rr * TILE;
+            const int r = rb + rr;
+            pi_[u] = (r < r1) ? A[(size_t)r * ldA + ti * TILE + cc] : 0.0;
+            pj_[u] = (r < r1) ? A[(size_t)r * ldA + tj * TILE + cc] : 0.0;
+        }
+    };
+    if (r0 < r1) fetch(r0);
     for (int rb = r0; rb < r1; rb += KB) {
         __syncthreads();
-        for (int q = tid; q < KB * TILE; q += 256) {
-            int rr = q / TILE, cc = q - rr * TILE;
-            int r = rb + rr;
-            Ai[rr][cc] = (r < r1) ? A[(size_t)r * ldA + ti * TILE + cc] : 0.0;
-            Aj[rr][cc] = (r < r1) ? A[(size_t)r * ldA + tj * TILE + cc] : 0.0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            const int rr = q / TILE, cc = q - rr * TILE;
+            Ai[rr][cc] = pi_[u];
+            Aj[rr][cc] = pj_[u];
         }
         __syncthreads();
+        if (rb + KB < r1) fetch(rb + KB);
 #pragma unroll
         for (int k0 = 0; k0 < KB; k0 += 4) {
             double af[2], bf[2];
