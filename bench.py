@@ -15,6 +15,12 @@ Workloads (``--workload``):
   desc  2,000 seeded structures, N ~ U{16..432}, bcc / fcc / liquid-like random
       packing at 0.085 A^-3, sigma ~ U[0.02, 0.2] A; one step = neighbour lists +
       B + dB/dr for the whole batch.
+  krr  Polynomial-kernel ridge regression: 313 bcc HEA cells (4x4x4, a = 3.1 A, five species
+      at random, sigma = 0.05 A) = 40,064 environments, SO(4) bispectrum (D = 55, r_cut =
+      5.0 A), 4,000 reference environments (a seeded random subset stands in for CUR
+      selection), normalised kernel_po with sigma = 1, power 2; rows [N, B, k] (4,056
+      columns), weights 1.0 / 0.1, ridge 1e-8, species-dependent Morse targets; forms fp64
+      and mixed (FP32 descriptors and kernel features, FP64 Gram and solve).
   serve  Poisson arrivals at 200 requests/s of 5-species bcc HEA structures (a = 3.1 A,
       random occupancy, sigma = 0.05 A), atom counts lognormal (median 128, clipped to
       8..2,000); each request asks for the energy, forces and per-atom Mahalanobis
@@ -339,6 +345,123 @@ def main_fit(args):
     print(json.dumps(out), flush=True)
 
 
+# ----------------------------------------------------------------------------- kernel ridge regression
+def main_krr(args):
+    import torch
+    import mld_amd as M
+    from mld_amd import fit, synth
+
+    rng = np.random.default_rng(SEED + 4)
+    n = max(2, int(round(313 * args.scale)))
+    structs = [synth.hea_bcc(rng, 4, a=3.1, nspecies=5, sigma=0.05) for _ in range(n)]
+    Dm, am, r0m = synth.morse_mixture_params(5)
+    nk_target, chunk = 4000, 64
+    # reference environments: FP64 descriptors of a seeded random subset
+    c64 = M.Context(j_max=4.0, r_cut=5.0, nspecies=5)
+    allB = []
+    for s0 in range(0, n, chunk):
+        part = structs[s0:s0 + chunk]
+        b = M.Batch(c64, [p for p, _, _ in part], [c for _, c, _ in part], [t for _, _, t in part])
+        b.neighbours()
+        e, _ = b.alloc_outputs(False)
+        b.bso4(e, None)
+        allB.append(e.cpu().numpy())
+    c64.close()
+    allB = np.concatenate(allB)
+    nk = min(nk_target, allB.shape[0])
+    zref = allB[rng.choice(allB.shape[0], nk, replace=False)]
+    length = float(np.sqrt(np.mean((allB ** 2).sum(1)) / 2.0))
+    # kernel columns of near-identical bcc environments are close to collinear: the ridge is
+    # relative, 1e-7 of the mean diagonal of the normal matrix (fixed on the first fit)
+    rel_lam = 1e-7
+    lam_box = []
+
+    def run(prec):
+        ctx = M.Context(j_max=4.0, r_cut=5.0, nspecies=5, precision=prec)
+        stream = torch.cuda.ExternalStream(ctx.stream())
+        fit.kernel_po_set(ctx, zref, 1.0, length, 2.0)
+        batches = []
+        for s0 in range(0, n, chunk):
+            part = structs[s0:s0 + chunk]
+            b = M.Batch(ctx, [p for p, _, _ in part], [c for _, c, _ in part], [t for _, _, t in part])
+            b.neighbours()
+            po, pi, pd = b.pair_tensors()
+            E, F = synth.morse_mixture_from_pairs(po, pi, pd, b.atom_off, b.itype, Dm, am, r0m, rc=5.0)
+            batches.append((b, E.cpu().numpy(), F.reshape(-1).contiguous()))
+        times, kpo_ms = [], []
+        w = diag = None
+        for it in range(args.warmup + args.steps):
+            ctx.synchronize()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            gf = fit.GramFit(ctx, order=3, nk=nk)
+            ev = []
+            for b, E, F in batches:
+                b.neighbours()
+                e, g = b.alloc_outputs(True)
+                b.bso4(e, g, sync=False)
+                ek, fk = fit.kernel_po_alloc(ctx, b)
+                e0, e1 = event_pair()
+                e0.record(stream)
+                fit.kernel_po(ctx, ek, fk)
+                e1.record(stream)
+                ev.append((e0, e1))
+                gf.add_batch(b, E, F, 1.0, 0.1)
+            if not lam_box:
+                G, _ = gf.export()
+                lam_box.append(rel_lam * float(np.trace(G)) / G.shape[0])
+            w, diag = gf.solve(lam_box[0])
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            if it >= args.warmup:
+                times.append(t1 - t0)
+                kpo_ms.append(sum(a.elapsed_time(c) for a, c in ev))
+        # training RMSE on the first batch through the device prediction path
+        fit.model_set(ctx, 3, w)
+        b, E, F = batches[0]
+        b.neighbours()
+        e, g = b.alloc_outputs(True)
+        b.bso4(e, g)
+        ek, fk = fit.kernel_po_alloc(ctx, b)
+        fit.kernel_po(ctx, ek, fk)
+        Ep, Fp, _ = fit.predict(ctx, b)
+        nat = torch.from_numpy(np.diff(b.atom_off).astype(np.float64)).to(Ep.device)
+        rmse = {"energy_meV_per_atom": 1e3 * float((((Ep - torch.from_numpy(E).to(Ep.device)) / nat) ** 2).mean().sqrt()),
+                "force_meV_per_A": 1e3 * float(((Fp.reshape(-1) - F) ** 2).mean().sqrt())}
+        natoms = sum(bb.natoms for bb, _, _ in batches)
+        npairs = sum(bb.npairs for bb, _, _ in batches)
+        out = dict(s=float(np.mean(times)), kpo_ms=float(np.mean(kpo_ms)), w=w, diag=diag, rmse=rmse,
+                   natoms=natoms, npairs=npairs)
+        del batches
+        ctx.close()
+        torch.cuda.empty_cache()
+        return out
+
+    r64 = run(0)
+    r32 = run(1)
+    natoms, npairs = r64["natoms"], r64["npairs"]
+    kflops = 2.0 * 55 * nk * (natoms + 3 * npairs)   # X Z^T and (dB/dr) Z^T
+    ach = kflops / (r32["kpo_ms"] * 1e-3) / 1e12   # all kernel_po launches of the step (f32 MFMA)
+    out = {
+        "metric": METRIC, "value": natoms / r32["s"], "unit": "atom-envs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * r32["s"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 desc + kernel, f64 Gram/solve", "data": f"synthetic (seed {SEED})",
+        "config": {"workload": f"krr: {n} bcc HEA 4x4x4 a=3.1 A 5 species, {natoms} environments, {nk} reference "
+                               f"points, kernel_po p=2, {1 + 55 + nk} columns, ridge {rel_lam} x mean diag "
+                               f"= {lam_box[0]:.3e}",
+                   "natoms": natoms, "npairs": npairs, "parallelism": "single GPU"},
+        "fp64": {"value": natoms / r64["s"], "ms_per_step": 1e3 * r64["s"], "kernel_po_ms_per_step": r64["kpo_ms"],
+                 "solve": r64["diag"], "rmse": r64["rmse"]},
+        "mixed": {"kernel_po_ms_per_step": r32["kpo_ms"], "solve": r32["diag"], "rmse": r32["rmse"],
+                  "coef_norm_rel_vs_fp64": float(np.linalg.norm(r32["w"] - r64["w"]) / np.linalg.norm(r64["w"]))},
+        "mixed_speedup_vs_fp64": r64["s"] / r32["s"],
+        "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
+                     "traffic": None, "kernel": "kpo_mfma_kernel<float> (v_mfma_f32_16x16x4f32)",
+                     "algorithmic_flops_per_step": kflops},
+    }
+    print(json.dumps(out), flush=True)
+
+
 # ----------------------------------------------------------------------------- serving
 def hea_request_pool(rng, n: int):
     from mld_amd import synth
@@ -453,7 +576,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="fit-quadratic", choices=["fit-quadratic", "fit-linear", "desc", "serve"])
+    ap.add_argument("--workload", default="fit-quadratic", choices=["fit-quadratic", "fit-linear", "desc", "serve", "krr"])
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of arrivals")
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0, help="serve: HBM admission budget")
@@ -480,6 +603,8 @@ def main():
         main_desc(args)
     elif args.workload == "serve":
         main_serve(args)
+    elif args.workload == "krr":
+        main_krr(args)
     else:
         main_fit(args)
 

@@ -105,7 +105,9 @@ int mld_gram_add_rows(mld_ctx* ctx, int D, int mc, const double* Ac, const doubl
 int mld_gram_add_rows_dev(mld_ctx* ctx, int D, int mc, const double* Ac_dev, const double* y_dev,
                           const double* w_dev);
 /* rows from the last batch descriptors: linear (order 1) or quadratic (order 2)
- * energy + force rows (mld_energy.F90:91-266, mld_forces.F90:222-500) with
+ * energy + force rows (mld_energy.F90:91-266, mld_forces.F90:222-500), or the rows of
+ * a kernel model (order 3: [N, B, k] from the last mld_kernel_po_dev features with
+ * gradients, kernel_pack_*_descriptor, mld_energy.F90:122-144, mld_forces.F90:268-320), with
  * per-structure targets E (nstruct, host) and forces F_dev (3*natoms, device),
  * weights w_e, w_f.  mixed = 1 runs the Gram in split-BF16 MFMA with FP32
  * accumulation (FP64 across tiles), 0 in FP64. */
@@ -135,7 +137,8 @@ int mld_kernel_po_dev(mld_ctx* ctx, void* ek_dev, void* fk_dev);
 /* ---- prediction with a fitted model resident on the device ----
  * mld_model_set: w_params (product_w_params_Amat, mld.F90:40-86): order 1 has
  * 1 + D coefficients [N, B]; order 2 either the unique layout 1 + D + D(D+1)/2
- * or the reference's full layout 1 + D + D^2 (mld.F90:771-780).  ref_energy
+ * or the reference's full layout 1 + D + D^2 (mld.F90:771-780); order 3 (kernel
+ * model) 1 + D + nk, predictions then read the last mld_kernel_po_dev features.  ref_energy
  * (nspecies, may be NULL) is ref_energy_per_element, added per atom
  * (md_mld_compute_energy, mld.F90:1610-1660).
  * mld_mahalanobis_set: mean (D) and Sigma^-1 (D, D) column-major of

@@ -165,6 +165,7 @@ class Context:
         self.L = L
         self.precision = precision
         self.nspecies = nspecies
+        self.nk = 0
         d = C.c_int()
         L.mld_descriptor_dim(h, C.byref(d))
         self.D = d.value

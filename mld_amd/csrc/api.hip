@@ -160,7 +160,7 @@ int mld_destroy(mld_ctx* ctx)
     predict_free(ctx);
     void* ptrs[] = {ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
                     ctx->d_geo, ctx->d_count, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_scan_tmp,
-                    ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_zref, ctx->d_zz};
+                    ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_zref, ctx->d_zz, ctx->d_zref32, ctx->d_kiz};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -282,6 +282,7 @@ int mld_batch_bso4_dev(mld_ctx* ctx, int want_grad, void* energy_dev, void* grad
     if (rc) return rc;
     ctx->last_energy = energy_dev;
     ctx->last_grad = want_grad ? grad_dev : nullptr;
+    ctx->last_ek = ctx->last_fk = nullptr;  // kernel features belong to the previous descriptors
     return MLD_OK;
 }
 
@@ -389,6 +390,7 @@ int mld_bso4(mld_ctx* ctx, int nat, const double* pos, const double* cell, const
     }
     ctx->d_itype = nullptr;
     ctx->last_energy = ctx->last_grad = nullptr;
+    ctx->last_ek = ctx->last_fk = nullptr;
     return MLD_OK;
 }
 

@@ -235,6 +235,84 @@ __global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int l
     if (tid < 3) y[row0 + tid] = F[3 * (size_t)k + tid] * swf;
 }
 
+// Rows of a kernel model (kernel_pack_energy_descriptor / kernel_pack_force_descriptor,
+// mld_energy.F90:122-144, mld_forces.F90:268-320): columns [N, B (D), k (nk)]; energy row
+// of structure s = [N_s, sum_a B_a, sum_a k_a]; force rows of atom k = [0, -sum_slots dB,
+// -sum_slots dk] over the same slots as the linear rows.  One block per structure /
+// atom; consecutive threads take consecutive columns of one feature row.
+template <typename T>
+__global__ void __launch_bounds__(256) rows_energy_krr_kernel(int D, int nk, int ldA, const int* __restrict__ atom_off,
+                                                              const T* __restrict__ energy, const T* __restrict__ ek,
+                                                              const double* __restrict__ E, double swe,
+                                                              double* __restrict__ A, double* __restrict__ y)
+{
+    const int s = blockIdx.x;
+    const int a0 = atom_off[s], a1 = atom_off[s + 1];
+    const size_t row = (size_t)s + 3 * (size_t)a0;
+    double* Ar = A + row * ldA;
+    for (int c = threadIdx.x; c < ldA; c += blockDim.x) {
+        double v = 0.0;
+        if (c == 0) v = (double)(a1 - a0);
+        else if (c <= D) for (int a = a0; a < a1; ++a) v += (double)energy[(size_t)a * D + c - 1];
+        else if (c <= D + nk) for (int a = a0; a < a1; ++a) v += (double)ek[(size_t)a * nk + c - 1 - D];
+        Ar[c] = v * swe;
+    }
+    if (threadIdx.x == 0) y[row] = E[s] * swe;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) rows_force_krr_kernel(int D, int nk, int ldA, const int* __restrict__ atom_struct,
+                                                             const int* __restrict__ atom_off, const int* __restrict__ pair_off,
+                                                             const int* __restrict__ rev_off, const int* __restrict__ rev_pair,
+                                                             const int* __restrict__ pair_src, const T* __restrict__ grad,
+                                                             const T* __restrict__ fk, const double* __restrict__ F, double swf,
+                                                             double* __restrict__ A, double* __restrict__ y)
+{
+    constexpr int CH = 256;
+    __shared__ long long slots[CH];
+    const int k = blockIdx.x;
+    const int s = atom_struct[k];
+    const size_t row0 = (size_t)s + 3 * (size_t)atom_off[s] + 1 + 3 * (size_t)(k - atom_off[s]);
+    const int r0 = rev_off[k], r1 = rev_off[k + 1];
+    const int nslot = 1 + (r1 - r0);
+    double* Ar = A + row0 * ldA;
+    auto stage = [&](int sc, int ns) {
+        if (threadIdx.x < ns) {
+            const int idx = sc + threadIdx.x;
+            if (idx == 0) slots[threadIdx.x] = (long long)pair_off[k] + k;
+            else {
+                const int p = rev_pair[r0 + idx - 1];
+                slots[threadIdx.x] = (long long)p + pair_src[p] + 1;
+            }
+        }
+    };
+    const bool once = nslot <= CH;  // the usual case: stage the slot list once
+    if (once) stage(0, nslot);
+    __syncthreads();
+    // column chunks of 256 per pass, all three directions
+    for (int x = 0; x < 3; ++x)
+        for (int c0 = 0; c0 < ldA; c0 += 256) {
+            const int c = c0 + threadIdx.x;
+            double v = 0.0;
+            for (int sc = 0; sc < nslot; sc += CH) {
+                const int ns = min(CH, nslot - sc);
+                if (!once) {
+                    __syncthreads();
+                    stage(sc, ns);
+                    __syncthreads();
+                }
+                if (c >= 1 && c <= D + nk) {
+                    for (int q = 0; q < ns; ++q) {
+                        const size_t srow = (size_t)slots[q] * 3 + x;
+                        v += (c <= D) ? (double)grad[srow * D + c - 1] : (double)fk[srow * nk + c - 1 - D];
+                    }
+                }
+            }
+            if (c < ldA) Ar[(size_t)x * ldA + c] = -v * swf;  // column 0 and padding are zero
+        }
+    if (threadIdx.x < 3) y[row0 + threadIdx.x] = F[3 * (size_t)k + threadIdx.x] * swf;
+}
+
 __global__ void scale_rows_kernel(int M, int Dc, int ldA, const double* __restrict__ Ain, const double* __restrict__ yin,
                                   const double* __restrict__ w, double* __restrict__ A, double* __restrict__ y)
 {
@@ -590,13 +668,39 @@ int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_de
     double* y = (double*)gs.y;
     double swe = std::sqrt(w_e), swf = std::sqrt(w_f);
     bool f32 = ctx->p.precision == MLD_PREC_FP32;
-    if (nstruct) {
+    if (nstruct && order != 3) {
         if (f32)
             hipLaunchKernelGGL(rows_energy_kernel<float>, dim3(nstruct), dim3(256), 0, st, order, D, ldA, ctx->d_atom_off,
                                (const float*)ctx->last_energy, E_dev, swe, A, y);
         else
             hipLaunchKernelGGL(rows_energy_kernel<double>, dim3(nstruct), dim3(256), 0, st, order, D, ldA, ctx->d_atom_off,
                                (const double*)ctx->last_energy, E_dev, swe, A, y);
+    }
+    if (order == 3) {
+        const int nk = ctx->nk;
+        if (nstruct) {
+            if (f32)
+                hipLaunchKernelGGL(rows_energy_krr_kernel<float>, dim3(nstruct), dim3(256), 0, st, D, nk, ldA, ctx->d_atom_off,
+                                   (const float*)ctx->last_energy, (const float*)ctx->last_ek, E_dev, swe, A, y);
+            else
+                hipLaunchKernelGGL(rows_energy_krr_kernel<double>, dim3(nstruct), dim3(256), 0, st, D, nk, ldA, ctx->d_atom_off,
+                                   (const double*)ctx->last_energy, (const double*)ctx->last_ek, E_dev, swe, A, y);
+        }
+        if (natoms) {
+            if (f32)
+                hipLaunchKernelGGL(rows_force_krr_kernel<float>, dim3(natoms), dim3(256), 0, st, D, nk, ldA, ctx->d_atom_struct,
+                                   ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src, (const float*)ctx->last_grad,
+                                   (const float*)ctx->last_fk, F_dev, swf, A, y);
+            else
+                hipLaunchKernelGGL(rows_force_krr_kernel<double>, dim3(natoms), dim3(256), 0, st, D, nk, ldA, ctx->d_atom_struct,
+                                   ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src, (const double*)ctx->last_grad,
+                                   (const double*)ctx->last_fk, F_dev, swf, A, y);
+        }
+        MLD_HIP(ctx, hipGetLastError());
+        *A_out = A;
+        *y_out = y;
+        *M_out = M;
+        return MLD_OK;
     }
     if (natoms) {
         const size_t fsm = rows_force_smem(D, ldA);
@@ -701,11 +805,14 @@ int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F
 {
     if (!ctx) return MLD_E_ARG;
     if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
-    if (order != 1 && order != 2) return mld_fail(ctx, MLD_E_ARG, "order must be 1 or 2");
+    if (order < 1 || order > 3) return mld_fail(ctx, MLD_E_ARG, "order must be 1, 2 or 3 (kernel)");
     const int D = ctx->D;
-    int Dc = 1 + D + (order == 2 ? D * (D + 1) / 2 : 0);
-    if (Dc != ctx->gD) return mld_fail(ctx, MLD_E_ARG, "mld_gram_begin D must be 1 + D (+ D(D+1)/2 for order 2)");
-    if (Dc > 2048) return mld_fail(ctx, MLD_E_UNSUPPORTED, "row packing is built for at most 2048 columns");
+    if (order == 3 && (!ctx->last_ek || !ctx->last_fk))
+        return mld_fail(ctx, MLD_E_STATE, "order 3 needs mld_kernel_po_dev features with gradients of this batch");
+    int Dc = 1 + D + (order == 2 ? D * (D + 1) / 2 : 0) + (order == 3 ? ctx->nk : 0);
+    if (Dc != ctx->gD)
+        return mld_fail(ctx, MLD_E_ARG, "mld_gram_begin D must be 1 + D (+ D(D+1)/2 for order 2, + nk for order 3)");
+    if (order == 2 && Dc > 2048) return mld_fail(ctx, MLD_E_UNSUPPORTED, "quadratic rows are built for at most 2048 columns");
     ctx->gram_order = order;
     if (!ctx->last_energy || !ctx->last_grad) return mld_fail(ctx, MLD_E_STATE, "mld_batch_bso4_dev with gradients first");
     if (!E || (ctx->natoms && !F_dev)) return mld_fail(ctx, MLD_E_ARG, "targets missing");

@@ -16,76 +16,118 @@ namespace {
 
 constexpr int T = 64;
 
-// C[r][c] = sum_d Arows[r][d] * Z[c][d] for r < nrow, c < nk; epilogue mode:
-//  mode 0: energies  -> out[r][c] = k(x_r, z_c), and dot[r][c] = x_r.z_c
-//  mode 1: gradients -> rows are (slot, x) of neighbour slots; out written at the compact slot index
+// Normalised-kernel factors: with u = s2 + t / l2,
+//   k(x, z)            = u(x.z)^p * ix * iz,       ix = u(x.x)^(-p/2), iz = u(z.z)^(-p/2)
+//   dk/dr (row g, dir) = (p u(x.z)^(p-1) / l2 * (g.z) - u(x.z)^p * (g.x) * fx) * ix * iz,
+//                        fx = p / (l2 u(x.x))
+// which is inner_kernel_po's expression (compute_kernel.F90:91-150) regrouped.
+// ipw >= 1: integer power evaluated by multiplication; 0: pow().
+__device__ inline double kpow(double u, double pw, int ipw)
+{
+    if (ipw > 0) {
+        double r = u;
+        for (int q = 1; q < ipw; ++q) r *= u;
+        return r;
+    }
+    return pow(u, pw);
+}
+
+template <typename TD> struct MfmaOps;
+template <> struct MfmaOps<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    __device__ static v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    // 16x16 f32 output: register v of lane l holds C[4 (l / 16) + v][l % 16]
+    __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <> struct MfmaOps<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    __device__ static v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    // 16x16 f64 output: register v of lane l holds C[l / 16 + 4 v][l % 16]
+    __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+// C[r][c] = sum_d Arows[r][d] Z[c][d] on MFMA (FP32 form: f32 16x16x4, FP64 form: f64 16x16x4),
+// 64 x 64 tiles, 4 wavefronts x (16 rows x 64 columns), K staged in LDS in chunks of 64.
+// Blocks are numbered column-tile fastest.  Epilogue:
+//  mode 0: energies  -> out[r][c] = k(x_r, z_c), dot_out[r][c] = x_r.z_c
+//  mode 1: gradients -> row r = (neighbour slot, direction), written at row_slot[r]
 template <typename TD>
-__global__ void __launch_bounds__(256) kpo_gemm_kernel(int mode, int nrow, int nk, int D, const TD* __restrict__ Arows,
-                                                       const double* __restrict__ Z, const double* __restrict__ zz,
-                                                       const double* __restrict__ xx_atom, const int* __restrict__ row_atom,
-                                                       const long long* __restrict__ row_slot, const double* __restrict__ gx,
-                                                       const double* __restrict__ dot, double s2, double l2, double pw,
+__global__ void __launch_bounds__(256) kpo_mfma_kernel(int mode, int nrow, int nk, int D, int nct, const TD* __restrict__ Arows,
+                                                       const TD* __restrict__ Z, const double* __restrict__ iz,
+                                                       const double* __restrict__ ix, const double* __restrict__ fx,
+                                                       const int* __restrict__ row_atom, const long long* __restrict__ row_slot,
+                                                       const double* __restrict__ gx, const double* __restrict__ dot,
+                                                       double s2, double l2, double pw, int ipw,
                                                        double* __restrict__ dot_out, TD* __restrict__ out)
 {
-    constexpr int KC = 32;
-    __shared__ double As[T][KC + 1];
-    __shared__ double Zs[T][KC + 1];
-    const int r0 = blockIdx.y * T, c0 = blockIdx.x * T;
-    const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
-    double acc[4][4] = {};
+    using O = MfmaOps<TD>;
+    constexpr int KC = 64;
+    __shared__ TD As[T][KC + 1];
+    __shared__ TD Zs[T][KC + 1];
+    const int ct = blockIdx.x % nct, rt = blockIdx.x / nct;
+    const int r0 = rt * T, c0 = ct * T;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    typename O::v4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = typename O::v4{0, 0, 0, 0};
     for (int d0 = 0; d0 < D; d0 += KC) {
         const int kc = min(KC, D - d0);
         __syncthreads();
         for (int q = threadIdx.x; q < T * KC; q += blockDim.x) {
-            int i = q / KC, d = q - i * KC;
-            bool okd = d < kc;
-            As[i][d] = (okd && r0 + i < nrow) ? (double)Arows[(size_t)(r0 + i) * D + d0 + d] : 0.0;
-            Zs[i][d] = (okd && c0 + i < nk) ? Z[(size_t)(c0 + i) * D + d0 + d] : 0.0;
+            const int i = q / KC, d = q - i * KC;
+            const bool okd = d < kc;
+            As[i][d] = (okd && r0 + i < nrow) ? Arows[(size_t)(r0 + i) * D + d0 + d] : (TD)0;
+            Zs[i][d] = (okd && c0 + i < nk) ? Z[(size_t)(c0 + i) * D + d0 + d] : (TD)0;
         }
         __syncthreads();
-        for (int d = 0; d < kc; ++d) {
-            double a[4], b[4];
+        for (int k = 0; k < kc; k += 4) {
+            // A 16 x 4: lane l holds A[l % 16][l / 16]; B 4 x 16: B[l / 16][l % 16]
+            const TD a = As[16 * w + (lane & 15)][k + (lane >> 4)];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) { a[u] = As[ty * 4 + u][d]; b[u] = Zs[tx * 4 + u][d]; }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+            for (int t = 0; t < 4; ++t) acc[t] = O::mma(a, Zs[16 * t + (lane & 15)][k + (lane >> 4)], acc[t]);
         }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        int r = r0 + ty * 4 + u;
-        if (r >= nrow) continue;
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            int c = c0 + tx * 4 + v;
-            if (c >= nk) continue;
+            const int r = r0 + 16 * w + O::row(lane, v);
+            const int c = c0 + 16 * t + (lane & 15);
+            if (r >= nrow || c >= nk) continue;
+            const double cv = (double)acc[t][v];
             if (mode == 0) {
-                int a = r;
-                double dxx = xx_atom[a];
-                double kxx = pow(s2 + dxx / l2, pw);
-                double dt = s2 + acc[u][v] / l2;
-                double kzz = pow(s2 + zz[c] / l2, pw);
-                double val = pow(dt, pw) / (sqrt(kzz) * sqrt(kxx));
-                if (row_atom[a] == 0) val = 0.0;  // row_atom holds the neighbour count in this mode
-                out[(size_t)a * nk + c] = (TD)val;
-                dot_out[(size_t)a * nk + c] = acc[u][v];
+                const double u = s2 + cv / l2;
+                double val = kpow(u, pw, ipw) * ix[r] * iz[c];
+                if (row_atom[r] == 0) val = 0.0;  // row_atom holds the neighbour count in this mode
+                out[(size_t)r * nk + c] = (TD)val;
+                dot_out[(size_t)r * nk + c] = cv;
             } else {
-                int a = row_atom[r];
-                double dxx = xx_atom[a];
-                double kxx = pow(s2 + dxx / l2, pw);
-                double dkxx = 2.0 * pw * pow(s2 + dxx / l2, pw - 1.0) / l2;
-                double dt = s2 + dot[(size_t)a * nk + c] / l2;
-                double kt = pow(dt, pw);
-                double kzz = pow(s2 + zz[c] / l2, pw);
-                double aaa = sqrt(kzz) * sqrt(kxx), bbb = 2.0 * kxx;
-                double tf1 = pw * pow(dt, pw - 1.0) / l2;
-                double dn = (tf1 * acc[u][v] - kt * (gx[r] * dkxx) / bbb) / aaa;
+                const int a = row_atom[r];
+                const double u = s2 + dot[(size_t)a * nk + c] / l2;
+                const double um1 = kpow(u, pw - 1.0, ipw - 1);
+                const double dn = (pw * um1 / l2 * cv - um1 * u * gx[r] * fx[a]) * ix[a] * iz[c];
                 out[(size_t)row_slot[r] * nk + c] = (TD)dn;
             }
         }
-    }
+}
+
+// per-atom factors ix = u(x.x)^(-p/2), fx = p / (l2 u(x.x)) and the neighbour count
+__global__ void kpo_atom_kernel(int natoms, const double* __restrict__ xx, double s2, double l2, double pw,
+                                double* __restrict__ ix, double* __restrict__ fx)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms) return;
+    const double u = s2 + xx[a] / l2;
+    ix[a] = pow(u, -0.5 * pw);
+    fx[a] = pw / (l2 * u);
+}
+
+__global__ void kpo_z_kernel(int n, const double* __restrict__ zin, double* __restrict__ zf64, float* __restrict__ zf32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    zf64[i] = zin[i];
+    zf32[i] = (float)zin[i];
 }
 
 // per-atom x.x and per (neighbour slot, x) g.x; rows of neighbour slots listed in order
@@ -153,8 +195,9 @@ int run(mld_ctx* ctx, TD* ek, TD* fk)
     const long long P = ctx->npairs;
     const double s2 = ctx->k_sigma * ctx->k_sigma, l2 = 2.0 * ctx->k_length * ctx->k_length, pw = ctx->k_power;
     size_t nrows = fk ? (size_t)3 * P : 0;
-    size_t need = (size_t)natoms * sizeof(double) + (size_t)natoms * sizeof(int) + nrows * (sizeof(int) + sizeof(long long) + sizeof(double)) +
-                  nrows * D * sizeof(TD) + (size_t)natoms * nk * sizeof(double) + 8 * 256;
+    size_t need = 3 * (size_t)natoms * sizeof(double) + (size_t)natoms * sizeof(int) +
+                  nrows * (sizeof(int) + sizeof(long long) + sizeof(double)) + nrows * D * sizeof(TD) +
+                  (size_t)natoms * nk * sizeof(double) + 16 * 256;
     if (need > ctx->cap_work) {
         if (ctx->d_work) MLD_HIP(ctx, hipFree(ctx->d_work));
         ctx->d_work = nullptr;
@@ -164,6 +207,8 @@ int run(mld_ctx* ctx, TD* ek, TD* fk)
     char* b = (char*)ctx->d_work;
     auto take = [&](size_t bytes) { char* p = b; b += ((bytes + 255) / 256) * 256; return (void*)p; };
     double* xx = (double*)take((size_t)natoms * sizeof(double));
+    double* ixa = (double*)take((size_t)natoms * sizeof(double));
+    double* fxa = (double*)take((size_t)natoms * sizeof(double));
     int* nn = (int*)take((size_t)natoms * sizeof(int));
     int* row_atom = (int*)take(nrows * sizeof(int) + 4);
     long long* row_slot = (long long*)take(nrows * sizeof(long long) + 8);
@@ -173,16 +218,23 @@ int run(mld_ctx* ctx, TD* ek, TD* fk)
     const TD* energy = (const TD*)ctx->last_energy;
     const TD* grad = fk ? (const TD*)ctx->last_grad : nullptr;
     if (natoms == 0) return MLD_OK;
+    const int ipw = (pw == std::floor(pw) && pw >= 1.0 && pw <= 16.0) ? (int)pw : 0;
+    const TD* Zt = (sizeof(TD) == 4) ? (const TD*)ctx->d_zref32 : (const TD*)ctx->d_zref;
     hipLaunchKernelGGL(kpo_prep_kernel<TD>, dim3(natoms), dim3(128), 0, st, natoms, D, ctx->d_pair_off, energy, grad, xx, nn,
                        row_atom, row_slot, gx, grows);
-    dim3 ge((nk + T - 1) / T, (natoms + T - 1) / T);
-    hipLaunchKernelGGL(kpo_gemm_kernel<TD>, ge, dim3(256), 0, st, 0, natoms, nk, D, energy, ctx->d_zref, ctx->d_zz, xx,
-                       nn, (const long long*)nullptr, (const double*)nullptr, (const double*)nullptr, s2, l2, pw, dot, ek);
+    hipLaunchKernelGGL(kpo_atom_kernel, dim3((natoms + 255) / 256), dim3(256), 0, st, natoms, xx, s2, l2, pw, ixa, fxa);
+    const int nct = (nk + T - 1) / T;
+    const long long ge = (long long)nct * ((natoms + T - 1) / T);
+    hipLaunchKernelGGL(kpo_mfma_kernel<TD>, dim3((unsigned)ge), dim3(256), 0, st, 0, natoms, nk, D, nct, energy, Zt,
+                       ctx->d_kiz, ixa, fxa, nn, (const long long*)nullptr, (const double*)nullptr,
+                       (const double*)nullptr, s2, l2, pw, ipw, dot, ek);
     MLD_HIP(ctx, hipGetLastError());
     if (fk && nrows) {
-        dim3 gf((nk + T - 1) / T, (unsigned)((nrows + T - 1) / T));
-        hipLaunchKernelGGL(kpo_gemm_kernel<TD>, gf, dim3(256), 0, st, 1, (int)nrows, nk, D, (const TD*)grows, ctx->d_zref,
-                           ctx->d_zz, xx, row_atom, row_slot, gx, dot, s2, l2, pw, (double*)nullptr, fk);
+        const long long gf = (long long)nct * (long long)((nrows + T - 1) / T);
+        if (gf > 0x7fffffffLL) return mld_fail(ctx, MLD_E_UNSUPPORTED, "kernel_po: batch too large for one launch");
+        hipLaunchKernelGGL(kpo_mfma_kernel<TD>, dim3((unsigned)gf), dim3(256), 0, st, 1, (int)nrows, nk, D, nct,
+                           (const TD*)grows, Zt, ctx->d_kiz, ixa, fxa, row_atom, row_slot, gx, dot, s2, l2, pw, ipw,
+                           (double*)nullptr, fk);
     }
     if (fk) hipLaunchKernelGGL(kpo_self_kernel<TD>, dim3((nk + 255) / 256, natoms), dim3(256), 0, st, natoms, nk, ctx->d_pair_off, fk);
     MLD_HIP(ctx, hipGetLastError());
@@ -203,13 +255,29 @@ extern "C" int mld_kernel_po_set(mld_ctx* ctx, int nk, const double* zref, doubl
         for (int d = 0; d < D; ++d) s += zref[(size_t)k * D + d] * zref[(size_t)k * D + d];
         zz[k] = s;
     }
-    if (ctx->d_zref) MLD_HIP(ctx, hipFree(ctx->d_zref));
-    if (ctx->d_zz) MLD_HIP(ctx, hipFree(ctx->d_zz));
-    ctx->d_zref = ctx->d_zz = nullptr;
+    // iz = u(z.z)^(-p/2), u(t) = sigma^2 + t / (2 l^2)
+    std::vector<double> izv(nk);
+    const double s2 = sigma * sigma, l2 = 2.0 * length * length;
+    for (int k = 0; k < nk; ++k) izv[k] = std::pow(s2 + zz[k] / l2, -0.5 * power);
+    void* olds[] = {ctx->d_zref, ctx->d_zz, ctx->d_zref32, ctx->d_kiz};
+    for (void* q : olds)
+        if (q) MLD_HIP(ctx, hipFree(q));
+    ctx->d_zref = ctx->d_zz = ctx->d_kiz = nullptr;
+    ctx->d_zref32 = nullptr;
     MLD_HIP(ctx, hipMalloc(&ctx->d_zref, (size_t)nk * D * sizeof(double)));
+    MLD_HIP(ctx, hipMalloc(&ctx->d_zref32, (size_t)nk * D * sizeof(float)));
     MLD_HIP(ctx, hipMalloc(&ctx->d_zz, (size_t)nk * sizeof(double)));
-    MLD_HIP(ctx, hipMemcpy(ctx->d_zref, zref, (size_t)nk * D * sizeof(double), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMalloc(&ctx->d_kiz, (size_t)nk * sizeof(double)));
+    double* zstage = nullptr;
+    MLD_HIP(ctx, hipMalloc(&zstage, (size_t)nk * D * sizeof(double)));
+    MLD_HIP(ctx, hipMemcpy(zstage, zref, (size_t)nk * D * sizeof(double), hipMemcpyHostToDevice));
+    const int nz = nk * D;
+    hipLaunchKernelGGL(kpo_z_kernel, dim3((nz + 255) / 256), dim3(256), 0, ctx->stream, nz, zstage, ctx->d_zref,
+                       ctx->d_zref32);
+    MLD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    MLD_HIP(ctx, hipFree(zstage));
     MLD_HIP(ctx, hipMemcpy(ctx->d_zz, zz.data(), (size_t)nk * sizeof(double), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMemcpy(ctx->d_kiz, izv.data(), (size_t)nk * sizeof(double), hipMemcpyHostToDevice));
     ctx->nk = nk;
     ctx->k_sigma = sigma;
     ctx->k_length = length;
@@ -224,6 +292,11 @@ extern "C" int mld_kernel_po_dev(mld_ctx* ctx, void* ek_dev, void* fk_dev)
     if (!ctx->last_energy) return mld_fail(ctx, MLD_E_STATE, "mld_batch_bso4_dev first");
     if (fk_dev && !ctx->last_grad) return mld_fail(ctx, MLD_E_STATE, "gradient features need mld_batch_bso4_dev with gradients");
     if (!ek_dev) return mld_fail(ctx, MLD_E_ARG, "null ek");
-    if (ctx->p.precision == MLD_PREC_FP32) return run<float>(ctx, (float*)ek_dev, (float*)fk_dev);
-    return run<double>(ctx, (double*)ek_dev, (double*)fk_dev);
+    int rc = (ctx->p.precision == MLD_PREC_FP32) ? run<float>(ctx, (float*)ek_dev, (float*)fk_dev)
+                                                 : run<double>(ctx, (double*)ek_dev, (double*)fk_dev);
+    if (rc == MLD_OK) {
+        ctx->last_ek = ek_dev;
+        ctx->last_fk = fk_dev;
+    }
+    return rc;
 }

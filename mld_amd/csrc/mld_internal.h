@@ -146,6 +146,9 @@ struct mld_ctx {
     // last descriptor outputs (device, owned by caller)
     const void* last_energy = nullptr;
     const void* last_grad = nullptr;
+    // last kernel features (mld_kernel_po_dev): ek (natoms, nk), fk ((P + natoms) * 3, nk)
+    const void* last_ek = nullptr;
+    const void* last_fk = nullptr;
     int* d_atom_off = nullptr;   // [nstruct + 1] device copy of atom_off
     size_t cap_struct_off = 0;
     // Gram state
@@ -164,6 +167,8 @@ struct mld_ctx {
     double k_sigma = 0, k_length = 1, k_power = 2;
     double* d_zref = nullptr;    // [nk][D] row-major
     double* d_zz = nullptr;      // [nk] |z|^2
+    float* d_zref32 = nullptr;   // [nk][D] FP32 copy for the FP32 form
+    double* d_kiz = nullptr;     // [nk] (sigma^2 + |z|^2 / (2 l^2))^(-p/2)
     // fitted model and selection-score state (predict.hip)
     void* predict_state = nullptr;
     // copies of the caller arrays named by p (species weights, inner cutoffs)

@@ -29,6 +29,8 @@ struct PredictState {
     double* d_ref = nullptr;   // [nsp] reference energy per species, or null
     double* d_mu = nullptr;    // [D]
     double* d_sinv = nullptr;  // [D*D] Sigma^-1, column-major (Sigma_inv(:, d2) contiguous)
+    double* d_wk = nullptr;    // [nk] kernel coefficients (order 3)
+    int nk = 0;
     void* work = nullptr; size_t cap_work = 0;
 };
 
@@ -117,6 +119,21 @@ __global__ void __launch_bounds__(256) slot_dot_kernel(int natoms, int D, const 
     }
 }
 
+// kernel part of a kernel model: v[r] += wk . feat[r] for rows r of nk features (one wave per row)
+template <typename T>
+__global__ void __launch_bounds__(256) krr_dot_kernel(long long nrow, int nk, const double* __restrict__ wk,
+                                                      const T* __restrict__ feat, double* __restrict__ v)
+{
+    const long long r = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= nrow) return;
+    const T* f = feat + (size_t)r * nk;
+    double s = 0.0;
+    for (int c = lane; c < nk; c += 64) s += wk[c] * (double)f[c];
+    s = wave_sum(s);
+    if (lane == 0) v[r] += s;
+}
+
 // one block per structure: E_s = w0 N_s + sum_a e_a (fixed tree order)
 __global__ void __launch_bounds__(256) energy_kernel(const int* __restrict__ atom_off, double w0,
                                                      const double* __restrict__ e_atom, double* __restrict__ E)
@@ -173,7 +190,7 @@ void predict_free(mld_ctx* ctx)
 {
     if (!ctx->predict_state) return;
     PredictState* ps = reinterpret_cast<PredictState*>(ctx->predict_state);
-    void* q[] = {ps->d_w, ps->d_U, ps->d_ref, ps->d_mu, ps->d_sinv, ps->work};
+    void* q[] = {ps->d_w, ps->d_U, ps->d_ref, ps->d_mu, ps->d_sinv, ps->d_wk, ps->work};
     for (void* p : q)
         if (p) (void)hipFree(p);
     delete ps;
@@ -204,6 +221,8 @@ int predict_clone(mld_ctx* dst, const mld_ctx* src)
     if (!rc) rc = cp(&pd->d_ref, ps->d_ref, (size_t)src->nsp);
     if (!rc) rc = cp(&pd->d_mu, ps->d_mu, D);
     if (!rc) rc = cp(&pd->d_sinv, ps->d_sinv, D * D);
+    pd->nk = ps->nk;
+    if (!rc) rc = cp(&pd->d_wk, ps->d_wk, (size_t)ps->nk);
     return rc;
 }
 
@@ -218,7 +237,9 @@ int mld_model_set(mld_ctx* ctx, int order, int ncol, const double* w, const doub
         return mld_fail(ctx, MLD_E_ARG, "mld_model_set: order 1 needs 1 + D coefficients");
     if (order == 2 && ncol != n_uni && ncol != n_full)
         return mld_fail(ctx, MLD_E_ARG, "mld_model_set: order 2 needs 1 + D + D(D+1)/2 or 1 + D + D^2 coefficients");
-    if (order != 1 && order != 2) return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_model_set: order must be 1 or 2");
+    if (order == 3 && (ctx->nk <= 0 || ncol != n_lin + ctx->nk))
+        return mld_fail(ctx, MLD_E_ARG, "mld_model_set: order 3 needs mld_kernel_po_set and 1 + D + nk coefficients");
+    if (order < 1 || order > 3) return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_model_set: order must be 1, 2 or 3");
     PredictState* ps = state(ctx);
     ps->order = order;
     ps->ncol = ncol;
@@ -246,6 +267,9 @@ int mld_model_set(mld_ctx* ctx, int order, int ncol, const double* w, const doub
     }
     int rc = upload(ctx, &ps->d_w, lin);
     if (!rc) rc = upload(ctx, &ps->d_U, U);
+    ps->nk = order == 3 ? ctx->nk : 0;
+    if (!rc) rc = upload(ctx, &ps->d_wk, order == 3 ? std::vector<double>(w + n_lin, w + n_lin + ctx->nk)
+                                                     : std::vector<double>());
     if (!rc) rc = upload(ctx, &ps->d_ref, ref_energy ? std::vector<double>(ref_energy, ref_energy + ctx->nsp)
                                                       : std::vector<double>());
     return rc;
@@ -274,6 +298,8 @@ int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev)
         return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: forces need descriptors with gradients");
     if (d2_dev && !ps->d_mu) return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: mld_mahalanobis_set first");
     if (ps->D != ctx->D) return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: model dimension changed");
+    if (ps->order == 3 && (ps->nk != ctx->nk || !ctx->last_ek || (F_dev && !ctx->last_fk)))
+        return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: kernel model needs mld_kernel_po_dev features of this batch");
     const int natoms = ctx->natoms, nstruct = ctx->nstruct, D = ctx->D;
     const long long P = ctx->npairs;
     hipStream_t st = ctx->stream;
@@ -296,6 +322,14 @@ int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev)
             hipLaunchKernelGGL(atom_kernel<double>, dim3(nblk), dim3(256), 0, st, natoms, D, (const double*)ctx->last_energy,
                                ctx->d_itype, ps->d_w, ps->d_U, ps->d_ref, mu, ps->d_sinv, g, e_atom, d2_dev);
     }
+    if (ps->order == 3 && natoms) {
+        if (f32)
+            hipLaunchKernelGGL(krr_dot_kernel<float>, dim3(nblk), dim3(256), 0, st, (long long)natoms, ps->nk, ps->d_wk,
+                               (const float*)ctx->last_ek, e_atom);
+        else
+            hipLaunchKernelGGL(krr_dot_kernel<double>, dim3(nblk), dim3(256), 0, st, (long long)natoms, ps->nk, ps->d_wk,
+                               (const double*)ctx->last_ek, e_atom);
+    }
     if (E_dev && nstruct)
         hipLaunchKernelGGL(energy_kernel, dim3(nstruct), dim3(256), 0, st, ctx->d_atom_off, ps->w0, e_atom, E_dev);
     if (F_dev && natoms) {
@@ -305,6 +339,16 @@ int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev)
         else
             hipLaunchKernelGGL(slot_dot_kernel<double>, dim3(nblk), dim3(256), 0, st, natoms, D, ctx->d_pair_off, g,
                                (const double*)ctx->last_grad, s);
+        if (ps->order == 3) {
+            const long long nrow = 3 * (P + natoms);
+            const unsigned nb2 = (unsigned)((nrow + 3) / 4);
+            if (f32)
+                hipLaunchKernelGGL(krr_dot_kernel<float>, dim3(nb2), dim3(256), 0, st, nrow, ps->nk, ps->d_wk,
+                                   (const float*)ctx->last_fk, s);
+            else
+                hipLaunchKernelGGL(krr_dot_kernel<double>, dim3(nb2), dim3(256), 0, st, nrow, ps->nk, ps->d_wk,
+                                   (const double*)ctx->last_fk, s);
+        }
         const int *rev_off, *rev_pair, *pair_src;
         rc = pair_transpose(ctx, &rev_off, &rev_pair, &pair_src);
         if (rc) return rc;

@@ -6,6 +6,8 @@ Cholesky with iterative refinement.
 Column layouts
   order 1 : [N, B_1..B_D]                                  (1 + D columns)
   order 2 : [N, B_1..B_D, (B_p B_q)_{p<=q}]                 (1 + D + D(D+1)/2)
+  order 3 : [N, B_1..B_D, k_1..k_nk]  kernel model           (1 + D + nk; kernel_pack_*,
+            mld_energy.F90:122-144, mld_forces.F90:268-320)
 The reference stores the quadratic block as the full D^2 outer product
 (mld.F90:771-780); ``expand_quadratic`` maps a unique-column solution to that
 layout (w_pq = w_qp = u_pq / 2 off the diagonal).
@@ -20,8 +22,9 @@ import numpy as np
 SOLVE_CHOL_IR, SOLVE_CHOL = 0, 1
 
 
-def ncols(D: int, order: int) -> int:
-    return 1 + D + (D * (D + 1) // 2 if order == 2 else 0)
+def ncols(D: int, order: int, nk: int = 0) -> int:
+    """1 + D (linear), + D(D+1)/2 (quadratic, unique columns), + nk (order 3: kernel model)."""
+    return 1 + D + (D * (D + 1) // 2 if order == 2 else 0) + (nk if order == 3 else 0)
 
 
 def unique_pairs(D: int):
@@ -52,17 +55,18 @@ def full_to_unique_rows(rows_full: np.ndarray, D: int) -> np.ndarray:
 
 
 class GramFit:
-    def __init__(self, ctx, order: int = 1, mixed: bool = False, keep_rows: bool = False):
+    def __init__(self, ctx, order: int = 1, mixed: bool = False, keep_rows: bool = False, nk: int = 0):
         self.ctx = ctx
         self.order = order
         self.mixed = bool(mixed)
         self.D = ctx.D
-        self.ncol = ncols(ctx.D, order)
+        self.ncol = ncols(ctx.D, order, nk)
         ctx.check(ctx.L.mld_gram_begin(ctx.h, self.ncol))
         ctx.check(ctx.L.mld_gram_set_keep_rows(ctx.h, int(keep_rows)))
 
     def add_batch(self, batch, E: np.ndarray, F_dev, w_e: float = 1.0, w_f: float = 1.0):
-        """Energy + force rows of the batch's last descriptors (mld_batch_bso4_dev with gradients)."""
+        """Energy + force rows of the batch's last descriptors (mld_batch_bso4_dev with gradients);
+        order 3 also uses the batch's last kernel features (kernel_po with gradients)."""
         from .batch import torch_ready
 
         E = np.ascontiguousarray(E, np.float64)
@@ -108,6 +112,18 @@ def kernel_po_set(ctx, zref: np.ndarray, sigma: float, length: float, power: flo
     """zref (nk, D): standardised reference points (global_kernel of init_kernel)."""
     z = np.ascontiguousarray(zref, np.float64)
     ctx.check(ctx.L.mld_kernel_po_set(ctx.h, z.shape[0], z.ctypes.data_as(C.c_void_p), sigma, length, power))
+    ctx.nk = z.shape[0]
+
+
+def kernel_po_alloc(ctx, batch, want_grad: bool = True):
+    """Device buffers for kernel features: ek (natoms, nk), fk ((P + natoms) * 3, nk)."""
+    import torch
+
+    dt = torch.float32 if ctx.precision == 1 else torch.float64
+    nk = ctx.nk
+    ek = torch.empty((batch.natoms, nk), dtype=dt, device=batch.device)
+    fk = torch.empty(((batch.npairs + batch.natoms) * 3, nk), dtype=dt, device=batch.device) if want_grad else None
+    return ek, fk
 
 
 def kernel_po(ctx, ek, fk=None):
@@ -116,12 +132,14 @@ def kernel_po(ctx, ek, fk=None):
     torch_ready(ek.device)
     ctx.check(ctx.L.mld_kernel_po_dev(ctx.h, C.c_void_p(ek.data_ptr()),
                                       None if fk is None else C.c_void_p(fk.data_ptr())))
+    ctx._kernel_features = (ek, fk)  # the context keeps raw pointers to them
     ctx.synchronize()
 
 
 def model_set(ctx, order: int, w: np.ndarray, ref_energy: Optional[np.ndarray] = None):
     """Keep a fitted model on the device: w in the unique (1 + D + D(D+1)/2) or the
-    reference's full (1 + D + D^2) quadratic layout, or 1 + D for order 1."""
+    reference's full (1 + D + D^2) quadratic layout, 1 + D for order 1, 1 + D + nk for
+    a kernel model (order 3, predictions then use the batch's last kernel features)."""
     w = np.ascontiguousarray(w, np.float64)
     ref = None if ref_energy is None else np.ascontiguousarray(ref_energy, np.float64)
     ctx.check(ctx.L.mld_model_set(ctx.h, order, w.shape[0], w.ctypes.data_as(C.c_void_p),

@@ -177,3 +177,42 @@ def morse_from_pairs(pair_off, pair_idx, pair_d, atom_off, D: float = 0.4, alpha
     sid = torch.repeat_interleave(torch.arange(nstruct, device=pair_d.device), ao[1:] - ao[:-1])
     E = torch.zeros(nstruct, dtype=torch.float64, device=pair_d.device).index_add_(0, sid, e_atom)
     return E, F
+
+
+def morse_mixture_params(nspecies: int = 5, seed: int = 11):
+    """Species-pair Morse parameters (symmetric): D in [0.3, 0.6] eV, alpha in [1.2, 1.6] / A,
+    r0 in [2.5, 2.9] A, drawn from a seeded generator."""
+    rng = np.random.default_rng(seed)
+
+    def sym(lo, hi):
+        m = rng.uniform(lo, hi, (nspecies, nspecies))
+        return np.triu(m) + np.triu(m, 1).T
+
+    return sym(0.3, 0.6), sym(1.2, 1.6), sym(2.5, 2.9)
+
+
+def morse_mixture_from_pairs(pair_off, pair_idx, pair_d, atom_off, itype, Dm, am, r0m, rc: float = 5.0):
+    """Species-dependent Morse energies / forces from a device pair list (torch); itype is a
+    device tensor of 0-based species, Dm / am / r0m (nsp, nsp) parameter tables."""
+    import torch
+
+    dev = pair_d.device
+    natoms = pair_off.shape[0] - 1
+    counts = (pair_off[1:] - pair_off[:-1]).long()
+    src = torch.repeat_interleave(torch.arange(natoms, device=dev), counts)
+    ti = itype.long()[src]
+    tj = itype.long()[pair_idx.long()]
+    Dt, at, rt = (torch.as_tensor(np.asarray(m, np.float64), device=dev) for m in (Dm, am, r0m))
+    D, alpha, r0 = Dt[ti, tj], at[ti, tj], rt[ti, tj]
+    r = pair_d[:, 3]
+    m = (r < rc).double()
+    ex = torch.exp(-alpha * (r - r0))
+    phi = D * (ex * ex - 2.0 * ex) * m
+    dphi = D * (-2.0 * alpha * ex * ex + 2.0 * alpha * ex) * m
+    e_atom = torch.zeros(natoms, dtype=torch.float64, device=dev).index_add_(0, src, 0.5 * phi)
+    F = torch.zeros((natoms, 3), dtype=torch.float64, device=dev)
+    F.index_add_(0, src, (dphi / r)[:, None] * pair_d[:, :3])
+    ao = torch.as_tensor(atom_off, device=dev).long()
+    sid = torch.repeat_interleave(torch.arange(ao.shape[0] - 1, device=dev), ao[1:] - ao[:-1])
+    E = torch.zeros(ao.shape[0] - 1, dtype=torch.float64, device=dev).index_add_(0, sid, e_atom)
+    return E, F

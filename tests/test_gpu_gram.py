@@ -165,3 +165,98 @@ def test_kernel_po_against_oracle(ctx64):
         assert rel(FK[s0:s0 + go.shape[0]], fo) < 1e-9
         a0 += len(p)
         s0 += go.shape[0]
+
+
+def krr_rows_numpy(e, g, nl, ek, fk):
+    """Kernel-model rows [N, B, k] (energy) and [0, -sum dB, -sum dk] (forces) from the
+    oracle's descriptors and kernel features, restating kernel_pack_energy_descriptor /
+    kernel_pack_force_descriptor (mld_energy.F90:122-144, mld_forces.F90:268-320) on the
+    slot layout: slot(a, 0) = self, slot(a, k) = k-th neighbour of a."""
+    off, nbr = nl[0], nl[1]
+    nat, D = e.shape
+    nk = ek.shape[1]
+    R = np.zeros((1 + 3 * nat, 1 + D + nk))
+    R[0, 0] = nat
+    R[0, 1:1 + D] = e.sum(0)
+    R[0, 1 + D:] = ek.sum(0)
+    gs = g.reshape(-1, 3, D)
+    fs = fk.reshape(-1, 3, nk)
+    for a in range(nat):
+        base = off[a] + a
+        for x in range(3):
+            R[1 + 3 * a + x, 1:1 + D] -= gs[base, x]
+            R[1 + 3 * a + x, 1 + D:] -= fs[base, x]
+        for k in range(1, off[a + 1] - off[a] + 1):
+            j = nbr[off[a] + k - 1]
+            for x in range(3):
+                R[1 + 3 * j + x, 1:1 + D] -= gs[base + k, x]
+                R[1 + 3 * j + x, 1 + D:] -= fs[base + k, x]
+    return R
+
+
+def test_kernel_model_rows_gram_and_prediction(ctx64):
+    """order 3: Gram of [N, B, k] rows against numpy on the oracle's rows; a kernel model's
+    E and F equal rows @ w."""
+    from mld_amd import fit
+
+    b, pos, cells, types, E, F = small_batch(ctx64, seed=9, nstruct=3)
+    rng = np.random.default_rng(3)
+    draft = np.concatenate([O.bso4(p, c, t, 8, 4.7, want_grad=False)[0] for p, c, t in zip(pos, cells, types)])
+    z = draft[rng.choice(len(draft), 37, replace=False)]
+    sigma, length, power = 1.0, float(np.sqrt(np.mean((draft ** 2).sum(1)) / 2)), 2.0
+    fit.kernel_po_set(ctx64, z, sigma, length, power)
+    ek, fk = fit.kernel_po_alloc(ctx64, b)
+    fit.kernel_po(ctx64, ek, fk)
+    import torch
+
+    Fd = torch.from_numpy(F.reshape(-1)).to("cuda:0")
+    gf = fit.GramFit(ctx64, order=3, nk=37)
+    gf.add_batch(b, E, Fd, 1.0, 0.25)
+    G, h = gf.export()
+    rows, ys, ws = [], [], []
+    a0 = 0
+    for s, (p, c, t) in enumerate(zip(pos, cells, types)):
+        eo, go, nl = O.bso4(p, c, t, 8, 4.7)
+        ko, fo = O.kernel_po(eo, go, nl, z, sigma, length, power)
+        rows.append(krr_rows_numpy(eo, go, nl, ko, fo))
+        ys.append(np.concatenate([[E[s]], F[a0:a0 + len(p)].reshape(-1)]))
+        ws.append(np.concatenate([[1.0], np.full(3 * len(p), 0.25)]))
+        a0 += len(p)
+    R, y, w = np.concatenate(rows), np.concatenate(ys), np.concatenate(ws)
+    Rw = R * np.sqrt(w)[:, None]
+    assert rel(G, Rw.T @ Rw) < 1e-12
+    assert rel(h, Rw.T @ (y * np.sqrt(w))) < 1e-12
+    coef = rng.standard_normal(R.shape[1])
+    fit.model_set(ctx64, 3, coef)
+    Ep, Fp, _ = fit.predict(ctx64, b)
+    pred = R @ coef
+    erow = np.cumsum([0] + [1 + 3 * len(p) for p in pos])[:-1]
+    frow = np.setdiff1d(np.arange(len(pred)), erow)
+    assert rel(Ep.cpu().numpy(), pred[erow]) < 1e-12
+    assert rel(Fp.cpu().numpy().reshape(-1), pred[frow]) < 1e-11
+
+
+def test_kernel_po_fp32_against_oracle():
+    """FP32 form of the kernel features (f32 MFMA dot products) against the FP64 oracle."""
+    import mld_amd as M
+    from mld_amd import fit
+
+    ctx = M.Context(j_max=4.0, r_cut=4.7, precision=M.PREC_FP32)
+    b, pos, cells, types, E, F = small_batch(ctx, seed=10, nstruct=2)
+    rng = np.random.default_rng(5)
+    draft = np.concatenate([O.bso4(p, c, t, 8, 4.7, want_grad=False)[0] for p, c, t in zip(pos, cells, types)])
+    z = draft[rng.choice(len(draft), 70, replace=True)] * (1 + 0.02 * rng.normal(size=(70, 55)))
+    sigma, length, power = 1.0, float(np.sqrt(np.mean((draft ** 2).sum(1)) / 2)), 2.0
+    fit.kernel_po_set(ctx, z, sigma, length, power)
+    ek, fk = fit.kernel_po_alloc(ctx, b)
+    fit.kernel_po(ctx, ek, fk)
+    EK, FK = ek.double().cpu().numpy(), fk.double().cpu().numpy().reshape(-1, 3, 70)
+    a0 = s0 = 0
+    for p, c, t in zip(pos, cells, types):
+        eo, go, nl = O.bso4(p, c, t, 8, 4.7)
+        ko, fo = O.kernel_po(eo, go, nl, z, sigma, length, power)
+        assert rel(EK[a0:a0 + len(p)], ko) < 2e-5
+        assert rel(FK[s0:s0 + go.shape[0]], fo) < 2e-4
+        a0 += len(p)
+        s0 += go.shape[0]
+    ctx.close()
