@@ -40,7 +40,7 @@ enum {
 };
 
 enum { MLD_PREC_FP64 = 0, MLD_PREC_FP32 = 1 };
-enum { MLD_DESC_BSO4 = 9 };
+enum { MLD_DESC_BSO4 = 9, MLD_DESC_ACE = 300 };  /* descriptor_bispectrum_so4, descriptor_ace */
 enum { MLD_SOLVE_CHOL_IR = 0, MLD_SOLVE_CHOL = 1 };
 
 typedef struct mld_params {
@@ -57,6 +57,14 @@ typedef struct mld_params {
     double r_cut_width_in;      /* &input_ml r_cut_width_in */
     int precision;              /* MLD_PREC_FP64 or MLD_PREC_FP32 */
     int device;                 /* HIP device ordinal */
+    /* ACE (descriptor_type 300), module_ace_desc keys: ace_numax (body order - 1, 1..3),
+     * ace_nmax_list / ace_lmax_list per nu (defaults "5 3 1" / "0 4 2", read_ml_file.F90:282-286),
+     * ace_lambda (expTcheb radial, ace_radial_poly = 2).  Chemical channels: nchannel 1
+     * (every neighbour) or 2 (plus one weighted by weight[species]). */
+    int ace_numax;
+    int ace_nmax[3];
+    int ace_lmax[3];
+    double ace_lambda;
 } mld_params;
 
 typedef struct mld_ctx mld_ctx;
@@ -76,9 +84,13 @@ int mld_max_neighbours(mld_ctx* ctx, int nat, const double* pos, const double* c
 
 /* Descriptors of one configuration: replaces compute_bispectrum_so4 plus the
  * force sign flip at descriptors.F90:296-302.  force may be NULL when
- * want_grad == 0.  n_neigh: (nat); kind_neigh: (nat, nn_max) 1-based. */
+ * want_grad == 0.  n_neigh: (nat); kind_neigh: (nat, nn_max) 1-based.
+ * mld_ace is the same call for an ACE context (compute_ace, descriptors.F90:293-294);
+ * either name runs the context's descriptor. */
 int mld_bso4(mld_ctx* ctx, int nat, const double* pos, const double* cell, const int* itype,
              int want_grad, double* energy, double* force, int* n_neigh, int* kind_neigh, int nn_max);
+int mld_ace(mld_ctx* ctx, int nat, const double* pos, const double* cell, const int* itype,
+            int want_grad, double* energy, double* force, int* n_neigh, int* kind_neigh, int nn_max);
 
 /* ---- device-resident batch of configurations (one launch per phase) ----
  * atom_off[nstruct+1] (host), pos_dev (3*natoms, [atom][xyz]), cell_dev
@@ -93,6 +105,7 @@ int mld_batch_neighbours(mld_ctx* ctx, int nstruct, const int* atom_off, const d
 int mld_batch_pairs_dev(mld_ctx* ctx, const int** pair_off_dev, const int** pair_idx_dev);
 /* pair geometry of the last neighbour build: P x (dx, dy, dz, r), dx = r_j - r_i (Angstrom) */
 int mld_batch_pair_geometry_dev(mld_ctx* ctx, const double** pair_d_dev);
+/* descriptors of the batch (the context's kind: SO(4) bispectrum or ACE) */
 int mld_batch_bso4_dev(mld_ctx* ctx, int want_grad, void* energy_dev, void* grad_dev);
 
 /* ---- streamed normal equations (gram_accumulate_config / gram_reduce_and_solve,

@@ -19,11 +19,11 @@ LIBPATH = os.path.join(_HERE, "libmldgpu.so")
 
 MLD_OK, MLD_E_ARG, MLD_E_DEVICE, MLD_E_NEIGH, MLD_E_SINGULAR, MLD_E_UNSUPPORTED, MLD_E_STATE = range(7)
 PREC_FP64, PREC_FP32 = 0, 1
-DESC_BSO4 = 9
+DESC_BSO4, DESC_ACE = 9, 300
 
 # Every symbol include/mldgpu.h declares.
 EXPORTS = [
-    "mld_create", "mld_destroy", "mld_last_error", "mld_descriptor_dim", "mld_max_neighbours", "mld_bso4",
+    "mld_create", "mld_destroy", "mld_last_error", "mld_descriptor_dim", "mld_max_neighbours", "mld_bso4", "mld_ace",
     "mld_batch_neighbours", "mld_batch_pairs_dev", "mld_batch_pair_geometry_dev", "mld_batch_bso4_dev", "mld_gram_begin", "mld_gram_add_rows",
     "mld_gram_add_rows_dev", "mld_gram_add_batch", "mld_gram_export", "mld_gram_set_keep_rows", "mld_gram_solve",
     "mld_kernel_po_set", "mld_kernel_po_dev", "mld_model_set", "mld_mahalanobis_set", "mld_predict_dev",
@@ -53,6 +53,10 @@ class mld_params(C.Structure):
         ("r_cut_width_in", C.c_double),
         ("precision", C.c_int),
         ("device", C.c_int),
+        ("ace_numax", C.c_int),
+        ("ace_nmax", C.c_int * 3),
+        ("ace_lmax", C.c_int * 3),
+        ("ace_lambda", C.c_double),
     ]
 
 
@@ -82,6 +86,7 @@ def load() -> C.CDLL:
         "mld_descriptor_dim": [vp, ip],
         "mld_max_neighbours": [vp, i, vp, vp, ip],
         "mld_bso4": [vp, i, vp, vp, vp, i, vp, vp, vp, vp, i],
+        "mld_ace": [vp, i, vp, vp, vp, i, vp, vp, vp, vp, i],
         "mld_batch_neighbours": [vp, i, vp, vp, vp, vp, i64p],
         "mld_batch_pairs_dev": [vp, vpp, vpp],
         "mld_batch_pair_geometry_dev": [vp, vpp],
@@ -137,12 +142,21 @@ class Context:
     def __init__(self, j_max: float = 4.0, r_cut: float = 4.7, inv_r0_input: Optional[float] = None,
                  lbso4_diag: bool = False, nspecies: int = 1, weight: Optional[Sequence[float]] = None,
                  weight_3ch: Optional[Sequence[float]] = None, r_cut_pair_in: Optional[np.ndarray] = None,
-                 r_cut_width_in: float = 0.4, precision: int = PREC_FP64, device: int = 0):
+                 r_cut_width_in: float = 0.4, precision: int = PREC_FP64, device: int = 0,
+                 descriptor: str = "bso4", ace_numax: int = 3, ace_nmax=(5, 3, 1), ace_lmax=(0, 4, 2),
+                 ace_lambda: float = 2.0):
+        """descriptor "bso4" (SO(4) bispectrum) or "ace" (ace_* keys: body order - 1, radial
+        functions and l per nu, expTcheb lambda; weight adds the species-weighted channel)."""
         L = load()
         nch = 1 + (weight is not None) + (weight_3ch is not None)
         self._keep = []
         p = mld_params()
-        p.descriptor_type = DESC_BSO4
+        p.descriptor_type = DESC_ACE if descriptor == "ace" else DESC_BSO4
+        p.ace_numax = ace_numax
+        for k in range(3):
+            p.ace_nmax[k] = int(ace_nmax[k]) if k < len(ace_nmax) else 0
+            p.ace_lmax[k] = int(ace_lmax[k]) if k < len(ace_lmax) else 0
+        p.ace_lambda = ace_lambda
         p.j_max = j_max
         p.r_cut = r_cut
         p.inv_r0_input = default_inv_r0_input() if inv_r0_input is None else inv_r0_input

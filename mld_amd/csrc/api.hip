@@ -60,13 +60,15 @@ int mld_create(const mld_params* p, mld_ctx** out)
     *out = nullptr;
     mld_ctx* ctx = new mld_ctx();
     ctx->p = *p;
-    if (p->descriptor_type != MLD_DESC_BSO4) {
+    const bool is_ace = p->descriptor_type == MLD_DESC_ACE;
+    if (p->descriptor_type != MLD_DESC_BSO4 && !is_ace) {
         delete ctx;
         return MLD_E_UNSUPPORTED;
     }
-    ctx->jj = (int)(2.0 * p->j_max);
-    if (ctx->jj < 1 || ctx->jj > mld::JJ_MAX || !(p->r_cut > 0.0) || !(p->inv_r0_input > 0.0 && p->inv_r0_input < 1.0) ||
-        p->nchannel < 1 || p->nchannel > mld::NCH_MAX || p->nspecies < 1 ||
+    ctx->jj = is_ace ? 0 : (int)(2.0 * p->j_max);
+    const bool bso4_ok = is_ace || (ctx->jj >= 1 && ctx->jj <= mld::JJ_MAX && p->inv_r0_input > 0.0 && p->inv_r0_input < 1.0);
+    const int nch_max = is_ace ? mld::ACE_NCH : mld::NCH_MAX;
+    if (!bso4_ok || !(p->r_cut > 0.0) || p->nchannel < 1 || p->nchannel > nch_max || p->nspecies < 1 ||
         (p->nchannel >= 2 && !p->weight) || (p->nchannel >= 3 && !p->weight_3ch)) {
         delete ctx;
         return MLD_E_ARG;
@@ -90,6 +92,42 @@ int mld_create(const mld_params* p, mld_ctx** out)
     };
     if (hipSetDevice(p->device) != hipSuccess) { delete ctx; return MLD_E_DEVICE; }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return MLD_E_DEVICE; }
+    if (is_ace) {
+        // basis tables (compute_ace_nlmu_tools.F90's role; this build's enumeration) and the
+        // species weights of the second chemical channel
+        int trc = mld::build_ace_tables(p->nchannel, p->ace_numax, p->ace_nmax, p->ace_lmax, p->ace_lambda, ctx->ace_h);
+        if (trc) {
+            fail(trc == 2 ? MLD_E_UNSUPPORTED : MLD_E_ARG,
+                 trc == 2 ? "ACE basis larger than 512 functions" : "ACE parameters out of range (numax 1..3, nmax 1..8, lmax 0..4)");
+            mld_destroy(ctx);
+            return rc;
+        }
+        ctx->D = ctx->ace_h.D;
+        auto upa = [&](void** dst, const void* src, size_t bytes) -> bool {
+            if (hipMalloc(dst, bytes ? bytes : 8) != hipSuccess) return false;
+            if (bytes && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return false;
+            return true;
+        };
+        std::vector<double> wf((size_t)2 * ctx->nsp, 1.0);
+        if (ctx->nch >= 2)
+            for (int s2 = 0; s2 < ctx->nsp; ++s2) wf[ctx->nsp + s2] = ctx->h_weight[s2];
+        bool ok = upa(&ctx->d_ace_terms, ctx->ace_h.terms.data(), ctx->ace_h.terms.size() * sizeof(mld::AceTerm)) &&
+                  upa(&ctx->d_ace_toff, ctx->ace_h.toff.data(), ctx->ace_h.toff.size() * sizeof(int)) &&
+                  upa((void**)&ctx->d_wfac, wf.data(), wf.size() * sizeof(double));
+        if (!ok) { fail(MLD_E_DEVICE, "device allocation of ACE tables failed"); mld_destroy(ctx); return rc; }
+        mld::AceDev& ad = ctx->ace_d;
+        ad.terms = (const mld::AceTerm*)ctx->d_ace_terms;
+        ad.toff = (const int*)ctx->d_ace_toff;
+        ad.D = ctx->ace_h.D;
+        ad.nch = ctx->ace_h.nch;
+        ad.NA = ctx->ace_h.NA;
+        ad.LA = ctx->ace_h.LA;
+        ad.LA2 = ctx->ace_h.LA2;
+        ad.nA = ctx->ace_h.nA;
+        ad.lam = ctx->ace_h.lam;
+        *out = ctx;
+        return MLD_OK;
+    }
     mld::build_tables(ctx->jj, p->lbso4_diag, ctx->ht);
     if (ctx->ht.nb > mld::NB_MAX) { fail(MLD_E_UNSUPPORTED, "too many bispectrum components"); mld_destroy(ctx); return rc; }
     int ncg_level = 0;
@@ -158,7 +196,7 @@ int mld_destroy(mld_ctx* ctx)
     gram_scratch_free(ctx);
     kept_rows_free(ctx);
     predict_free(ctx);
-    void* ptrs[] = {ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
+    void* ptrs[] = {ctx->d_ace_terms, ctx->d_ace_toff, ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
                     ctx->d_geo, ctx->d_count, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_scan_tmp,
                     ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_zref, ctx->d_zz, ctx->d_zref32, ctx->d_kiz};
     for (void* q : ptrs)
@@ -278,7 +316,8 @@ int mld_batch_bso4_dev(mld_ctx* ctx, int want_grad, void* energy_dev, void* grad
     if (!ctx) return MLD_E_ARG;
     if (!energy_dev || (want_grad && !grad_dev)) return mld_fail(ctx, MLD_E_ARG, "mld_batch_bso4_dev: null output");
     if (ctx->natoms > 0 && !ctx->d_pair_off) return mld_fail(ctx, MLD_E_STATE, "mld_batch_neighbours first");
-    int rc = launch_bso4(ctx, want_grad, energy_dev, grad_dev);
+    int rc = (ctx->p.descriptor_type == MLD_DESC_ACE) ? launch_ace(ctx, want_grad, energy_dev, grad_dev)
+                                                      : launch_bso4(ctx, want_grad, energy_dev, grad_dev);
     if (rc) return rc;
     ctx->last_energy = energy_dev;
     ctx->last_grad = want_grad ? grad_dev : nullptr;
@@ -392,6 +431,14 @@ int mld_bso4(mld_ctx* ctx, int nat, const double* pos, const double* cell, const
     ctx->last_energy = ctx->last_grad = nullptr;
     ctx->last_ek = ctx->last_fk = nullptr;
     return MLD_OK;
+}
+
+int mld_ace(mld_ctx* ctx, int nat, const double* pos, const double* cell, const int* itype, int want_grad,
+            double* energy, double* force, int* n_neigh, int* kind_neigh, int nn_max)
+{
+    if (ctx && ctx->p.descriptor_type != MLD_DESC_ACE)
+        return mld_fail(ctx, MLD_E_STATE, "mld_ace: the context was created for another descriptor");
+    return mld_bso4(ctx, nat, pos, cell, itype, want_grad, energy, force, n_neigh, kind_neigh, nn_max);
 }
 
 }  // extern "C"

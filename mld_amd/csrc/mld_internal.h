@@ -112,6 +112,42 @@ struct StructGeo {
 };
 void struct_geometry(const double* cell_colmajor, double rcut, StructGeo& g);
 
+// Wigner 3j symbol (j1 j2 j3; m1 m2 m3) with integer arguments (0 outside the selection rules)
+double wigner3j(int j1, int m1, int j2, int m2, int j3, int m3);
+
+// ---- ACE (atomic cluster expansion) tables ----
+constexpr int ACE_NMAX = 8;     // radial functions per nu
+constexpr int ACE_LMAX = 4;     // l of Y_lm
+constexpr int ACE_LM = (ACE_LMAX + 1) * (ACE_LMAX + 1);
+constexpr int ACE_NCH = 2;      // chemical channels: every neighbour / species-weighted
+constexpr int ACE_DMAX = 512;   // basis functions per atom (two per thread of a 256-thread block)
+
+// one product term of a basis function: coef * prod_i A[e_i] (e = -1: unused); the A index is
+// e = (c * NA + n) * LA2 + l * l + l + m
+struct AceTerm {
+    double coef;
+    int e[3];
+};
+
+struct AceHost {
+    int nch = 1, numax = 3, NA = 0, LA = 0, LA2 = 0, nA = 0, D = 0;
+    int nmax[3] = {0, 0, 0}, lmax[3] = {0, 0, 0};
+    double lam = 2.0;
+    std::vector<int> toff;   // [D + 1]
+    std::vector<int> nu;     // [D]
+    std::vector<AceTerm> terms;
+};
+
+struct AceDev {
+    const AceTerm* terms;
+    const int* toff;
+    int D, nch, NA, LA, LA2, nA;
+    double lam;
+};
+
+// basis enumeration (nu-major; k1 <= k2 <= k3 over keys (c, n); then l, l1, l2, l3)
+int build_ace_tables(int nch, int numax, const int* nmax, const int* lmax, double lam, AceHost& t);
+
 }  // namespace mld
 
 struct mld_ctx {
@@ -173,6 +209,11 @@ struct mld_ctx {
     void* predict_state = nullptr;
     // copies of the caller arrays named by p (species weights, inner cutoffs)
     std::vector<double> h_weight, h_weight3, h_rin;
+    // ACE contexts (descriptor_type MLD_DESC_ACE)
+    mld::AceHost ace_h;
+    mld::AceDev ace_d{};
+    void* d_ace_terms = nullptr;
+    void* d_ace_toff = nullptr;
 };
 
 int mld_fail(mld_ctx* ctx, int code, const std::string& msg);
@@ -186,6 +227,7 @@ int mld_fail(mld_ctx* ctx, int code, const std::string& msg);
 // launchers implemented in the .hip units
 int launch_neighbours_pos(mld_ctx* ctx, const double* pos);
 int launch_bso4(mld_ctx* ctx, int want_grad, void* energy, void* grad);
+int launch_ace(mld_ctx* ctx, int want_grad, void* energy, void* grad);
 // ncg_level: largest CG1 range of one output level (cgbase[j+1] - cgbase[j]); ncg: all tables
 bool bso4_tables_fit(int jj, int ncg_level, int ncg, int wsize);
 int launch_to_reference_layout(mld_ctx* ctx, int nat, int nn_max, const void* grad, double* force_dev,
