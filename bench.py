@@ -24,8 +24,10 @@ Workloads (``--workload``):
   serve  Poisson arrivals at 200 requests/s of 5-species bcc HEA structures (a = 3.1 A,
       random occupancy, sigma = 0.05 A), atom counts lognormal (median 128, clipped to
       8..2,000); each request asks for the energy, forces and per-atom Mahalanobis
-      scores of a linear model fitted beforehand to Morse targets.  Descriptor: the
-      two-channel species-weighted SO(4) bispectrum (D = 110), r_cut = 5.0 A, FP32.
+      scores of a linear model fitted beforehand to species-dependent Morse targets.
+      Descriptor: ACE with the Chebyshev (expTcheb) radial basis, n_max = 8 / 4 / 2 and
+      l_max = 0 / 4 / 2 for body orders 2 / 3 / 4, two chemical channels (D = 356),
+      r_cut = 5.0 A, FP32.
       Reported: p50 / p99 latency at that load, and the throughput of the same
       requests submitted all at once.
 
@@ -484,13 +486,15 @@ def main_serve(args):
 
     rng = np.random.default_rng(SEED + 5)
     weights = [1.0, 1.25, 1.5, 1.75, 2.0]
+    ace = dict(descriptor="ace", ace_numax=3, ace_nmax=(8, 4, 2), ace_lmax=(0, 4, 2), ace_lambda=2.0)
+    Dm, am, r0m = synth.morse_mixture_params(5)
     # fit the served model (linear, FP64) on 200 structures of the same kind
     train = hea_request_pool(rng, 200)
-    c64 = M.Context(j_max=4.0, r_cut=5.0, nspecies=5, weight=weights)
+    c64 = M.Context(r_cut=5.0, nspecies=5, weight=weights, **ace)
     b = M.Batch(c64, [p for p, _, _ in train], [c for _, c, _ in train], [t for _, _, t in train])
     b.neighbours()
     po, pi, pd = b.pair_tensors()
-    E, F = synth.morse_from_pairs(po, pi, pd, b.atom_off, rc=5.0)
+    E, F = synth.morse_mixture_from_pairs(po, pi, pd, b.atom_off, b.itype, Dm, am, r0m, rc=5.0)
     e, g = b.alloc_outputs(True)
     b.bso4(e, g)
     gf = fit.GramFit(c64, order=1)
@@ -498,13 +502,15 @@ def main_serve(args):
     w, _ = gf.solve(1e-8)
     Bh = e.cpu().numpy()
     mu = Bh.mean(0)
-    cov = np.cov(Bh.T) + 1e-8 * np.eye(Bh.shape[1])
+    cov = np.cov(Bh.T)
+    cov += 1e-6 * np.trace(cov) / cov.shape[0] * np.eye(cov.shape[0])
     c64.close()
-    ctx = M.Context(j_max=4.0, r_cut=5.0, nspecies=5, weight=weights, precision=M.PREC_FP32)
+    ctx = M.Context(r_cut=5.0, nspecies=5, weight=weights, precision=M.PREC_FP32, **ace)
     fit.model_set(ctx, 1, w)
     fit.mahalanobis_set(ctx, mu, np.linalg.inv(cov))
     nreq = int(round(args.rate * args.duration))
     pool = hea_request_pool(rng, nreq)
+    ctx_D = ctx.D
     sv = Server(ctx, hbm_budget_bytes=int(args.hbm_budget_gib * (1 << 30)), max_batch_structs=64,
                 max_batch_atoms=16384)
     for p, c, t in pool[:20]:   # warm-up: buffers reach their working size
@@ -558,7 +564,7 @@ def main_serve(args):
         "dtype": "f32 desc, f64 prediction", "data": f"synthetic (seed {SEED})",
         "config": {"workload": f"serve: {nreq} requests, Poisson {args.rate:g}/s, 5-species bcc HEA a=3.1 A, "
                                "N lognormal median 128 clipped 8..2000, E + F + Mahalanobis d2, "
-                               "weighted SO(4) bispectrum D=110 r_cut=5.0 A FP32",
+                               f"ACE expTcheb n_max 8/4/2 l_max 0/4/2 two channels D={ctx_D} r_cut=5.0 A FP32",
                    "natoms": natoms, "hbm_budget_gib": args.hbm_budget_gib, "max_batch_structs": 64,
                    "parallelism": "single GPU"},
         "at_load": {"rate_req_s": args.rate, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],

@@ -95,13 +95,14 @@ __global__ void rows_energy_kernel(int order, int D, int ldA, const int* __restr
 // pair order), g = dB/dr of the slot and b = B of the slot's centre atom.  The
 // quadratic part is register-blocked in 4 x 4 (p, q) tiles; the three rows are staged
 // in LDS and written as one contiguous stream.  Dynamic LDS:
-//   sg[CH][3][DP] + sb[CH][DP] + out[3][ldA] doubles, DP = D rounded up to 4.
-constexpr int ROWS_CH = 16;
+//   sg[CH][3][DP] + sb[CH][DP] + out[3][ldA] doubles, DP = D rounded up to 4, CH slots per
+// pass (16, or 4 for descriptors wider than 192 components).
+__host__ __device__ inline int rows_force_ch(int D) { return D > 192 ? 4 : 16; }
 
 __host__ __device__ inline size_t rows_force_smem(int D, int ldA)
 {
-    const int DP = (D + 3) & ~3;
-    return sizeof(double) * ((size_t)ROWS_CH * 3 * DP + (size_t)ROWS_CH * DP + 3 * (size_t)ldA);
+    const int DP = (D + 3) & ~3, CH = rows_force_ch(D);
+    return sizeof(double) * ((size_t)CH * 3 * DP + (size_t)CH * DP + 3 * (size_t)ldA);
 }
 
 template <typename T>
@@ -113,10 +114,10 @@ __global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int l
                                                          double swf, double* __restrict__ A, double* __restrict__ y)
 {
     extern __shared__ __align__(16) double rsm[];
-    const int DP = (D + 3) & ~3;
+    const int DP = (D + 3) & ~3, CH = rows_force_ch(D);
     double* sg = rsm;                                  // [CH][3][DP]
-    double* sb = sg + ROWS_CH * 3 * DP;                // [CH][DP]
-    double* out = sb + ROWS_CH * DP;                   // [3][ldA]
+    double* sb = sg + CH * 3 * DP;                     // [CH][DP]
+    double* out = sb + CH * DP;                        // [3][ldA]
     const int k = blockIdx.x;
     const int tid = threadIdx.x;
     const int s = atom_struct[k];
@@ -149,10 +150,11 @@ __global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int l
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[u][i][j] = 0.0;
-    // linear items: (x, d) = tid + 256 v over 3 * D
-    double lin[3] = {0.0, 0.0, 0.0};
-    for (int c0 = 0; c0 < nslot; c0 += ROWS_CH) {
-        const int nc = min(ROWS_CH, nslot - c0);
+    // linear items: (x, d) = tid + 256 v over 3 * D (D <= 512)
+    constexpr int NLIN = 6;
+    double lin[NLIN] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int c0 = 0; c0 < nslot; c0 += CH) {
+        const int nc = min(CH, nslot - c0);
         __syncthreads();
         for (int q = tid; q < nc * DP; q += 256) {
             const int sl = q / DP, d = q - sl * DP;
@@ -174,7 +176,7 @@ __global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int l
         }
         __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 3; ++v) {
+        for (int v = 0; v < NLIN; ++v) {
             const int it = tid + 256 * v;
             if (it < 3 * D) {
                 const int x = it / D, d = it - x * D;
@@ -207,7 +209,7 @@ __global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int l
     for (int q = tid; q < 3 * ldA; q += 256) out[q] = 0.0;
     __syncthreads();
 #pragma unroll
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < NLIN; ++v) {
         const int it = tid + 256 * v;
         if (it < 3 * D) {
             const int x = it / D, d = it - x * D;
@@ -813,6 +815,7 @@ int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F
     if (Dc != ctx->gD)
         return mld_fail(ctx, MLD_E_ARG, "mld_gram_begin D must be 1 + D (+ D(D+1)/2 for order 2, + nk for order 3)");
     if (order == 2 && Dc > 2048) return mld_fail(ctx, MLD_E_UNSUPPORTED, "quadratic rows are built for at most 2048 columns");
+    if (order == 1 && D > 512) return mld_fail(ctx, MLD_E_UNSUPPORTED, "linear rows are built for descriptors of at most 512 components");
     ctx->gram_order = order;
     if (!ctx->last_energy || !ctx->last_grad) return mld_fail(ctx, MLD_E_STATE, "mld_batch_bso4_dev with gradients first");
     if (!E || (ctx->natoms && !F_dev)) return mld_fail(ctx, MLD_E_ARG, "targets missing");

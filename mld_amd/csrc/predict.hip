@@ -19,7 +19,7 @@
 
 namespace {
 
-constexpr int DMAX = mld::NB_MAX * mld::NCH_MAX;  // 192 = 3 values per lane
+constexpr int DMAX = mld::ACE_DMAX;  // 512 = 8 values per lane
 
 struct PredictState {
     int order = 0, ncol = 0, D = 0;
@@ -97,9 +97,9 @@ __global__ void __launch_bounds__(256) slot_dot_kernel(int natoms, int D, const 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int a = blockIdx.x * 4 + wv;
     if (a >= natoms) return;
-    double gr[3];
+    double gr[DMAX / 64];
 #pragma unroll
-    for (int u = 0; u < 3; ++u) {
+    for (int u = 0; u < DMAX / 64; ++u) {
         int d = lane + 64 * u;
         gr[u] = d < D ? g[(size_t)a * D + d] : 0.0;
     }
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(256) slot_dot_kernel(int natoms, int D, const 
         const T* src = grad + row * D;
         double v = 0.0;
 #pragma unroll
-        for (int u = 0; u < 3; ++u) {
+        for (int u = 0; u < DMAX / 64; ++u) {
             int d = lane + 64 * u;
             if (d < D) v += gr[u] * (double)src[d];
         }
@@ -298,6 +298,7 @@ int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev)
         return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: forces need descriptors with gradients");
     if (d2_dev && !ps->d_mu) return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: mld_mahalanobis_set first");
     if (ps->D != ctx->D) return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: model dimension changed");
+    if (ctx->D > DMAX) return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_predict_dev: descriptor dimension above 512");
     if (ps->order == 3 && (ps->nk != ctx->nk || !ctx->last_ek || (F_dev && !ctx->last_fk)))
         return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: kernel model needs mld_kernel_po_dev features of this batch");
     const int natoms = ctx->natoms, nstruct = ctx->nstruct, D = ctx->D;
