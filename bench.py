@@ -68,6 +68,12 @@ METRIC = "atom-envs/s (desc+grads+Gram) mixed vs FP64 at stated err; serve p50/p
 #                        (pack_cmm_into_ZcmmABC, :1014-1143)
 CFMA_PAIR_REC, CFMA_PAIR_GRAD, CFMA_ATOM = 3976, 17721, 104838
 PEAK_FP32, PEAK_FP64 = 157.3, 78.6   # TFLOP/s, MI355X dense
+# HBM bytes of one bso4_kernel<float,8> launch on the first 500-structure batch of the
+# quadratic-fit workload (96,582 atoms, 3,020,350 pairs), from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE passes scaled by a 1 GiB copy (factors 2.0 and 1.0):
+# (69,043 x 2.0 + 2,035,417 x 1.0) KiB; the launch's algorithmic bytes are 2,177,392,328.
+TRAFFIC_BSO4_F32_FIT_BATCH = (69043 * 2.0 + 2035417 * 1.0) * 1024
+ALGO_BYTES_BSO4_F32_FIT_BATCH = 2177392328.0
 
 
 def algorithmic_flops(natoms: int, npairs: int) -> float:
@@ -323,6 +329,7 @@ def main_fit(args):
                 "solve": r["solve"], "rmse": r["rmse"]}
 
     ach = flops / (rmx["bso4_ms_per_fit"] * 1e-3) / 1e12
+    quad = args.workload == "fit-quadratic"
     out = {
         "metric": METRIC, "value": natoms / rmx["s_per_fit"], "unit": "atom-envs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * rmx["s_per_fit"],
@@ -337,7 +344,10 @@ def main_fit(args):
         "mixed_speedup_vs_fp64": r64["s_per_fit"] / rmx["s_per_fit"],
         "bf16_speedup_vs_fp64": r64["s_per_fit"] / res["bf16"]["s_per_fit"],
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
-                     "traffic": None, "kernel": "bso4_kernel<float,8>",
+                     "traffic": TRAFFIC_BSO4_F32_FIT_BATCH if quad else None, "kernel": "bso4_kernel<float,8>",
+                     "traffic_note": ("HBM bytes of the first 500-structure launch (profiled), "
+                                      f"{TRAFFIC_BSO4_F32_FIT_BATCH / ALGO_BYTES_BSO4_F32_FIT_BATCH:.3f} x its "
+                                      "algorithmic bytes") if quad else "not profiled for this workload",
                      "kernel_ms_per_launch": rmx["bso4_ms_per_fit"] / rmx["nbatch"],
                      "algorithmic_flops_per_step": flops,
                      "algorithmic_bytes_per_step": algorithmic_bytes(natoms, npairs, 55, 4)},
