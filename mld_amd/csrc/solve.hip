@@ -20,66 +20,97 @@ namespace {
 
 constexpr int NB = 64;
 
-// L (n x n row-major, ld) lower, factor the diagonal block k in place
-__global__ void __launch_bounds__(256) chol_diag_kernel(int n, int ld, double* __restrict__ L, int k, int* __restrict__ info)
+// L (n x n row-major, ld) lower: factor the diagonal block k in place and write the
+// inverse of the factored block to Linv[k] (64 x 64 row-major; rows and columns past n
+// carry the identity).  One wavefront: lane i owns row i of the block.
+__global__ void __launch_bounds__(64) chol_diag_kernel(int n, int ld, double* __restrict__ L, int k,
+                                                       int* __restrict__ info, double* __restrict__ Linv)
 {
     __shared__ double a[NB][NB + 1];
+    __shared__ double x[NB][NB + 1];
     const int k0 = k * NB, nb = min(NB, n - k0);
-    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
+    const int t = threadIdx.x;
+    for (int q = t; q < NB * NB; q += NB) {
         int i = q / NB, j = q % NB;
-        a[i][j] = (i < nb && j < nb) ? L[(size_t)(k0 + i) * ld + k0 + j] : 0.0;
+        a[i][j] = (i < nb && j < nb) ? (j <= i ? L[(size_t)(k0 + i) * ld + k0 + j] : 0.0) : (i == j ? 1.0 : 0.0);
     }
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        if (threadIdx.x == 0) {
+    for (int j = 0; j < NB; ++j) {
+        if (t == j) {
             double d = a[j][j];
             if (!(d > 0.0)) {
                 if (*info == 0) *info = k0 + j + 1;
-                d = 1e-300;  // keep going without NaN; the caller reports the failure
+                d = 1e-300;  // keep going; the caller reports the failure
             }
             a[j][j] = sqrt(d);
         }
         __syncthreads();
-        double djj = a[j][j];
-        for (int i = j + 1 + threadIdx.x; i < nb; i += blockDim.x) a[i][j] /= djj;
-        __syncthreads();
-        for (int q = threadIdx.x; q < (nb - j - 1) * (nb - j - 1); q += blockDim.x) {
-            int i = j + 1 + q / (nb - j - 1), c = j + 1 + q % (nb - j - 1);
-            if (c <= i) a[i][c] -= a[i][j] * a[c][j];
+        double lij = 0.0;
+        if (t > j) {
+            lij = a[t][j] / a[j][j];
+            a[t][j] = lij;
         }
         __syncthreads();
+        if (t > j)
+            for (int c = j + 1; c <= t; ++c) a[t][c] -= lij * a[c][j];
+        __syncthreads();
     }
-    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
+    // column t of the inverse by forward substitution (lane t touches only column t of x)
+    for (int i = 0; i < NB; ++i) {
+        double v = 0.0;
+        if (i == t) {
+            v = 1.0 / a[i][i];
+        } else if (i > t) {
+            double s = 0.0;
+            for (int c = t; c < i; ++c) s += a[i][c] * x[c][t];
+            v = -s / a[i][i];
+        }
+        x[i][t] = v;
+    }
+    __syncthreads();
+    double* Lk = Linv + (size_t)k * NB * NB;
+    for (int q = t; q < NB * NB; q += NB) {
         int i = q / NB, j = q % NB;
-        if (i < nb && j < nb && j <= i) L[(size_t)(k0 + i) * ld + k0 + j] = a[i][j];
+        Lk[q] = x[i][j];
+        if (i < nb && j <= i) L[(size_t)(k0 + i) * ld + k0 + j] = a[i][j];
     }
 }
 
-// panel: rows below block k: X L_kk^T = A_ik  (one block of 64 rows per workgroup)
-__global__ void __launch_bounds__(256) chol_panel_kernel(int n, int ld, double* __restrict__ L, int k)
+// panel below block k: X = A_ik L_kk^-T, one 64-row block per workgroup, from the
+// inverse of the diagonal block (4 x 4 outputs per thread)
+__global__ void __launch_bounds__(256) chol_panel_kernel(int n, int ld, double* __restrict__ L, int k,
+                                                         const double* __restrict__ Linv)
 {
-    __shared__ double lk[NB][NB + 1];
-    __shared__ double x[NB][NB + 1];
+    __shared__ double ai[NB][NB + 1];
+    __shared__ double li[NB][NB + 1];
     const int k0 = k * NB, nb = min(NB, n - k0);
     const int i0 = (k + 1 + blockIdx.x) * NB, mb = min(NB, n - i0);
+    const double* Lk = Linv + (size_t)k * NB * NB;
     for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
         int i = q / NB, j = q % NB;
-        lk[i][j] = (i < nb && j < nb) ? L[(size_t)(k0 + i) * ld + k0 + j] : 0.0;
-        x[i][j] = (i < mb && j < nb) ? L[(size_t)(i0 + i) * ld + k0 + j] : 0.0;
+        ai[i][j] = (i < mb && j < nb) ? L[(size_t)(i0 + i) * ld + k0 + j] : 0.0;
+        li[i][j] = Lk[q];
     }
     __syncthreads();
-    // row r of X: x[r][j] = (a[r][j] - sum_{c<j} x[r][c] lk[j][c]) / lk[j][j]
-    for (int r = threadIdx.x; r < mb; r += blockDim.x)
-        for (int j = 0; j < nb; ++j) {
-            double s = x[r][j];
-            for (int c = 0; c < j; ++c) s -= x[r][c] * lk[j][c];
-            x[r][j] = s / lk[j][j];
+    // X[r][j] = sum_c A[r][c] Linv[j][c]
+    const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+    double acc[4][4] = {};
+    for (int c = 0; c < NB; ++c) {
+        double u4[4], v4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { u4[u] = ai[ty * 4 + u][c]; v4[u] = li[tx * 4 + u][c]; }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += u4[u] * v4[v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int i = ty * 4 + u, j = tx * 4 + v;
+            if (i < mb && j < nb) L[(size_t)(i0 + i) * ld + k0 + j] = acc[u][v];
         }
-    __syncthreads();
-    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
-        int i = q / NB, j = q % NB;
-        if (i < mb && j < nb) L[(size_t)(i0 + i) * ld + k0 + j] = x[i][j];
-    }
 }
 
 // trailing update A_ij -= L_ik L_jk^T for k < j <= i (lower tiles)
@@ -120,33 +151,57 @@ __global__ void __launch_bounds__(256) chol_update_kernel(int n, int ld, double*
         }
 }
 
-// one workgroup: forward (trans = 0, L z = b) or backward (trans = 1, L^T x = z) substitution, in place
-__global__ void __launch_bounds__(256) trsv_kernel(int n, int ld, const double* __restrict__ L, double* __restrict__ b, int trans)
+// One block step of a triangular solve with the inverted diagonal blocks.
+// Forward (trans = 0, L z = b), step k = 0 .. nb-1: x_k = Linv_k b_k; workgroup g >= 1
+// updates row block k + g: b_i -= L_ik x_k.  Backward (trans = 1, L^T x = b), step
+// k = nb-1 .. 0: x_k = Linv_k^T b_k; workgroup g >= 1 updates row block k - g:
+// b_i -= L_ki^T x_k.  Workgroup 0 writes x_k.  b_k is read, never written, in step k.
+__global__ void __launch_bounds__(256) trsv_step_kernel(int n, int ld, const double* __restrict__ L,
+                                                        const double* __restrict__ Linv, double* __restrict__ b,
+                                                        double* __restrict__ x, int k, int trans)
 {
-    __shared__ double red[256];
-    __shared__ double xv;
+    __shared__ double bk[NB];
+    __shared__ double xk[NB];
+    __shared__ double part[4][NB];
+    const int k0 = k * NB, nb = min(NB, n - k0);
+    const int t = threadIdx.x;
+    if (t < NB) bk[t] = t < nb ? b[k0 + t] : 0.0;
+    __syncthreads();
+    const double* Lk = Linv + (size_t)k * NB * NB;
+    {
+        const int r = t & (NB - 1), qd = t >> 6;
+        double s = 0.0;
+        for (int c = qd * 16; c < qd * 16 + 16; ++c) s += (trans ? Lk[c * NB + r] : Lk[r * NB + c]) * bk[c];
+        part[qd][r] = s;
+    }
+    __syncthreads();
+    if (t < NB) xk[t] = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        if (t < nb) x[k0 + t] = xk[t];
+        return;
+    }
     if (!trans) {
-        for (int i = 0; i < n; ++i) {
-            double s = 0.0;
-            for (int c = threadIdx.x; c < i; c += blockDim.x) s += L[(size_t)i * ld + c] * b[c];
-            red[threadIdx.x] = s;
-            __syncthreads();
-            for (int o = 128; o > 0; o >>= 1) {
-                if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-                __syncthreads();
-            }
-            if (threadIdx.x == 0) b[i] = (b[i] - red[0]) / L[(size_t)i * ld + i];
-            __syncthreads();
+        const int i0 = (k + blockIdx.x) * NB, mi = min(NB, n - i0);
+        const int r = t >> 2, qd = t & 3;
+        double s = 0.0;
+        if (r < mi) {
+            const double* row = L + (size_t)(i0 + r) * ld + k0 + qd * 16;
+            for (int c = 0; c < 16; ++c) s += row[c] * xk[qd * 16 + c];
         }
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if (r < mi && qd == 0) b[i0 + r] -= s;
     } else {
-        for (int i = n - 1; i >= 0; --i) {
-            if (threadIdx.x == 0) xv = b[i] / L[(size_t)i * ld + i];
-            __syncthreads();
-            double xi = xv;
-            for (int c = threadIdx.x; c < i; c += blockDim.x) b[c] -= L[(size_t)i * ld + c] * xi;
-            if (threadIdx.x == 0) b[i] = xi;
-            __syncthreads();
-        }
+        const int i0 = (k - blockIdx.x) * NB;  // a full block: i < k
+        const int r = t & (NB - 1), qd = t >> 6;
+        double s = 0.0;
+        for (int c = qd * 16; c < qd * 16 + 16; ++c)
+            if (c < nb) s += L[(size_t)(k0 + c) * ld + i0 + r] * xk[c];
+        __syncthreads();
+        part[qd][r] = s;
+        __syncthreads();
+        if (t < NB) b[i0 + t] -= (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
     }
 }
 
@@ -325,7 +380,9 @@ extern "C" int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w
     size_t nvec = (size_t)ld;
     size_t maxM = 1;
     for (int M : ctx->kept_M) maxM = std::max(maxM, (size_t)M);
-    size_t need = bytes_L + 8 * nvec * sizeof(double) + maxM * sizeof(double) + 256 * nvec * sizeof(double) + 4096;
+    size_t bytes_inv = nb * NB * NB * sizeof(double);  // inverted diagonal blocks of the factor
+    size_t need = bytes_L + 8 * nvec * sizeof(double) + maxM * sizeof(double) + 256 * nvec * sizeof(double) +
+                  bytes_inv + 4096;
     if (need > ctx->cap_work) {
         if (ctx->d_work) MLD_HIP(ctx, hipFree(ctx->d_work));
         ctx->d_work = nullptr;
@@ -340,9 +397,10 @@ extern "C" int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w
     double* r = (double*)base; base += nvec * sizeof(double);
     double* d = (double*)base; base += nvec * sizeof(double);
     int* info = (int*)base; base += nvec * sizeof(double);
-    base += 2 * nvec * sizeof(double);
+    double* tmpv = (double*)base; base += 2 * nvec * sizeof(double);
     double* tbuf = (double*)base; base += maxM * sizeof(double);
-    double* part = (double*)base;
+    double* part = (double*)base; base += 256 * nvec * sizeof(double);
+    double* Linv = (double*)base;
     MLD_HIP(ctx, hipMemcpyAsync(dridge, ridge.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
     // Factorise M = S^-1 (S (G + Lambda) S + shift I) S^-1.  The FP64 Gram is factorised
     // as is (shift 0); a split-BF16 Gram carries a relative error of about 1e-6 of its
@@ -355,10 +413,10 @@ extern "C" int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w
         MLD_HIP(ctx, hipMemsetAsync(L, 0, bytes_L, st));
         hipLaunchKernelGGL(prep_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, dridge, shift, L, S);
         for (size_t k = 0; k < nb; ++k) {
-            hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, st, n, ld, L, (int)k, info);
+            hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(NB), 0, st, n, ld, L, (int)k, info, Linv);
             int nrem = (int)(nb - k - 1);
             if (nrem > 0) {
-                hipLaunchKernelGGL(chol_panel_kernel, dim3(nrem), dim3(256), 0, st, n, ld, L, (int)k);
+                hipLaunchKernelGGL(chol_panel_kernel, dim3(nrem), dim3(256), 0, st, n, ld, L, (int)k, Linv);
                 hipLaunchKernelGGL(chol_update_kernel, dim3(nrem * (nrem + 1) / 2), dim3(256), 0, st, n, ld, L, (int)k, nrem);
             }
         }
@@ -374,8 +432,10 @@ extern "C" int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w
     }
     auto precond = [&](double* v) {  // v <- M^-1 v using S L L^T S
         hipLaunchKernelGGL(scale_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, S, v);
-        hipLaunchKernelGGL(trsv_kernel, dim3(1), dim3(256), 0, st, n, ld, L, v, 0);
-        hipLaunchKernelGGL(trsv_kernel, dim3(1), dim3(256), 0, st, n, ld, L, v, 1);
+        for (int k = 0; k < (int)nb; ++k)  // L z = v: z into tmpv, v is scratch
+            hipLaunchKernelGGL(trsv_step_kernel, dim3((int)nb - k), dim3(256), 0, st, n, ld, L, Linv, v, tmpv, k, 0);
+        for (int k = (int)nb - 1; k >= 0; --k)  // L^T v = z: tmpv is scratch
+            hipLaunchKernelGGL(trsv_step_kernel, dim3(k + 1), dim3(256), 0, st, n, ld, L, Linv, tmpv, v, k, 1);
         hipLaunchKernelGGL(scale_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, S, v);
     };
     auto to_host = [&](std::vector<double>& hv, const double* dv) -> int {
