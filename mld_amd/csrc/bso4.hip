@@ -15,6 +15,8 @@
 //      dB_bi/dr_k = Re sum_{j,e} W_j[bi][e] V_j[e]      (:505-652, sign flip of
 //      descriptors.F90:298-302 folded in), self slot = -sum_k (:692-695).
 // Precision T = double (plain FP64 form) or float (FP32 form; set-up geometry in FP64).
+#include <algorithm>
+
 #include "mld_internal.h"
 
 namespace {
@@ -187,6 +189,76 @@ __device__ __forceinline__ Geo<T> make_geo(double4 d, double rcut, double r0, do
     return g;
 }
 
+// Packed per-pair geometry: zp, xp, dzp[3], dxp[3] (complex), fc, dfc, rhat[3]; zm, xm,
+// dzm, dxm are their conjugates (make_geo builds them with the same magnitudes and
+// opposite signs).  Written once per pair by pair_geo_kernel, read by every wavefront
+// that walks the pair.
+constexpr int GEO_STRIDE = 24;
+
+template <typename T>
+__device__ __forceinline__ Geo<T> load_geo(const T* __restrict__ o, bool grad)
+{
+    Geo<T> g;
+    g.zp = cx<T>{o[0], o[1]};
+    g.zm = cx<T>{o[0], -o[1]};
+    g.xp = cx<T>{o[2], o[3]};
+    g.xm = cx<T>{o[2], -o[3]};
+    g.fc = o[16];
+    g.dfc = o[17];
+    g.drr[0] = o[18];
+    g.drr[1] = o[19];
+    g.drr[2] = o[20];
+    if (grad) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            g.dzp[k] = cx<T>{o[4 + 2 * k], o[5 + 2 * k]};
+            g.dzm[k] = cx<T>{o[4 + 2 * k], -o[5 + 2 * k]};
+            g.dxp[k] = cx<T>{o[10 + 2 * k], o[11 + 2 * k]};
+            g.dxm[k] = cx<T>{o[10 + 2 * k], -o[11 + 2 * k]};
+        }
+    }
+    return g;
+}
+
+// One wavefront per atom, lanes over its neighbours: the FP64 trigonometry of
+// make_geo runs once per pair instead of once per wavefront and phase.
+template <typename T>
+__global__ void __launch_bounds__(256) pair_geo_kernel(int natoms, const int* __restrict__ pair_off,
+                                                       const int* __restrict__ pair_idx,
+                                                       const double4* __restrict__ pair_d,
+                                                       const int* __restrict__ itype,
+                                                       const double* __restrict__ rin_tab, int nsp, double win,
+                                                       double rcut, double r0, T* __restrict__ geo)
+{
+    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a >= natoms) return;
+    const int lane = threadIdx.x & 63;
+    const int p1 = pair_off[a + 1];
+    const int ti = itype[a];
+    for (int p = pair_off[a] + lane; p < p1; p += 64) {
+        const double rin = rin_tab ? rin_tab[ti * nsp + itype[pair_idx[p]]] : 0.0;
+        const Geo<T> g = make_geo<T>(pair_d[p], rcut, r0, rin, win, true);
+        T* o = geo + (size_t)p * GEO_STRIDE;
+        o[0] = g.zp.r;
+        o[1] = g.zp.i;
+        o[2] = g.xp.r;
+        o[3] = g.xp.i;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            o[4 + 2 * k] = g.dzp[k].r;
+            o[5 + 2 * k] = g.dzp[k].i;
+            o[10 + 2 * k] = g.dxp[k].r;
+            o[11 + 2 * k] = g.dxp[k].i;
+        }
+        o[16] = g.fc;
+        o[17] = g.dfc;
+        o[18] = g.drr[0];
+        o[19] = g.drr[1];
+        o[20] = g.drr[2];
+        o[21] = o[22] = o[23] = (T)0;
+    }
+}
+
 // One recursion step for entry (a, b) of level j from level j-1 (Up, dUp).
 template <typename T, bool GRAD>
 __device__ __forceinline__ void rec_step(const RecEntry& R, int b, const Geo<T>& g, const cx<T>* Up,
@@ -271,10 +343,9 @@ struct Smem {
 template <typename T, int JJ>
 __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, const int* __restrict__ pair_off,
                                                   const int* __restrict__ pair_idx,
-                                                  const double4* __restrict__ pair_d,
+                                                  const T* __restrict__ geo,
                                                   const int* __restrict__ itype, const double* __restrict__ wfac,
-                                                  const double* __restrict__ rin_tab, int nsp, double win,
-                                                  double rcut, double r0, int nch, int want_grad,
+                                                  int nsp, int nch, int want_grad,
                                                   T* __restrict__ energy, T* __restrict__ grad)
 {
     using S = Smem<T, JJ>;
@@ -286,14 +357,13 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
     const int g_atom = blockIdx.x;
     if (g_atom >= natoms) return;
     const int tid = threadIdx.x;
-    const int w = tid >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wavefront index (uniform)
     const int lane = tid & 63;
     const int p0 = pair_off[g_atom];
     const int nn = pair_off[g_atom + 1] - p0;
     const int nb = tb.nb;
     const int D = nb * nch;
     const size_t base = (size_t)p0 + g_atom;  // slot 0 of this atom
-    const int ti = itype[g_atom];
 
     if (nn == 0) {
         for (int d = tid; d < D; d += BS) energy[(size_t)g_atom * D + d] = (T)0;
@@ -311,10 +381,8 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
         // wavefront's cpart row without conflicts
         for (int e = lane; e < NU; e += 64) sm.u.a.cpart[w][e] = cx<T>{(T)0, (T)0};
         for (int n = w; n < nn; n += NW) {
-            double4 d = pair_d[p0 + n];
-            int tj = itype[pair_idx[p0 + n]];
-            double rin = rin_tab ? rin_tab[ti * nsp + tj] : 0.0;
-            Geo<T> g = make_geo<T>(d, rcut, r0, rin, win, false);
+            const int tj = itype[pair_idx[p0 + n]];
+            const Geo<T> g = load_geo<T>(geo + (size_t)(p0 + n) * GEO_STRIDE, false);
             T fw = g.fc * (T)wfac[ch * nsp + tj];
             cx<T>* cp = sm.u.a.cpart[w];
             if (lane == 0) {
@@ -382,16 +450,23 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                 const cx<T>* c2 = sm.c + mld::uoff(J2);
                 const int ialo = (max(-J1, m1 - J2) + J1) >> 1, iahi = (min(J1, m1 + J2) + J1) >> 1;
                 const int iblo = (max(-J1, m2 - J2) + J1) >> 1, ibhi = (min(J1, m2 + J2) + J1) >> 1;
-                const int nbq = ibhi - iblo + 1;
-                const int nterm = (iahi - ialo + 1) * nbq;
+                // c2 index (ka, kb) = ((m1 + J1 + J2) / 2 - ia, (m2 + J1 + J2) / 2 - ib)
+                const int ka0 = (m1 + J1 + J2) >> 1, kb0 = (m2 + J1 + J2) >> 1;
                 cx<T> sacc{(T)0, (T)0};
-                for (int t = 0; t < nterm; ++t) {
-                    const int ia = ialo + t / nbq, ib = iblo + t % nbq;
-                    const int ka = (m1 + J1 - 2 * ia + J2) >> 1, kb = (m2 + J1 - 2 * ib + J2) >> 1;
-                    const T cgab = cg[ia * (j + 1) + A] * cg[ib * (j + 1) + B];
-                    cx<T> pr = cmul(c1[ia * (J1 + 1) + ib], c2[ka * (J2 + 1) + kb]);
-                    sacc.r += cgab * pr.r;
-                    sacc.i += cgab * pr.i;
+                for (int ia = ialo; ia <= iahi; ++ia) {
+                    const cx<T>* c1r = c1 + ia * (J1 + 1);
+                    const cx<T>* c2r = c2 + (ka0 - ia) * (J2 + 1) + kb0;
+                    const T* cgb = cg + B;
+                    cx<T> s{(T)0, (T)0};
+                    for (int ib = iblo; ib <= ibhi; ++ib) {
+                        const T cb = cgb[ib * (j + 1)];
+                        const cx<T> pr = cmul(c1r[ib], c2r[-ib]);
+                        s.r += cb * pr.r;
+                        s.i += cb * pr.i;
+                    }
+                    const T ca = cg[ia * (j + 1) + A];
+                    sacc.r += ca * s.r;
+                    sacc.i += ca * s.i;
                 }
                 wv = cadd(wv, cscale((T)zt.scale, sacc));
             }
@@ -440,10 +515,8 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
         if constexpr (!S::CG_BY_LEVEL)
             for (int q = lane; q < nb * 3; q += 64) selfw[q] = (T)0;
         for (int n = w; n < nn; n += NW) {
-            double4 d = pair_d[p0 + n];
-            int tj = itype[pair_idx[p0 + n]];
-            double rin = rin_tab ? rin_tab[ti * nsp + tj] : 0.0;
-            Geo<T> g = make_geo<T>(d, rcut, r0, rin, win, true);
+            const int tj = itype[pair_idx[p0 + n]];
+            const Geo<T> g = load_geo<T>(geo + (size_t)(p0 + n) * GEO_STRIDE, true);
             const T wn = (T)wfac[ch * nsp + tj];
             T* outw = &sm.u.g.outw[w][0][0];
             for (int q = lane; q < nb * 3; q += 64) outw[q] = (T)0;
@@ -603,9 +676,16 @@ int launch_t(mld_ctx* ctx, int want_grad, void* energy, void* grad)
         attr_done = true;
     }
     if (ctx->natoms == 0) return MLD_OK;
+    const size_t npairs = (size_t)std::max<long long>(ctx->npairs, 1);
+    int rc = device_ensure(ctx, &ctx->d_pgeo, &ctx->cap_pgeo, npairs * GEO_STRIDE * sizeof(T));
+    if (rc) return rc;
+    T* geo = (T*)ctx->d_pgeo;
+    hipLaunchKernelGGL(pair_geo_kernel<T>, dim3((ctx->natoms + 3) / 4), dim3(256), 0, ctx->stream, ctx->natoms,
+                       ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_itype, ctx->d_rin, ctx->nsp,
+                       ctx->p.r_cut_width_in, ctx->p.r_cut, ctx->r0, geo);
     hipLaunchKernelGGL(k, dim3(ctx->natoms), dim3(BS), sh, ctx->stream, ctx->dt, ctx->natoms, ctx->d_pair_off,
-                       ctx->d_pair_idx, ctx->d_pair_d, ctx->d_itype, ctx->d_wfac, ctx->d_rin, ctx->nsp,
-                       ctx->p.r_cut_width_in, ctx->p.r_cut, ctx->r0, ctx->nch, want_grad, (T*)energy, (T*)grad);
+                       ctx->d_pair_idx, (const T*)geo, ctx->d_itype, ctx->d_wfac, ctx->nsp, ctx->nch, want_grad,
+                       (T*)energy, (T*)grad);
     MLD_HIP(ctx, hipGetLastError());
     return MLD_OK;
 }

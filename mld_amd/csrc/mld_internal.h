@@ -198,6 +198,7 @@ struct mld_ctx {
     std::vector<double*> kept_A, kept_y;
     std::vector<int> kept_M;
     void* d_work = nullptr; size_t cap_work = 0;
+    void* d_pgeo = nullptr; size_t cap_pgeo = 0;  // per-pair SO(4) geometry of the last batch (bso4.hip)
     // kernel_po state
     int nk = 0;
     double k_sigma = 0, k_length = 1, k_power = 2;
