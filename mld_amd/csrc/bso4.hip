@@ -313,25 +313,27 @@ struct Smem {
     static constexpr int NU = mld::ucount(JJ);
     static constexpr int NL = mld::hcount(JJ);  // canonical half of the largest level
     static constexpr int WMAX = wsize_bound(JJ);
-    // FP32: every CG1 table is staged at once (fits); FP64: one output level at a time,
-    // which keeps the workgroup under 80 KB of LDS (two workgroups per CU)
-    static constexpr bool CG_BY_LEVEL = sizeof(T) == 8;
-    static constexpr int NCG = CG_BY_LEVEL ? cg_level_max(JJ) : cg1_size(JJ);
+    // every CG1 table is staged at once for the Z -> W phase; the recursion table shares
+    // that space with it (restaged before each recursion phase), which keeps the FP64
+    // workgroup under 80 KB of LDS (two workgroups per CU)
+    static constexpr int NCG = cg1_size(JJ);
     static constexpr int HU = mld::hoff(JJ + 1);
     // FP32 accumulates the self slot per wavefront in LDS (registers are the scarcer
     // resource there); FP64 keeps it in registers (LDS is)
-    static constexpr int NSELF = CG_BY_LEVEL ? 1 : NW * mld::NB_MAX * 3;
+    static constexpr bool SELF_LDS = sizeof(T) == 4;
+    static constexpr int NSELF = SELF_LDS ? NW * mld::NB_MAX * 3 : 1;
     T selfw[NSELF];
     cx<T> c[NU];
     cx<T> W[WMAX];
-    RecEntry rec[HU];
     union {
         struct {
+            RecEntry rec[HU];
             cx<T> cpart[NW][NU];
             cx<T> ub[NW][2][NL];
         } a;
         T cg[NCG];
         struct {
+            RecEntry rec[HU];  // same offset as a.rec
             cx<T> ub[NW][2][NL];
             cx<T> dub[NW][2][NL][3];
             cx<T> vb[NW][NL][3];
@@ -341,7 +343,7 @@ struct Smem {
 };
 
 template <typename T, int JJ>
-__global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, const int* __restrict__ pair_off,
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) bso4_kernel(DevTables tb, int natoms, const int* __restrict__ pair_off,
                                                   const int* __restrict__ pair_idx,
                                                   const T* __restrict__ geo,
                                                   const int* __restrict__ itype, const double* __restrict__ wfac,
@@ -372,14 +374,13 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
         return;
     }
 
-    for (int e = tid; e < S::HU; e += BS) sm.rec[e] = tb.rec[e];
-    __syncthreads();
-
     for (int ch = 0; ch < nch; ++ch) {
         // ---------------- A: density ----------------
         // each lane owns fixed entries of every level, so it accumulates them in its
         // wavefront's cpart row without conflicts
+        for (int e = tid; e < S::HU; e += BS) sm.u.a.rec[e] = tb.rec[e];
         for (int e = lane; e < NU; e += 64) sm.u.a.cpart[w][e] = cx<T>{(T)0, (T)0};
+        __syncthreads();
         for (int n = w; n < nn; n += NW) {
             const int tj = itype[pair_idx[p0 + n]];
             const Geo<T> g = load_geo<T>(geo + (size_t)(p0 + n) * GEO_STRIDE, false);
@@ -389,7 +390,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                 sm.u.a.ub[w][0][0] = cx<T>{(T)1, (T)0};
                 cp[0].r += fw;
             }
-#pragma unroll 1
+#pragma unroll
             for (int j = 1; j <= JJ; ++j) {
                 wave_sync();
                 const cx<T>* Up = sm.u.a.ub[w][(j - 1) & 1];
@@ -400,7 +401,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                     int e = lane + 64 * s2;
                     if (e < K) {
                         int b = e % (j + 1);
-                        RecEntry R = sm.rec[mld::hoff(j) + e];
+                        RecEntry R = sm.u.a.rec[mld::hoff(j) + e];
                         cx<T> u;
                         rec_step<T, false>(R, b, g, Up, nullptr, u, nullptr);
                         Un[e] = u;
@@ -474,19 +475,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
             const T pw = (2 * e + 1 == (j + 1) * (j + 1)) ? (T)1 : (T)2;
             sm.W[tb.wofs[j] + e * M + r] = cscale(pw, wv);
         };
-        if constexpr (S::CG_BY_LEVEL) {
-            // level by level: stage the level's CG tables, then its W entries, the k-th
-            // task of every thread costing about the same
-#pragma unroll 1
-            for (int j = 0; j <= JJ; ++j) {
-                const int cg0 = tb.cgbase[j], ncg = tb.cgbase[j + 1] - cg0;
-                for (int q = tid; q < ncg; q += BS) sm.u.cg[q] = (T)tb.cg1[cg0 + q];
-                __syncthreads();
-                const int t0 = tb.tbase[j], nt = tb.tbase[j + 1] - t0;
-                for (int k = 0; k * BS + tid < nt; ++k) w_task(tb.tasks[t0 + k * BS + tid], cg0);
-                __syncthreads();
-            }
-        } else {
+        {
             for (int q = tid; q < tb.cgbase[JJ + 1]; q += BS) sm.u.cg[q] = (T)tb.cg1[q];
             __syncthreads();
             for (int k = 0; k * BS + tid < tb.ntask; ++k) w_task(tb.tasks_all[k * BS + tid], 0);
@@ -511,9 +500,11 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
 
         // ---------------- C: gradients ----------------
         T selfacc[3] = {(T)0, (T)0, (T)0};  // -sum over this wavefront's neighbours, q = lane + 64 u
-        T* selfw = &sm.selfw[S::CG_BY_LEVEL ? 0 : w * mld::NB_MAX * 3];
-        if constexpr (!S::CG_BY_LEVEL)
+        T* selfw = &sm.selfw[S::SELF_LDS ? w * mld::NB_MAX * 3 : 0];
+        if constexpr (S::SELF_LDS)
             for (int q = lane; q < nb * 3; q += 64) selfw[q] = (T)0;
+        for (int e = tid; e < S::HU; e += BS) sm.u.g.rec[e] = tb.rec[e];
+        __syncthreads();
         for (int n = w; n < nn; n += NW) {
             const int tj = itype[pair_idx[p0 + n]];
             const Geo<T> g = load_geo<T>(geo + (size_t)(p0 + n) * GEO_STRIDE, true);
@@ -521,8 +512,9 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
             T* outw = &sm.u.g.outw[w][0][0];
             for (int q = lane; q < nb * 3; q += 64) outw[q] = (T)0;
             cx<T>* vb = &sm.u.g.vb[w][0][0];
-            // level 0: U = 1, dU = 0, V = dfcut rhat (the level loop below is not unrolled
-            // to keep the register footprint at two wavefronts per SIMD)
+            // level 0: U = 1, dU = 0, V = dfcut rhat.  The level loop below is unrolled (level
+            // sizes become constants); the FP32 kernel is held to 168 VGPRs, three wavefronts
+            // per SIMD, which its LDS footprint allows anyway
             if (lane == 0) {
                 sm.u.g.ub[w][0][0] = cx<T>{(T)1, (T)0};
 #pragma unroll
@@ -531,7 +523,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                     vb[k] = cx<T>{g.dfc * g.drr[k], (T)0};
                 }
             }
-#pragma unroll 1
+#pragma unroll
             for (int j = 0; j <= JJ; ++j) {
                 if (j > 0) {
                     wave_sync();
@@ -546,7 +538,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                         int e = lane + 64 * s;
                         if (e < mld::hcount(j)) {
                             int b = e % (j + 1);
-                            RecEntry R = sm.rec[mld::hoff(j) + e];
+                            RecEntry R = sm.u.g.rec[mld::hoff(j) + e];
                             rec_step<T, true>(R, b, g, Up, dUp, us[s], dus[s]);
                         }
                     }
@@ -567,12 +559,12 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
                     }
                 }
                 wave_sync();
-                // contraction of level j: the K_j entries are cut into NS slices;
-                // lane r + M*s sums slice s of row r (two interleaved partial sums
-                // per direction), then the slices are added in slice order.
                 const int M = tb.M[j];
                 const int K = mld::hcount(j);
                 const cx<T>* Wj = sm.W + tb.wofs[j];
+                // contraction of level j: the K_j entries are cut into NS slices;
+                // lane r + M*s sums slice s of row r (two interleaved partial sums
+                // per direction), then the slices are added in slice order.
                 const int NS = (M > 0) ? max(1, min(4, 64 / M)) : 1;
                 const int slice = (K + NS - 1) / NS;
                 const int r = lane % max(M, 1), sidx = lane / max(M, 1);
@@ -622,7 +614,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
             }
             // flush neighbour n: grad slot (base + 1 + n), layout [slot][x][D]
             T* gout = grad + (base + 1 + n) * 3 * (size_t)D + ch * nb;
-            if constexpr (S::CG_BY_LEVEL) {
+            if constexpr (!S::SELF_LDS) {
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
                     const int q = lane + 64 * u;
@@ -648,7 +640,7 @@ __global__ void __launch_bounds__(BS) bso4_kernel(DevTables tb, int natoms, cons
             T* outw = &sm.u.g.outw[w][0][0];
 #pragma unroll
             for (int u = 0; u < 3; ++u)
-                if (lane + 64 * u < nb * 3) outw[lane + 64 * u] = S::CG_BY_LEVEL ? selfacc[u] : selfw[lane + 64 * u];
+                if (lane + 64 * u < nb * 3) outw[lane + 64 * u] = S::SELF_LDS ? selfw[lane + 64 * u] : selfacc[u];
         }
         __syncthreads();
         // self slot = -sum of the neighbour slots (compute_bispectrum_so4.F90:692-695), the
