@@ -275,21 +275,32 @@ __device__ __forceinline__ void rec_step(const RecEntry& R, int b, const Geo<T>&
     cx<T> U1 = Up[i1], U2 = Up[i2];
     U1 = cx<T>{ng1 * U1.r, ng1 * cj1 * U1.i};
     U2 = cx<T>{ng2 * U2.r, ng2 * cj2 * U2.i};
-    cx<T> t1 = cscale(c1, zq);
-    cx<T> t2 = cx<T>{-c2 * xq.i, c2 * xq.r};  // (i c2) xq
-    u = csub(cmul(t1, U1), cmul(t2, U2));
+    const cx<T> t1 = cscale(c1, zq);
+    const cx<T> t2 = cx<T>{-c2 * xq.i, c2 * xq.r};  // (i c2) xq
+    // u = t1 U1 - t2 U2 as one chain of fused multiply-adds per component
+    u.r = fma(t1.r, U1.r, fma(-t1.i, U1.i, fma(-t2.r, U2.r, t2.i * U2.i)));
+    u.i = fma(t1.r, U1.i, fma(t1.i, U1.r, fma(-t2.r, U2.i, -(t2.i * U2.r))));
     if (GRAD) {
-        cx<T> ic2 = cx<T>{(T)0, c2};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            cx<T> dzk = k0 ? g.dzp[k] : g.dzm[k];
-            cx<T> dxk = k0 ? g.dxm[k] : g.dxp[k];
+            const cx<T> dzk = k0 ? g.dzp[k] : g.dzm[k];
+            const cx<T> dxk = k0 ? g.dxm[k] : g.dxp[k];
             cx<T> dU1 = dUp[3 * i1 + k], dU2 = dUp[3 * i2 + k];
             dU1 = cx<T>{ng1 * dU1.r, ng1 * cj1 * dU1.i};
             dU2 = cx<T>{ng2 * dU2.r, ng2 * cj2 * dU2.i};
-            cx<T> a = cadd(cmul(dzk, U1), cmul(zq, dU1));
-            cx<T> bq = cadd(cmul(dxk, U2), cmul(xq, dU2));
-            du[k] = csub(cscale(c1, a), cmul(ic2, bq));
+            // du = c1 (dz U1 + zq dU1) - i c2 (dx U2 + xq dU2)
+            //    = (c1 dz) U1 + t1 dU1 - (i c2 dx) U2 - t2 dU2
+            const cx<T> a1 = cscale(c1, dzk);
+            const cx<T> a2 = cx<T>{-c2 * dxk.i, c2 * dxk.r};
+            T r = fma(a1.r, U1.r, -(a1.i * U1.i));
+            r = fma(t1.r, dU1.r, fma(-t1.i, dU1.i, r));
+            r = fma(-a2.r, U2.r, fma(a2.i, U2.i, r));
+            r = fma(-t2.r, dU2.r, fma(t2.i, dU2.i, r));
+            T im = fma(a1.r, U1.i, a1.i * U1.r);
+            im = fma(t1.r, dU1.i, fma(t1.i, dU1.r, im));
+            im = fma(-a2.r, U2.i, fma(-a2.i, U2.r, im));
+            im = fma(-t2.r, dU2.i, fma(-t2.i, dU2.r, im));
+            du[k] = cx<T>{r, im};
         }
     }
 }
@@ -576,19 +587,19 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                         cx<T> wv = Wj[e * M + r], wu = Wj[(e + 1) * M + r];
                         cx<T> v0 = vb[3 * e], v1 = vb[3 * e + 1], v2 = vb[3 * e + 2];
                         cx<T> u0 = vb[3 * e + 3], u1 = vb[3 * e + 4], u2 = vb[3 * e + 5];
-                        a0 += wv.r * v0.r - wv.i * v0.i;
-                        a1 += wv.r * v1.r - wv.i * v1.i;
-                        a2 += wv.r * v2.r - wv.i * v2.i;
-                        b0 += wu.r * u0.r - wu.i * u0.i;
-                        b1 += wu.r * u1.r - wu.i * u1.i;
-                        b2 += wu.r * u2.r - wu.i * u2.i;
+                        a0 = fma(wv.r, v0.r, fma(-wv.i, v0.i, a0));
+                        a1 = fma(wv.r, v1.r, fma(-wv.i, v1.i, a1));
+                        a2 = fma(wv.r, v2.r, fma(-wv.i, v2.i, a2));
+                        b0 = fma(wu.r, u0.r, fma(-wu.i, u0.i, b0));
+                        b1 = fma(wu.r, u1.r, fma(-wu.i, u1.i, b1));
+                        b2 = fma(wu.r, u2.r, fma(-wu.i, u2.i, b2));
                     }
                     if (e < e1) {
                         cx<T> wv = Wj[e * M + r];
                         cx<T> v0 = vb[3 * e], v1 = vb[3 * e + 1], v2 = vb[3 * e + 2];
-                        a0 += wv.r * v0.r - wv.i * v0.i;
-                        a1 += wv.r * v1.r - wv.i * v1.i;
-                        a2 += wv.r * v2.r - wv.i * v2.i;
+                        a0 = fma(wv.r, v0.r, fma(-wv.i, v0.i, a0));
+                        a1 = fma(wv.r, v1.r, fma(-wv.i, v1.i, a1));
+                        a2 = fma(wv.r, v2.r, fma(-wv.i, v2.i, a2));
                     }
                     a0 += b0;
                     a1 += b1;
