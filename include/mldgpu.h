@@ -41,7 +41,7 @@ enum {
 
 enum { MLD_PREC_FP64 = 0, MLD_PREC_FP32 = 1 };
 enum { MLD_DESC_BSO4 = 9, MLD_DESC_ACE = 300 };  /* descriptor_bispectrum_so4, descriptor_ace */
-enum { MLD_SOLVE_CHOL_IR = 0, MLD_SOLVE_CHOL = 1 };
+enum { MLD_SOLVE_CHOL_IR = 0, MLD_SOLVE_CHOL = 1, MLD_SOLVE_SVD = 2 };
 
 typedef struct mld_params {
     int descriptor_type;        /* 9 = SO(4) bispectrum (ml_in_ndm_module descriptor_bispectrum_so4) */
@@ -137,8 +137,18 @@ int mld_gram_set_keep_rows(mld_ctx* ctx, int keep);
  * preconditions FP64 conjugate-gradient refinement whose operator comes from the kept
  * FP64 rows (or the FP64 Gram); a Gram accumulated with mixed = 1 is factorised with a
  * diagonal shift of 1e-6 of its diagonal (raised tenfold while a pivot fails).
+ * MLD_SOLVE_SVD: the minimum-norm solution of serial_lsystem_by_svd (stats_tools.F90:1153-1213,
+ * dgelsd with rcond = machine epsilon): eigen-decomposition of the symmetric system by
+ * cyclic Jacobi on the device, eigenvalues below eps * max |eigenvalue| dropped; with
+ * unique quadratic columns it returns the reference's full-layout minimum-norm solution
+ * folded onto those columns.  diag_out then holds [Jacobi sweeps, relative residual,
+ * smallest kept |eigenvalue|, largest |eigenvalue|].
  * diag_out (optional, 4 doubles): [iterations, final relative residual, min pivot, max pivot]. */
 int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w, double* diag_out);
+/* The MLD_SOLVE_SVD solve with an explicit relative threshold: the &input_ml key svd_rcond
+ * (read_ml_file.F90:353-354, passed to dgelsd by gram_reduce_and_solve,
+ * module_online_fit.F90:580-583); rcond < 0 means machine epsilon, as in dgelsd. */
+int mld_gram_solve_svd(mld_ctx* ctx, double lambda, double rcond, double* w, double* diag_out);
 
 /* ---- kernel_po features (compute/compute_kernel.F90:91-150, 476-866) ----
  * zref (D, nk) column-major: the standardised reference points (global_kernel).

@@ -12,6 +12,8 @@
 // when the FP64 rows were kept resident (mixed form), r = A^T (y - A w) - Lambda w
 // straight from the rows, so the refined w solves the FP64 normal equations even
 // when the factorised Gram came from split-BF16 products.
+#include <algorithm>
+#include <cfloat>
 #include <cmath>
 
 #include "mld_internal.h"
@@ -293,6 +295,139 @@ __global__ void axpy_kernel(int n, const double* __restrict__ d, double* __restr
     if (c < n) w[c] += d[c];
 }
 
+// ---- minimum-norm solve: cyclic two-sided Jacobi on the symmetric matrix ----
+// A = S (G + Lambda) S (n2 x n2, n2 = n rounded up to even, padding rows / columns zero),
+// b = S h, V = I.  S = sqrt(2) on the off-diagonal quadratic columns (the reference's
+// full D^2 layout carries those columns twice), 1 elsewhere.
+__global__ void mn_build_kernel(int n, int n2, int ld, const double* __restrict__ G, const double* __restrict__ ridge,
+                                const double* __restrict__ sc, const double* __restrict__ h, double* __restrict__ A,
+                                double* __restrict__ V, double* __restrict__ b)
+{
+    const int i = blockIdx.x;
+    for (int j = threadIdx.x; j < n2; j += blockDim.x) {
+        double a = 0.0;
+        if (i < n && j < n) {
+            const double g = (j >= i) ? G[(size_t)i * ld + j] : G[(size_t)j * ld + i];
+            a = sc[i] * (g + (i == j ? ridge[i] : 0.0)) * sc[j];
+        }
+        A[(size_t)i * n2 + j] = a;
+        V[(size_t)i * n2 + j] = (i == j) ? 1.0 : 0.0;
+    }
+    if (threadIdx.x == 0) b[i] = i < n ? sc[i] * h[i] : 0.0;
+}
+
+// round-robin pairing of n2 indices, step s in [0, n2 - 1): pair k = (pos k, pos n2-1-k),
+// position 0 fixed, the others rotated by s
+__device__ __forceinline__ int rr_player(int n2, int s, int pos)
+{
+    return pos == 0 ? 0 : 1 + (pos - 1 + s) % (n2 - 1);
+}
+
+// rotation (c, s) that zeroes A_pq of every pair of step `step` (Golub-Van Loan sym.schur2)
+__global__ void mn_params_kernel(int n2, int step, const double* __restrict__ A, double2* __restrict__ cs,
+                                 int2* __restrict__ pq)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n2 / 2) return;
+    const int p = rr_player(n2, step, k), q = rr_player(n2, step, n2 - 1 - k);
+    const double apq = A[(size_t)p * n2 + q], app = A[(size_t)p * n2 + p], aqq = A[(size_t)q * n2 + q];
+    double c = 1.0, sn = 0.0;
+    if (fabs(apq) > 1e-300 && fabs(apq) > 1e-18 * sqrt(fabs(app * aqq))) {
+        const double tau = (aqq - app) / (2.0 * apq);
+        const double at = fabs(tau);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (at + (at < 1e150 ? sqrt(1.0 + tau * tau) : at));
+        c = 1.0 / sqrt(1.0 + t * t);
+        sn = t * c;
+    }
+    cs[k] = make_double2(c, sn);
+    pq[k] = make_int2(p, q);
+}
+
+// A <- J^T A J on 2 x 2 blocks (rows of pair P, columns of pair Q), V <- V J on the
+// column pairs; every element is read and written by one thread
+__global__ void mn_rotate_kernel(int n2, double* __restrict__ A, double* __restrict__ V,
+                                 const double2* __restrict__ cs, const int2* __restrict__ pq)
+{
+    const int h2 = n2 / 2;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nA = (long long)h2 * h2;
+    if (t < nA) {
+        const int P = (int)(t / h2), Q = (int)(t % h2);
+        const int2 rp = pq[P], rq = pq[Q];
+        const double2 j1 = cs[P], j2 = cs[Q];
+        double* r1 = A + (size_t)rp.x * n2;
+        double* r2 = A + (size_t)rp.y * n2;
+        const double a00 = r1[rq.x], a01 = r1[rq.y], a10 = r2[rq.x], a11 = r2[rq.y];
+        const double x00 = j1.x * a00 - j1.y * a10, x01 = j1.x * a01 - j1.y * a11;
+        const double x10 = j1.y * a00 + j1.x * a10, x11 = j1.y * a01 + j1.x * a11;
+        r1[rq.x] = x00 * j2.x - x01 * j2.y;
+        r1[rq.y] = x00 * j2.y + x01 * j2.x;
+        r2[rq.x] = x10 * j2.x - x11 * j2.y;
+        r2[rq.y] = x10 * j2.y + x11 * j2.x;
+    } else if (t < nA + (long long)n2 * h2) {
+        const long long u = t - nA;
+        const int i = (int)(u / h2), Q = (int)(u % h2);
+        const int2 rq = pq[Q];
+        const double2 j2 = cs[Q];
+        double* row = V + (size_t)i * n2;
+        const double v0 = row[rq.x], v1 = row[rq.y];
+        row[rq.x] = v0 * j2.x - v1 * j2.y;
+        row[rq.y] = v0 * j2.y + v1 * j2.x;
+    }
+}
+
+// per-row sums of squares: off-diagonal and all elements
+__global__ void mn_norm_kernel(int n2, const double* __restrict__ A, double2* __restrict__ part)
+{
+    const int i = blockIdx.x;
+    __shared__ double so[256], sa[256];
+    double o = 0.0, a = 0.0;
+    for (int j = threadIdx.x; j < n2; j += blockDim.x) {
+        const double v = A[(size_t)i * n2 + j];
+        a += v * v;
+        if (j != i) o += v * v;
+    }
+    so[threadIdx.x] = o;
+    sa[threadIdx.x] = a;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            so[threadIdx.x] += so[threadIdx.x + k];
+            sa[threadIdx.x] += sa[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[i] = make_double2(so[0], sa[0]);
+}
+
+__global__ void mn_diag_kernel(int n2, const double* __restrict__ A, double* __restrict__ d)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n2) d[i] = A[(size_t)i * n2 + i];
+}
+
+// y_j = sum_i V_ij x_i (one thread per column)
+__global__ void mn_vt_kernel(int n2, const double* __restrict__ V, const double* __restrict__ x, double* __restrict__ y)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n2) return;
+    double s = 0.0;
+    for (int i = 0; i < n2; ++i) s += V[(size_t)i * n2 + j] * x[i];
+    y[j] = s;
+}
+
+// y_i = sum_j V_ij x_j (one wavefront per row)
+__global__ void mn_v_kernel(int n2, const double* __restrict__ V, const double* __restrict__ x, double* __restrict__ y)
+{
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n2) return;
+    double s = 0.0;
+    for (int j = lane; j < n2; j += 64) s += V[(size_t)i * n2 + j] * x[j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) y[i] = s;
+}
+
 }  // namespace
 
 int keep_rows_copy(mld_ctx* ctx, int M, const double* A, const double* y)
@@ -355,12 +490,124 @@ extern "C" int mld_gram_set_keep_rows(mld_ctx* ctx, int keep)
     return MLD_OK;
 }
 
-extern "C" int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w_out, double* diag_out)
+// Minimum-norm solution of (G + Lambda) w = h in the sense of the reference's SVD
+// solve (serial_lsystem_by_svd, stats_tools.F90:1153-1213, LAPACK dgelsd with rcond =
+// machine epsilon for rcond < 0): eigenvalues at or below rcond * max |eigenvalue| are treated as zero.  For
+// the unique quadratic columns the solve runs on S (G + Lambda) S with S = sqrt(2) on the
+// off-diagonal columns, which is the full-layout system restricted to its range: the
+// result equals the reference's minimum-norm solution on the D^2 layout folded back.
+static int solve_min_norm(mld_ctx* ctx, const std::vector<double>& ridge, double rcond, double* w_out,
+                          double* diag_out)
+{
+    hipStream_t st = ctx->stream;
+    const int n = ctx->gD, ld = ctx->gld, n2 = n + (n & 1), h2 = n2 / 2;
+    std::vector<double> sc(n, 1.0);
+    if (ctx->gram_order == 2)
+        for (int c = 1 + ctx->D; c < n; ++c) {
+            int q = c - 1 - ctx->D, p = 0;
+            while (q >= ctx->D - p) { q -= ctx->D - p; ++p; }
+            q += p;
+            if (p != q) sc[c] = std::sqrt(2.0);
+        }
+    const size_t nn = (size_t)n2 * n2;
+    double *A = nullptr, *V = nullptr, *vec = nullptr;
+    MLD_HIP(ctx, hipMalloc(&A, nn * sizeof(double)));
+    MLD_HIP(ctx, hipMalloc(&V, nn * sizeof(double)));
+    MLD_HIP(ctx, hipMalloc(&vec, (size_t)(8 * n2 + 8) * sizeof(double) + (size_t)h2 * (sizeof(double2) + sizeof(int2))));
+    struct Free {
+        double *a, *v, *x;
+        ~Free() { (void)hipFree(a); (void)hipFree(v); (void)hipFree(x); }
+    } fr{A, V, vec};
+    double* dr = vec;           // ridge (n)
+    double* dsc = dr + n2;      // scale (n)
+    double* b = dsc + n2;       // S h (n2)
+    double* y = b + n2;         // V^T b, then scaled (n2)
+    double* x = y + n2;         // V y (n2)
+    double* r = x + n2;         // residual (n2)
+    double2* part = (double2*)(r + n2);        // n2 pairs
+    double2* cs = (double2*)(r + 3 * n2);      // h2
+    int2* pq = (int2*)(cs + h2);               // h2
+    MLD_HIP(ctx, hipMemcpyAsync(dr, ridge.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+    MLD_HIP(ctx, hipMemcpyAsync(dsc, sc.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(mn_build_kernel, dim3(n2), dim3(256), 0, st, n, n2, ld, ctx->d_G, dr, dsc, ctx->d_h, A, V, b);
+    MLD_HIP(ctx, hipGetLastError());
+    const long long nthr = (long long)h2 * h2 + (long long)n2 * h2;
+    const unsigned nblk = (unsigned)((nthr + 255) / 256);
+    std::vector<double2> hp(n2);
+    int sweeps = 0;
+    const int max_sweeps = 40;
+    for (; sweeps < max_sweeps; ++sweeps) {
+        hipLaunchKernelGGL(mn_norm_kernel, dim3(n2), dim3(256), 0, st, n2, A, part);
+        MLD_HIP(ctx, hipMemcpyAsync(hp.data(), part, (size_t)n2 * sizeof(double2), hipMemcpyDeviceToHost, st));
+        MLD_HIP(ctx, hipStreamSynchronize(st));
+        long double off = 0.0L, all = 0.0L;
+        for (const double2& v : hp) {
+            off += v.x;
+            all += v.y;
+        }
+        // converged when the off-diagonal part is at the rounding level of the rotations,
+        // about n eps of the matrix norm
+        const long double tol = 4.0L * n2 * DBL_EPSILON;
+        if (n2 < 2 || off <= tol * tol * all) break;
+        for (int step = 0; step < n2 - 1; ++step) {
+            hipLaunchKernelGGL(mn_params_kernel, dim3((h2 + 255) / 256), dim3(256), 0, st, n2, step, A, cs, pq);
+            hipLaunchKernelGGL(mn_rotate_kernel, dim3(nblk), dim3(256), 0, st, n2, A, V, cs, pq);
+        }
+        MLD_HIP(ctx, hipGetLastError());
+    }
+    // eigenvalues on the diagonal; y = V^T b scaled by the pseudo-inverse; x = V y; w = S x
+    std::vector<double> ev(n2), hy(n2);
+    hipLaunchKernelGGL(mn_diag_kernel, dim3((n2 + 255) / 256), dim3(256), 0, st, n2, A, r);
+    MLD_HIP(ctx, hipMemcpyAsync(ev.data(), r, (size_t)n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(mn_vt_kernel, dim3((n2 + 255) / 256), dim3(256), 0, st, n2, V, b, y);
+    MLD_HIP(ctx, hipMemcpyAsync(hy.data(), y, (size_t)n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MLD_HIP(ctx, hipStreamSynchronize(st));
+    double emax = 0.0;
+    for (double e : ev) emax = std::max(emax, std::fabs(e));
+    const double thr = (rcond < 0.0 ? DBL_EPSILON : rcond) * emax;
+    double kept_min = emax;
+    for (int i = 0; i < n2; ++i) {
+        if (std::fabs(ev[i]) > thr) {
+            hy[i] /= ev[i];
+            kept_min = std::min(kept_min, std::fabs(ev[i]));
+        } else {
+            hy[i] = 0.0;
+        }
+    }
+    MLD_HIP(ctx, hipMemcpyAsync(y, hy.data(), (size_t)n2 * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(mn_v_kernel, dim3((n2 + 3) / 4), dim3(256), 0, st, n2, V, y, x);
+    std::vector<double> hx(n2);
+    MLD_HIP(ctx, hipMemcpyAsync(hx.data(), x, (size_t)n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MLD_HIP(ctx, hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) w_out[i] = sc[i] * hx[i];
+    if (diag_out) {
+        // relative residual of (G + Lambda) w = h
+        MLD_HIP(ctx, hipMemcpyAsync(x, w_out, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(resid_gram_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, dr, ctx->d_h, x, r);
+        std::vector<double> hr(n), hh(n);
+        MLD_HIP(ctx, hipMemcpyAsync(hr.data(), r, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        MLD_HIP(ctx, hipMemcpyAsync(hh.data(), ctx->d_h, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        MLD_HIP(ctx, hipStreamSynchronize(st));
+        long double rr = 0.0L, hn = 0.0L;
+        for (int i = 0; i < n; ++i) {
+            rr += (long double)hr[i] * hr[i];
+            hn += (long double)hh[i] * hh[i];
+        }
+        diag_out[0] = sweeps;
+        diag_out[1] = hn > 0 ? (double)std::sqrt(rr / hn) : 0.0;
+        diag_out[2] = kept_min;
+        diag_out[3] = emax;
+    }
+    return MLD_OK;
+}
+
+static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond, double* w_out, double* diag_out)
 {
     if (!ctx) return MLD_E_ARG;
     if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
     if (!w_out) return mld_fail(ctx, MLD_E_ARG, "mld_gram_solve: null w");
-    if (method != MLD_SOLVE_CHOL_IR && method != MLD_SOLVE_CHOL) return mld_fail(ctx, MLD_E_ARG, "unknown method");
+    if (method != MLD_SOLVE_CHOL_IR && method != MLD_SOLVE_CHOL && method != MLD_SOLVE_SVD)
+        return mld_fail(ctx, MLD_E_ARG, "unknown method");
     hipStream_t st = ctx->stream;
     const int n = ctx->gD, ld = ctx->gld;
     if (lambda < 0.0) lambda = 0.0;  // lambda_krr < 0 -> 0 (module_online_fit.F90:571)
@@ -375,6 +622,7 @@ extern "C" int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w
             if (p != q) ridge[c] = 0.5 * lambda;
         }
     }
+    if (method == MLD_SOLVE_SVD) return solve_min_norm(ctx, ridge, svd_rcond, w_out, diag_out);
     size_t nb = (size_t)((n + NB - 1) / NB);
     size_t bytes_L = (size_t)ld * ld * sizeof(double);
     size_t nvec = (size_t)ld;
@@ -545,4 +793,14 @@ extern "C" int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w
         diag_out[3] = mx;
     }
     return MLD_OK;
+}
+
+extern "C" int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w_out, double* diag_out)
+{
+    return gram_solve(ctx, lambda, method, -1.0, w_out, diag_out);
+}
+
+extern "C" int mld_gram_solve_svd(mld_ctx* ctx, double lambda, double rcond, double* w_out, double* diag_out)
+{
+    return gram_solve(ctx, lambda, MLD_SOLVE_SVD, rcond, w_out, diag_out);
 }

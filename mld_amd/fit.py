@@ -19,7 +19,7 @@ from typing import Optional, Tuple
 
 import numpy as np
 
-SOLVE_CHOL_IR, SOLVE_CHOL = 0, 1
+SOLVE_CHOL_IR, SOLVE_CHOL, SOLVE_SVD = 0, 1, 2
 
 
 def ncols(D: int, order: int, nk: int = 0) -> int:
@@ -55,12 +55,15 @@ def full_to_unique_rows(rows_full: np.ndarray, D: int) -> np.ndarray:
 
 
 class GramFit:
-    def __init__(self, ctx, order: int = 1, mixed: bool = False, keep_rows: bool = False, nk: int = 0):
+    def __init__(self, ctx, order: int = 1, mixed: bool = False, keep_rows: bool = False, nk: int = 0,
+                 ncol: Optional[int] = None):
+        """ncol: width of caller-packed rows (add_rows only); default the width of the
+        order's rows [N, B (, quadratic | kernel)]."""
         self.ctx = ctx
         self.order = order
         self.mixed = bool(mixed)
         self.D = ctx.D
-        self.ncol = ncols(ctx.D, order, nk)
+        self.ncol = ncols(ctx.D, order, nk) if ncol is None else int(ncol)
         ctx.check(ctx.L.mld_gram_begin(ctx.h, self.ncol))
         ctx.check(ctx.L.mld_gram_set_keep_rows(ctx.h, int(keep_rows)))
 
@@ -80,6 +83,8 @@ class GramFit:
         y = np.ascontiguousarray(y, np.float64)
         w = np.ascontiguousarray(w, np.float64)
         mc = A.shape[0]
+        if A.ndim != 2 or A.shape[1] != self.ncol or y.shape != (mc,) or w.shape != (mc,):
+            raise ValueError(f"add_rows: rows must be (mc, {self.ncol}) with y, w of length mc")
         self.ctx.check(self.ctx.L.mld_gram_add_rows(self.ctx.h, self.ncol, mc, A.ctypes.data_as(C.c_void_p),
                                                     y.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p)))
 
@@ -90,11 +95,20 @@ class GramFit:
                                                   h.ctypes.data_as(C.c_void_p)))
         return G, h
 
-    def solve(self, lam: float, method: int = SOLVE_CHOL_IR):
+    def solve(self, lam: float, method: int = SOLVE_CHOL_IR, rcond: float = -1.0):
+        """(G + lambda) w = h; method SOLVE_SVD gives the minimum-norm solution with eigenvalues at
+        or below rcond * max dropped (rcond < 0: machine epsilon, the reference's svd_rcond = -1)."""
         w = np.zeros(self.ncol)
         diag = np.zeros(4)
-        self.ctx.check(self.ctx.L.mld_gram_solve(self.ctx.h, lam, method, w.ctypes.data_as(C.c_void_p),
-                                                 diag.ctypes.data_as(C.c_void_p)))
+        if method == SOLVE_SVD:
+            self.ctx.check(self.ctx.L.mld_gram_solve_svd(self.ctx.h, lam, rcond, w.ctypes.data_as(C.c_void_p),
+                                                         diag.ctypes.data_as(C.c_void_p)))
+        else:
+            self.ctx.check(self.ctx.L.mld_gram_solve(self.ctx.h, lam, method, w.ctypes.data_as(C.c_void_p),
+                                                     diag.ctypes.data_as(C.c_void_p)))
+        if method == SOLVE_SVD:
+            return w, {"sweeps": int(diag[0]), "rel_residual": float(diag[1]), "min_kept_eigenvalue": float(diag[2]),
+                       "max_eigenvalue": float(diag[3])}
         return w, {"iterations": int(diag[0]), "rel_residual": float(diag[1]), "min_pivot": float(diag[2]),
                    "max_pivot": float(diag[3])}
 

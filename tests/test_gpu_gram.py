@@ -108,12 +108,7 @@ def test_solve_matches_numpy_and_refines(ctx64):
 
     rng = np.random.default_rng(3)
     D = 120
-    gf = GramFit(ctx64, order=1) if False else None
-    import mld_amd as M
-
-    gf = GramFit.__new__(GramFit)
-    gf.ctx, gf.order, gf.mixed, gf.D, gf.ncol = ctx64, 1, False, ctx64.D, D
-    ctx64.check(ctx64.L.mld_gram_begin(ctx64.h, D))
+    gf = GramFit(ctx64, ncol=D)
     A = rng.normal(size=(800, D)) @ np.diag(np.logspace(0, 3, D))
     y = rng.normal(size=800)
     w = np.ones(800)
