@@ -25,9 +25,9 @@ Workloads (``--workload``):
       random occupancy, sigma = 0.05 A), atom counts lognormal (median 128, clipped to
       8..2,000); each request asks for the energy, forces and per-atom Mahalanobis
       scores of a linear model fitted beforehand to species-dependent Morse targets.
-      Descriptor: ACE with the Chebyshev (expTcheb) radial basis, n_max = 8 / 4 / 2 and
-      l_max = 0 / 4 / 2 for body orders 2 / 3 / 4, two chemical channels (D = 356),
-      r_cut = 5.0 A, FP32.
+      Descriptor: ACE with the Chebyshev (expTcheb) radial basis, n_max = 8 / 4 / 2 / 1 and
+      l_max = 0 / 4 / 2 / 1 for nu = 1 / 2 / 3 / 4 (body orders 2 to 5), two chemical
+      channels (D = 394), r_cut = 5.0 A, FP32.
       Reported: p50 / p99 latency at that load, and the throughput of the same
       requests submitted all at once.
 
@@ -496,7 +496,7 @@ def main_serve(args):
 
     rng = np.random.default_rng(SEED + 5)
     weights = [1.0, 1.25, 1.5, 1.75, 2.0]
-    ace = dict(descriptor="ace", ace_numax=3, ace_nmax=(8, 4, 2), ace_lmax=(0, 4, 2), ace_lambda=2.0)
+    ace = dict(descriptor="ace", ace_numax=4, ace_nmax=(8, 4, 2, 1), ace_lmax=(0, 4, 2, 1), ace_lambda=2.0)
     Dm, am, r0m = synth.morse_mixture_params(5)
     # fit the served model (linear, FP64) on 200 structures of the same kind
     train = hea_request_pool(rng, 200)
@@ -509,7 +509,9 @@ def main_serve(args):
     b.bso4(e, g)
     gf = fit.GramFit(c64, order=1)
     gf.add_batch(b, E.cpu().numpy(), F.reshape(-1).contiguous(), 1.0, 0.1)
-    w, _ = gf.solve(1e-8)
+    # minimum-norm solve (the reference's svd_rcond = -1 semantics): the nu = 4 couplings
+    # through different intermediate L are not all linearly independent
+    w, _ = gf.solve(1e-8, fit.SOLVE_SVD)
     Bh = e.cpu().numpy()
     mu = Bh.mean(0)
     cov = np.cov(Bh.T)
@@ -574,7 +576,7 @@ def main_serve(args):
         "dtype": "f32 desc, f64 prediction", "data": f"synthetic (seed {SEED})",
         "config": {"workload": f"serve: {nreq} requests, Poisson {args.rate:g}/s, 5-species bcc HEA a=3.1 A, "
                                "N lognormal median 128 clipped 8..2000, E + F + Mahalanobis d2, "
-                               f"ACE expTcheb n_max 8/4/2 l_max 0/4/2 two channels D={ctx_D} r_cut=5.0 A FP32",
+                               f"ACE expTcheb nu<=4 n_max 8/4/2/1 l_max 0/4/2/1 two channels D={ctx_D} r_cut=5.0 A FP32",
                    "natoms": natoms, "hbm_budget_gib": args.hbm_budget_gib, "max_batch_structs": 64,
                    "parallelism": "single GPU"},
         "at_load": {"rate_req_s": args.rate, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],

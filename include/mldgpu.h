@@ -57,13 +57,13 @@ typedef struct mld_params {
     double r_cut_width_in;      /* &input_ml r_cut_width_in */
     int precision;              /* MLD_PREC_FP64 or MLD_PREC_FP32 */
     int device;                 /* HIP device ordinal */
-    /* ACE (descriptor_type 300), module_ace_desc keys: ace_numax (body order - 1, 1..3),
+    /* ACE (descriptor_type 300), module_ace_desc keys: ace_numax (body order - 1, 1..4),
      * ace_nmax_list / ace_lmax_list per nu (defaults "5 3 1" / "0 4 2", read_ml_file.F90:282-286),
      * ace_lambda (expTcheb radial, ace_radial_poly = 2).  Chemical channels: nchannel 1
      * (every neighbour) or 2 (plus one weighted by weight[species]). */
     int ace_numax;
-    int ace_nmax[3];
-    int ace_lmax[3];
+    int ace_nmax[4];
+    int ace_lmax[4];
     double ace_lambda;
 } mld_params;
 

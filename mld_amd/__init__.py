@@ -54,8 +54,8 @@ class mld_params(C.Structure):
         ("precision", C.c_int),
         ("device", C.c_int),
         ("ace_numax", C.c_int),
-        ("ace_nmax", C.c_int * 3),
-        ("ace_lmax", C.c_int * 3),
+        ("ace_nmax", C.c_int * 4),
+        ("ace_lmax", C.c_int * 4),
         ("ace_lambda", C.c_double),
     ]
 
@@ -154,7 +154,7 @@ class Context:
         p = mld_params()
         p.descriptor_type = DESC_ACE if descriptor == "ace" else DESC_BSO4
         p.ace_numax = ace_numax
-        for k in range(3):
+        for k in range(4):
             p.ace_nmax[k] = int(ace_nmax[k]) if k < len(ace_nmax) else 0
             p.ace_lmax[k] = int(ace_lmax[k]) if k < len(ace_lmax) else 0
         p.ace_lambda = ace_lambda

@@ -198,6 +198,7 @@ __global__ void __launch_bounds__(NTH) ace_kernel(AceDev tb, int natoms, const i
             cx2<T> pr = sm.A[term.e[0]];
             if (term.e[1] >= 0) pr = cmulT(pr, sm.A[term.e[1]]);
             if (term.e[2] >= 0) pr = cmulT(pr, sm.A[term.e[2]]);
+            if (term.e[3] >= 0) pr = cmulT(pr, sm.A[term.e[3]]);
             s.r += (T)term.coef * pr.r;
             s.i += (T)term.coef * pr.i;
         }
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(NTH) ace_kernel(AceDev tb, int natoms, const i
                 T d[3] = {(T)0, (T)0, (T)0};
                 for (int t = tb.toff[b]; t < tb.toff[b + 1]; ++t) {
                     const AceTerm term = tb.terms[t];
-                    const int np = term.e[1] < 0 ? 1 : (term.e[2] < 0 ? 2 : 3);
+                    const int np = term.e[1] < 0 ? 1 : (term.e[2] < 0 ? 2 : (term.e[3] < 0 ? 3 : 4));
                     for (int i = 0; i < np; ++i) {
                         cx2<T> other{(T)term.coef, (T)0};
                         for (int i2 = 0; i2 < np; ++i2)

@@ -126,12 +126,12 @@ constexpr int ACE_DMAX = 512;   // basis functions per atom (two per thread of a
 // e = (c * NA + n) * LA2 + l * l + l + m
 struct AceTerm {
     double coef;
-    int e[3];
+    int e[4];
 };
 
 struct AceHost {
     int nch = 1, numax = 3, NA = 0, LA = 0, LA2 = 0, nA = 0, D = 0;
-    int nmax[3] = {0, 0, 0}, lmax[3] = {0, 0, 0};
+    int nmax[4] = {0, 0, 0, 0}, lmax[4] = {0, 0, 0, 0};
     double lam = 2.0;
     std::vector<int> toff;   // [D + 1]
     std::vector<int> nu;     // [D]
@@ -145,7 +145,8 @@ struct AceDev {
     double lam;
 };
 
-// basis enumeration (nu-major; k1 <= k2 <= k3 over keys (c, n); then l, l1, l2, l3)
+// basis enumeration (nu-major; k1 <= k2 <= ... over keys (c, n); then l1, l2, ...; then the
+// intermediate L of nu = 4)
 int build_ace_tables(int nch, int numax, const int* nmax, const int* lmax, double lam, AceHost& t);
 
 }  // namespace mld

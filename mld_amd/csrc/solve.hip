@@ -323,24 +323,32 @@ __device__ __forceinline__ int rr_player(int n2, int s, int pos)
     return pos == 0 ? 0 : 1 + (pos - 1 + s) % (n2 - 1);
 }
 
-// rotation (c, s) that zeroes A_pq of every pair of step `step` (Golub-Van Loan sym.schur2)
-__global__ void mn_params_kernel(int n2, int step, const double* __restrict__ A, double2* __restrict__ cs,
-                                 int2* __restrict__ pq)
+// rotation (c, s) that zeroes A_pq of every pair of step `step` (Golub-Van Loan sym.schur2);
+// pairs with |A_pq| <= eps sqrt(|A_pp A_qq|), or below the floor (eps times the largest
+// diagonal: rounding noise between null directions), are left alone; the others are
+// counted in *nrot
+__global__ void mn_params_kernel(int n2, int step, const double* __restrict__ A, double floor_abs,
+                                 double2* __restrict__ cs, int2* __restrict__ pq, int* __restrict__ nrot)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n2 / 2) return;
-    const int p = rr_player(n2, step, k), q = rr_player(n2, step, n2 - 1 - k);
-    const double apq = A[(size_t)p * n2 + q], app = A[(size_t)p * n2 + p], aqq = A[(size_t)q * n2 + q];
-    double c = 1.0, sn = 0.0;
-    if (fabs(apq) > 1e-300 && fabs(apq) > 1e-18 * sqrt(fabs(app * aqq))) {
-        const double tau = (aqq - app) / (2.0 * apq);
-        const double at = fabs(tau);
-        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (at + (at < 1e150 ? sqrt(1.0 + tau * tau) : at));
-        c = 1.0 / sqrt(1.0 + t * t);
-        sn = t * c;
+    bool rot = false;
+    if (k < n2 / 2) {
+        const int p = rr_player(n2, step, k), q = rr_player(n2, step, n2 - 1 - k);
+        const double apq = A[(size_t)p * n2 + q], app = A[(size_t)p * n2 + p], aqq = A[(size_t)q * n2 + q];
+        double c = 1.0, sn = 0.0;
+        if (fabs(apq) > floor_abs && fabs(apq) > DBL_EPSILON * sqrt(fabs(app * aqq))) {
+            rot = true;
+            const double tau = (aqq - app) / (2.0 * apq);
+            const double at = fabs(tau);
+            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (at + (at < 1e150 ? sqrt(1.0 + tau * tau) : at));
+            c = 1.0 / sqrt(1.0 + t * t);
+            sn = t * c;
+        }
+        cs[k] = make_double2(c, sn);
+        pq[k] = make_int2(p, q);
     }
-    cs[k] = make_double2(c, sn);
-    pq[k] = make_int2(p, q);
+    const unsigned long long m = __ballot(rot);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(nrot, (int)__popcll(m));
 }
 
 // A <- J^T A J on 2 x 2 blocks (rows of pair P, columns of pair Q), V <- V J on the
@@ -374,30 +382,6 @@ __global__ void mn_rotate_kernel(int n2, double* __restrict__ A, double* __restr
         row[rq.x] = v0 * j2.x - v1 * j2.y;
         row[rq.y] = v0 * j2.y + v1 * j2.x;
     }
-}
-
-// per-row sums of squares: off-diagonal and all elements
-__global__ void mn_norm_kernel(int n2, const double* __restrict__ A, double2* __restrict__ part)
-{
-    const int i = blockIdx.x;
-    __shared__ double so[256], sa[256];
-    double o = 0.0, a = 0.0;
-    for (int j = threadIdx.x; j < n2; j += blockDim.x) {
-        const double v = A[(size_t)i * n2 + j];
-        a += v * v;
-        if (j != i) o += v * v;
-    }
-    so[threadIdx.x] = o;
-    sa[threadIdx.x] = a;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if (threadIdx.x < k) {
-            so[threadIdx.x] += so[threadIdx.x + k];
-            sa[threadIdx.x] += sa[threadIdx.x + k];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[i] = make_double2(so[0], sa[0]);
 }
 
 __global__ void mn_diag_kernel(int n2, const double* __restrict__ A, double* __restrict__ d)
@@ -533,27 +517,32 @@ static int solve_min_norm(mld_ctx* ctx, const std::vector<double>& ridge, double
     MLD_HIP(ctx, hipGetLastError());
     const long long nthr = (long long)h2 * h2 + (long long)n2 * h2;
     const unsigned nblk = (unsigned)((nthr + 255) / 256);
-    std::vector<double2> hp(n2);
+    // sweeps until one applies no rotation: every |A_pq| <= eps sqrt(|A_pp A_qq|), which
+    // also gives the small eigenvalues to high relative accuracy
+    int* nrot = (int*)part;
+    double floor_abs = 0.0;
+    {
+        std::vector<double> d0(n2);
+        hipLaunchKernelGGL(mn_diag_kernel, dim3((n2 + 255) / 256), dim3(256), 0, st, n2, A, r);
+        MLD_HIP(ctx, hipMemcpyAsync(d0.data(), r, (size_t)n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        MLD_HIP(ctx, hipStreamSynchronize(st));
+        for (double v : d0) floor_abs = std::max(floor_abs, std::fabs(v));
+        floor_abs = std::max(floor_abs * DBL_EPSILON, 1e-300);
+    }
     int sweeps = 0;
     const int max_sweeps = 40;
-    for (; sweeps < max_sweeps; ++sweeps) {
-        hipLaunchKernelGGL(mn_norm_kernel, dim3(n2), dim3(256), 0, st, n2, A, part);
-        MLD_HIP(ctx, hipMemcpyAsync(hp.data(), part, (size_t)n2 * sizeof(double2), hipMemcpyDeviceToHost, st));
-        MLD_HIP(ctx, hipStreamSynchronize(st));
-        long double off = 0.0L, all = 0.0L;
-        for (const double2& v : hp) {
-            off += v.x;
-            all += v.y;
-        }
-        // converged when the off-diagonal part is at the rounding level of the rotations,
-        // about n eps of the matrix norm
-        const long double tol = 4.0L * n2 * DBL_EPSILON;
-        if (n2 < 2 || off <= tol * tol * all) break;
+    for (; sweeps < max_sweeps && n2 >= 2; ++sweeps) {
+        MLD_HIP(ctx, hipMemsetAsync(nrot, 0, sizeof(int), st));
         for (int step = 0; step < n2 - 1; ++step) {
-            hipLaunchKernelGGL(mn_params_kernel, dim3((h2 + 255) / 256), dim3(256), 0, st, n2, step, A, cs, pq);
+            hipLaunchKernelGGL(mn_params_kernel, dim3((h2 + 255) / 256), dim3(256), 0, st, n2, step, A, floor_abs,
+                               cs, pq, nrot);
             hipLaunchKernelGGL(mn_rotate_kernel, dim3(nblk), dim3(256), 0, st, n2, A, V, cs, pq);
         }
         MLD_HIP(ctx, hipGetLastError());
+        int hrot = 0;
+        MLD_HIP(ctx, hipMemcpyAsync(&hrot, nrot, sizeof(int), hipMemcpyDeviceToHost, st));
+        MLD_HIP(ctx, hipStreamSynchronize(st));
+        if (hrot == 0) break;
     }
     // eigenvalues on the diagonal; y = V^T b scaled by the pseudo-inverse; x = V y; w = S x
     std::vector<double> ev(n2), hy(n2);

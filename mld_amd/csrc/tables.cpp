@@ -185,7 +185,7 @@ double wigner3j(int j1, int m1, int j2, int m2, int j3, int m3)
 int build_ace_tables(int nch, int numax, const int* nmax, const int* lmax, double lam, AceHost& t)
 {
     t = AceHost();
-    if (nch < 1 || nch > ACE_NCH || numax < 1 || numax > 3) return 1;
+    if (nch < 1 || nch > ACE_NCH || numax < 1 || numax > 4) return 1;
     t.nch = nch;
     t.numax = numax;
     t.lam = lam;
@@ -212,7 +212,7 @@ int build_ace_tables(int nch, int numax, const int* nmax, const int* lmax, doubl
     };
     if (numax >= 1)
         for (auto k : keys(0)) {
-            t.terms.push_back(AceTerm{1.0, {eidx(k.first, k.second, 0, 0), -1, -1}});
+            t.terms.push_back(AceTerm{1.0, {eidx(k.first, k.second, 0, 0), -1, -1, -1}});
             close(1);
         }
     if (numax >= 2) {
@@ -222,7 +222,7 @@ int build_ace_tables(int nch, int numax, const int* nmax, const int* lmax, doubl
                 for (int l = 0; l <= t.lmax[1]; ++l) {
                     for (int m = -l; m <= l; ++m)
                         t.terms.push_back(AceTerm{(m & 1) ? -1.0 : 1.0,
-                                                  {eidx(K[i1].first, K[i1].second, l, m), eidx(K[i2].first, K[i2].second, l, -m), -1}});
+                                                  {eidx(K[i1].first, K[i1].second, l, m), eidx(K[i2].first, K[i2].second, l, -m), -1, -1}});
                     close(2);
                 }
     }
@@ -248,10 +248,61 @@ int build_ace_tables(int nch, int numax, const int* nmax, const int* lmax, doubl
                                         if (w == 0.0) continue;
                                         t.terms.push_back(AceTerm{w, {eidx(K[i1].first, K[i1].second, l1, m1),
                                                                       eidx(K[i2].first, K[i2].second, l2, m2),
-                                                                      eidx(K[i3].first, K[i3].second, l3, m3)}});
+                                                                      eidx(K[i3].first, K[i3].second, l3, m3), -1}});
                                     }
                                 close(3);
                             }
+    }
+    if (numax >= 4) {
+        // pairs (k1 l1)(k2 l2) and (k3 l3)(k4 l4) coupled to L:
+        // sum_M (-1)^(L-M) (l1 l2 L; m1 m2 -M) (l3 l4 L; m3 m4 M) A1 A2 A3 A4
+        auto K = keys(3);
+        const int L = t.lmax[3];
+        const int nk = (int)K.size();
+        auto lt2 = [](int a, int la, int b, int lb) { return a < b || (a == b && la <= lb); };
+        for (int i1 = 0; i1 < nk; ++i1)
+            for (int i2 = i1; i2 < nk; ++i2)
+                for (int i3 = i1; i3 < nk; ++i3)
+                    for (int i4 = i3; i4 < nk; ++i4)
+                        for (int l1 = 0; l1 <= L; ++l1)
+                            for (int l2 = 0; l2 <= L; ++l2)
+                                for (int l3 = 0; l3 <= L; ++l3)
+                                    for (int l4 = 0; l4 <= L; ++l4) {
+                                        if ((l1 + l2 + l3 + l4) & 1) continue;
+                                        if (!lt2(i1, l1, i2, l2) || !lt2(i3, l3, i4, l4)) continue;
+                                        // (i1, l1, i2, l2) <= (i3, l3, i4, l4) lexicographically
+                                        const int ka[4] = {i1, l1, i2, l2}, kb[4] = {i3, l3, i4, l4};
+                                        int cmp = 0;
+                                        for (int z = 0; z < 4 && !cmp; ++z) cmp = (ka[z] < kb[z]) ? -1 : (ka[z] > kb[z] ? 1 : 0);
+                                        if (cmp > 0) continue;
+                                        const int lo = std::max(std::abs(l1 - l2), std::abs(l3 - l4));
+                                        const int hi = std::min(l1 + l2, l3 + l4);
+                                        for (int Lc = lo; Lc <= hi; ++Lc) {
+                                            // a coupled pair of equal (k, l) has no odd-L part
+                                            if (i1 == i2 && l1 == l2 && (Lc & 1)) continue;
+                                            if (i3 == i4 && l3 == l4 && (Lc & 1)) continue;
+                                            for (int M = -Lc; M <= Lc; ++M)
+                                                for (int m1 = -l1; m1 <= l1; ++m1) {
+                                                    const int m2 = M - m1;
+                                                    if (std::abs(m2) > l2) continue;
+                                                    const double w12 = wigner3j(l1, m1, l2, m2, Lc, -M);
+                                                    if (w12 == 0.0) continue;
+                                                    for (int m3 = -l3; m3 <= l3; ++m3) {
+                                                        const int m4 = -M - m3;
+                                                        if (std::abs(m4) > l4) continue;
+                                                        const double w34 = wigner3j(l3, m3, l4, m4, Lc, M);
+                                                        if (w34 == 0.0) continue;
+                                                        const double sg = ((Lc - M) & 1) ? -1.0 : 1.0;
+                                                        t.terms.push_back(AceTerm{sg * w12 * w34,
+                                                                                  {eidx(K[i1].first, K[i1].second, l1, m1),
+                                                                                   eidx(K[i2].first, K[i2].second, l2, m2),
+                                                                                   eidx(K[i3].first, K[i3].second, l3, m3),
+                                                                                   eidx(K[i4].first, K[i4].second, l4, m4)}});
+                                                    }
+                                                }
+                                            close(4);
+                                        }
+                                    }
     }
     t.D = (int)t.nu.size();
     return (t.D > ACE_DMAX) ? 2 : 0;

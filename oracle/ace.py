@@ -16,8 +16,12 @@ The reference's basis enumeration and generalised Clebsch-Gordan tables
   nu = 2 : B = Re sum_m (-1)^m A_{k1,l,m} A_{k2,l,-m}               k1 <= k2, n < nmax[1], l <= lmax[1]
   nu = 3 : B = Re sum_{m1,m2} (l1 l2 l3; m1 m2 m3) A_{k1 l1 m1} A_{k2 l2 m2} A_{k3 l3 m3},
            m3 = -m1 - m2, (k1,l1) <= (k2,l2) <= (k3,l3), l1+l2+l3 even, triangle, n < nmax[2], l <= lmax[2]
+  nu = 4 : B = Re sum_M (-1)^(L-M) sum (l1 l2 L; m1 m2 -M) (l3 l4 L; m3 m4 M) A1 A2 A3 A4, the pairs
+           (k1,l1),(k2,l2) and (k3,l3),(k4,l4) coupled to L: (k1,l1) <= (k2,l2), (k3,l3) <= (k4,l4),
+           (k1,l1,k2,l2) <= (k3,l3,k4,l4), l1+l2+l3+l4 even, L of the right parity for a coupled pair
+           of equal (k,l) (L even), n < nmax[3], l <= lmax[3]
 with chemical channels c = 0 (every neighbour, weight 1) and c = 1 (species weight), in
-that order (nu-major, then k1, k2, k3, then l, l1, l2, l3).
+that order (nu-major, then k1, k2, ..., then l, l1, l2, ..., then L).
 """
 from __future__ import annotations
 
@@ -131,6 +135,32 @@ def basis(nch: int, numax: int, nmax, lmax):
                                 if not ((i1, l1) <= (i2, l2) <= (i3, l3)):
                                     continue
                                 out.append((3, (K[i1], K[i2], K[i3]), (l1, l2, l3)))
+    if numax >= 4:
+        K = keys[3]
+        L = lmax[3]
+        nk = len(K)
+        for i1 in range(nk):
+            for i2 in range(i1, nk):
+                for i3 in range(i1, nk):
+                    for i4 in range(i3, nk):
+                        for l1 in range(L + 1):
+                            for l2 in range(L + 1):
+                                for l3 in range(L + 1):
+                                    for l4 in range(L + 1):
+                                        if (l1 + l2 + l3 + l4) % 2:
+                                            continue
+                                        if not ((i1, l1) <= (i2, l2) and (i3, l3) <= (i4, l4)):
+                                            continue
+                                        if not ((i1, l1, i2, l2) <= (i3, l3, i4, l4)):
+                                            continue
+                                        lo = max(abs(l1 - l2), abs(l3 - l4))
+                                        hi = min(l1 + l2, l3 + l4)
+                                        for Lc in range(lo, hi + 1):
+                                            if (i1, l1) == (i2, l2) and Lc % 2:
+                                                continue
+                                            if (i3, l3) == (i4, l4) and Lc % 2:
+                                                continue
+                                            out.append((4, (K[i1], K[i2], K[i3], K[i4]), (l1, l2, l3, l4, Lc)))
     return out
 
 
@@ -142,6 +172,28 @@ def terms(b):
     if nu == 2:
         l = ls[0]
         return [((-1.0) ** m, ((ks[0][0], ks[0][1], l, m), (ks[1][0], ks[1][1], l, -m))) for m in range(-l, l + 1)]
+    if nu == 4:
+        l1, l2, l3, l4, Lc = ls
+        out = []
+        for M in range(-Lc, Lc + 1):
+            for m1 in range(-l1, l1 + 1):
+                m2 = M - m1
+                if abs(m2) > l2:
+                    continue
+                w12 = wigner3j(l1, m1, l2, m2, Lc, -M)
+                if w12 == 0.0:
+                    continue
+                for m3 in range(-l3, l3 + 1):
+                    m4 = -M - m3
+                    if abs(m4) > l4:
+                        continue
+                    w34 = wigner3j(l3, m3, l4, m4, Lc, M)
+                    if w34 == 0.0:
+                        continue
+                    out.append(((-1.0) ** (Lc - M) * w12 * w34,
+                                ((ks[0][0], ks[0][1], l1, m1), (ks[1][0], ks[1][1], l2, m2),
+                                 (ks[2][0], ks[2][1], l3, m3), (ks[3][0], ks[3][1], l4, m4))))
+        return out
     l1, l2, l3 = ls
     out = []
     for m1 in range(-l1, l1 + 1):

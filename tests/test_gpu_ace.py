@@ -48,6 +48,25 @@ def test_ace_matches_oracle(n, nch):
         ctx.close()
 
 
+def test_ace_body_order_4_matches_oracle():
+    """ace_numax = 4: the four-A couplings through an intermediate L (oracle/ace.py) on the GPU."""
+    import mld_amd as M
+
+    rng = np.random.default_rng(12)
+    pos, cell, it = synth.hea_bcc(rng, 2, a=3.1, nspecies=5, sigma=0.1)
+    w = [1.0, 1.3, 0.8, 1.6, 1.1]
+    nu4 = dict(numax=4, nmax=(3, 2, 1, 1), lmax=(0, 2, 1, 2))
+    Eo, Go, nl = A.ace(pos, cell, it, 5.0, nch=2, weight=w, **nu4)
+    for prec, tb, tg in ((M.PREC_FP64, 1e-11, 1e-10), (M.PREC_FP32, 2e-5, 1e-4)):
+        ctx = M.Context(r_cut=5.0, nspecies=5, precision=prec, descriptor="ace", weight=w, ace_numax=4,
+                        ace_nmax=nu4["nmax"], ace_lmax=nu4["lmax"])
+        assert ctx.D == Eo.shape[1]
+        b, E, G = gpu_ace(ctx, [(pos, cell, it)])
+        assert rel(E, Eo) < tb, (prec, rel(E, Eo))
+        assert rel(G.reshape(Go.shape), Go) < tg, (prec, rel(G.reshape(Go.shape), Go))
+        ctx.close()
+
+
 def test_ace_batch_invariance_and_large_box():
     import mld_amd as M
 

@@ -99,3 +99,59 @@ def test_rotation_permutation_invariance_and_fd_gradient():
                 if nbr[off[a] + k - 1] == j:
                     an += G[off[a] + a + k, x]
             assert np.abs(fd - an).max() < 1e-6 * max(1.0, np.abs(an).max()), (j, x)
+
+
+NU4 = dict(numax=4, nmax=(3, 2, 1, 1), lmax=(0, 2, 1, 2))
+
+
+def test_nu4_single_neighbour_closed_form():
+    """nu = 4 with one neighbour: only M = 0 survives after rotating the bond onto z, so
+    B = (-1)^L (l1 l2 L; 0 0 0) (l3 l4 L; 0 0 0) prod_i g_i sqrt((2 l_i + 1) / 4pi)."""
+    pos = np.array([[0.0, 0.0, 0.0], [-0.8, 1.9, 1.1]])
+    cell = np.eye(3) * 30.0
+    t = np.zeros(2, np.int32)
+    E, _, _ = A.ace(pos, cell, t, 5.0, want_grad=False, **NU4)
+    r = np.linalg.norm(pos[1])
+    g, _ = A.radial(r, 5.0, 2.0, 3)
+    bas = A.basis(1, 4, NU4["nmax"], NU4["lmax"])
+    n4 = 0
+    for b, (nu, ks, ls) in enumerate(bas):
+        if nu != 4:
+            continue
+        n4 += 1
+        l1, l2, l3, l4, L = ls
+        ref = (-1.0) ** L * A.wigner3j(l1, 0, l2, 0, L, 0) * A.wigner3j(l3, 0, l4, 0, L, 0)
+        for k, l in zip(ks, (l1, l2, l3, l4)):
+            ref *= g[k[1]] * math.sqrt((2 * l + 1) / (4 * math.pi))
+        assert abs(E[0, b] - ref) < 1e-13 * max(1.0, abs(ref)), (b, ls, E[0, b], ref)
+    assert n4 >= 8
+
+
+def test_nu4_rotation_invariance_and_fd_gradient():
+    rng = np.random.default_rng(2)
+    pos, cell, it = synth.hea_bcc(rng, 2, a=3.1, nspecies=3, sigma=0.12)
+    w = [1.0, 1.6, 0.7]
+    E, G, nl = A.ace(pos, cell, it, 5.0, nch=2, weight=w, **NU4)
+    nu = [b[0] for b in A.basis(2, 4, NU4["nmax"], NU4["lmax"])]
+    assert nu.count(4) > 0 and np.abs(E[:, np.array(nu) == 4]).max() > 1e-6
+    R = rotation(rng)
+    E2, _, _ = A.ace(pos @ R.T, cell @ R.T, it, 5.0, nch=2, weight=w, want_grad=False, **NU4)
+    assert np.abs(E2 - E).max() < 1e-11 * np.abs(E).max()
+    # dB_a/dr_j by central differences, images of j summed as in the bispectrum-free test above
+    off, nbr = nl[0], nl[1]
+    h = 1e-5
+    a = 0
+    for j in (a, int(nbr[off[a] + 3])):
+        for x in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[j, x] += h
+            pm[j, x] -= h
+            fd = (A.ace(pp, cell, it, 5.0, nch=2, weight=w, want_grad=False, **NU4)[0][a] -
+                  A.ace(pm, cell, it, 5.0, nch=2, weight=w, want_grad=False, **NU4)[0][a]) / (2 * h)
+            an = np.zeros(E.shape[1])
+            if j == a:
+                an += G[off[a] + a, x]
+            for k in range(1, off[a + 1] - off[a] + 1):
+                if nbr[off[a] + k - 1] == j:
+                    an += G[off[a] + a + k, x]
+            assert np.abs(fd - an).max() < 1e-6 * max(1.0, np.abs(an).max()), (j, x)
