@@ -7,8 +7,10 @@
 //   2. A_{c n l m} = sum_j w_c(j) g_n(r_j) Y_lm(r_j)  (compute_uniqueA_neigh_ja,
 //      compute/compute_ace_AB.F90:2475-2563), accumulated in neighbour order;
 //   3. B_b = Re sum_terms coef prod_i A_{e_i}  (one thread per basis function);
-//   4. dB_b/dr_j = Re sum_terms coef sum_i (prod_{i' != i} A) dA_{e_i}(j),
-//      dA_e(j) = w_c(j) (dg_n r^ Y_lm + g_n grad Y_lm); the self slot is -sum_j (:329-333).
+//   4. G_s = dB_b/dA_e for the distinct A entries e of each b (sum over terms and factors of
+//      coef * prod of the other factors), once per atom in LDS; then
+//      dB_b/dr_j = Re sum_s G_s dA_e(j), dA_e(j) = w_c(j) (dg_n r^ Y_lm + g_n grad Y_lm),
+//      one work item per (neighbour, b); the self slot is -sum_j (:329-333).
 // Channels: c = 0 every neighbour (weight 1), c = 1 weighted by the neighbour's species
 // weight.  Precision T = double (FP64 form) or float (FP32 form, geometry in FP64).
 #include "mld_internal.h"
@@ -205,71 +207,94 @@ __global__ void __launch_bounds__(NTH) ace_kernel(AceDev tb, int natoms, const i
         energy[(size_t)a * D + b] = s.r;
     }
     if (!want_grad) return;
-    // ---- dB/dr, two basis functions per thread at most (D <= 512)
-    T self[2][3] = {{(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}};
+    // ---- dB/dr: G_s = dB_b / dA_e for the slots s (distinct A entries) of every b, once per
+    // atom; then per neighbour dB_b/dr_j = Re sum_s G_s dA_e(j), work items (neighbour, b)
+    extern __shared__ __align__(16) unsigned char ace_dyn[];
+    cx2<T>* Gs = reinterpret_cast<cx2<T>*>(ace_dyn);
+    int* sc = reinterpret_cast<int*>(ace_dyn + (size_t)tb.nslot * sizeof(cx2<T>));
+    for (int s = tid; s < tb.nslot; s += NTH) {
+        cx2<T> gsum{(T)0, (T)0};
+        for (int t = tb.dtoff[s]; t < tb.dtoff[s + 1]; ++t) {
+            const AceTerm dt = tb.dterms[t];
+            cx2<T> pr{(T)dt.coef, (T)0};
+            for (int i = 0; i < 3 && dt.e[i] >= 0; ++i) pr = cmulT(pr, sm.A[dt.e[i]]);
+            gsum.r += pr.r;
+            gsum.i += pr.i;
+        }
+        Gs[s] = gsum;
+        sc[s] = tb.scode[s];
+    }
     for (int q0 = 0; q0 < nn; q0 += NBC) {
         const int nq = min(NBC, nn - q0);
         __syncthreads();
         stage_chunk<T>(tb, sm, p0, q0, nq, pair_d, pair_idx, itype, wfac, nsp, rcut);
         __syncthreads();
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int b = tid + NTH * u;
-            if (b >= D) continue;
-            for (int q = 0; q < nq; ++q) {
-                T d[3] = {(T)0, (T)0, (T)0};
-                for (int t = tb.toff[b]; t < tb.toff[b + 1]; ++t) {
-                    const AceTerm term = tb.terms[t];
-                    const int np = term.e[1] < 0 ? 1 : (term.e[2] < 0 ? 2 : (term.e[3] < 0 ? 3 : 4));
-                    for (int i = 0; i < np; ++i) {
-                        cx2<T> other{(T)term.coef, (T)0};
-                        for (int i2 = 0; i2 < np; ++i2)
-                            if (i2 != i) other = cmulT(other, sm.A[term.e[i2]]);
-                        const int e = term.e[i];
-                        const int lm = e % tb.LA2, cn = e / tb.LA2, n = cn % tb.NA, c = cn / tb.NA;
-                        const T wq = (c == 0) ? (T)1 : sm.w1[q];
-                        const T gq = sm.g[q][n], dgq = sm.dg[q][n];
-                        const cx2<T> Y = sm.Y[q][lm];
-#pragma unroll
-                        for (int x = 0; x < 3; ++x) {
-                            const cx2<T> dY = sm.dY[q][lm][x];
-                            const cx2<T> dA{wq * (dgq * sm.rh[q][x] * Y.r + gq * dY.r),
-                                            wq * (dgq * sm.rh[q][x] * Y.i + gq * dY.i)};
-                            d[x] += other.r * dA.r - other.i * dA.i;
-                        }
-                    }
-                }
-                T* out = grad + (base + 1 + q0 + q) * 3 * (size_t)D + b;
-#pragma unroll
-                for (int x = 0; x < 3; ++x) {
-                    out[(size_t)x * D] = d[x];
-                    self[u][x] -= d[x];
-                }
+        for (int it = tid; it < nq * D; it += NTH) {
+            const int q = it / D, b = it - q * D;
+            const T w1q = sm.w1[q];
+            const T r0 = sm.rh[q][0], r1 = sm.rh[q][1], r2 = sm.rh[q][2];
+            T d0 = (T)0, d1 = (T)0, d2 = (T)0;
+            for (int s = tb.soff[b]; s < tb.soff[b + 1]; ++s) {
+                const int code = sc[s];
+                const int c = code >> 16, n = (code >> 8) & 255, lm = code & 255;
+                const T wq = (c == 0) ? (T)1 : w1q;
+                const T gq = wq * sm.g[q][n], dgq = wq * sm.dg[q][n];
+                const cx2<T> G = Gs[s], Y = sm.Y[q][lm];
+                const cx2<T> dY0 = sm.dY[q][lm][0], dY1 = sm.dY[q][lm][1], dY2 = sm.dY[q][lm][2];
+                // Re(G dA_x), dA_x = dg rhat_x Y + g dY_x
+                const T gy = dgq * (G.r * Y.r - G.i * Y.i);
+                d0 += gy * r0 + gq * (G.r * dY0.r - G.i * dY0.i);
+                d1 += gy * r1 + gq * (G.r * dY1.r - G.i * dY1.i);
+                d2 += gy * r2 + gq * (G.r * dY2.r - G.i * dY2.i);
             }
+            T* out = grad + (base + 1 + q0 + q) * 3 * (size_t)D + b;
+            out[0] = d0;
+            out[D] = d1;
+            out[2 * (size_t)D] = d2;
         }
     }
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int b = tid + NTH * u;
-        if (b >= D) continue;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) grad[(base * 3 + x) * (size_t)D + b] = self[u][x];
-    }
+    // self slot = -sum of the neighbour slots, in neighbour order (the slots written above by
+    // this workgroup are visible after the barrier)
+    __syncthreads();
+    for (int b = tid; b < D; b += NTH)
+        for (int x = 0; x < 3; ++x) {
+            T acc = (T)0;
+            for (int q = 0; q < nn; ++q) acc -= grad[((base + 1 + q) * 3 + x) * (size_t)D + b];
+            grad[(base * 3 + x) * (size_t)D + b] = acc;
+        }
+}
+
+size_t ace_dyn_bytes(int nslot, size_t cxsize)
+{
+    return (size_t)nslot * (cxsize + sizeof(int)) + 16;
 }
 
 }  // namespace
 
+bool ace_tables_fit(int nslot)
+{
+    // FP64 workgroup: static tables + slot values and codes within the 160 KB of LDS of a CU
+    return sizeof(AceSmem<double>) + ace_dyn_bytes(nslot, sizeof(cx2<double>)) <= 160 * 1024;
+}
+
 int launch_ace(mld_ctx* ctx, int want_grad, void* energy, void* grad)
 {
     if (ctx->natoms == 0) return MLD_OK;
-    if (ctx->p.precision == MLD_PREC_FP32)
-        hipLaunchKernelGGL(ace_kernel<float>, dim3(ctx->natoms), dim3(NTH), 0, ctx->stream, ctx->ace_d, ctx->natoms,
+    if (ctx->p.precision == MLD_PREC_FP32) {
+        const size_t dyn = ace_dyn_bytes(ctx->ace_d.nslot, sizeof(cx2<float>));
+        MLD_HIP(ctx, hipFuncSetAttribute((const void*)ace_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)dyn));
+        hipLaunchKernelGGL(ace_kernel<float>, dim3(ctx->natoms), dim3(NTH), dyn, ctx->stream, ctx->ace_d, ctx->natoms,
                            ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_itype, ctx->d_wfac, ctx->nsp,
                            ctx->p.r_cut, want_grad, (float*)energy, (float*)grad);
-    else
-        hipLaunchKernelGGL(ace_kernel<double>, dim3(ctx->natoms), dim3(NTH), 0, ctx->stream, ctx->ace_d, ctx->natoms,
+    } else {
+        const size_t dyn = ace_dyn_bytes(ctx->ace_d.nslot, sizeof(cx2<double>));
+        MLD_HIP(ctx, hipFuncSetAttribute((const void*)ace_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)dyn));
+        hipLaunchKernelGGL(ace_kernel<double>, dim3(ctx->natoms), dim3(NTH), dyn, ctx->stream, ctx->ace_d, ctx->natoms,
                            ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_itype, ctx->d_wfac, ctx->nsp,
                            ctx->p.r_cut, want_grad, (double*)energy, (double*)grad);
+    }
     MLD_HIP(ctx, hipGetLastError());
     return MLD_OK;
 }

@@ -98,11 +98,17 @@ int mld_create(const mld_params* p, mld_ctx** out)
         int trc = mld::build_ace_tables(p->nchannel, p->ace_numax, p->ace_nmax, p->ace_lmax, p->ace_lambda, ctx->ace_h);
         if (trc) {
             fail(trc == 2 ? MLD_E_UNSUPPORTED : MLD_E_ARG,
-                 trc == 2 ? "ACE basis larger than 512 functions" : "ACE parameters out of range (numax 1..3, nmax 1..8, lmax 0..4)");
+                 trc == 2 ? "ACE basis larger than 512 functions" : "ACE parameters out of range (numax 1..4, nmax 1..8, lmax 0..4)");
             mld_destroy(ctx);
             return rc;
         }
         ctx->D = ctx->ace_h.D;
+        if (!ace_tables_fit((int)ctx->ace_h.scode.size())) {
+            fail(MLD_E_UNSUPPORTED, "ACE basis too large for the gradient tables (" +
+                                        std::to_string(ctx->ace_h.scode.size()) + " A slots)");
+            mld_destroy(ctx);
+            return rc;
+        }
         auto upa = [&](void** dst, const void* src, size_t bytes) -> bool {
             if (hipMalloc(dst, bytes ? bytes : 8) != hipSuccess) return false;
             if (bytes && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return false;
@@ -111,13 +117,29 @@ int mld_create(const mld_params* p, mld_ctx** out)
         std::vector<double> wf((size_t)2 * ctx->nsp, 1.0);
         if (ctx->nch >= 2)
             for (int s2 = 0; s2 < ctx->nsp; ++s2) wf[ctx->nsp + s2] = ctx->h_weight[s2];
+        const mld::AceHost& ah = ctx->ace_h;
+        // gradient tables packed: dterms (8-byte aligned) first, then soff, scode, dtoff
+        std::vector<char> gpack(ah.dterms.size() * sizeof(mld::AceTerm) +
+                                (ah.soff.size() + ah.scode.size() + ah.dtoff.size()) * sizeof(int));
+        size_t o_dt = 0, o_so = o_dt + ah.dterms.size() * sizeof(mld::AceTerm);
+        size_t o_sc = o_so + ah.soff.size() * sizeof(int), o_do = o_sc + ah.scode.size() * sizeof(int);
+        std::memcpy(gpack.data() + o_dt, ah.dterms.data(), ah.dterms.size() * sizeof(mld::AceTerm));
+        std::memcpy(gpack.data() + o_so, ah.soff.data(), ah.soff.size() * sizeof(int));
+        std::memcpy(gpack.data() + o_sc, ah.scode.data(), ah.scode.size() * sizeof(int));
+        std::memcpy(gpack.data() + o_do, ah.dtoff.data(), ah.dtoff.size() * sizeof(int));
         bool ok = upa(&ctx->d_ace_terms, ctx->ace_h.terms.data(), ctx->ace_h.terms.size() * sizeof(mld::AceTerm)) &&
                   upa(&ctx->d_ace_toff, ctx->ace_h.toff.data(), ctx->ace_h.toff.size() * sizeof(int)) &&
+                  upa(&ctx->d_ace_grad, gpack.data(), gpack.size()) &&
                   upa((void**)&ctx->d_wfac, wf.data(), wf.size() * sizeof(double));
         if (!ok) { fail(MLD_E_DEVICE, "device allocation of ACE tables failed"); mld_destroy(ctx); return rc; }
         mld::AceDev& ad = ctx->ace_d;
         ad.terms = (const mld::AceTerm*)ctx->d_ace_terms;
         ad.toff = (const int*)ctx->d_ace_toff;
+        ad.dterms = (const mld::AceTerm*)((const char*)ctx->d_ace_grad + o_dt);
+        ad.soff = (const int*)((const char*)ctx->d_ace_grad + o_so);
+        ad.scode = (const int*)((const char*)ctx->d_ace_grad + o_sc);
+        ad.dtoff = (const int*)((const char*)ctx->d_ace_grad + o_do);
+        ad.nslot = (int)ah.scode.size();
         ad.D = ctx->ace_h.D;
         ad.nch = ctx->ace_h.nch;
         ad.NA = ctx->ace_h.NA;
@@ -196,7 +218,7 @@ int mld_destroy(mld_ctx* ctx)
     gram_scratch_free(ctx);
     kept_rows_free(ctx);
     predict_free(ctx);
-    void* ptrs[] = {ctx->d_ace_terms, ctx->d_ace_toff, ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
+    void* ptrs[] = {ctx->d_ace_terms, ctx->d_ace_toff, ctx->d_ace_grad, ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
                     ctx->d_geo, ctx->d_count, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_scan_tmp,
                     ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_pgeo, ctx->d_zref, ctx->d_zz, ctx->d_zref32, ctx->d_kiz};
     for (void* q : ptrs)

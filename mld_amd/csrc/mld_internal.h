@@ -129,6 +129,9 @@ struct AceTerm {
     int e[4];
 };
 
+// Gradient tables: basis function b depends on the distinct A entries ("slots")
+// soff[b] .. soff[b+1]; slot s has code (c << 16) | (n << 8) | lm and the terms of
+// dB_b / dA_e (coef * prod of the other factors, unused factors -1) dtoff[s] .. dtoff[s+1].
 struct AceHost {
     int nch = 1, numax = 3, NA = 0, LA = 0, LA2 = 0, nA = 0, D = 0;
     int nmax[4] = {0, 0, 0, 0}, lmax[4] = {0, 0, 0, 0};
@@ -136,12 +139,20 @@ struct AceHost {
     std::vector<int> toff;   // [D + 1]
     std::vector<int> nu;     // [D]
     std::vector<AceTerm> terms;
+    std::vector<int> soff;   // [D + 1]
+    std::vector<int> scode;  // [nslot]
+    std::vector<int> dtoff;  // [nslot + 1]
+    std::vector<AceTerm> dterms;
 };
 
 struct AceDev {
     const AceTerm* terms;
     const int* toff;
-    int D, nch, NA, LA, LA2, nA;
+    const int* soff;
+    const int* scode;
+    const int* dtoff;
+    const AceTerm* dterms;
+    int D, nch, NA, LA, LA2, nA, nslot;
     double lam;
 };
 
@@ -216,6 +227,7 @@ struct mld_ctx {
     mld::AceDev ace_d{};
     void* d_ace_terms = nullptr;
     void* d_ace_toff = nullptr;
+    void* d_ace_grad = nullptr;  // soff, scode, dtoff, dterms in one allocation
 };
 
 int mld_fail(mld_ctx* ctx, int code, const std::string& msg);
@@ -230,6 +242,8 @@ int mld_fail(mld_ctx* ctx, int code, const std::string& msg);
 int launch_neighbours_pos(mld_ctx* ctx, const double* pos);
 int launch_bso4(mld_ctx* ctx, int want_grad, void* energy, void* grad);
 int launch_ace(mld_ctx* ctx, int want_grad, void* energy, void* grad);
+// the ACE gradient tables (slots) fit the LDS of one workgroup
+bool ace_tables_fit(int nslot);
 // ncg_level: largest CG1 range of one output level (cgbase[j+1] - cgbase[j]); ncg: all tables
 bool bso4_tables_fit(int jj, int ncg_level, int ncg, int wsize);
 int launch_to_reference_layout(mld_ctx* ctx, int nat, int nn_max, const void* grad, double* force_dev,

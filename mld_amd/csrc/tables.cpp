@@ -305,6 +305,33 @@ int build_ace_tables(int nch, int numax, const int* nmax, const int* lmax, doubl
                                     }
     }
     t.D = (int)t.nu.size();
+    // slots and derivative terms: dB_b/dA_e = sum over terms and factors i with e_i = e of
+    // coef * prod_{j != i} A_{e_j}
+    t.soff.assign(1, 0);
+    t.dtoff.assign(1, 0);
+    for (int b = 0; b < t.D; ++b) {
+        std::vector<int> es;
+        for (int q = t.toff[b]; q < t.toff[b + 1]; ++q)
+            for (int i = 0; i < 4 && t.terms[q].e[i] >= 0; ++i)
+                if (std::find(es.begin(), es.end(), t.terms[q].e[i]) == es.end()) es.push_back(t.terms[q].e[i]);
+        for (int e : es) {
+            const int lm = e % t.LA2, cn = e / t.LA2, n = cn % t.NA, c = cn / t.NA;
+            t.scode.push_back((c << 16) | (n << 8) | lm);
+            for (int q = t.toff[b]; q < t.toff[b + 1]; ++q) {
+                const AceTerm& tm = t.terms[q];
+                for (int i = 0; i < 4 && tm.e[i] >= 0; ++i) {
+                    if (tm.e[i] != e) continue;
+                    AceTerm d{tm.coef, {-1, -1, -1, -1}};
+                    int k = 0;
+                    for (int j = 0; j < 4 && tm.e[j] >= 0; ++j)
+                        if (j != i) d.e[k++] = tm.e[j];
+                    t.dterms.push_back(d);
+                }
+            }
+            t.dtoff.push_back((int)t.dterms.size());
+        }
+        t.soff.push_back((int)t.scode.size());
+    }
     return (t.D > ACE_DMAX) ? 2 : 0;
 }
 
