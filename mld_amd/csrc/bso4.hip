@@ -607,8 +607,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                 }
                 // gather the slices of row r into lane r (fixed order)
                 for (int q = 1; q < 4; ++q) {
-                    int src = lane + q * max(M, 1);
-                    T t0 = __shfl(a0, src & 63, 64), t1 = __shfl(a1, src & 63, 64), t2 = __shfl(a2, src & 63, 64);
+                    const unsigned dl = (unsigned)(q * max(M, 1));  // lane + dl holds slice q of row r
+                    T t0 = __shfl_down(a0, dl, 64), t1 = __shfl_down(a1, dl, 64), t2 = __shfl_down(a2, dl, 64);
                     if (q < NS && sidx == 0) {
                         a0 += t0;
                         a1 += t1;
