@@ -127,6 +127,11 @@ int mld_gram_add_rows_dev(mld_ctx* ctx, int D, int mc, const double* Ac_dev, con
 int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F_dev, double w_e,
                        double w_f, int mixed);
 int mld_gram_export(mld_ctx* ctx, double* G, double* h);
+/* G (D x D symmetric, either layout) and h from the host into the device Gram: replaces it
+ * (accumulate = 0), e.g. with the MPI_Allreduce of every rank's mld_gram_export before the
+ * root's mld_gram_solve (module_online_fit.F90:563-567), or adds to it (accumulate = 1).
+ * Rows kept for the refinement are released (the solve then iterates on the Gram). */
+int mld_gram_import(mld_ctx* ctx, const double* G, const double* h, int accumulate);
 /* keep = 1: mld_gram_add_batch keeps its FP64 rows resident in HBM so that the
  * refinement residual of mld_gram_solve is r = A^T (y - A w) - Lambda w from the
  * rows themselves (used with the mixed Gram; bytes = rows * padded columns * 8).

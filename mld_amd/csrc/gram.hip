@@ -869,4 +869,27 @@ int mld_gram_export(mld_ctx* ctx, double* G, double* h)
     return MLD_OK;
 }
 
+int mld_gram_import(mld_ctx* ctx, const double* G, const double* h, int accumulate)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
+    if (!G || !h) return mld_fail(ctx, MLD_E_ARG, "mld_gram_import: null G or h");
+    const int D = ctx->gD, ld = ctx->gld;
+    MLD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<double> tmp((size_t)ld * ld, 0.0), th((size_t)ld, 0.0);
+    if (accumulate) {
+        MLD_HIP(ctx, hipMemcpy(tmp.data(), ctx->d_G, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        MLD_HIP(ctx, hipMemcpy(th.data(), ctx->d_h, th.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    // the upper triangle is the accumulated part of the device Gram
+    for (int i = 0; i < D; ++i)
+        for (int j = i; j < D; ++j) tmp[(size_t)i * ld + j] += G[(size_t)i * D + j];
+    for (int i = 0; i < D; ++i) th[i] += h[i];
+    MLD_HIP(ctx, hipMemcpy(ctx->d_G, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMemcpy(ctx->d_h, th.data(), th.size() * sizeof(double), hipMemcpyHostToDevice));
+    // rows kept for the refinement no longer describe the whole system
+    kept_rows_free(ctx);
+    return MLD_OK;
+}
+
 }  // extern "C"

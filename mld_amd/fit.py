@@ -88,6 +88,15 @@ class GramFit:
         self.ctx.check(self.ctx.L.mld_gram_add_rows(self.ctx.h, self.ncol, mc, A.ctypes.data_as(C.c_void_p),
                                                     y.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p)))
 
+    def import_(self, G: np.ndarray, h: np.ndarray, accumulate: bool = False):
+        """mld_gram_import: set (or add to) the device Gram from host G (ncol x ncol) and h."""
+        G = np.ascontiguousarray(G, np.float64)
+        h = np.ascontiguousarray(h, np.float64)
+        if G.shape != (self.ncol, self.ncol) or h.shape != (self.ncol,):
+            raise ValueError(f"import_: G must be ({self.ncol}, {self.ncol}) and h ({self.ncol},)")
+        self.ctx.check(self.ctx.L.mld_gram_import(self.ctx.h, G.ctypes.data_as(C.c_void_p),
+                                                  h.ctypes.data_as(C.c_void_p), int(accumulate)))
+
     def export(self) -> Tuple[np.ndarray, np.ndarray]:
         G = np.zeros((self.ncol, self.ncol))
         h = np.zeros(self.ncol)

@@ -255,3 +255,37 @@ def test_kernel_po_fp32_against_oracle():
         a0 += len(p)
         s0 += go.shape[0]
     ctx.close()
+
+
+def test_gram_import_replace_and_accumulate(ctx64):
+    """mld_gram_import: the reduce-then-solve path (export on two contexts, sum on the host,
+    import, solve) equals one context that saw every row."""
+    import mld_amd as M
+    from mld_amd.fit import GramFit
+
+    rng = np.random.default_rng(9)
+    D = 48
+    A = rng.normal(size=(700, D))
+    y = rng.normal(size=700)
+    w = rng.uniform(0.5, 2.0, size=700)
+    full = GramFit(ctx64, ncol=D)
+    full.add_rows(A, y, w)
+    G0, h0 = full.export()
+    w0, _ = full.solve(1e-6)
+    ctx2 = M.Context(j_max=4.0, r_cut=4.7, precision=M.PREC_FP64)
+    try:
+        g1 = GramFit(ctx64, ncol=D)
+        g1.add_rows(A[:300], y[:300], w[:300])
+        g2 = GramFit(ctx2, ncol=D)
+        g2.add_rows(A[300:], y[300:], w[300:])
+        Ga, ha = g1.export()
+        Gb, hb = g2.export()
+        g1.import_(Ga + Gb, ha + hb)               # replace with the reduced system
+        G1, h1 = g1.export()
+        assert rel(G1, G0) < 1e-13 and rel(h1, h0) < 1e-13
+        assert rel(g1.solve(1e-6)[0], w0) < 1e-10
+        g2.import_(Ga, ha, accumulate=True)        # add the other part
+        G2, h2 = g2.export()
+        assert rel(G2, G0) < 1e-13 and rel(h2, h0) < 1e-13
+    finally:
+        ctx2.close()
