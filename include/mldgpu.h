@@ -129,7 +129,7 @@ int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F
 int mld_gram_export(mld_ctx* ctx, double* G, double* h);
 /* G (D x D symmetric, either layout) and h from the host into the device Gram: replaces it
  * (accumulate = 0), e.g. with the MPI_Allreduce of every rank's mld_gram_export before the
- * root's mld_gram_solve (module_online_fit.F90:563-567), or adds to it (accumulate = 1).
+ * root's mld_gram_solve (module_online_fit.F90:566-567), or adds to it (accumulate = 1).
  * Rows kept for the refinement are released (the solve then iterates on the Gram). */
 int mld_gram_import(mld_ctx* ctx, const double* G, const double* h, int accumulate);
 /* keep = 1: mld_gram_add_batch keeps its FP64 rows resident in HBM so that the
