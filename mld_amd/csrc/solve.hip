@@ -24,21 +24,22 @@ constexpr int NB = 64;
 
 // L (n x n row-major, ld) lower: factor the diagonal block k in place and write the
 // inverse of the factored block to Linv[k] (64 x 64 row-major; rows and columns past n
-// carry the identity).  One wavefront: lane i owns row i of the block.
-__global__ void __launch_bounds__(64) chol_diag_kernel(int n, int ld, double* __restrict__ L, int k,
-                                                       int* __restrict__ info, double* __restrict__ Linv)
+// carry the identity).  256 threads factor (right-looking, the trailing update spread over
+// the workgroup); wavefront 0 then forms the inverse with lane c holding column c in
+// registers (fully unrolled forward substitution, L read by broadcast).
+__global__ void __launch_bounds__(256) chol_diag_kernel(int n, int ld, double* __restrict__ L, int k,
+                                                        int* __restrict__ info, double* __restrict__ Linv)
 {
     __shared__ double a[NB][NB + 1];
-    __shared__ double x[NB][NB + 1];
     const int k0 = k * NB, nb = min(NB, n - k0);
     const int t = threadIdx.x;
-    for (int q = t; q < NB * NB; q += NB) {
+    for (int q = t; q < NB * NB; q += blockDim.x) {
         int i = q / NB, j = q % NB;
         a[i][j] = (i < nb && j < nb) ? (j <= i ? L[(size_t)(k0 + i) * ld + k0 + j] : 0.0) : (i == j ? 1.0 : 0.0);
     }
     __syncthreads();
     for (int j = 0; j < NB; ++j) {
-        if (t == j) {
+        if (t == 0) {
             double d = a[j][j];
             if (!(d > 0.0)) {
                 if (*info == 0) *info = k0 + j + 1;
@@ -47,33 +48,32 @@ __global__ void __launch_bounds__(64) chol_diag_kernel(int n, int ld, double* __
             a[j][j] = sqrt(d);
         }
         __syncthreads();
-        double lij = 0.0;
-        if (t > j) {
-            lij = a[t][j] / a[j][j];
-            a[t][j] = lij;
+        const double djj = a[j][j];
+        for (int i = j + 1 + t; i < NB; i += blockDim.x) a[i][j] /= djj;
+        __syncthreads();
+        const int m = NB - j - 1;
+        for (int q = t; q < m * m; q += blockDim.x) {
+            const int i = j + 1 + q / m, c = j + 1 + q % m;
+            if (c <= i) a[i][c] -= a[i][j] * a[c][j];
         }
         __syncthreads();
-        if (t > j)
-            for (int c = j + 1; c <= t; ++c) a[t][c] -= lij * a[c][j];
-        __syncthreads();
     }
-    // column t of the inverse by forward substitution (lane t touches only column t of x)
-    for (int i = 0; i < NB; ++i) {
-        double v = 0.0;
-        if (i == t) {
-            v = 1.0 / a[i][i];
-        } else if (i > t) {
-            double s = 0.0;
-            for (int c = t; c < i; ++c) s += a[i][c] * x[c][t];
-            v = -s / a[i][i];
-        }
-        x[i][t] = v;
-    }
-    __syncthreads();
     double* Lk = Linv + (size_t)k * NB * NB;
-    for (int q = t; q < NB * NB; q += NB) {
-        int i = q / NB, j = q % NB;
-        Lk[q] = x[i][j];
+    if (t < NB) {
+        // column t of the inverse: x_i = (delta_it - sum_{s < i} a_is x_s) / a_ii
+        double x[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            double sacc = (i == t) ? 1.0 : 0.0;
+#pragma unroll
+            for (int s2 = 0; s2 < i; ++s2) sacc = fma(-a[i][s2], x[s2], sacc);
+            x[i] = (i < t) ? 0.0 : sacc / a[i][i];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) Lk[(size_t)i * NB + t] = x[i];
+    }
+    for (int q = t; q < NB * NB; q += blockDim.x) {
+        const int i = q / NB, j = q % NB;
         if (i < nb && j <= i) L[(size_t)(k0 + i) * ld + k0 + j] = a[i][j];
     }
 }
@@ -650,7 +650,7 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
         MLD_HIP(ctx, hipMemsetAsync(L, 0, bytes_L, st));
         hipLaunchKernelGGL(prep_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, dridge, shift, L, S);
         for (size_t k = 0; k < nb; ++k) {
-            hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(NB), 0, st, n, ld, L, (int)k, info, Linv);
+            hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, st, n, ld, L, (int)k, info, Linv);
             int nrem = (int)(nb - k - 1);
             if (nrem > 0) {
                 hipLaunchKernelGGL(chol_panel_kernel, dim3(nrem), dim3(256), 0, st, n, ld, L, (int)k, Linv);
