@@ -64,19 +64,17 @@ __global__ void __launch_bounds__(256) atom_kernel(int natoms, int D, const T* _
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     double epart = 0.0, dpart = 0.0;
     for (int d = lane; d < D; d += 64) {
+        // lanes take consecutive d, so each step over q reads one contiguous row of U / Sigma^-1
         double hd = 0.0;
-        if (U) {
-            const double* Ur = U + (size_t)d * D;
-            for (int q = 0; q < D; ++q) hd += Ur[q] * sB[wv][q];
-        }
+        if (U)  // U symmetric: (U B)_d = sum_q U[q][d] B_q
+            for (int q = 0; q < D; ++q) hd += U[(size_t)q * D + d] * sB[wv][q];
         const double gd = w[d] + hd;
         g[(size_t)a * D + d] = gd;
         epart += sB[wv][d] * (w[d] + 0.5 * hd);
         if (mu) {
-            // column d of Sigma^-1 dotted with v, times v_d (the reference's loop over d2)
-            const double* Sc = sinv + (size_t)d * D;
+            // d2 = sum_d v_d (Sigma^-1 v)_d = sum_d v_d sum_q v_q S[q][d] (S as given, any symmetry)
             double t = 0.0;
-            for (int q = 0; q < D; ++q) t += sV[wv][q] * Sc[q];
+            for (int q = 0; q < D; ++q) t += sV[wv][q] * sinv[(size_t)q * D + d];
             dpart += t * sV[wv][d];
         }
     }
@@ -104,18 +102,26 @@ __global__ void __launch_bounds__(256) slot_dot_kernel(int natoms, int D, const 
         gr[u] = d < D ? g[(size_t)a * D + d] : 0.0;
     }
     const size_t s0 = (size_t)pair_off[a] + a;
-    const int nslot = pair_off[a + 1] - pair_off[a] + 1;
-    for (int r = 0; r < 3 * nslot; ++r) {
-        const size_t row = s0 * 3 + r;
-        const T* src = grad + row * D;
-        double v = 0.0;
+    const int nrow = 3 * (pair_off[a + 1] - pair_off[a] + 1);
+    // four rows at a time: their loads are in flight together, their reductions interleave
+    for (int r = 0; r < nrow; r += 4) {
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int u = 0; u < DMAX / 64; ++u) {
-            int d = lane + 64 * u;
-            if (d < D) v += gr[u] * (double)src[d];
+        for (int k = 0; k < 4; ++k) {
+            if (r + k < nrow) {
+                const T* src = grad + (s0 * 3 + r + k) * D;
+#pragma unroll
+                for (int u = 0; u < DMAX / 64; ++u) {
+                    const int d = lane + 64 * u;
+                    if (d < D) v[k] += gr[u] * (double)src[d];
+                }
+            }
         }
-        v = wave_sum(v);
-        if (lane == 0) s[row] = v;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] += __shfl_xor(v[k], o, 64);
+        if (lane < 4 && r + lane < nrow) s[s0 * 3 + r + lane] = lane == 0 ? v[0] : lane == 1 ? v[1] : lane == 2 ? v[2] : v[3];
     }
 }
 
