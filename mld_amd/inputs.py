@@ -1,0 +1,187 @@
+"""The &input_ml namelist of a MILADY run and its mapping onto this library's parameters.
+
+Reads the keys of the fitting path (read_ml_file.F90: defaults :282-590, namelist
+:1800-1835; species weights :1618-1690 with the renormalisation of build_subdata.F90:2966-2987)
+and turns them into the arguments of ``mld_amd.Context`` and of the fit.  Keys outside the
+path are kept in the returned dict but not interpreted.
+"""
+from __future__ import annotations
+
+import math
+import re
+from typing import Any, Dict, List, Optional, Tuple
+
+import numpy as np
+
+# defaults of the keys this module interprets (read_ml_file.F90)
+DEFAULTS: Dict[str, Any] = {
+    "ml_type": 0,                 # -1 descriptors only, 0 linear / quadratic (snap), 1 kernel (:307)
+    "db_file": "db_model.in",     # :310
+    "db_path": "DB/",             # :311
+    "desc_forces": True,          # :323
+    "lambda_krr": -1.0,           # :332 (< 0: 0)
+    "kernel_type": 4,             # :338 (4 = polynomial kernel_po)
+    "kernel_power": 2.0,          # :345
+    "length_kernel": math.sqrt(0.5),  # :346
+    "sigma_kernel": 0.0,          # :347
+    "svd_rcond": -1.0,            # :354
+    "descriptor_type": 1,         # :372
+    "j_max": 1.5,                 # :424
+    "inv_r0_input": 1.0 - 0.02 / math.pi,  # :425
+    "lbso4_diag": False,          # :426
+    "r_cut": 5.0,                 # :428
+    "r_cut_width_in": 0.4,        # :443
+    "selection_type": 3,          # :489
+    "write_desc": False,          # :493
+    "weighted": False,            # :504
+    "weighted_3ch": False,        # :505
+    "weighted_auto": False,
+    "chemical_elements": "Fe",    # :508
+    "weight_per_element": "1.d0",      # :510
+    "weight_per_element_3ch": "1.d0",  # :511
+    "snap_order": -777,           # :591
+    "ace_numax": 3,               # :282
+    "ace_nmax_list": " 5 3 1",    # :283
+    "ace_lmax_list": " 0 4 2",    # :284
+    "ace_lambda_list": " 2.d0 2.d0 2.d0",  # :286
+}
+
+DESCRIPTOR_BSO4, DESCRIPTOR_ACE = 9, 300  # ml_in_ndm_module.F90:1312, 1320
+
+
+def _strip_comment(line: str) -> str:
+    out, q = [], None
+    for ch in line:
+        if q:
+            out.append(ch)
+            if ch == q:
+                q = None
+        elif ch in "'\"":
+            q = ch
+            out.append(ch)
+        elif ch == "!":
+            break
+        else:
+            out.append(ch)
+    return "".join(out)
+
+
+def _value(tok: str) -> Any:
+    t = tok.strip().rstrip(",").strip()
+    if len(t) >= 2 and t[0] == t[-1] and t[0] in "'\"":
+        return t[1:-1]
+    low = t.lower()
+    if low in (".true.", "t", ".t."):
+        return True
+    if low in (".false.", "f", ".f."):
+        return False
+    if re.fullmatch(r"[+-]?\d+", t):
+        return int(t)
+    try:
+        return float(low.replace("d", "e"))
+    except ValueError:
+        return t
+
+
+def parse_namelist(text: str, group: str = "input_ml") -> Dict[str, Any]:
+    """key = value pairs of the namelist group (Fortran literals: .true./.false., 1.d-8,
+    quoted strings; '!' comments; terminated by '&end' or '/')."""
+    body, inside = [], False
+    for raw in text.splitlines():
+        line = _strip_comment(raw).strip()
+        if not inside:
+            if line.lower().startswith("&" + group):
+                inside = True
+                line = line[len(group) + 1:].strip()
+            else:
+                continue
+        if line.lower().startswith("&end") or line == "/":
+            break
+        body.append(line)
+    joined = " ".join(body)
+    out: Dict[str, Any] = {}
+    # split at "key =" boundaries outside quotes
+    parts = re.split(r"""(?:^|[\s,])([A-Za-z_][A-Za-z0-9_%]*)\s*=(?=(?:[^'"]|'[^']*'|"[^"]*")*$)""", joined)
+    for k in range(1, len(parts) - 1, 2):
+        out[parts[k].lower()] = _value(parts[k + 1])
+    return out
+
+
+def read_input_ml(path: str) -> Dict[str, Any]:
+    """The namelist of a MILADY input file merged over the defaults of the interpreted keys."""
+    with open(path) as fh:
+        nml = parse_namelist(fh.read())
+    return {**DEFAULTS, **nml}
+
+
+def elements(nml: Dict[str, Any]) -> List[str]:
+    return str(nml["chemical_elements"]).split()
+
+
+def _floats(s: Any) -> List[float]:
+    return [float(str(t).lower().replace("d", "e")) for t in str(s).split()]
+
+
+def species_weights(nml: Dict[str, Any]) -> Tuple[Optional[np.ndarray], Optional[np.ndarray]]:
+    """Weights of the second and third chemical channels per species (read_ml_file.F90:1618-1690):
+    explicit weight_per_element(_3ch), or with weighted_auto Z and mass^2 + rcov^2 renormalised
+    by their root mean square / mean (build_subdata.F90:2966-2987)."""
+    from . import io
+
+    if not nml["weighted_3ch"] and not nml["weighted"]:
+        return None, None
+    els = elements(nml)
+    if nml["weighted_auto"]:
+        z = np.array([io.element_z(e) for e in els])
+        w1 = z / np.sqrt((z ** 2).mean())
+        w3 = None
+        if nml["weighted_3ch"]:
+            v = np.array([io.element_mass(e) ** 2 + io.element_rcov(e) ** 2 for e in els])
+            w3 = v / v.mean()
+        return w1, w3
+    w1 = np.array(_floats(nml["weight_per_element"]))
+    if len(w1) != len(els):
+        raise ValueError("weight_per_element and chemical_elements differ in length")
+    w3 = None
+    if nml["weighted_3ch"]:
+        w3 = np.array(_floats(nml["weight_per_element_3ch"]))
+        if len(w3) != len(els):
+            raise ValueError("weight_per_element_3ch and chemical_elements differ in length")
+    return w1, w3
+
+
+def context_kwargs(nml: Dict[str, Any], precision: int = 0) -> Dict[str, Any]:
+    """Arguments of mld_amd.Context for the run's descriptor (SO(4) bispectrum or ACE)."""
+    dt = int(nml["descriptor_type"])
+    kw: Dict[str, Any] = dict(r_cut=float(nml["r_cut"]), nspecies=max(1, len(elements(nml))), precision=precision)
+    w1, w3 = species_weights(nml)
+    if w1 is not None:
+        kw["weight"] = list(w1)
+    if w3 is not None:
+        kw["weight_3ch"] = list(w3)
+    if dt == DESCRIPTOR_BSO4:
+        kw.update(j_max=float(nml["j_max"]), inv_r0_input=float(nml["inv_r0_input"]),
+                  lbso4_diag=bool(nml["lbso4_diag"]))
+    elif dt == DESCRIPTOR_ACE:
+        numax = int(nml["ace_numax"])
+        kw.update(descriptor="ace", ace_numax=numax,
+                  ace_nmax=[int(x) for x in _floats(nml["ace_nmax_list"])][:numax],
+                  ace_lmax=[int(x) for x in _floats(nml["ace_lmax_list"])][:numax],
+                  ace_lambda=_floats(nml["ace_lambda_list"])[0])
+    else:
+        raise NotImplementedError(f"descriptor_type {dt}: only 9 (SO(4) bispectrum) and 300 (ACE) are built")
+    return kw
+
+
+def fit_order(nml: Dict[str, Any]) -> int:
+    """1 linear, 2 quadratic (snap_order), 3 kernel model (ml_type 1, read_ml_file.F90:846-851)."""
+    if int(nml["ml_type"]) == 1:
+        return 3
+    so = int(nml["snap_order"])
+    return 2 if so == 2 else 1
+
+
+def ridge(nml: Dict[str, Any]) -> float:
+    """lambda_krr < 0 means no ridge (module_online_fit.F90:571)."""
+    lam = float(nml["lambda_krr"])
+    return lam if lam > 0.0 else 0.0

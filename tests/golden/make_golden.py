@@ -3,6 +3,8 @@
 Run in a container where the reference tree is mounted (reads it as data only):
     python tests/golden/make_golden.py /root/reference
 Outputs (inputs + expected outputs, no code):
+  fit17_vacancy.ml, fit13_vacancy.ml, fit17_DB/*.poscar -- the namelists of fit17 / fit13
+                     and the 20 configurations fit17's db_model.in selects (input files)
   fit17_bso4.npz  -- tests/train_tests/fit17: 20 W configurations (DB_one poscars)
                      and REF/descDB/*.eml, SO(4) bispectrum j_max=4, r_cut=4.7, D=55
   fit19_bso4_3ch.npz -- tests/train_tests/fit19: 3 of the 22 Ta-Ti-V-W configurations
@@ -12,6 +14,7 @@ offsets (no pickled objects).
 """
 import glob
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -53,6 +56,15 @@ def main(ref):
     pairs = [(os.path.join(tt, "data_for_tests", "DB_HEA", s + ".poscar"),
               os.path.join(tt, "fit19", "REF", "descDB", s + ".eml")) for s in sel]
     np.savez_compressed(os.path.join(here, "fit19_bso4_3ch.npz"), **pack(pairs, ["Ta", "Ti", "V", "W"]))
+    # input files of fit17 and fit13 as the reference's runs read them: the namelists, and
+    # the 20 poscars fit17's db_model.in selects (into fit17_DB/), for the input-driven path
+    for fit in ("fit17", "fit13"):
+        shutil.copyfile(os.path.join(tt, fit, "vacancy.ml"), os.path.join(here, f"{fit}_vacancy.ml"))
+    db = os.path.join(here, "fit17_DB")
+    os.makedirs(db, exist_ok=True)
+    for eml in sorted(glob.glob(os.path.join(tt, "fit17", "REF", "descDB", "*.eml"))):
+        stem = os.path.basename(eml)[:-4]
+        shutil.copyfile(os.path.join(tt, "fit17", "DB", stem + ".poscar"), os.path.join(db, stem + ".poscar"))
     print("written")
 
 

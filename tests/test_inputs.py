@@ -1,0 +1,66 @@
+"""The &input_ml reader (mld_amd/inputs.py) on the reference's own input files (fit17, fit13) and on
+the Fortran literal forms, and the configuration selection of db_model.in (mld_amd/driver.py)."""
+import math
+import os
+
+import numpy as np
+
+from mld_amd import driver, inputs
+
+G = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_parse_reference_inputs():
+    n17 = inputs.read_input_ml(os.path.join(G, "fit17_vacancy.ml"))
+    assert n17["ml_type"] == -1 and n17["write_desc"] is True and n17["desc_forces"] is False
+    assert n17["descriptor_type"] == 9 and n17["j_max"] == 4.0 and n17["r_cut"] == 4.7
+    assert inputs.elements(n17) == ["W"]
+    assert n17["inv_r0_input"] == 1.0 - 0.02 / math.pi          # default, read_ml_file.F90:425
+    kw = inputs.context_kwargs(n17)
+    assert kw == dict(r_cut=4.7, nspecies=1, precision=0, j_max=4.0, inv_r0_input=1.0 - 0.02 / math.pi,
+                      lbso4_diag=False)
+    n13 = inputs.read_input_ml(os.path.join(G, "fit13_vacancy.ml"))
+    assert n13["ml_type"] == 1 and n13["snap_order"] == 1 and n13["lambda_krr"] == 1e-8
+    assert n13["kernel_type"] == 4 and n13["kernel_power"] == 4 and n13["length_kernel"] == 0.05
+    assert n13["classes_for_mcd"] == "07 08 13" and n13["db_path"] == "DB/"
+    assert inputs.fit_order(n13) == 3 and inputs.ridge(n13) == 1e-8
+    assert inputs.ridge(n17) == 0.0                              # lambda_krr = -1 -> 0
+
+
+def test_fortran_literals_and_comments():
+    text = """ junk before
+    &input_ml
+    a=1, b = 2.5d-3  c='x ! not a comment'   ! a comment
+    flag=.true. off=.FALSE.  name = "W  Ta"
+    e = -1.D+2
+    &end
+    z=5
+    """
+    n = inputs.parse_namelist(text)
+    assert n == {"a": 1, "b": 2.5e-3, "c": "x ! not a comment", "flag": True, "off": False, "name": "W  Ta",
+                 "e": -100.0}
+
+
+def test_weights_explicit_and_auto():
+    n = {**inputs.DEFAULTS, "chemical_elements": "Ta Ti V W", "weighted": True, "weighted_3ch": True,
+         "weight_per_element": "1.d0 2.d0 3.d0 4.d0", "weight_per_element_3ch": "4 3 2 1"}
+    w1, w3 = inputs.species_weights(n)
+    assert list(w1) == [1, 2, 3, 4] and list(w3) == [4, 3, 2, 1]
+    from conftest import fit19_weights
+
+    a1, a3 = inputs.species_weights({**n, "weighted_auto": True})
+    r1, r3 = fit19_weights()
+    assert np.allclose(a1, r1, rtol=1e-15) and np.allclose(a3, r3, rtol=1e-15)
+
+
+def test_select_fit17_configurations():
+    n = {**inputs.read_input_ml(os.path.join(G, "fit17_vacancy.ml")), "db_file": "fit17_db_model.in",
+         "db_path": "fit17_DB/"}
+    cfg = driver.select_configs(n, G)
+    gold = np.load(os.path.join(G, "fit17_bso4.npz"))
+    assert [c.name[:-7] for c, _ in cfg] == list(gold["names"])
+    off = gold["atom_off"]
+    for s, (c, cl) in enumerate(cfg):
+        assert np.array_equal(c.pos, gold["pos"][off[s]:off[s + 1]])
+        assert np.array_equal(c.cell, gold["cell"][s])
+    assert {cl.cls for _, cl in cfg} == {"01", "02", "03", "04", "05", "10", "12"}
