@@ -398,6 +398,101 @@ __global__ void __launch_bounds__(256) syrk_f64_kernel(int M, int ldA, int ntile
             }
 }
 
+// ---------------------------------------------------------------- FP64 SYRK, 128 x 128 tiles
+// grid.x = upper 128-tiles (ti <= tj) over ceil(ldA / 128) tiles, grid.y = row slices; 256
+// threads; wave (wi, wj) owns a 64 x 64 quarter = 4 x 4 MFMA 16 x 16 tiles.  Columns past ldA
+// read as zero.  Twice the work of a 64 x 64 tile per byte staged: the panel traffic from L2
+// falls to half, which the 64 x 64 kernel was bound by.
+constexpr int T2 = 128;
+__global__ void __launch_bounds__(256) syrk128_f64_kernel(int M, int ldA, int ntile, const double* __restrict__ A,
+                                                          int rows_per_slice, double* __restrict__ part)
+{
+    int t = blockIdx.x, ti = 0;
+    while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
+    const int tj = ti + t;
+    const int r0 = blockIdx.y * rows_per_slice;
+    const int r1 = min(M, r0 + rows_per_slice);
+    constexpr int KB = 16;
+    __shared__ double Ai[KB][T2 + 1];
+    __shared__ double Aj[KB][T2 + 1];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int wi = w >> 1, wj = w & 1;
+    d4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = d4{0, 0, 0, 0};
+    constexpr int PER = KB * T2 / 256;  // 8 elements per thread and panel
+    double pi_[PER], pj_[PER];
+    const int ci0 = ti * T2, cj0 = tj * T2;
+    auto fetch = [&](int rb) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = tid + 256 * u;
+            const int rr = q / T2, cc = q - rr * T2;
+            const int r = rb + rr;
+            const bool rin = r < r1;
+            pi_[u] = (rin && ci0 + cc < ldA) ? A[(size_t)r * ldA + ci0 + cc] : 0.0;
+            pj_[u] = (rin && cj0 + cc < ldA) ? A[(size_t)r * ldA + cj0 + cc] : 0.0;
+        }
+    };
+    if (r0 < r1) fetch(r0);
+    for (int rb = r0; rb < r1; rb += KB) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = tid + 256 * u;
+            const int rr = q / T2, cc = q - rr * T2;
+            Ai[rr][cc] = pi_[u];
+            Aj[rr][cc] = pj_[u];
+        }
+        __syncthreads();
+        if (rb + KB < r1) fetch(rb + KB);
+#pragma unroll
+        for (int k0 = 0; k0 < KB; k0 += 4) {
+            double af[4], bf[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) af[a] = Ai[k0 + (lane >> 4)][wi * 64 + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bf[b] = Aj[k0 + (lane >> 4)][wj * 64 + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+        }
+    }
+    // f64 16x16x4 output: register v of lane l holds C[l/16 + 4v][l%16]
+    double* P = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * T2 * T2;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = wi * 64 + a * 16 + (lane >> 4) + 4 * v;
+                const int j = wj * 64 + b * 16 + (lane & 15);
+                P[i * T2 + j] = acc[a][b][v];
+            }
+}
+
+// G (ldG x ldG, row-major) += sum over slices of the 128-tile partials, entries inside ldG
+__global__ void syrk128_reduce_kernel(int ntile, int nslice, int ldG, const double* __restrict__ part,
+                                      double* __restrict__ G)
+{
+    int t = blockIdx.x, ti = 0;
+    while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
+    const int tj = ti + t;
+    const int ntiles_up = gridDim.x;
+    for (int q = threadIdx.x; q < T2 * T2; q += blockDim.x) {
+        const int i = q / T2, j = q - i * T2;
+        const int gi = ti * T2 + i, gj = tj * T2 + j;
+        if (gi >= ldG || gj >= ldG) continue;
+        double s = 0.0;
+        for (int sl = 0; sl < nslice; ++sl) s += part[((size_t)sl * ntiles_up + blockIdx.x) * T2 * T2 + q];
+        G[(size_t)gi * ldG + gj] += s;
+    }
+}
+
 // ---------------------------------------------------------------- mixed SYRK
 __global__ void split_bf16_kernel(size_t n, const double* __restrict__ A, __bf16* __restrict__ hi,
                                   __bf16* __restrict__ mid, __bf16* __restrict__ lo)
@@ -573,19 +668,41 @@ int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, 
     if (M <= 0) return MLD_OK;
     hipStream_t st = ctx->stream;
     const int ldA = ctx->gld;
-    const int ntile = ldA / TILE;
-    const int nup = ntile * (ntile + 1) / 2;
-    int nslice = std::max(1, std::min(1024, (2048 + nup - 1) / nup));
-    int rps = (M + nslice - 1) / nslice;
-    rps = ((rps + 31) / 32) * 32;
-    nslice = (M + rps - 1) / rps;
     GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
-    int rc = device_ensure(ctx, &gs.part, &gs.cap_part, (size_t)nslice * nup * TILE * TILE * sizeof(double));
-    if (rc) return rc;
+    int rc = MLD_OK;
     if (!mixed) {
-        hipLaunchKernelGGL(syrk_f64_kernel, dim3(nup, nslice), dim3(256), 0, st, M, ldA, ntile, A, rps,
+        // 128 x 128 tiles over the columns rounded up to 128 (reads past ldA are zero)
+        const int nt2 = (ldA + T2 - 1) / T2, nup2 = nt2 * (nt2 + 1) / 2;
+        // row slices: the workgroup count closest to whole waves of 512 resident workgroups
+        // (two per CU at this kernel's register use), at least 256 rows a slice
+        const int cap = 512;
+        const int ns_hi = std::max(1, std::min(64, M / 256));
+        int ns2 = 1;
+        double best = -1.0;
+        for (int ns = 1; ns <= ns_hi; ++ns) {
+            const long long wg = (long long)nup2 * ns;
+            const double eff = (double)wg / (double)(((wg + cap - 1) / cap) * cap);
+            if (eff > best + 1e-9) { best = eff; ns2 = ns; }
+        }
+        int rps2 = (M + ns2 - 1) / ns2;
+        rps2 = ((rps2 + 31) / 32) * 32;
+        ns2 = (M + rps2 - 1) / rps2;
+        rc = device_ensure(ctx, &gs.part, &gs.cap_part, (size_t)ns2 * nup2 * T2 * T2 * sizeof(double));
+        if (rc) return rc;
+        hipLaunchKernelGGL(syrk128_f64_kernel, dim3(nup2, ns2), dim3(256), 0, st, M, ldA, nt2, A, rps2,
                            (double*)gs.part);
+        MLD_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(syrk128_reduce_kernel, dim3(nup2), dim3(256), 0, st, nt2, ns2, ldA,
+                           (const double*)gs.part, ctx->d_G);
     } else {
+        const int ntile = ldA / TILE;
+        const int nup = ntile * (ntile + 1) / 2;
+        int nslice = std::max(1, std::min(1024, (2048 + nup - 1) / nup));
+        int rps = (M + nslice - 1) / nslice;
+        rps = ((rps + 31) / 32) * 32;
+        nslice = (M + rps - 1) / rps;
+        rc = device_ensure(ctx, &gs.part, &gs.cap_part, (size_t)nslice * nup * TILE * TILE * sizeof(double));
+        if (rc) return rc;
         size_t n = (size_t)M * ldA;
         rc = device_ensure(ctx, &gs.split, &gs.cap_split, 3 * n * sizeof(__bf16));
         if (rc) return rc;
@@ -593,10 +710,11 @@ int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, 
         hipLaunchKernelGGL(split_bf16_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, A, h, h + n, h + 2 * n);
         hipLaunchKernelGGL(syrk_bf16_kernel<6>, dim3(nup, nslice), dim3(256), 0, st, M, ldA, ntile, h, h + n,
                            h + 2 * n, rps, 512, (double*)gs.part);
+        MLD_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(syrk_reduce_kernel, dim3(nup), dim3(256), 0, st, ntile, nslice, ldA,
+                           (const double*)gs.part, ctx->d_G);
     }
     MLD_HIP(ctx, hipGetLastError());
-    hipLaunchKernelGGL(syrk_reduce_kernel, dim3(nup), dim3(256), 0, st, ntile, nslice, ldA, (const double*)gs.part,
-                       ctx->d_G);
     // h
     int hs = std::max(1, std::min(512, M / 256));
     int hrps = (M + hs - 1) / hs;
