@@ -99,14 +99,19 @@ __global__ void rows_energy_kernel(int order, int D, int ldA, const int* __restr
 // pass (16, or 4 for descriptors wider than 192 components).
 __host__ __device__ inline int rows_force_ch(int D) { return D > 192 ? 4 : 16; }
 
-__host__ __device__ inline size_t rows_force_smem(int D, int ldA)
+// threads of rows_force_kernel: one 4 x 4 quadratic block each (3 * 136 blocks at D = 62)
+constexpr int RF_THREADS = 512;
+
+// LDS of rows_force_kernel: one staged output row (doubles), then the chunk's gradients and
+// descriptors in the descriptor precision (tsize bytes)
+__host__ __device__ inline size_t rows_force_smem(int D, int ldA, size_t tsize)
 {
     const int DP = (D + 3) & ~3, CH = rows_force_ch(D);
-    return sizeof(double) * ((size_t)CH * 3 * DP + (size_t)CH * DP + 3 * (size_t)ldA);
+    return sizeof(double) * (size_t)ldA + tsize * ((size_t)CH * 3 * DP + (size_t)CH * DP);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int ldA, const int* __restrict__ atom_struct,
+__global__ void __launch_bounds__(RF_THREADS) rows_force_kernel(int order, int D, int ldA, const int* __restrict__ atom_struct,
                                                          const int* __restrict__ atom_off, const int* __restrict__ pair_off,
                                                          const int* __restrict__ rev_off, const int* __restrict__ rev_pair,
                                                          const int* __restrict__ pair_src, const T* __restrict__ energy,
@@ -115,9 +120,9 @@ __global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int l
 {
     extern __shared__ __align__(16) double rsm[];
     const int DP = (D + 3) & ~3, CH = rows_force_ch(D);
-    double* sg = rsm;                                  // [CH][3][DP]
-    double* sb = sg + CH * 3 * DP;                     // [CH][DP]
-    double* out = sb + CH * DP;                        // [3][ldA]
+    double* out = rsm;                                 // [ldA], one force row at a time
+    T* sg = reinterpret_cast<T*>(out + ldA);           // [CH][3][DP]
+    T* sb = sg + CH * 3 * DP;                          // [CH][DP]
     const int k = blockIdx.x;
     const int tid = threadIdx.x;
     const int s = atom_struct[k];
@@ -127,36 +132,29 @@ __global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int l
     const int nb4 = DP >> 2;
     const int ntri = nb4 * (nb4 + 1) / 2;
     const int nquad = (order == 2) ? 3 * ntri : 0;
-    // quadratic items owned by this thread: it = tid, tid + 256 (x, bp, bq)
-    int qx[2], qp[2], qq[2];
-    bool qv[2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int it = tid + 256 * u;
-        qv[u] = it < nquad;
-        int x = 0, bp = 0, bq = 0;
-        if (qv[u]) {
-            x = it / ntri;
-            int t = it - x * ntri;
-            while (t >= nb4 - bp) { t -= nb4 - bp; ++bp; }
-            bq = bp + t;
-        }
-        qx[u] = x; qp[u] = bp; qq[u] = bq;
+    // the quadratic item of this thread: direction x, 4 x 4 block (bp, bq) of the upper triangle
+    const bool qv = tid < nquad;
+    int qx = 0, qp = 0, qq = 0;
+    if (qv) {
+        qx = tid / ntri;
+        int t = tid - qx * ntri;
+        while (t >= nb4 - qp) { t -= nb4 - qp; ++qp; }
+        qq = qp + t;
     }
-    double acc[2][4][4];
+    double acc[4][4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    // linear items: (x, d) = tid + RF_THREADS v over 3 * D (D <= 512)
+    constexpr int NLIN = 3 * 512 / RF_THREADS;
+    double lin[NLIN];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[u][i][j] = 0.0;
-    // linear items: (x, d) = tid + 256 v over 3 * D (D <= 512)
-    constexpr int NLIN = 6;
-    double lin[NLIN] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int v = 0; v < NLIN; ++v) lin[v] = 0.0;
     for (int c0 = 0; c0 < nslot; c0 += CH) {
         const int nc = min(CH, nslot - c0);
         __syncthreads();
-        for (int q = tid; q < nc * DP; q += 256) {
+        for (int q = tid; q < nc * DP; q += RF_THREADS) {
             const int sl = q / DP, d = q - sl * DP;
             const int idx = c0 + sl;
             size_t slot;
@@ -171,25 +169,23 @@ __global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int l
             }
             const bool in = d < D;
 #pragma unroll
-            for (int x = 0; x < 3; ++x) sg[(sl * 3 + x) * DP + d] = in ? (double)grad[(slot * 3 + x) * D + d] : 0.0;
-            sb[sl * DP + d] = in ? (double)energy[(size_t)centre * D + d] : 0.0;
+            for (int x = 0; x < 3; ++x) sg[(sl * 3 + x) * DP + d] = in ? grad[(slot * 3 + x) * D + d] : (T)0;
+            sb[sl * DP + d] = in ? energy[(size_t)centre * D + d] : (T)0;
         }
         __syncthreads();
 #pragma unroll
         for (int v = 0; v < NLIN; ++v) {
-            const int it = tid + 256 * v;
+            const int it = tid + RF_THREADS * v;
             if (it < 3 * D) {
                 const int x = it / D, d = it - x * D;
                 for (int sl = 0; sl < nc; ++sl) lin[v] += sg[(sl * 3 + x) * DP + d];
             }
         }
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            if (!qv[u]) continue;
-            const int p0 = 4 * qp[u], q0 = 4 * qq[u];
+        if (qv) {
+            const int p0 = 4 * qp, q0 = 4 * qq;
             for (int sl = 0; sl < nc; ++sl) {
-                const double* g = sg + (sl * 3 + qx[u]) * DP;
-                const double* bb = sb + sl * DP;
+                const T* g = sg + (sl * 3 + qx) * DP;
+                const T* bb = sb + sl * DP;
                 double gp[4], gq[4], bp[4], bq[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -201,39 +197,40 @@ __global__ void __launch_bounds__(256) rows_force_kernel(int order, int D, int l
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[u][i][j] += gp[i] * bq[j] + gq[j] * bp[i];
+                    for (int j = 0; j < 4; ++j) acc[i][j] += gp[i] * bq[j] + gq[j] * bp[i];
             }
         }
     }
-    // stage the three rows: zero, then linear and quadratic columns
-    for (int q = tid; q < 3 * ldA; q += 256) out[q] = 0.0;
-    __syncthreads();
-#pragma unroll
-    for (int v = 0; v < NLIN; ++v) {
-        const int it = tid + 256 * v;
-        if (it < 3 * D) {
-            const int x = it / D, d = it - x * D;
-            out[x * ldA + 1 + d] = -lin[v] * swf;
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        if (!qv[u]) continue;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int p = 4 * qp[u] + i;
-            if (p >= D) continue;
-            const int offp = 1 + D + p * D - p * (p - 1) / 2 - p;  // column of (p, q) = offp + q
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int q = 4 * qq[u] + j;
-                if (q < D && q >= p) out[qx[u] * ldA + offp + q] = -acc[u][i][j] * swf;
-            }
-        }
-    }
-    __syncthreads();
+    // stage and write the three rows one at a time: zero, linear and quadratic columns, copy out
     double* Ar = A + row0 * ldA;
-    for (int q = tid; q < 3 * ldA; q += 256) Ar[q] = out[q];
+    for (int x = 0; x < 3; ++x) {
+        __syncthreads();
+        for (int q = tid; q < ldA; q += RF_THREADS) out[q] = 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NLIN; ++v) {
+            const int it = tid + RF_THREADS * v;
+            if (it < 3 * D) {
+                const int xx = it / D, d = it - xx * D;
+                if (xx == x) out[1 + d] = -lin[v] * swf;
+            }
+        }
+        if (qv && qx == x) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int p = 4 * qp + i;
+                if (p >= D) continue;
+                const int offp = 1 + D + p * D - p * (p - 1) / 2 - p;  // column of (p, q) = offp + q
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int q = 4 * qq + j;
+                    if (q < D && q >= p) out[offp + q] = -acc[i][j] * swf;
+                }
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < ldA; q += RF_THREADS) Ar[(size_t)x * ldA + q] = out[q];
+    }
     if (tid < 3) y[row0 + tid] = F[3 * (size_t)k + tid] * swf;
 }
 
@@ -823,8 +820,8 @@ int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_de
         return MLD_OK;
     }
     if (natoms) {
-        const size_t fsm = rows_force_smem(D, ldA);
-        static bool attr_done = false;  // the largest layout (D = 192 linear, D = 62 quadratic) is < 110 KB
+        const size_t fsm = rows_force_smem(D, ldA, f32 ? sizeof(float) : sizeof(double));
+        static bool attr_done = false;  // the largest layout (D = 192 linear, D = 62 quadratic) is < 100 KB
         if (!attr_done) {
             MLD_HIP(ctx, hipFuncSetAttribute((const void*)rows_force_kernel<float>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
@@ -833,11 +830,11 @@ int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_de
             attr_done = true;
         }
         if (f32)
-            hipLaunchKernelGGL(rows_force_kernel<float>, dim3(natoms), dim3(256), fsm, st, order, D, ldA, ctx->d_atom_struct,
+            hipLaunchKernelGGL(rows_force_kernel<float>, dim3(natoms), dim3(RF_THREADS), fsm, st, order, D, ldA, ctx->d_atom_struct,
                                ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src,
                                (const float*)ctx->last_energy, (const float*)ctx->last_grad, F_dev, swf, A, y);
         else
-            hipLaunchKernelGGL(rows_force_kernel<double>, dim3(natoms), dim3(256), fsm, st, order, D, ldA, ctx->d_atom_struct,
+            hipLaunchKernelGGL(rows_force_kernel<double>, dim3(natoms), dim3(RF_THREADS), fsm, st, order, D, ldA, ctx->d_atom_struct,
                                ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src,
                                (const double*)ctx->last_energy, (const double*)ctx->last_grad, F_dev, swf, A, y);
     }
