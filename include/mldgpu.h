@@ -126,6 +126,16 @@ int mld_gram_add_rows_dev(mld_ctx* ctx, int D, int mc, const double* Ac_dev, con
  * accumulation (FP64 across tiles), 0 in FP64. */
 int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F_dev, double w_e,
                        double w_f, int mixed);
+/* A batch of configurations from host buffers, end to end: neighbours, descriptors with
+ * gradients (the context's descriptor and precision), (order 3: kernel_po features,) energy
+ * and force rows and the FP64 Gram update -- the per-configuration loop of
+ * module_online_fit.F90:992-1009 (calc_neighbours, compute_descriptors, pack_*_descriptor,
+ * gram%accumulate) for nstruct configurations at once.  nat (nstruct); pos (3, sum nat)
+ * column-major [atom][xyz]; cell (3, 3, nstruct) column-major, columns = lattice vectors;
+ * itype (sum nat) 0-based species; E (nstruct); F (3, sum nat) like pos; weights w_e, w_f. */
+int mld_gram_add_structures(mld_ctx* ctx, int order, int nstruct, const int* nat, const double* pos,
+                            const double* cell, const int* itype, const double* E, const double* F, double w_e,
+                            double w_f);
 int mld_gram_export(mld_ctx* ctx, double* G, double* h);
 /* G (D x D symmetric, either layout) and h from the host into the device Gram: replaces it
  * (accumulate = 0), e.g. with the MPI_Allreduce of every rank's mld_gram_export before the
@@ -161,6 +171,16 @@ int mld_gram_solve_svd(mld_ctx* ctx, double lambda, double rcond, double* w, dou
  * ((P+natoms)*3, nk) in the slot order above, or NULL for energies only. */
 int mld_kernel_po_set(mld_ctx* ctx, int nk, const double* zref, double sigma, double length, double power);
 int mld_kernel_po_dev(mld_ctx* ctx, void* ek_dev, void* fk_dev);
+/* One configuration from host buffers in the reference layouts: replaces the call of
+ * compute_kernel at descriptors.F90:67-75 (kernel_po branch, compute_kernel.F90:476-866).
+ * energy (D, nat) and force (D, nat, 0:nn_max, 3) column-major as mld_bso4 / mld_ace write
+ * them (config_desc%energy / %force), n_neigh (nat); out energy_kernel (nk, nat) and
+ * force_kernel (nk, nat, 0:nn_max, 3) column-major (config_desc%energy_kernel /
+ * %force_kernel, compute_kernel.F90:465-469; slots beyond n_neigh are zero).  force and
+ * force_kernel may both be NULL for energies only; n_neigh is always needed (an atom without
+ * neighbours has zero features, compute_kernel.F90:598-616).  Runs in the context's precision. */
+int mld_kernel_po(mld_ctx* ctx, int nat, const double* energy, const double* force, const int* n_neigh, int nn_max,
+                  double* energy_kernel, double* force_kernel);
 
 /* ---- prediction with a fitted model resident on the device ----
  * mld_model_set: w_params (product_w_params_Amat, mld.F90:40-86): order 1 has

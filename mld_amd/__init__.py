@@ -25,8 +25,8 @@ DESC_BSO4, DESC_ACE = 9, 300
 EXPORTS = [
     "mld_create", "mld_destroy", "mld_last_error", "mld_descriptor_dim", "mld_max_neighbours", "mld_bso4", "mld_ace",
     "mld_batch_neighbours", "mld_batch_pairs_dev", "mld_batch_pair_geometry_dev", "mld_batch_bso4_dev", "mld_gram_begin", "mld_gram_add_rows",
-    "mld_gram_add_rows_dev", "mld_gram_add_batch", "mld_gram_export", "mld_gram_import", "mld_gram_set_keep_rows", "mld_gram_solve", "mld_gram_solve_svd",
-    "mld_kernel_po_set", "mld_kernel_po_dev", "mld_model_set", "mld_mahalanobis_set", "mld_predict_dev",
+    "mld_gram_add_rows_dev", "mld_gram_add_batch", "mld_gram_add_structures", "mld_gram_export", "mld_gram_import", "mld_gram_set_keep_rows", "mld_gram_solve", "mld_gram_solve_svd",
+    "mld_kernel_po_set", "mld_kernel_po_dev", "mld_kernel_po", "mld_model_set", "mld_mahalanobis_set", "mld_predict_dev",
     "mld_serve_create", "mld_serve_submit", "mld_serve_poll", "mld_serve_stats", "mld_serve_destroy",
     "mld_get_stream", "mld_synchronize",
 ]
@@ -95,6 +95,7 @@ def load() -> C.CDLL:
         "mld_gram_add_rows": [vp, i, i, vp, vp, vp],
         "mld_gram_add_rows_dev": [vp, i, i, vp, vp, vp],
         "mld_gram_add_batch": [vp, i, vp, vp, d, d, i],
+        "mld_gram_add_structures": [vp, i, i, vp, vp, vp, vp, vp, vp, d, d],
         "mld_gram_export": [vp, vp, vp],
         "mld_gram_set_keep_rows": [vp, i],
         "mld_gram_import": [vp, vp, vp, i],
@@ -102,6 +103,7 @@ def load() -> C.CDLL:
         "mld_gram_solve_svd": [vp, d, d, vp, vp],
         "mld_kernel_po_set": [vp, i, vp, d, d, d],
         "mld_kernel_po_dev": [vp, vp, vp],
+        "mld_kernel_po": [vp, i, vp, vp, vp, i, vp, vp],
         "mld_model_set": [vp, i, i, vp, vp],
         "mld_mahalanobis_set": [vp, vp, vp],
         "mld_predict_dev": [vp, vp, vp, vp],
@@ -233,6 +235,36 @@ class Context:
                                    None if force is None else force.ctypes.data_as(C.c_void_p),
                                    _ptr(n_neigh), kind.ctypes.data_as(C.c_void_p), nn_max))
         return energy, force, n_neigh, kind
+
+    def kernel_po(self, energy: np.ndarray, force: Optional[np.ndarray], n_neigh: np.ndarray):
+        """compute_kernel (kernel_po) for one configuration from the arrays bso4 returns:
+        energy (D, nat), force (D, nat, nn_max+1, 3) Fortran-ordered or None for energies
+        only, n_neigh (nat; atoms without neighbours get zero features).
+        Returns energy_kernel (nk, nat) and force_kernel (nk, nat, nn_max+1, 3), Fortran-
+        ordered (config_desc%energy_kernel / %force_kernel); the reference points and
+        kernel parameters are those of fit.kernel_po_set."""
+        nk = getattr(self, "nk", 0)
+        if not nk:
+            raise MldError(MLD_E_STATE, "kernel_po_set first")
+        energy = np.asfortranarray(energy, np.float64)
+        D, nat = energy.shape
+        ek = np.zeros((nk, nat), order="F")
+        nnb = np.ascontiguousarray(n_neigh, np.int32)
+        if nnb.shape != (nat,):
+            raise ValueError(f"n_neigh must have {nat} entries")
+        fkr, nn_max = None, 0
+        if force is not None:
+            force = np.asfortranarray(force, np.float64)
+            if force.shape[:2] != (D, nat) or force.shape[3] != 3:
+                raise ValueError(f"force shape {force.shape} does not match energy {energy.shape}")
+            nn_max = force.shape[2] - 1
+            fkr = np.zeros((nk, nat, nn_max + 1, 3), order="F")
+        self.check(self.L.mld_kernel_po(self.h, nat, energy.ctypes.data_as(C.c_void_p),
+                                        None if force is None else force.ctypes.data_as(C.c_void_p),
+                                        nnb.ctypes.data_as(C.c_void_p), nn_max,
+                                        ek.ctypes.data_as(C.c_void_p),
+                                        None if fkr is None else fkr.ctypes.data_as(C.c_void_p)))
+        return ek, fkr
 
     def stream(self) -> int:
         s = C.c_void_p()

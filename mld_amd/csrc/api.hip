@@ -463,4 +463,63 @@ int mld_ace(mld_ctx* ctx, int nat, const double* pos, const double* cell, const 
     return mld_bso4(ctx, nat, pos, cell, itype, want_grad, energy, force, n_neigh, kind_neigh, nn_max);
 }
 
+int mld_gram_add_structures(mld_ctx* ctx, int order, int nstruct, const int* nat, const double* pos,
+                            const double* cell, const int* itype, const double* E, const double* F, double w_e,
+                            double w_f)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (nstruct < 0 || (nstruct > 0 && (!nat || !cell || !E)))
+        return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_structures: bad arguments");
+    if (order < 1 || order > 3) return mld_fail(ctx, MLD_E_ARG, "order must be 1, 2 or 3 (kernel)");
+    if (order == 3 && !ctx->d_zref) return mld_fail(ctx, MLD_E_STATE, "order 3 needs mld_kernel_po_set first");
+    std::vector<int> off((size_t)nstruct + 1, 0);
+    for (int s = 0; s < nstruct; ++s) {
+        if (nat[s] < 0) return mld_fail(ctx, MLD_E_ARG, "negative atom count");
+        if ((long long)off[s] + nat[s] > 0x7fffffffLL) return mld_fail(ctx, MLD_E_UNSUPPORTED, "too many atoms");
+        off[s + 1] = off[s] + nat[s];
+    }
+    const int N = off[nstruct];
+    if (N > 0 && (!pos || !itype || !F)) return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_structures: bad arguments");
+    for (int a = 0; a < N; ++a)
+        if (itype[a] < 0 || itype[a] >= ctx->nsp) return mld_fail(ctx, MLD_E_ARG, "itype out of range");
+    if (nstruct == 0) return MLD_OK;
+    DevBuf dpos, dcell, dtype, dF, den, dgr, dek, dfk;
+    // the batch state points into the buffers above: synchronise and clear it before they are freed
+    struct StateGuard {
+        mld_ctx* c;
+        ~StateGuard()
+        {
+            (void)hipStreamSynchronize(c->stream);
+            c->d_itype = nullptr;
+            c->last_energy = c->last_grad = nullptr;
+            c->last_ek = c->last_fk = nullptr;
+        }
+    } guard{ctx};
+    MLD_HIP(ctx, hipMalloc(&dpos.p, (size_t)3 * N * sizeof(double) + 8));
+    MLD_HIP(ctx, hipMalloc(&dcell.p, (size_t)9 * nstruct * sizeof(double)));
+    MLD_HIP(ctx, hipMalloc(&dtype.p, (size_t)N * sizeof(int) + 8));
+    MLD_HIP(ctx, hipMalloc(&dF.p, (size_t)3 * N * sizeof(double) + 8));
+    MLD_HIP(ctx, hipMemcpy(dpos.p, pos, (size_t)3 * N * sizeof(double), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMemcpy(dcell.p, cell, (size_t)9 * nstruct * sizeof(double), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMemcpy(dtype.p, itype, (size_t)N * sizeof(int), hipMemcpyHostToDevice));
+    MLD_HIP(ctx, hipMemcpy(dF.p, F, (size_t)3 * N * sizeof(double), hipMemcpyHostToDevice));
+    long long P = 0;
+    int rc = mld_batch_neighbours(ctx, nstruct, off.data(), (const double*)dpos.p, (const double*)dcell.p,
+                                  (const int*)dtype.p, &P);
+    if (rc) return rc;
+    const size_t esz = ctx->p.precision == MLD_PREC_FP32 ? sizeof(float) : sizeof(double);
+    const size_t nslot = (size_t)P + N;
+    MLD_HIP(ctx, hipMalloc(&den.p, (size_t)N * ctx->D * esz + 8));
+    MLD_HIP(ctx, hipMalloc(&dgr.p, nslot * 3 * ctx->D * esz + 8));
+    rc = mld_batch_bso4_dev(ctx, 1, den.p, dgr.p);
+    if (rc) return rc;
+    if (order == 3) {
+        MLD_HIP(ctx, hipMalloc(&dek.p, (size_t)N * ctx->nk * esz + 8));
+        MLD_HIP(ctx, hipMalloc(&dfk.p, nslot * 3 * ctx->nk * esz + 8));
+        rc = mld_kernel_po_dev(ctx, dek.p, dfk.p);
+        if (rc) return rc;
+    }
+    return mld_gram_add_batch(ctx, order, E, (const double*)dF.p, w_e, w_f, 0);
+}
+
 }  // extern "C"

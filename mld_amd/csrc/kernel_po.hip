@@ -8,6 +8,7 @@
 //   with g = dB/dr of that slot; slot 0 = -sum over the neighbour slots.
 // The two dot-product sets X Z^T and (dB/dr) Z^T are tiled GEMMs (64 x 64 tiles,
 // K = D staged in LDS); the kernel normalisation is the epilogue.
+#include <algorithm>
 #include <cmath>
 
 #include "mld_internal.h"
@@ -187,12 +188,52 @@ __global__ void kpo_self_kernel(int natoms, int nk, const int* __restrict__ pair
     }
 }
 
+// Compact slot-major gradients [(slot * 3 + x) * D + d] from the reference layout
+// force(D, nat, 0:nn_max, 3) (column-major), slot = pair_off[a] + a + k for k <= n_neigh(a).
 template <typename TD>
-int run(mld_ctx* ctx, TD* ek, TD* fk)
+__global__ void from_reference_layout(int nat, int nn_max, int D, const int* __restrict__ pair_off,
+                                      const double* __restrict__ force, TD* __restrict__ grad)
+{
+    const int a = blockIdx.x, k = blockIdx.y;
+    if (k > pair_off[a + 1] - pair_off[a]) return;
+    const size_t slot = (size_t)pair_off[a] + a + k;
+    for (int q = threadIdx.x; q < 3 * D; q += blockDim.x) {
+        const int x = q / D, d = q - x * D;
+        grad[(slot * 3 + x) * D + d] =
+            (TD)force[(size_t)d + (size_t)D * ((size_t)a + (size_t)nat * ((size_t)k + (size_t)(nn_max + 1) * x))];
+    }
+}
+
+// force_kernel(nk, nat, 0:nn_max, 3) (column-major) from the slot-major features
+// [(slot * 3 + x) * nk + c]; slots beyond n_neigh(a) are zero (compute_kernel.F90:528-529).
+template <typename TD>
+__global__ void features_to_reference_layout(int nat, int nn_max, int nk, const int* __restrict__ pair_off,
+                                             const TD* __restrict__ fk, double* __restrict__ out)
+{
+    const int a = blockIdx.x, k = blockIdx.y;
+    const bool in = k <= pair_off[a + 1] - pair_off[a];
+    const size_t slot = (size_t)pair_off[a] + a + k;
+    for (int q = threadIdx.x; q < 3 * nk; q += blockDim.x) {
+        const int x = q / nk, c = q - x * nk;
+        out[(size_t)c + (size_t)nk * ((size_t)a + (size_t)nat * ((size_t)k + (size_t)(nn_max + 1) * x))] =
+            in ? (double)fk[(slot * 3 + x) * nk + c] : 0.0;
+    }
+}
+
+template <typename TS, typename TD>
+__global__ void convert_kernel(size_t n, const TS* __restrict__ in, TD* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (TD)in[i];
+}
+
+// Features of natoms environments: energy [a][D], grad in the compact slot order over
+// pair_off (device, natoms + 1), P = pair_off[natoms].
+template <typename TD>
+int run(mld_ctx* ctx, int natoms, long long P, const int* pair_off, const TD* energy, const TD* grad, TD* ek, TD* fk)
 {
     hipStream_t st = ctx->stream;
-    const int natoms = ctx->natoms, D = ctx->D, nk = ctx->nk;
-    const long long P = ctx->npairs;
+    const int D = ctx->D, nk = ctx->nk;
     const double s2 = ctx->k_sigma * ctx->k_sigma, l2 = 2.0 * ctx->k_length * ctx->k_length, pw = ctx->k_power;
     size_t nrows = fk ? (size_t)3 * P : 0;
     size_t need = 3 * (size_t)natoms * sizeof(double) + (size_t)natoms * sizeof(int) +
@@ -215,12 +256,11 @@ int run(mld_ctx* ctx, TD* ek, TD* fk)
     double* gx = (double*)take(nrows * sizeof(double) + 8);
     TD* grows = (TD*)take(nrows * D * sizeof(TD) + 8);
     double* dot = (double*)take((size_t)natoms * nk * sizeof(double) + 8);
-    const TD* energy = (const TD*)ctx->last_energy;
-    const TD* grad = fk ? (const TD*)ctx->last_grad : nullptr;
+    if (!fk) grad = nullptr;
     if (natoms == 0) return MLD_OK;
     const int ipw = (pw == std::floor(pw) && pw >= 1.0 && pw <= 16.0) ? (int)pw : 0;
     const TD* Zt = (sizeof(TD) == 4) ? (const TD*)ctx->d_zref32 : (const TD*)ctx->d_zref;
-    hipLaunchKernelGGL(kpo_prep_kernel<TD>, dim3(natoms), dim3(128), 0, st, natoms, D, ctx->d_pair_off, energy, grad, xx, nn,
+    hipLaunchKernelGGL(kpo_prep_kernel<TD>, dim3(natoms), dim3(128), 0, st, natoms, D, pair_off, energy, grad, xx, nn,
                        row_atom, row_slot, gx, grows);
     hipLaunchKernelGGL(kpo_atom_kernel, dim3((natoms + 255) / 256), dim3(256), 0, st, natoms, xx, s2, l2, pw, ixa, fxa);
     const int nct = (nk + T - 1) / T;
@@ -236,7 +276,7 @@ int run(mld_ctx* ctx, TD* ek, TD* fk)
                            (const TD*)grows, Zt, ctx->d_kiz, ixa, fxa, row_atom, row_slot, gx, dot, s2, l2, pw, ipw,
                            (double*)nullptr, fk);
     }
-    if (fk) hipLaunchKernelGGL(kpo_self_kernel<TD>, dim3((nk + 255) / 256, natoms), dim3(256), 0, st, natoms, nk, ctx->d_pair_off, fk);
+    if (fk) hipLaunchKernelGGL(kpo_self_kernel<TD>, dim3((nk + 255) / 256, natoms), dim3(256), 0, st, natoms, nk, pair_off, fk);
     MLD_HIP(ctx, hipGetLastError());
     return MLD_OK;
 }
@@ -292,11 +332,101 @@ extern "C" int mld_kernel_po_dev(mld_ctx* ctx, void* ek_dev, void* fk_dev)
     if (!ctx->last_energy) return mld_fail(ctx, MLD_E_STATE, "mld_batch_bso4_dev first");
     if (fk_dev && !ctx->last_grad) return mld_fail(ctx, MLD_E_STATE, "gradient features need mld_batch_bso4_dev with gradients");
     if (!ek_dev) return mld_fail(ctx, MLD_E_ARG, "null ek");
-    int rc = (ctx->p.precision == MLD_PREC_FP32) ? run<float>(ctx, (float*)ek_dev, (float*)fk_dev)
-                                                 : run<double>(ctx, (double*)ek_dev, (double*)fk_dev);
+    int rc = (ctx->p.precision == MLD_PREC_FP32)
+                 ? run<float>(ctx, ctx->natoms, ctx->npairs, ctx->d_pair_off, (const float*)ctx->last_energy,
+                              (const float*)ctx->last_grad, (float*)ek_dev, (float*)fk_dev)
+                 : run<double>(ctx, ctx->natoms, ctx->npairs, ctx->d_pair_off, (const double*)ctx->last_energy,
+                               (const double*)ctx->last_grad, (double*)ek_dev, (double*)fk_dev);
     if (rc == MLD_OK) {
         ctx->last_ek = ek_dev;
         ctx->last_fk = fk_dev;
     }
     return rc;
+}
+
+namespace {
+
+struct KpoBuf {
+    void* p = nullptr;
+    ~KpoBuf() { if (p) (void)hipFree(p); }
+};
+
+// Host-buffer kernel features of one configuration in the reference layouts.
+template <typename TD>
+int kernel_po_host(mld_ctx* ctx, int nat, const double* energy, const double* force, const std::vector<int>& po,
+                   int nn_max, double* energy_kernel, double* force_kernel)
+{
+    hipStream_t st = ctx->stream;
+    const int D = ctx->D, nk = ctx->nk;
+    const long long P = po[nat];
+    const size_t nslot = (size_t)P + nat, ne = (size_t)nat * D;
+    KpoBuf dpo, de64, de, df64, dg, dek, dfk, dout;
+    MLD_HIP(ctx, hipMalloc(&dpo.p, (size_t)(nat + 1) * sizeof(int)));
+    MLD_HIP(ctx, hipMemcpyAsync(dpo.p, po.data(), (size_t)(nat + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+    // energy (D, nat) column-major is [atom][D] row-major
+    MLD_HIP(ctx, hipMalloc(&de64.p, ne * sizeof(double) + 8));
+    MLD_HIP(ctx, hipMemcpyAsync(de64.p, energy, ne * sizeof(double), hipMemcpyHostToDevice, st));
+    const TD* e = (const TD*)de64.p;
+    if (sizeof(TD) != sizeof(double)) {
+        MLD_HIP(ctx, hipMalloc(&de.p, ne * sizeof(TD) + 8));
+        if (ne) hipLaunchKernelGGL((convert_kernel<double, TD>), dim3((ne + 255) / 256), dim3(256), 0, st, ne,
+                                   (const double*)de64.p, (TD*)de.p);
+        e = (const TD*)de.p;
+    }
+    const TD* g = nullptr;
+    if (force) {
+        const size_t nf = (size_t)D * nat * (nn_max + 1) * 3;
+        MLD_HIP(ctx, hipMalloc(&df64.p, nf * sizeof(double) + 8));
+        MLD_HIP(ctx, hipMemcpyAsync(df64.p, force, nf * sizeof(double), hipMemcpyHostToDevice, st));
+        MLD_HIP(ctx, hipMalloc(&dg.p, nslot * 3 * D * sizeof(TD) + 8));
+        hipLaunchKernelGGL(from_reference_layout<TD>, dim3(nat, nn_max + 1), dim3(256), 0, st, nat, nn_max, D,
+                           (const int*)dpo.p, (const double*)df64.p, (TD*)dg.p);
+        g = (const TD*)dg.p;
+    }
+    const size_t nek = (size_t)nat * nk;
+    MLD_HIP(ctx, hipMalloc(&dek.p, nek * sizeof(TD) + 8));
+    if (force) MLD_HIP(ctx, hipMalloc(&dfk.p, nslot * 3 * nk * sizeof(TD) + 8));
+    int rc = run<TD>(ctx, nat, P, (const int*)dpo.p, e, g, (TD*)dek.p, force ? (TD*)dfk.p : nullptr);
+    if (rc) return rc;
+    // energy_kernel (nk, nat) column-major is [atom][nk] row-major
+    const size_t nout = force ? std::max(nek, (size_t)nk * nat * (nn_max + 1) * 3) : nek;
+    MLD_HIP(ctx, hipMalloc(&dout.p, nout * sizeof(double) + 8));
+    if (nek) hipLaunchKernelGGL((convert_kernel<TD, double>), dim3((nek + 255) / 256), dim3(256), 0, st, nek,
+                                (const TD*)dek.p, (double*)dout.p);
+    MLD_HIP(ctx, hipGetLastError());
+    MLD_HIP(ctx, hipMemcpyAsync(energy_kernel, dout.p, nek * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (force) {
+        MLD_HIP(ctx, hipStreamSynchronize(st));
+        hipLaunchKernelGGL(features_to_reference_layout<TD>, dim3(nat, nn_max + 1), dim3(256), 0, st, nat, nn_max, nk,
+                           (const int*)dpo.p, (const TD*)dfk.p, (double*)dout.p);
+        MLD_HIP(ctx, hipGetLastError());
+        MLD_HIP(ctx, hipMemcpyAsync(force_kernel, dout.p, (size_t)nk * nat * (nn_max + 1) * 3 * sizeof(double),
+                                    hipMemcpyDeviceToHost, st));
+    }
+    MLD_HIP(ctx, hipStreamSynchronize(st));
+    return MLD_OK;
+}
+
+}  // namespace
+
+extern "C" int mld_kernel_po(mld_ctx* ctx, int nat, const double* energy, const double* force, const int* n_neigh,
+                             int nn_max, double* energy_kernel, double* force_kernel)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (!ctx->d_zref) return mld_fail(ctx, MLD_E_STATE, "mld_kernel_po_set first");
+    if (nat < 0 || nn_max < 0 || (nat > 0 && (!energy || !energy_kernel || !n_neigh)) || (force && !force_kernel) ||
+        (!force && force_kernel))
+        return mld_fail(ctx, MLD_E_ARG, "mld_kernel_po: bad arguments");
+    // n_neigh is needed for energies too: an atom without neighbours has zero features
+    // (compute_kernel.F90:598-616)
+    std::vector<int> po((size_t)nat + 1, 0);
+    for (int a = 0; a < nat; ++a) {
+        const int n = n_neigh[a];
+        if (n < 0 || (force && n > nn_max)) return mld_fail(ctx, MLD_E_ARG, "mld_kernel_po: n_neigh outside 0..nn_max");
+        po[a + 1] = po[a] + n;
+    }
+    if (nat == 0) return MLD_OK;
+    return (ctx->p.precision == MLD_PREC_FP32)
+               ? kernel_po_host<float>(ctx, nat, energy, force, po, nn_max, energy_kernel, force_kernel)
+               : kernel_po_host<double>(ctx, nat, energy, force, po, nn_max, energy_kernel, force_kernel);
 }

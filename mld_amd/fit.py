@@ -15,7 +15,7 @@ layout (w_pq = w_qp = u_pq / 2 off the diagonal).
 from __future__ import annotations
 
 import ctypes as C
-from typing import Optional, Tuple
+from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -76,6 +76,31 @@ class GramFit:
         torch_ready(F_dev.device)
         self.ctx.check(self.ctx.L.mld_gram_add_batch(self.ctx.h, self.order, E.ctypes.data_as(C.c_void_p),
                                                      C.c_void_p(F_dev.data_ptr()), w_e, w_f, int(self.mixed)))
+
+    def add_structures(self, pos: Sequence[np.ndarray], cells: Sequence[np.ndarray], types: Sequence[np.ndarray],
+                       E: np.ndarray, F: Sequence[np.ndarray], w_e: float = 1.0, w_f: float = 1.0):
+        """Configurations from host arrays, end to end (mld_gram_add_structures): pos[s] (nat, 3),
+        cells[s] (3, 3) with the lattice vectors as rows, types[s] (nat) 0-based, E (nstruct),
+        F[s] (nat, 3).  Order 3 uses the reference points of kernel_po_set."""
+        if self.mixed:
+            raise ValueError("add_structures forms the Gram in FP64; use add_batch for the split-BF16 form")
+        nat = np.array([len(p) for p in pos], np.int32)
+        P = np.ascontiguousarray(np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p in pos])
+                                 if len(pos) else np.zeros((0, 3)))
+        # (3, 3, nstruct) column-major with columns = lattice vectors: row-major rows per structure
+        Cc = np.ascontiguousarray(np.stack([np.asarray(c, np.float64).reshape(3, 3) for c in cells])
+                                  if len(cells) else np.zeros((0, 3, 3)))
+        T = np.ascontiguousarray(np.concatenate([np.asarray(t, np.int32) for t in types])
+                                 if len(types) else np.zeros(0, np.int32))
+        Fh = np.ascontiguousarray(np.concatenate([np.asarray(f, np.float64).reshape(-1, 3) for f in F])
+                                  if len(F) else np.zeros((0, 3)))
+        E = np.ascontiguousarray(E, np.float64)
+        if len(cells) != len(pos) or len(types) != len(pos) or E.shape != (len(pos),) or Fh.shape != P.shape \
+                or T.shape != (P.shape[0],):
+            raise ValueError("add_structures: pos, cells, types, E and F describe different configurations")
+        v = lambda a: a.ctypes.data_as(C.c_void_p)
+        self.ctx.check(self.ctx.L.mld_gram_add_structures(self.ctx.h, self.order, len(pos), v(nat), v(P), v(Cc), v(T),
+                                                          v(E), v(Fh), w_e, w_f))
 
     def add_rows(self, Ac: np.ndarray, y: np.ndarray, w: np.ndarray):
         """gram_accumulate_config for caller-packed columns Ac (D, mc) (numpy: shape (mc, D) C-order)."""

@@ -162,6 +162,93 @@ def test_kernel_po_against_oracle(ctx64):
         s0 += go.shape[0]
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+def test_kernel_po_host_buffers_reference_layout(fp32):
+    """mld_kernel_po on mld_bso4's reference-layout arrays equals the oracle's kernel_po of the
+    same descriptors, slot by slot; slots beyond n_neigh are zero; energies-only agrees."""
+    import mld_amd as M
+    from mld_amd.fit import kernel_po_set
+
+    ctx = M.Context(j_max=4.0, r_cut=4.7, precision=M.PREC_FP32 if fp32 else M.PREC_FP64)
+    try:
+        rng = np.random.default_rng(31)
+        pos, cell = synth.bcc_supercell(rng, 2.855, 2, 0.1)
+        types = np.zeros(len(pos), np.int32)
+        nat = len(pos)
+        e, f, nn, _ = ctx.bso4(pos, cell, types, nn_max=ctx.max_neighbours(pos, cell) + 2)
+        draft = e.T[rng.choice(nat, 6, replace=False)] * (1 + 0.05 * rng.normal(size=(6, e.shape[0])))
+        if fp32:  # FP32 dot products: a well-conditioned kernel (as the FP32 batch test)
+            z = draft
+            prm = (1.0, float(np.sqrt(np.mean((e ** 2).sum(0)) / 2)), 2.0)
+        else:
+            z = O.standardise_kernel_points(draft)
+            prm = (0.0, 0.05, 4.0)
+        kernel_po_set(ctx, z, *prm)
+        ek, fk = ctx.kernel_po(e, f, nn)
+        off = np.concatenate([[0], np.cumsum(nn)]).astype(np.int32)
+        slots = [(a, k) for a in range(nat) for k in range(nn[a] + 1)]
+        grad = np.ascontiguousarray(np.array([[f[:, a, k, x] for x in range(3)] for a, k in slots]))
+        ko, fo = O.kernel_po(np.ascontiguousarray(e.T), grad, (off,), z, *prm)
+        tol_e, tol_f = (2e-5, 2e-4) if fp32 else (1e-11, 1e-9)
+        assert rel(ek.T, ko) < tol_e
+        got = np.array([fk[:, a, k, :].T for a, k in slots])
+        assert rel(got, fo) < tol_f
+        for a in range(nat):
+            assert not fk[:, a, nn[a] + 1:, :].any()
+        ek2, none = ctx.kernel_po(e, None, nn)
+        assert none is None and np.array_equal(ek2, ek)
+        # an atom without neighbours has zero features (compute_kernel.F90:598-616)
+        nn0 = nn.copy()
+        nn0[0] = 0
+        ek3, _ = ctx.kernel_po(e, None, nn0)
+        assert not ek3[:, 0].any() and np.array_equal(ek3[:, 1:], ek[:, 1:])
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("order,fp32", [(1, False), (2, True), (3, False)])
+def test_add_structures_equals_device_batch(order, fp32):
+    """mld_gram_add_structures (host arrays, end to end) gives the same G and h, bit for bit, as
+    the device-batch sequence neighbours -> descriptors (-> kernel features) -> add_batch."""
+    import mld_amd as M
+    from mld_amd import fit
+
+    ctx = M.Context(j_max=4.0, r_cut=4.7, precision=M.PREC_FP32 if fp32 else M.PREC_FP64)
+    try:
+        b, pos, cells, types, E, F = small_batch(ctx, seed=40 + order, nstruct=3)
+        nk = 0
+        if order == 3:
+            rng = np.random.default_rng(3)
+            draft = np.concatenate([O.bso4(p, c, t, 8, 4.7, want_grad=False)[0] for p, c, t in zip(pos, cells, types)])
+            z = draft[rng.choice(len(draft), 9, replace=False)]
+            fit.kernel_po_set(ctx, z, 1.0, float(np.sqrt(np.mean((draft ** 2).sum(1)) / 2)), 2.0)
+            nk = 9
+        import torch
+
+        Fd = torch.from_numpy(np.ascontiguousarray(F.reshape(-1))).to(b.device)
+        ref = fit.GramFit(ctx, order=order, nk=nk)
+        if order == 3:
+            ek, fk = fit.kernel_po_alloc(ctx, b)
+            fit.kernel_po(ctx, ek, fk)
+        ref.add_batch(b, E, Fd, 1.0, 0.3)
+        G0, h0 = ref.export()
+        gf = fit.GramFit(ctx, order=order, nk=nk)
+        n = [len(p) for p in pos]
+        Fs = np.split(F.reshape(-1, 3), np.cumsum(n)[:-1])
+        gf.add_structures(pos, cells, types, E, Fs, 1.0, 0.3)
+        G1, h1 = gf.export()
+        assert np.array_equal(G1, G0) and np.array_equal(h1, h0)
+        assert np.abs(G0).max() > 0
+        # a second call accumulates
+        gf.add_structures(pos[:1], cells[:1], types[:1], E[:1], Fs[:1], 1.0, 0.3)
+        G2, _ = gf.export()
+        assert np.abs(G2 - G1).max() > 0
+        with pytest.raises(ValueError):
+            gf.add_structures(pos, cells, types, E[:2], Fs)
+    finally:
+        ctx.close()
+
+
 def krr_rows_numpy(e, g, nl, ek, fk):
     """Kernel-model rows [N, B, k] (energy) and [0, -sum dB, -sum dk] (forces) from the
     oracle's descriptors and kernel features, restating kernel_pack_energy_descriptor /
