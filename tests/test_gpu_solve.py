@@ -118,3 +118,36 @@ def test_svd_quadratic_fold_matches_full_layout_solution(ctx64, lam):
     if lam == 0.0:
         plain = np.linalg.lstsq(Ru, yu, rcond=1e-10)[0]
         assert rel(expand_quadratic(plain, D), wf) > 1e-3
+
+
+def test_dense_fit_phi_formulation(ctx64):
+    """train_snap_get_parameters_linear (parameters.F90:1064-1146): phi = Amat W Amat^T + lambda I,
+    rhs = Amat W ymat, solved by dgelsd (fit_lapack_svd) -- restated with scipy's gelsd on a
+    rank-deficient Amat (a repeated column), against add_rows + the minimum-norm solve; and the
+    full-rank fit types against the Cholesky solve."""
+    import scipy.linalg as sl
+    from mld_amd.fit import GramFit, SOLVE_CHOL_IR, SOLVE_SVD
+
+    rng = np.random.default_rng(17)
+    D, M = 23, 400
+    Amat = rng.normal(size=(D, M)) * np.linspace(0.5, 4.0, D)[:, None]   # (dim, observations)
+    ymat = rng.normal(size=M)
+    wts = rng.uniform(0.2, 3.0, size=M)
+    lam = 1e-3
+    phi = (Amat * wts) @ Amat.T + lam * np.eye(D)
+    rhs = (Amat * wts) @ ymat
+    gf = GramFit(ctx64, ncol=D)
+    gf.add_rows(Amat.T, ymat, wts)
+    w_chol, _ = gf.solve(lam, SOLVE_CHOL_IR)
+    assert rel(w_chol, np.linalg.solve(phi, rhs)) < 1e-10
+    # rank-deficient: column 7 repeated as column 11, lambda = 0, svd_rcond above the rounding floor
+    Ad = Amat.copy()
+    Ad[11] = Ad[7]
+    phi0 = (Ad * wts) @ Ad.T
+    rhs0 = (Ad * wts) @ ymat
+    ref = sl.lstsq(phi0, rhs0, cond=1e-12, lapack_driver="gelsd")[0]
+    gd = GramFit(ctx64, ncol=D)
+    gd.add_rows(Ad.T, ymat, wts)
+    w_svd, _ = gd.solve(0.0, SOLVE_SVD, rcond=1e-12)
+    assert rel(w_svd, ref) < 1e-8, rel(w_svd, ref)
+    assert abs(w_svd[7] - w_svd[11]) < 1e-9 * abs(w_svd).max()
