@@ -121,8 +121,8 @@ constexpr int cg_level_max(int jj)
 
 template <typename T>
 struct Geo {
-    cx<T> zp, zm, xp, xm;
-    cx<T> dzp[3], dzm[3], dxp[3], dxm[3];
+    cx<T> zp, xp;  // z- = conj(z+), x- = conj(x+) (and their derivatives) are not stored
+    cx<T> dzp[3], dxp[3];
     T fc, dfc;
     T drr[3];
 };
@@ -162,9 +162,7 @@ __device__ __forceinline__ Geo<T> make_geo(double4 d, double rcut, double r0, do
     double l0 = rr * sin_i, l0i = 1.0 / l0, z0 = rr * tan_i;
     Geo<T> g;
     g.zp = cx<T>{(T)(z0 * l0i), (T)(z * l0i)};
-    g.zm = cx<T>{(T)(z0 * l0i), (T)(-z * l0i)};
     g.xp = cx<T>{(T)(x * l0i), (T)(y * l0i)};
-    g.xm = cx<T>{(T)(x * l0i), (T)(-y * l0i)};
     g.fc = (T)fc;
     g.dfc = (T)dfc;
     double drr[3] = {x / rr, y / rr, z / rr};
@@ -175,24 +173,18 @@ __device__ __forceinline__ Geo<T> make_geo(double4 d, double rcut, double r0, do
         for (int k = 0; k < 3; ++k) {
             double a = dz * l0i;
             g.dzp[k] = cx<T>{(T)((z0 * dil0 + a) * drr[k]), (T)(z * dil0 * drr[k])};
-            g.dzm[k] = cx<T>{(T)((z0 * dil0 + a) * drr[k]), (T)(-z * dil0 * drr[k])};
             g.dxp[k] = cx<T>{(T)(x * dil0 * drr[k]), (T)(y * dil0 * drr[k])};
-            g.dxm[k] = cx<T>{(T)(x * dil0 * drr[k]), (T)(-y * dil0 * drr[k])};
         }
         g.dzp[2].i += (T)l0i;
-        g.dzm[2].i -= (T)l0i;
         g.dxp[0].r += (T)l0i;
         g.dxp[1].i += (T)l0i;
-        g.dxm[0].r += (T)l0i;
-        g.dxm[1].i -= (T)l0i;
     }
     return g;
 }
 
-// Packed per-pair geometry: zp, xp, dzp[3], dxp[3] (complex), fc, dfc, rhat[3]; zm, xm,
-// dzm, dxm are their conjugates (make_geo builds them with the same magnitudes and
-// opposite signs).  Written once per pair by pair_geo_kernel, read by every wavefront
-// that walks the pair.
+// Packed per-pair geometry: zp, xp, dzp[3], dxp[3] (complex), fc, dfc, rhat[3]; z-, x- and
+// their derivatives are the conjugates.  Written once per pair by pair_geo_kernel, read by
+// every wavefront that walks the pair.
 constexpr int GEO_STRIDE = 24;
 
 template <typename T>
@@ -200,9 +192,7 @@ __device__ __forceinline__ Geo<T> load_geo(const T* __restrict__ o, bool grad)
 {
     Geo<T> g;
     g.zp = cx<T>{o[0], o[1]};
-    g.zm = cx<T>{o[0], -o[1]};
     g.xp = cx<T>{o[2], o[3]};
-    g.xm = cx<T>{o[2], -o[3]};
     g.fc = o[16];
     g.dfc = o[17];
     g.drr[0] = o[18];
@@ -212,9 +202,7 @@ __device__ __forceinline__ Geo<T> load_geo(const T* __restrict__ o, bool grad)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             g.dzp[k] = cx<T>{o[4 + 2 * k], o[5 + 2 * k]};
-            g.dzm[k] = cx<T>{o[4 + 2 * k], -o[5 + 2 * k]};
             g.dxp[k] = cx<T>{o[10 + 2 * k], o[11 + 2 * k]};
-            g.dxm[k] = cx<T>{o[10 + 2 * k], -o[11 + 2 * k]};
         }
     }
     return g;
@@ -259,39 +247,50 @@ __global__ void __launch_bounds__(256) pair_geo_kernel(int natoms, const int* __
     }
 }
 
-// One recursion step for entry (a, b) of level j from level j-1 (Up, dUp).
+// A recursion entry in the working precision, staged from the table once per phase:
+// c1, c2 with the negation of their source folded in; s1, s2 = (index << 2) | (conjugate << 1).
+template <typename T>
+struct RecT {
+    T c1, c2;
+    int s1, s2;
+};
+
+template <typename T>
+__device__ __forceinline__ RecT<T> stage_rec(const mld::RecEntry& e)
+{
+    return RecT<T>{(T)((e.s1 & 1) ? -e.c1 : e.c1), (T)((e.s2 & 1) ? -e.c2 : e.c2), e.s1, e.s2};
+}
+
+// One recursion step for entry (a, b) of level j from level j-1 (Up, dUp):
+//   u = c1 zq U1 - i c2 xq U2, du = c1 (dzq U1 + zq dU1) - i c2 (dxq U2 + xq dU2),
+// zq = z+ and xq = x- for b = 0, z- and x+ otherwise.  z- = conj(z+), x- = conj(x+) and
+// likewise their derivatives, so the choice is a sign on the imaginary parts, folded into
+// the coefficients together with the source negations (both exact).
 template <typename T, bool GRAD>
-__device__ __forceinline__ void rec_step(const RecEntry& R, int b, const Geo<T>& g, const cx<T>* Up,
+__device__ __forceinline__ void rec_step(const RecT<T>& R, int b, const Geo<T>& g, const cx<T>* Up,
                                          const cx<T>* dUp, cx<T>& u, cx<T>* du)
 {
-    bool k0 = (b == 0);
-    cx<T> zq = k0 ? g.zp : g.zm;
-    cx<T> xq = k0 ? g.xm : g.xp;
-    T c1 = (T)R.c1, c2 = (T)R.c2;
-    // sources through the symmetry: (index << 2) | (conjugate << 1) | negate
+    const T sg = (b == 0) ? (T)1 : (T)-1;
+    const T A1 = R.c1, A2 = R.c2, A1s = sg * A1, A2s = sg * A2;
     const int i1 = R.s1 >> 2, i2 = R.s2 >> 2;
-    const T cj1 = (R.s1 & 2) ? (T)-1 : (T)1, ng1 = (R.s1 & 1) ? (T)-1 : (T)1;
-    const T cj2 = (R.s2 & 2) ? (T)-1 : (T)1, ng2 = (R.s2 & 1) ? (T)-1 : (T)1;
+    const T cj1 = (R.s1 & 2) ? (T)-1 : (T)1, cj2 = (R.s2 & 2) ? (T)-1 : (T)1;
     cx<T> U1 = Up[i1], U2 = Up[i2];
-    U1 = cx<T>{ng1 * U1.r, ng1 * cj1 * U1.i};
-    U2 = cx<T>{ng2 * U2.r, ng2 * cj2 * U2.i};
-    const cx<T> t1 = cscale(c1, zq);
-    const cx<T> t2 = cx<T>{-c2 * xq.i, c2 * xq.r};  // (i c2) xq
+    U1.i *= cj1;
+    U2.i *= cj2;
+    const cx<T> t1{A1 * g.zp.r, A1s * g.zp.i};  // c1 zq
+    const cx<T> t2{A2s * g.xp.i, A2 * g.xp.r};  // (i c2) xq
     // u = t1 U1 - t2 U2 as one chain of fused multiply-adds per component
     u.r = fma(t1.r, U1.r, fma(-t1.i, U1.i, fma(-t2.r, U2.r, t2.i * U2.i)));
     u.i = fma(t1.r, U1.i, fma(t1.i, U1.r, fma(-t2.r, U2.i, -(t2.i * U2.r))));
     if (GRAD) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const cx<T> dzk = k0 ? g.dzp[k] : g.dzm[k];
-            const cx<T> dxk = k0 ? g.dxm[k] : g.dxp[k];
             cx<T> dU1 = dUp[3 * i1 + k], dU2 = dUp[3 * i2 + k];
-            dU1 = cx<T>{ng1 * dU1.r, ng1 * cj1 * dU1.i};
-            dU2 = cx<T>{ng2 * dU2.r, ng2 * cj2 * dU2.i};
-            // du = c1 (dz U1 + zq dU1) - i c2 (dx U2 + xq dU2)
-            //    = (c1 dz) U1 + t1 dU1 - (i c2 dx) U2 - t2 dU2
-            const cx<T> a1 = cscale(c1, dzk);
-            const cx<T> a2 = cx<T>{-c2 * dxk.i, c2 * dxk.r};
+            dU1.i *= cj1;
+            dU2.i *= cj2;
+            // du = (c1 dz) U1 + t1 dU1 - (i c2 dx) U2 - t2 dU2
+            const cx<T> a1{A1 * g.dzp[k].r, A1s * g.dzp[k].i};
+            const cx<T> a2{A2s * g.dxp[k].i, A2 * g.dxp[k].r};
             T r = fma(a1.r, U1.r, -(a1.i * U1.i));
             r = fma(t1.r, dU1.r, fma(-t1.i, dU1.i, r));
             r = fma(-a2.r, U2.r, fma(a2.i, U2.i, r));
@@ -334,17 +333,20 @@ struct Smem {
     static constexpr bool SELF_LDS = sizeof(T) == 4;
     static constexpr int NSELF = SELF_LDS ? NW * mld::NB_MAX * 3 : 1;
     T selfw[NSELF];
+    // contraction lane map of each level: r | slice << 6 | NS << 9 | e0 << 12 | e1 << 18 | bi << 24
+    // (row, slice index, slice count, entry range, output component of the row); e0 = e1 for an idle lane
+    unsigned lmap[JJ + 1][64];
     cx<T> c[NU];
     cx<T> W[WMAX];
     union {
         struct {
-            RecEntry rec[HU];
+            RecT<T> rec[HU];
             cx<T> cpart[NW][NU];
             cx<T> ub[NW][2][NL];
         } a;
         T cg[NCG];
         struct {
-            RecEntry rec[HU];  // same offset as a.rec
+            RecT<T> rec[HU];  // same offset as a.rec
             cx<T> ub[NW][2][NL];
             cx<T> dub[NW][2][NL][3];
             cx<T> vb[NW][NL][3];
@@ -385,11 +387,32 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
         return;
     }
 
+    for (int q = tid; q < (JJ + 1) * 64; q += BS) {
+        const int j = q >> 6, ln = q & 63;
+        const int M = tb.M[j], K = mld::hcount(j);
+        unsigned v = 0;
+        if (M > 0) {
+            const int NS = max(1, min(4, 64 / M));
+            const int slice = (K + NS - 1) / NS;
+            const int r = ln % M, sidx = ln / M;
+            int e0 = 0, e1 = 0, bi = 0;
+            if (sidx < NS) {
+                e0 = min(K, sidx * slice);
+                e1 = min(K, e0 + slice);
+            }
+            if (sidx == 0) bi = tb.rows[tb.rowbase[j] + r].bi;
+            v = (unsigned)r | ((unsigned)min(sidx, 7) << 6) | ((unsigned)NS << 9) | ((unsigned)e0 << 12) |
+                ((unsigned)e1 << 18) | ((unsigned)bi << 24);
+        }
+        sm.lmap[j][ln] = v;
+    }
+    __syncthreads();
+
     for (int ch = 0; ch < nch; ++ch) {
         // ---------------- A: density ----------------
         // each lane owns fixed entries of every level, so it accumulates them in its
         // wavefront's cpart row without conflicts
-        for (int e = tid; e < S::HU; e += BS) sm.u.a.rec[e] = tb.rec[e];
+        for (int e = tid; e < S::HU; e += BS) sm.u.a.rec[e] = stage_rec<T>(tb.rec[e]);
         for (int e = lane; e < NU; e += 64) sm.u.a.cpart[w][e] = cx<T>{(T)0, (T)0};
         __syncthreads();
         for (int n = w; n < nn; n += NW) {
@@ -412,7 +435,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                     int e = lane + 64 * s2;
                     if (e < K) {
                         int b = e % (j + 1);
-                        RecEntry R = sm.u.a.rec[mld::hoff(j) + e];
+                        const RecT<T> R = sm.u.a.rec[mld::hoff(j) + e];
                         cx<T> u;
                         rec_step<T, false>(R, b, g, Up, nullptr, u, nullptr);
                         Un[e] = u;
@@ -514,7 +537,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
         T* selfw = &sm.selfw[S::SELF_LDS ? w * mld::NB_MAX * 3 : 0];
         if constexpr (S::SELF_LDS)
             for (int q = lane; q < nb * 3; q += 64) selfw[q] = (T)0;
-        for (int e = tid; e < S::HU; e += BS) sm.u.g.rec[e] = tb.rec[e];
+        for (int e = tid; e < S::HU; e += BS) sm.u.g.rec[e] = stage_rec<T>(tb.rec[e]);
         __syncthreads();
         for (int n = w; n < nn; n += NW) {
             const int tj = itype[pair_idx[p0 + n]];
@@ -549,7 +572,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                         int e = lane + 64 * s;
                         if (e < mld::hcount(j)) {
                             int b = e % (j + 1);
-                            RecEntry R = sm.u.g.rec[mld::hoff(j) + e];
+                            const RecT<T> R = sm.u.g.rec[mld::hoff(j) + e];
                             rec_step<T, true>(R, b, g, Up, dUp, us[s], dus[s]);
                         }
                     }
@@ -571,17 +594,15 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                 }
                 wave_sync();
                 const int M = tb.M[j];
-                const int K = mld::hcount(j);
                 const cx<T>* Wj = sm.W + tb.wofs[j];
                 // contraction of level j: the K_j entries are cut into NS slices;
                 // lane r + M*s sums slice s of row r (two interleaved partial sums
                 // per direction), then the slices are added in slice order.
-                const int NS = (M > 0) ? max(1, min(4, 64 / M)) : 1;
-                const int slice = (K + NS - 1) / NS;
-                const int r = lane % max(M, 1), sidx = lane / max(M, 1);
+                const unsigned lm = sm.lmap[j][lane];
+                const int r = lm & 63, sidx = (lm >> 6) & 7, NS = (lm >> 9) & 7;
+                const int e0 = (lm >> 12) & 63, e1 = (lm >> 18) & 63;
                 T a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
-                if (M > 0 && sidx < NS) {
-                    const int e0 = sidx * slice, e1 = min(K, e0 + slice);
+                {
                     int e = e0;
                     for (; e + 1 < e1; e += 2) {
                         cx<T> wv = Wj[e * M + r], wu = Wj[(e + 1) * M + r];
@@ -616,7 +637,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                     }
                 }
                 if (M > 0 && sidx == 0 && lane < M) {
-                    const int bi = tb.rows[tb.rowbase[j] + r].bi;
+                    const int bi = lm >> 24;
                     outw[bi * 3 + 0] += a0;
                     outw[bi * 3 + 1] += a1;
                     outw[bi * 3 + 2] += a2;
