@@ -324,39 +324,37 @@ struct Smem {
     static constexpr int NL = mld::hcount(JJ);  // canonical half of the largest level
     static constexpr int WMAX = wsize_bound(JJ);
     // every CG1 table is staged at once for the Z -> W phase; the recursion table shares
-    // that space with it (restaged before each recursion phase), which keeps the FP64
-    // workgroup under 80 KB of LDS (two workgroups per CU)
+    // that space with it (restaged before each recursion phase).  Everything but W lives in
+    // the union of the three phases, which keeps the FP32 workgroup under 40 KB of LDS (four
+    // workgroups per CU) and the FP64 one under 80 KB (two)
     static constexpr int NCG = cg1_size(JJ);
     static constexpr int HU = mld::hoff(JJ + 1);
-    // FP32 accumulates the self slot per wavefront in LDS (registers are the scarcer
-    // resource there); FP64 keeps it in registers (LDS is)
-    static constexpr bool SELF_LDS = sizeof(T) == 4;
-    static constexpr int NSELF = SELF_LDS ? NW * mld::NB_MAX * 3 : 1;
-    T selfw[NSELF];
-    // contraction lane map of each level: r | slice << 6 | NS << 9 | e0 << 12 | e1 << 18 | bi << 24
-    // (row, slice index, slice count, entry range, output component of the row); e0 = e1 for an idle lane
-    unsigned lmap[JJ + 1][64];
-    cx<T> c[NU];
     cx<T> W[WMAX];
     union {
-        struct {
+        struct {  // A: density
             RecT<T> rec[HU];
             cx<T> cpart[NW][NU];
             cx<T> ub[NW][2][NL];
         } a;
-        T cg[NCG];
-        struct {
+        struct {  // B: Z -> W; c is written at the end of A, past the end of a
+            T cg[NCG];
+            cx<T> c[NU];
+        } b;
+        struct {  // C: gradients
             RecT<T> rec[HU];  // same offset as a.rec
             cx<T> ub[NW][2][NL];
             cx<T> dub[NW][2][NL][3];
             cx<T> vb[NW][NL][3];
             T outw[NW][mld::NB_MAX][3];
+            // contraction lane map of each level: r | slice << 6 | NS << 9 | e0 << 12 | e1 << 18 | bi << 24
+            // (row, slice index, slice count, entry range, output component of the row); e0 = e1 for an idle lane
+            unsigned lmap[JJ + 1][64];
         } g;
     } u;
 };
 
 template <typename T, int JJ>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) bso4_kernel(DevTables tb, int natoms, const int* __restrict__ pair_off,
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) bso4_kernel(DevTables tb, int natoms, const int* __restrict__ pair_off,
                                                   const int* __restrict__ pair_idx,
                                                   const T* __restrict__ geo,
                                                   const int* __restrict__ itype, const double* __restrict__ wfac,
@@ -386,27 +384,6 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
             for (int d = tid; d < 3 * D; d += BS) grad[base * 3 * D + d] = (T)0;
         return;
     }
-
-    for (int q = tid; q < (JJ + 1) * 64; q += BS) {
-        const int j = q >> 6, ln = q & 63;
-        const int M = tb.M[j], K = mld::hcount(j);
-        unsigned v = 0;
-        if (M > 0) {
-            const int NS = max(1, min(4, 64 / M));
-            const int slice = (K + NS - 1) / NS;
-            const int r = ln % M, sidx = ln / M;
-            int e0 = 0, e1 = 0, bi = 0;
-            if (sidx < NS) {
-                e0 = min(K, sidx * slice);
-                e1 = min(K, e0 + slice);
-            }
-            if (sidx == 0) bi = tb.rows[tb.rowbase[j] + r].bi;
-            v = (unsigned)r | ((unsigned)min(sidx, 7) << 6) | ((unsigned)NS << 9) | ((unsigned)e0 << 12) |
-                ((unsigned)e1 << 18) | ((unsigned)bi << 24);
-        }
-        sm.lmap[j][ln] = v;
-    }
-    __syncthreads();
 
     for (int ch = 0; ch < nch; ++ch) {
         // ---------------- A: density ----------------
@@ -457,12 +434,12 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
             // self term on the diagonal a == b (compute_bispectrum_so4.F90:431-439)
             const int a = le / (j + 1), b = le % (j + 1);
             if (a == b) s.r += (T)1;
-            sm.c[e] = s;
+            sm.u.b.c[e] = s;
             // the partner entry: (-1)^(a-b) conj
             const int lp = (j + 1) * (j + 1) - 1 - le;
             if (lp != le) {
                 const T sg = ((a - b) & 1) ? (T)-1 : (T)1;
-                sm.c[mld::uoff(j) + lp] = cx<T>{sg * s.r, -sg * s.i};
+                sm.u.b.c[mld::uoff(j) + lp] = cx<T>{sg * s.r, -sg * s.i};
             }
         }
         __syncthreads();
@@ -480,9 +457,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
             for (int q = 0; q < row.nz; ++q) {
                 const ZTerm zt = row.z[q];
                 const int J1 = zt.J1, J2 = zt.J2;
-                const T* cg = sm.u.cg + (zt.cgoff - cgs);
-                const cx<T>* c1 = sm.c + mld::uoff(J1);
-                const cx<T>* c2 = sm.c + mld::uoff(J2);
+                const T* cg = sm.u.b.cg + (zt.cgoff - cgs);
+                const cx<T>* c1 = sm.u.b.c + mld::uoff(J1);
+                const cx<T>* c2 = sm.u.b.c + mld::uoff(J2);
                 const int ialo = (max(-J1, m1 - J2) + J1) >> 1, iahi = (min(J1, m1 + J2) + J1) >> 1;
                 const int iblo = (max(-J1, m2 - J2) + J1) >> 1, ibhi = (min(J1, m2 + J2) + J1) >> 1;
                 // c2 index (ka, kb) = ((m1 + J1 + J2) / 2 - ia, (m2 + J1 + J2) / 2 - ib)
@@ -510,7 +487,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
             sm.W[tb.wofs[j] + e * M + r] = cscale(pw, wv);
         };
         {
-            for (int q = tid; q < tb.cgbase[JJ + 1]; q += BS) sm.u.cg[q] = (T)tb.cg1[q];
+            for (int q = tid; q < tb.cgbase[JJ + 1]; q += BS) sm.u.b.cg[q] = (T)tb.cg1[q];
             __syncthreads();
             for (int k = 0; k * BS + tid < tb.ntask; ++k) w_task(tb.tasks_all[k * BS + tid], 0);
             __syncthreads();
@@ -522,7 +499,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
             const int M = tb.M[l];
             T bpart = (T)0;
             for (int e = lane; e < mld::hcount(l); e += 64) {
-                cx<T> cl = sm.c[mld::uoff(l) + e];
+                cx<T> cl = sm.u.b.c[mld::uoff(l) + e];
                 cx<T> wv = sm.W[tb.wofs[l] + e * M + r];
                 bpart += cl.r * wv.r + cl.i * wv.i;
             }
@@ -534,10 +511,26 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
 
         // ---------------- C: gradients ----------------
         T selfacc[3] = {(T)0, (T)0, (T)0};  // -sum over this wavefront's neighbours, q = lane + 64 u
-        T* selfw = &sm.selfw[S::SELF_LDS ? w * mld::NB_MAX * 3 : 0];
-        if constexpr (S::SELF_LDS)
-            for (int q = lane; q < nb * 3; q += 64) selfw[q] = (T)0;
         for (int e = tid; e < S::HU; e += BS) sm.u.g.rec[e] = stage_rec<T>(tb.rec[e]);
+        for (int q = tid; q < (JJ + 1) * 64; q += BS) {
+            const int j = q >> 6, ln = q & 63;
+            const int M = tb.M[j], K = mld::hcount(j);
+            unsigned v = 0;
+            if (M > 0) {
+                const int NS = max(1, min(4, 64 / M));
+                const int slice = (K + NS - 1) / NS;
+                const int r = ln % M, sidx = ln / M;
+                int e0 = 0, e1 = 0, bi = 0;
+                if (sidx < NS) {
+                    e0 = min(K, sidx * slice);
+                    e1 = min(K, e0 + slice);
+                }
+                if (sidx == 0) bi = tb.rows[tb.rowbase[j] + r].bi;
+                v = (unsigned)r | ((unsigned)min(sidx, 7) << 6) | ((unsigned)NS << 9) | ((unsigned)e0 << 12) |
+                    ((unsigned)e1 << 18) | ((unsigned)bi << 24);
+            }
+            sm.u.g.lmap[j][ln] = v;
+        }
         __syncthreads();
         for (int n = w; n < nn; n += NW) {
             const int tj = itype[pair_idx[p0 + n]];
@@ -598,7 +591,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                 // contraction of level j: the K_j entries are cut into NS slices;
                 // lane r + M*s sums slice s of row r (two interleaved partial sums
                 // per direction), then the slices are added in slice order.
-                const unsigned lm = sm.lmap[j][lane];
+                const unsigned lm = sm.u.g.lmap[j][lane];
                 const int r = lm & 63, sidx = (lm >> 6) & 7, NS = (lm >> 9) & 7;
                 const int e0 = (lm >> 12) & 63, e1 = (lm >> 18) & 63;
                 T a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
@@ -646,23 +639,14 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
             }
             // flush neighbour n: grad slot (base + 1 + n), layout [slot][x][D]
             T* gout = grad + (base + 1 + n) * 3 * (size_t)D + ch * nb;
-            if constexpr (!S::SELF_LDS) {
 #pragma unroll
-                for (int u = 0; u < 3; ++u) {
-                    const int q = lane + 64 * u;
-                    if (q < nb * 3) {
-                        const int x = q / nb, bi = q - x * nb;
-                        const T v = wn * outw[bi * 3 + x];
-                        gout[(size_t)x * D + bi] = v;
-                        selfacc[u] -= v;
-                    }
-                }
-            } else {
-                for (int q = lane; q < nb * 3; q += 64) {
+            for (int u = 0; u < 3; ++u) {
+                const int q = lane + 64 * u;
+                if (q < nb * 3) {
                     const int x = q / nb, bi = q - x * nb;
                     const T v = wn * outw[bi * 3 + x];
                     gout[(size_t)x * D + bi] = v;
-                    selfw[q] -= v;
+                    selfacc[u] -= v;
                 }
             }
             wave_sync();
@@ -672,7 +656,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
             T* outw = &sm.u.g.outw[w][0][0];
 #pragma unroll
             for (int u = 0; u < 3; ++u)
-                if (lane + 64 * u < nb * 3) outw[lane + 64 * u] = S::SELF_LDS ? selfw[lane + 64 * u] : selfacc[u];
+                if (lane + 64 * u < nb * 3) outw[lane + 64 * u] = selfacc[u];
         }
         __syncthreads();
         // self slot = -sum of the neighbour slots (compute_bispectrum_so4.F90:692-695), the
