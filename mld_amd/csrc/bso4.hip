@@ -639,12 +639,13 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
             }
             // flush neighbour n: grad slot (base + 1 + n), layout [slot][x][D]
             T* gout = grad + (base + 1 + n) * 3 * (size_t)D + ch * nb;
+            // lane + 64 u walks outw in its own (component, direction) order: q = 3 bi + x
 #pragma unroll
             for (int u = 0; u < 3; ++u) {
                 const int q = lane + 64 * u;
                 if (q < nb * 3) {
-                    const int x = q / nb, bi = q - x * nb;
-                    const T v = wn * outw[bi * 3 + x];
+                    const int bi = q / 3, x = q - 3 * bi;
+                    const T v = wn * outw[q];
                     gout[(size_t)x * D + bi] = v;
                     selfacc[u] -= v;
                 }
@@ -663,7 +664,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
         // wavefront partials added in wavefront order
         T* gself = grad + base * 3 * (size_t)D + ch * nb;
         for (int q = tid; q < nb * 3; q += BS) {
-            const int x = q / nb, bi = q - x * nb;
+            const int bi = q / 3, x = q - 3 * bi;
             T s = sm.u.g.outw[0][0][q];
 #pragma unroll
             for (int qq = 1; qq < NW; ++qq) s += sm.u.g.outw[qq][0][q];
