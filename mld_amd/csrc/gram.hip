@@ -201,12 +201,14 @@ __global__ void __launch_bounds__(RF_THREADS) rows_force_kernel(int order, int D
             }
         }
     }
-    // stage and write the three rows one at a time: zero, linear and quadratic columns, copy out
+    // stage and write the three rows one at a time: the linear and quadratic columns cover
+    // 1 .. ncol - 1 exactly once, column 0 and the padding are zero; then copy out
+    const int ncol = 1 + D + (order == 2 ? D * (D + 1) / 2 : 0);
     double* Ar = A + row0 * ldA;
     for (int x = 0; x < 3; ++x) {
         __syncthreads();
-        for (int q = tid; q < ldA; q += RF_THREADS) out[q] = 0.0;
-        __syncthreads();
+        if (tid == 0) out[0] = 0.0;
+        for (int q = ncol + tid; q < ldA; q += RF_THREADS) out[q] = 0.0;
 #pragma unroll
         for (int v = 0; v < NLIN; ++v) {
             const int it = tid + RF_THREADS * v;

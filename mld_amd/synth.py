@@ -161,22 +161,35 @@ def morse_from_pairs(pair_off, pair_idx, pair_d, atom_off, D: float = 0.4, alpha
     pairs are directed (i -> j, d = r_j - r_i) and all have r < rc."""
     import torch
 
-    natoms = pair_off.shape[0] - 1
-    counts = (pair_off[1:] - pair_off[:-1]).long()
-    src = torch.repeat_interleave(torch.arange(natoms, device=pair_d.device), counts)
+    dev = pair_d.device
+    po = pair_off.long()
     r = pair_d[:, 3]
     m = (r < rc).double()
     ex = torch.exp(-alpha * (r - r0))
     phi = D * (ex * ex - 2.0 * ex) * m
     dphi = D * (-2.0 * alpha * ex * ex + 2.0 * alpha * ex) * m
-    e_atom = torch.zeros(natoms, dtype=torch.float64, device=pair_d.device).index_add_(0, src, 0.5 * phi)
-    F = torch.zeros((natoms, 3), dtype=torch.float64, device=pair_d.device)
-    F.index_add_(0, src, (dphi / r)[:, None] * pair_d[:, :3])
-    ao = torch.as_tensor(atom_off, device=pair_d.device).long()
-    nstruct = ao.shape[0] - 1
-    sid = torch.repeat_interleave(torch.arange(nstruct, device=pair_d.device), ao[1:] - ao[:-1])
-    E = torch.zeros(nstruct, dtype=torch.float64, device=pair_d.device).index_add_(0, sid, e_atom)
+    e_atom = _segment_sum(0.5 * phi[:, None], po)[:, 0]
+    F = _segment_sum((dphi / r)[:, None] * pair_d[:, :3], po)
+    ao = torch.as_tensor(np.asarray(atom_off), device=dev).long()
+    E = _segment_sum(e_atom[:, None], ao)[:, 0]
     return E, F
+
+
+def _segment_sum(vals, off):
+    """Sums of the contiguous segments [off[s], off[s+1]) of vals (n, k), in a fixed order (the
+    segments padded into a dense block and reduced), so the targets are the same on every run."""
+    import torch
+
+    nseg = off.shape[0] - 1
+    counts = off[1:] - off[:-1]
+    if vals.shape[0] == 0 or nseg == 0:
+        return torch.zeros((nseg, vals.shape[1]), dtype=vals.dtype, device=vals.device)
+    width = int(counts.max())
+    seg = torch.repeat_interleave(torch.arange(nseg, device=vals.device), counts)
+    pos = torch.arange(vals.shape[0], device=vals.device) - off[seg]
+    dense = torch.zeros((nseg, width, vals.shape[1]), dtype=vals.dtype, device=vals.device)
+    dense[seg, pos] = vals
+    return dense.sum(dim=1)
 
 
 def morse_mixture_params(nspecies: int = 5, seed: int = 11):
