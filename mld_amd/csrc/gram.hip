@@ -424,6 +424,10 @@ __global__ void __launch_bounds__(256) syrk128_f64_kernel(int M, int ldA, int nt
     constexpr int PER = KB * T2 / 256;  // 8 elements per thread and panel
     double pi_[PER], pj_[PER];
     const int ci0 = ti * T2, cj0 = tj * T2;
+    // a wavefront whose 64 x 64 block lies wholly in the padding columns (the last tile column or
+    // row when ld is not a multiple of 128) only helps stage the panels: the reduction never
+    // reads its block
+    const bool live = (ci0 + wi * 64 < ldA) && (cj0 + wj * 64 < ldA);
     auto fetch = [&](int rb) {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
@@ -447,6 +451,7 @@ __global__ void __launch_bounds__(256) syrk128_f64_kernel(int M, int ldA, int nt
         }
         __syncthreads();
         if (rb + KB < r1) fetch(rb + KB);
+        if (!live) continue;
 #pragma unroll
         for (int k0 = 0; k0 < KB; k0 += 4) {
             double af[4], bf[4];
@@ -460,6 +465,7 @@ __global__ void __launch_bounds__(256) syrk128_f64_kernel(int M, int ldA, int nt
                 for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
         }
     }
+    if (!live) return;
     // f64 16x16x4 output: register v of lane l holds C[l/16 + 4v][l%16]
     double* P = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * T2 * T2;
 #pragma unroll
