@@ -117,10 +117,12 @@ void build_tables(int jj, int diag, HostTables& t)
             if (j != l && j != l1 && j != l2) continue;
             WRow row{};
             row.bi = bi;
-            // terms with the same (J1, J2 -> j) are the same Z: merge them by adding scales
+            // terms with the same {J1, J2} -> j are the same Z (Z(J1, J2 -> j) = Z(J2, J1 -> j): the
+            // two Clebsch-Gordan factors change sign together under the exchange): merge them by
+            // adding scales
             auto add = [&](int J1, int J2, int isA, double sc) {
                 for (int q = 0; q < row.nz; ++q)
-                    if (row.z[q].J1 == J1 && row.z[q].J2 == J2) {
+                    if ((row.z[q].J1 == J1 && row.z[q].J2 == J2) || (row.z[q].J1 == J2 && row.z[q].J2 == J1)) {
                         row.z[q].scale += sc;
                         row.z[q].isA |= isA;
                         return;
