@@ -387,19 +387,23 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
 
     for (int ch = 0; ch < nch; ++ch) {
         // ---------------- A: density ----------------
-        // each lane owns fixed entries of every level, so it accumulates them in its
-        // wavefront's cpart row without conflicts
+        // each lane owns fixed entries of every level and accumulates them in registers over its
+        // wavefront's neighbours; the wavefront's sums go to its cpart row at the end
+        constexpr int LS = level_slots<JJ>(JJ);
         for (int e = tid; e < S::HU; e += BS) sm.u.a.rec[e] = stage_rec<T>(tb.rec[e]);
-        for (int e = lane; e < NU; e += 64) sm.u.a.cpart[w][e] = cx<T>{(T)0, (T)0};
+        cx<T> dacc[JJ + 1][LS];
+#pragma unroll
+        for (int j = 0; j <= JJ; ++j)
+#pragma unroll
+            for (int s2 = 0; s2 < LS; ++s2) dacc[j][s2] = cx<T>{(T)0, (T)0};
         __syncthreads();
         for (int n = w; n < nn; n += NW) {
             const int tj = itype[pair_idx[p0 + n]];
             const Geo<T> g = load_geo<T>(geo + (size_t)(p0 + n) * GEO_STRIDE, false);
             T fw = g.fc * (T)wfac[ch * nsp + tj];
-            cx<T>* cp = sm.u.a.cpart[w];
             if (lane == 0) {
                 sm.u.a.ub[w][0][0] = cx<T>{(T)1, (T)0};
-                cp[0].r += fw;
+                dacc[0][0].r += fw;
             }
 #pragma unroll
             for (int j = 1; j <= JJ; ++j) {
@@ -408,7 +412,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                 cx<T>* Un = sm.u.a.ub[w][j & 1];
                 const int K = mld::hcount(j);
 #pragma unroll
-                for (int s2 = 0; s2 < level_slots<JJ>(JJ); ++s2) {
+                for (int s2 = 0; s2 < LS; ++s2) {
                     int e = lane + 64 * s2;
                     if (e < K) {
                         int b = e % (j + 1);
@@ -416,11 +420,21 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                         cx<T> u;
                         rec_step<T, false>(R, b, g, Up, nullptr, u, nullptr);
                         Un[e] = u;
-                        cp[mld::uoff(j) + e] = cadd(cp[mld::uoff(j) + e], cscale(fw, u));
+                        dacc[j][s2] = cadd(dacc[j][s2], cscale(fw, u));
                     }
                 }
             }
             wave_sync();
+        }
+        {
+            cx<T>* cp = sm.u.a.cpart[w];
+#pragma unroll
+            for (int j = 0; j <= JJ; ++j)
+#pragma unroll
+                for (int s2 = 0; s2 < LS; ++s2) {
+                    const int e = lane + 64 * s2;
+                    if (e < mld::hcount(j)) cp[mld::uoff(j) + e] = dacc[j][s2];
+                }
         }
         __syncthreads();
         for (int e = tid; e < NU; e += BS) {
