@@ -71,9 +71,9 @@ PEAK_FP32, PEAK_FP64 = 157.3, 78.6   # TFLOP/s, MI355X dense
 # HBM bytes of one bso4_kernel<float,8> launch on the first 500-structure batch of the
 # quadratic-fit workload (96,582 atoms, 3,020,350 pairs), from rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE passes scaled by a 1 GiB copy (factors 2.0 and 1.0):
-# (316,364 x 2.0 + 2,036,930 x 1.0) KiB; the launch's algorithmic bytes are 2,177,392,328
+# (315,683 x 2.0 + 2,037,535 x 1.0) KiB; the launch's algorithmic bytes are 2,177,392,328
 # (the excess is the per-pair geometry record, read by the density and gradient phases).
-TRAFFIC_BSO4_F32_FIT_BATCH = (316364 * 2.0 + 2036930 * 1.0) * 1024
+TRAFFIC_BSO4_F32_FIT_BATCH = (315683 * 2.0 + 2037535 * 1.0) * 1024
 ALGO_BYTES_BSO4_F32_FIT_BATCH = 2177392328.0
 
 
