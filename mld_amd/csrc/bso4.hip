@@ -312,6 +312,25 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
+// Slices per contraction row: the rows of a level times the slices fill at most one wavefront,
+// in groups of 1, 2 or 4 adjacent lanes (one quad-permute step per halving).
+__host__ __device__ constexpr int slice_count(int M) { return M <= 16 ? 4 : (M <= 32 ? 2 : 1); }
+
+// v plus its partner lane under a quad permute (CTRL = 0xB1: lane ^ 1, 0x4E: lane ^ 2)
+template <int CTRL>
+__device__ __forceinline__ float quad_add(float v)
+{
+    return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double quad_add(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return v + __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 template <int JJ>
 constexpr int level_slots(int j)
 {
@@ -531,15 +550,16 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
             const int M = tb.M[j], K = mld::hcount(j);
             unsigned v = 0;
             if (M > 0) {
-                const int NS = max(1, min(4, 64 / M));
+                // slices of a row sit in adjacent lanes (slice index fastest), summed by quad permutes
+                const int NS = slice_count(M);
                 const int slice = (K + NS - 1) / NS;
-                const int r = ln % M, sidx = ln / M;
+                const int r = ln / NS, sidx = ln % NS;
                 int e0 = 0, e1 = 0, bi = 0;
-                if (sidx < NS) {
+                if (r < M) {
                     e0 = min(K, sidx * slice);
                     e1 = min(K, e0 + slice);
+                    if (sidx == 0) bi = tb.rows[tb.rowbase[j] + r].bi;
                 }
-                if (sidx == 0) bi = tb.rows[tb.rowbase[j] + r].bi;
                 v = (unsigned)r | ((unsigned)min(sidx, 7) << 6) | ((unsigned)NS << 9) | ((unsigned)e0 << 12) |
                     ((unsigned)e1 << 18) | ((unsigned)bi << 24);
             }
@@ -603,10 +623,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                 const int M = tb.M[j];
                 const cx<T>* Wj = sm.W + tb.wofs[j];
                 // contraction of level j: the K_j entries are cut into NS slices;
-                // lane r + M*s sums slice s of row r (two interleaved partial sums
-                // per direction), then the slices are added in slice order.
+                // lane NS*r + s sums slice s of row r (two interleaved partial sums
+                // per direction), then the slices are added across the lane group.
                 const unsigned lm = sm.u.g.lmap[j][lane];
-                const int r = lm & 63, sidx = (lm >> 6) & 7, NS = (lm >> 9) & 7;
+                const int r = lm & 63, sidx = (lm >> 6) & 7;
                 const int e0 = (lm >> 12) & 63, e1 = (lm >> 18) & 63;
                 T a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
                 {
@@ -633,17 +653,19 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                     a1 += b1;
                     a2 += b2;
                 }
-                // gather the slices of row r into lane r (fixed order)
-                for (int q = 1; q < 4; ++q) {
-                    const unsigned dl = (unsigned)(q * max(M, 1));  // lane + dl holds slice q of row r
-                    T t0 = __shfl_down(a0, dl, 64), t1 = __shfl_down(a1, dl, 64), t2 = __shfl_down(a2, dl, 64);
-                    if (q < NS && sidx == 0) {
-                        a0 += t0;
-                        a1 += t1;
-                        a2 += t2;
-                    }
+                // add the slices of row r into its slice-0 lane: (s0 + s1) + (s2 + s3)
+                const int NSw = slice_count(M);  // uniform across the wavefront
+                if (NSw >= 2) {
+                    a0 = quad_add<0xB1>(a0);
+                    a1 = quad_add<0xB1>(a1);
+                    a2 = quad_add<0xB1>(a2);
                 }
-                if (M > 0 && sidx == 0 && lane < M) {
+                if (NSw == 4) {
+                    a0 = quad_add<0x4E>(a0);
+                    a1 = quad_add<0x4E>(a1);
+                    a2 = quad_add<0x4E>(a2);
+                }
+                if (M > 0 && sidx == 0 && r < M) {
                     const int bi = lm >> 24;
                     outw[bi * 3 + 0] += a0;
                     outw[bi * 3 + 1] += a1;
