@@ -425,9 +425,9 @@ __global__ void __launch_bounds__(256) syrk128_f64_kernel(int M, int ldA, int nt
     double pi_[PER], pj_[PER];
     const int ci0 = ti * T2, cj0 = tj * T2;
     // a wavefront whose 64 x 64 block lies wholly in the padding columns (the last tile column or
-    // row when ld is not a multiple of 128) only helps stage the panels: the reduction never
-    // reads its block
-    const bool live = (ci0 + wi * 64 < ldA) && (cj0 + wj * 64 < ldA);
+    // row when ld is not a multiple of 128) or below the diagonal (of a diagonal tile) only helps
+    // stage the panels: the reduction never reads its block
+    const bool live = (ci0 + wi * 64 < ldA) && (cj0 + wj * 64 < ldA) && !(ti == tj && wi > wj);
     auto fetch = [&](int rb) {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
@@ -480,7 +480,8 @@ __global__ void __launch_bounds__(256) syrk128_f64_kernel(int M, int ldA, int nt
             }
 }
 
-// G (ldG x ldG, row-major) += sum over slices of the 128-tile partials, entries inside ldG
+// G (ldG x ldG, row-major) += sum over slices of the 128-tile partials, upper-triangle entries
+// inside ldG (the strictly lower part of G stays as it is; every reader takes the upper one)
 __global__ void syrk128_reduce_kernel(int ntile, int nslice, int ldG, const double* __restrict__ part,
                                       double* __restrict__ G)
 {
@@ -491,7 +492,7 @@ __global__ void syrk128_reduce_kernel(int ntile, int nslice, int ldG, const doub
     for (int q = threadIdx.x; q < T2 * T2; q += blockDim.x) {
         const int i = q / T2, j = q - i * T2;
         const int gi = ti * T2 + i, gj = tj * T2 + j;
-        if (gi >= ldG || gj >= ldG) continue;
+        if (gi >= ldG || gj >= ldG || gi > gj) continue;
         double s = 0.0;
         for (int sl = 0; sl < nslice; ++sl) s += part[((size_t)sl * ntiles_up + blockIdx.x) * T2 * T2 + q];
         G[(size_t)gi * ldG + gj] += s;
