@@ -372,6 +372,10 @@ struct Smem {
     } u;
 };
 
+// LDS per workgroup at 2j_max = 8: four FP32 workgroups or two FP64 ones per CU (160 KB)
+static_assert(sizeof(Smem<float, 8>) <= 40 * 1024, "FP32 descriptor workgroup above 40 KB of LDS");
+static_assert(sizeof(Smem<double, 8>) <= 80 * 1024, "FP64 descriptor workgroup above 80 KB of LDS");
+
 template <typename T, int JJ>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) bso4_kernel(DevTables tb, int natoms, const int* __restrict__ pair_off,
                                                   const int* __restrict__ pair_idx,
