@@ -53,7 +53,7 @@ template <> struct MfmaOps<double> {
 //  mode 0: energies  -> out[r][c] = k(x_r, z_c), dot_out[r][c] = x_r.z_c
 //  mode 1: gradients -> row r = (neighbour slot, direction), written at row_slot[r]
 template <typename TD>
-__global__ void __launch_bounds__(256) kpo_mfma_kernel(int mode, int nrow, int nk, int D, int nct, const TD* __restrict__ Arows,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(TD) == 4 ? 4 : 2))) kpo_mfma_kernel(int mode, int nrow, int nk, int D, int nct, const TD* __restrict__ Arows,
                                                        const TD* __restrict__ Z, const double* __restrict__ iz,
                                                        const double* __restrict__ ix, const double* __restrict__ fx,
                                                        const int* __restrict__ row_atom, const long long* __restrict__ row_slot,
@@ -88,28 +88,74 @@ __global__ void __launch_bounds__(256) kpo_mfma_kernel(int mode, int nrow, int n
             for (int t = 0; t < 4; ++t) acc[t] = O::mma(a, Zs[16 * t + (lane & 15)][k + (lane >> 4)], acc[t]);
         }
     }
+    // epilogue: the per-row, per-atom and per-column factors of this lane's 4 rows and 4 columns
+    // are loaded up front (one round of independent loads), then the 16 outputs are formed
+    int rr[4], cc[4];
+    bool rok[4], cok[4];
+    double izc[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int v = 0; v < 4; ++v) {
+        rr[v] = r0 + 16 * w + O::row(lane, v);
+        rok[v] = rr[v] < nrow;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        cc[t] = c0 + 16 * t + (lane & 15);
+        cok[t] = cc[t] < nk;
+        izc[t] = cok[t] ? iz[cc[t]] : 0.0;
+    }
+    if (mode == 0) {
+        double ixr[4];
+        int nnr[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int r = r0 + 16 * w + O::row(lane, v);
-            const int c = c0 + 16 * t + (lane & 15);
-            if (r >= nrow || c >= nk) continue;
-            const double cv = (double)acc[t][v];
-            if (mode == 0) {
-                const double u = s2 + cv / l2;
-                double val = kpow(u, pw, ipw) * ix[r] * iz[c];
-                if (row_atom[r] == 0) val = 0.0;  // row_atom holds the neighbour count in this mode
-                out[(size_t)r * nk + c] = (TD)val;
-                dot_out[(size_t)r * nk + c] = cv;
-            } else {
-                const int a = row_atom[r];
-                const double u = s2 + dot[(size_t)a * nk + c] / l2;
-                const double um1 = kpow(u, pw - 1.0, ipw - 1);
-                const double dn = (pw * um1 / l2 * cv - um1 * u * gx[r] * fx[a]) * ix[a] * iz[c];
-                out[(size_t)row_slot[r] * nk + c] = (TD)dn;
-            }
+            ixr[v] = rok[v] ? ix[rr[v]] : 0.0;
+            nnr[v] = rok[v] ? row_atom[rr[v]] : 0;  // row_atom holds the neighbour count in this mode
         }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (!rok[v] || !cok[t]) continue;
+                const double cv = (double)acc[t][v];
+                const double u = s2 + cv / l2;
+                double val = kpow(u, pw, ipw) * ixr[v] * izc[t];
+                if (nnr[v] == 0) val = 0.0;
+                out[(size_t)rr[v] * nk + cc[t]] = (TD)val;
+                dot_out[(size_t)rr[v] * nk + cc[t]] = cv;
+            }
+    } else {
+        int av[4];
+        long long sl[4];
+        double gxr[4], fxa[4], ixa[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            av[v] = rok[v] ? row_atom[rr[v]] : 0;
+            sl[v] = rok[v] ? row_slot[rr[v]] : 0;
+            gxr[v] = rok[v] ? gx[rr[v]] : 0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            fxa[v] = fx[av[v]];
+            ixa[v] = ix[av[v]];
+        }
+        double dv[4][4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) dv[t][v] = (rok[v] && cok[t]) ? dot[(size_t)av[v] * nk + cc[t]] : 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (!rok[v] || !cok[t]) continue;
+                const double cv = (double)acc[t][v];
+                const double u = s2 + dv[t][v] / l2;
+                const double um1 = kpow(u, pw - 1.0, ipw - 1);
+                const double dn = (pw * um1 / l2 * cv - um1 * u * gxr[v] * fxa[v]) * ixa[v] * izc[t];
+                out[(size_t)sl[v] * nk + cc[t]] = (TD)dn;
+            }
+    }
 }
 
 // per-atom factors ix = u(x.x)^(-p/2), fx = p / (l2 u(x.x)) and the neighbour count
