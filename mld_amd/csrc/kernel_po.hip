@@ -16,6 +16,7 @@
 namespace {
 
 constexpr int T = 64;
+constexpr int KPO_CTB = 8;  // column tiles per kpo_mfma_kernel block
 
 // Normalised-kernel factors: with u = s2 + t / l2,
 //   k(x, z)            = u(x.z)^p * ix * iz,       ix = u(x.x)^(-p/2), iz = u(z.z)^(-p/2)
@@ -53,108 +54,113 @@ template <> struct MfmaOps<double> {
 //  mode 0: energies  -> out[r][c] = k(x_r, z_c), dot_out[r][c] = x_r.z_c
 //  mode 1: gradients -> row r = (neighbour slot, direction), written at row_slot[r]
 template <typename TD>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(TD) == 4 ? 4 : 2))) kpo_mfma_kernel(int mode, int nrow, int nk, int D, int nct, const TD* __restrict__ Arows,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(TD) == 4 ? 3 : 2))) kpo_mfma_kernel(int mode, int nrow, int nk, int D, int nct, const TD* __restrict__ Arows,
                                                        const TD* __restrict__ Z, const double* __restrict__ iz,
                                                        const double* __restrict__ ix, const double* __restrict__ fx,
                                                        const int* __restrict__ row_atom, const long long* __restrict__ row_slot,
-                                                       const double* __restrict__ gx, const double* __restrict__ dot,
+                                                       const double* __restrict__ gx, const TD* __restrict__ dot,
                                                        double s2, double l2, double pw, int ipw,
-                                                       double* __restrict__ dot_out, TD* __restrict__ out)
+                                                       TD* __restrict__ dot_out, TD* __restrict__ out)
 {
     using O = MfmaOps<TD>;
     constexpr int KC = 64;
     __shared__ TD As[T][KC + 1];
     __shared__ TD Zs[T][KC + 1];
-    const int ct = blockIdx.x % nct, rt = blockIdx.x / nct;
-    const int r0 = rt * T, c0 = ct * T;
+    // block = (row tile, chunk of KPO_CTB column tiles): the row tile (and, for D <= 64, its
+    // staged rows) and the per-row factors serve every column tile of the chunk
+    const int nchunk = (nct + KPO_CTB - 1) / KPO_CTB;
+    const int cb = blockIdx.x % nchunk, rt = blockIdx.x / nchunk;
+    const int ct0 = cb * KPO_CTB, ct1 = min(nct, ct0 + KPO_CTB);
+    const int r0 = rt * T;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    typename O::v4 acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = typename O::v4{0, 0, 0, 0};
-    for (int d0 = 0; d0 < D; d0 += KC) {
-        const int kc = min(KC, D - d0);
-        __syncthreads();
-        for (int q = threadIdx.x; q < T * KC; q += blockDim.x) {
-            const int i = q / KC, d = q - i * KC;
-            const bool okd = d < kc;
-            As[i][d] = (okd && r0 + i < nrow) ? Arows[(size_t)(r0 + i) * D + d0 + d] : (TD)0;
-            Zs[i][d] = (okd && c0 + i < nk) ? Z[(size_t)(c0 + i) * D + d0 + d] : (TD)0;
-        }
-        __syncthreads();
-        for (int k = 0; k < kc; k += 4) {
-            // A 16 x 4: lane l holds A[l % 16][l / 16]; B 4 x 16: B[l / 16][l % 16]
-            const TD a = As[16 * w + (lane & 15)][k + (lane >> 4)];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = O::mma(a, Zs[16 * t + (lane & 15)][k + (lane >> 4)], acc[t]);
-        }
-    }
-    // epilogue: the per-row, per-atom and per-column factors of this lane's 4 rows and 4 columns
-    // are loaded up front (one round of independent loads), then the 16 outputs are formed
-    int rr[4], cc[4];
-    bool rok[4], cok[4];
-    double izc[4];
+    const bool resident = D <= KC;
+    // per-row factors of this lane's 4 rows
+    int rr[4];
+    bool rok[4];
+    double rf0[4], rf1[4], rf2[4];  // mode 0: ix, -, -   mode 1: gx, fx, ix of the row's atom
+    int ra[4];                      // mode 0: neighbour count   mode 1: atom
+    long long sl[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         rr[v] = r0 + 16 * w + O::row(lane, v);
         rok[v] = rr[v] < nrow;
+        ra[v] = rok[v] ? row_atom[rr[v]] : 0;  // row_atom holds the neighbour count in mode 0
+        sl[v] = (mode == 1 && rok[v]) ? row_slot[rr[v]] : 0;
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        cc[t] = c0 + 16 * t + (lane & 15);
-        cok[t] = cc[t] < nk;
-        izc[t] = cok[t] ? iz[cc[t]] : 0.0;
+    for (int v = 0; v < 4; ++v) {
+        if (mode == 0) {
+            rf0[v] = rok[v] ? ix[rr[v]] : 0.0;
+            rf1[v] = rf2[v] = 0.0;
+        } else {
+            rf0[v] = rok[v] ? gx[rr[v]] : 0.0;
+            rf1[v] = fx[ra[v]];
+            rf2[v] = ix[ra[v]];
+        }
     }
-    if (mode == 0) {
-        double ixr[4];
-        int nnr[4];
+    for (int ct = ct0; ct < ct1; ++ct) {
+        const int c0 = ct * T;
+        typename O::v4 acc[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            ixr[v] = rok[v] ? ix[rr[v]] : 0.0;
-            nnr[v] = rok[v] ? row_atom[rr[v]] : 0;  // row_atom holds the neighbour count in this mode
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                if (!rok[v] || !cok[t]) continue;
-                const double cv = (double)acc[t][v];
-                const double u = s2 + cv / l2;
-                double val = kpow(u, pw, ipw) * ixr[v] * izc[t];
-                if (nnr[v] == 0) val = 0.0;
-                out[(size_t)rr[v] * nk + cc[t]] = (TD)val;
-                dot_out[(size_t)rr[v] * nk + cc[t]] = cv;
+        for (int t = 0; t < 4; ++t) acc[t] = typename O::v4{0, 0, 0, 0};
+        for (int d0 = 0; d0 < D; d0 += KC) {
+            const int kc = min(KC, D - d0);
+            const bool stage_a = !resident || ct == ct0;
+            __syncthreads();
+            for (int q = threadIdx.x; q < T * KC; q += blockDim.x) {
+                const int i = q / KC, d = q - i * KC;
+                const bool okd = d < kc;
+                if (stage_a) As[i][d] = (okd && r0 + i < nrow) ? Arows[(size_t)(r0 + i) * D + d0 + d] : (TD)0;
+                Zs[i][d] = (okd && c0 + i < nk) ? Z[(size_t)(c0 + i) * D + d0 + d] : (TD)0;
             }
-    } else {
-        int av[4];
-        long long sl[4];
-        double gxr[4], fxa[4], ixa[4];
+            __syncthreads();
+            for (int k = 0; k < kc; k += 4) {
+                // A 16 x 4: lane l holds A[l % 16][l / 16]; B 4 x 16: B[l / 16][l % 16]
+                const TD a = As[16 * w + (lane & 15)][k + (lane >> 4)];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            av[v] = rok[v] ? row_atom[rr[v]] : 0;
-            sl[v] = rok[v] ? row_slot[rr[v]] : 0;
-            gxr[v] = rok[v] ? gx[rr[v]] : 0.0;
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            fxa[v] = fx[av[v]];
-            ixa[v] = ix[av[v]];
-        }
-        double dv[4][4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) dv[t][v] = (rok[v] && cok[t]) ? dot[(size_t)av[v] * nk + cc[t]] : 0.0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                if (!rok[v] || !cok[t]) continue;
-                const double cv = (double)acc[t][v];
-                const double u = s2 + dv[t][v] / l2;
-                const double um1 = kpow(u, pw - 1.0, ipw - 1);
-                const double dn = (pw * um1 / l2 * cv - um1 * u * gxr[v] * fxa[v]) * ixa[v] * izc[t];
-                out[(size_t)sl[v] * nk + cc[t]] = (TD)dn;
+                for (int t = 0; t < 4; ++t) acc[t] = O::mma(a, Zs[16 * t + (lane & 15)][k + (lane >> 4)], acc[t]);
             }
+        }
+        // epilogue: column factors and (mode 1) the atoms' dot products loaded in one round first
+        int cc[4];
+        bool cok[4];
+        double izc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            cc[t] = c0 + 16 * t + (lane & 15);
+            cok[t] = cc[t] < nk;
+            izc[t] = cok[t] ? iz[cc[t]] : 0.0;
+        }
+        if (mode == 0) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if (!rok[v] || !cok[t]) continue;
+                    const double cv = (double)acc[t][v];
+                    const double u = s2 + cv / l2;
+                    double val = kpow(u, pw, ipw) * rf0[v] * izc[t];
+                    if (ra[v] == 0) val = 0.0;
+                    out[(size_t)rr[v] * nk + cc[t]] = (TD)val;
+                    dot_out[(size_t)rr[v] * nk + cc[t]] = acc[t][v];  // x.z in the working precision
+                }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                TD dv[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dv[v] = (rok[v] && cok[t]) ? dot[(size_t)ra[v] * nk + cc[t]] : (TD)0;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if (!rok[v] || !cok[t]) continue;
+                    const double cv = (double)acc[t][v];
+                    const double u = s2 + (double)dv[v] / l2;
+                    const double um1 = kpow(u, pw - 1.0, ipw - 1);
+                    const double dn = (pw * um1 / l2 * cv - um1 * u * rf0[v] * rf1[v]) * rf2[v] * izc[t];
+                    out[(size_t)sl[v] * nk + cc[t]] = (TD)dn;
+                }
+            }
+        }
     }
 }
 
@@ -301,7 +307,7 @@ int run(mld_ctx* ctx, int natoms, long long P, const int* pair_off, const TD* en
     long long* row_slot = (long long*)take(nrows * sizeof(long long) + 8);
     double* gx = (double*)take(nrows * sizeof(double) + 8);
     TD* grows = (TD*)take(nrows * D * sizeof(TD) + 8);
-    double* dot = (double*)take((size_t)natoms * nk * sizeof(double) + 8);
+    TD* dot = (TD*)take((size_t)natoms * nk * sizeof(TD) + 8);  // x.z per (atom, reference point)
     if (!fk) grad = nullptr;
     if (natoms == 0) return MLD_OK;
     const int ipw = (pw == std::floor(pw) && pw >= 1.0 && pw <= 16.0) ? (int)pw : 0;
@@ -310,17 +316,18 @@ int run(mld_ctx* ctx, int natoms, long long P, const int* pair_off, const TD* en
                        row_atom, row_slot, gx, grows);
     hipLaunchKernelGGL(kpo_atom_kernel, dim3((natoms + 255) / 256), dim3(256), 0, st, natoms, xx, s2, l2, pw, ixa, fxa);
     const int nct = (nk + T - 1) / T;
-    const long long ge = (long long)nct * ((natoms + T - 1) / T);
+    const int nchunk = (nct + KPO_CTB - 1) / KPO_CTB;
+    const long long ge = (long long)nchunk * ((natoms + T - 1) / T);
     hipLaunchKernelGGL(kpo_mfma_kernel<TD>, dim3((unsigned)ge), dim3(256), 0, st, 0, natoms, nk, D, nct, energy, Zt,
                        ctx->d_kiz, ixa, fxa, nn, (const long long*)nullptr, (const double*)nullptr,
-                       (const double*)nullptr, s2, l2, pw, ipw, dot, ek);
+                       (const TD*)nullptr, s2, l2, pw, ipw, dot, ek);
     MLD_HIP(ctx, hipGetLastError());
     if (fk && nrows) {
-        const long long gf = (long long)nct * (long long)((nrows + T - 1) / T);
+        const long long gf = (long long)nchunk * (long long)((nrows + T - 1) / T);
         if (gf > 0x7fffffffLL) return mld_fail(ctx, MLD_E_UNSUPPORTED, "kernel_po: batch too large for one launch");
         hipLaunchKernelGGL(kpo_mfma_kernel<TD>, dim3((unsigned)gf), dim3(256), 0, st, 1, (int)nrows, nk, D, nct,
-                           (const TD*)grows, Zt, ctx->d_kiz, ixa, fxa, row_atom, row_slot, gx, dot, s2, l2, pw, ipw,
-                           (double*)nullptr, fk);
+                           (const TD*)grows, Zt, ctx->d_kiz, ixa, fxa, row_atom, row_slot, gx, (const TD*)dot, s2, l2, pw, ipw,
+                           (TD*)nullptr, fk);
     }
     if (fk) hipLaunchKernelGGL(kpo_self_kernel<TD>, dim3((nk + 255) / 256, natoms), dim3(256), 0, st, natoms, nk, pair_off, fk);
     MLD_HIP(ctx, hipGetLastError());
