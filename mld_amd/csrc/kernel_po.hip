@@ -63,6 +63,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
                                                        TD* __restrict__ dot_out, TD* __restrict__ out)
 {
     using O = MfmaOps<TD>;
+    const double il2 = 1.0 / l2;  // one division per thread instead of two per output
     constexpr int KC = 64;
     __shared__ TD As[T][KC + 1];
     __shared__ TD Zs[T][KC + 1];
@@ -138,7 +139,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
                 for (int v = 0; v < 4; ++v) {
                     if (!rok[v] || !cok[t]) continue;
                     const double cv = (double)acc[t][v];
-                    const double u = s2 + cv / l2;
+                    const double u = fma(cv, il2, s2);
                     double val = kpow(u, pw, ipw) * rf0[v] * izc[t];
                     if (ra[v] == 0) val = 0.0;
                     out[(size_t)rr[v] * nk + cc[t]] = (TD)val;
@@ -154,9 +155,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
                 for (int v = 0; v < 4; ++v) {
                     if (!rok[v] || !cok[t]) continue;
                     const double cv = (double)acc[t][v];
-                    const double u = s2 + (double)dv[v] / l2;
+                    const double u = fma((double)dv[v], il2, s2);
                     const double um1 = kpow(u, pw - 1.0, ipw - 1);
-                    const double dn = (pw * um1 / l2 * cv - um1 * u * rf0[v] * rf1[v]) * rf2[v] * izc[t];
+                    const double dn = (pw * um1 * il2 * cv - um1 * u * rf0[v] * rf1[v]) * rf2[v] * izc[t];
                     out[(size_t)sl[v] * nk + cc[t]] = (TD)dn;
                 }
             }
