@@ -220,7 +220,7 @@ int mld_destroy(mld_ctx* ctx)
     predict_free(ctx);
     void* ptrs[] = {ctx->d_ace_terms, ctx->d_ace_toff, ctx->d_ace_grad, ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
                     ctx->d_geo, ctx->d_count, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_scan_tmp,
-                    ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_pgeo, ctx->d_zref, ctx->d_zz, ctx->d_zref32, ctx->d_kiz};
+                    ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_pgeo, ctx->d_etgt, ctx->d_zref, ctx->d_zz, ctx->d_zref32, ctx->d_kiz};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

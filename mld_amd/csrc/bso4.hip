@@ -719,11 +719,8 @@ int launch_t(mld_ctx* ctx, int want_grad, void* energy, void* grad)
 {
     size_t sh = sizeof(Smem<T, JJ>);
     auto k = bso4_kernel<T, JJ>;
-    static bool attr_done = false;
-    if (!attr_done) {
-        MLD_HIP(ctx, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-        attr_done = true;
-    }
+    static std::atomic<unsigned long long> attr_done{0};
+    if (int rc0 = set_lds_limit_once(ctx, (const void*)k, (int)sh, attr_done)) return rc0;
     if (ctx->natoms == 0) return MLD_OK;
     const size_t npairs = (size_t)std::max<long long>(ctx->npairs, 1);
     int rc = device_ensure(ctx, &ctx->d_pgeo, &ctx->cap_pgeo, npairs * GEO_STRIDE * sizeof(T));

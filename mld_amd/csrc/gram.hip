@@ -830,14 +830,10 @@ int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_de
     }
     if (natoms) {
         const size_t fsm = rows_force_smem(D, ldA, f32 ? sizeof(float) : sizeof(double));
-        static bool attr_done = false;  // the largest layout (D = 192 linear, D = 62 quadratic) is < 100 KB
-        if (!attr_done) {
-            MLD_HIP(ctx, hipFuncSetAttribute((const void*)rows_force_kernel<float>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            MLD_HIP(ctx, hipFuncSetAttribute((const void*)rows_force_kernel<double>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            attr_done = true;
-        }
+        // the largest layout (D = 192 linear, D = 62 quadratic) is < 100 KB
+        static std::atomic<unsigned long long> attr_f32{0}, attr_f64{0};
+        if (int rc0 = set_lds_limit_once(ctx, (const void*)rows_force_kernel<float>, 160 * 1024, attr_f32)) return rc0;
+        if (int rc0 = set_lds_limit_once(ctx, (const void*)rows_force_kernel<double>, 160 * 1024, attr_f64)) return rc0;
         if (f32)
             hipLaunchKernelGGL(rows_force_kernel<float>, dim3(natoms), dim3(RF_THREADS), fsm, st, order, D, ldA, ctx->d_atom_struct,
                                ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src,
@@ -945,8 +941,8 @@ int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F
     if (!E || (ctx->natoms && !F_dev)) return mld_fail(ctx, MLD_E_ARG, "targets missing");
     if (!(w_e >= 0.0) || !(w_f >= 0.0)) return mld_fail(ctx, MLD_E_ARG, "weights must be non-negative");
     // per-structure energies to the device
-    double* dE = nullptr;
-    MLD_HIP(ctx, hipMalloc(&dE, (size_t)(ctx->nstruct + 1) * sizeof(double)));
+    if (int rc0 = device_ensure(ctx, &ctx->d_etgt, &ctx->cap_etgt, (size_t)(ctx->nstruct + 1) * sizeof(double))) return rc0;
+    double* dE = (double*)ctx->d_etgt;
     MLD_HIP(ctx, hipMemcpyAsync(dE, E, (size_t)ctx->nstruct * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     double *A = nullptr, *y = nullptr;
     int M = 0;
@@ -955,7 +951,6 @@ int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F
     if (!rc) rc = gram_accumulate_rows(ctx, M, A, y, mixed);
     if (!rc && ctx->keep_rows) rc = keep_rows_copy(ctx, M, A, y);
     hipError_t e = hipStreamSynchronize(ctx->stream);
-    (void)hipFree(dE);
     if (rc) return rc;
     if (e != hipSuccess) return mld_fail(ctx, MLD_E_DEVICE, hipGetErrorString(e));
     return MLD_OK;

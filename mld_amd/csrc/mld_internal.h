@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -211,6 +212,7 @@ struct mld_ctx {
     std::vector<int> kept_M;
     void* d_work = nullptr; size_t cap_work = 0;
     void* d_pgeo = nullptr; size_t cap_pgeo = 0;  // per-pair SO(4) geometry of the last batch (bso4.hip)
+    void* d_etgt = nullptr; size_t cap_etgt = 0;  // per-structure energy targets of mld_gram_add_batch
     // kernel_po state
     int nk = 0;
     double k_sigma = 0, k_length = 1, k_power = 2;
@@ -237,6 +239,17 @@ int mld_fail(mld_ctx* ctx, int code, const std::string& msg);
         if (e_ != hipSuccess)                                                                    \
             return mld_fail((ctx), MLD_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// Raises a kernel's dynamic-LDS limit to bytes on the context's device, once per device and
+// kernel (the attribute is held per device; done holds one bit per device ordinal).
+inline int set_lds_limit_once(mld_ctx* ctx, const void* fn, int bytes, std::atomic<unsigned long long>& done)
+{
+    const unsigned long long bit = 1ull << (ctx->p.device & 63);
+    if (done.load(std::memory_order_acquire) & bit) return MLD_OK;
+    MLD_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    done.fetch_or(bit, std::memory_order_acq_rel);
+    return MLD_OK;
+}
 
 // launchers implemented in the .hip units
 int launch_neighbours_pos(mld_ctx* ctx, const double* pos);
