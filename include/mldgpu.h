@@ -65,6 +65,9 @@ typedef struct mld_params {
     int ace_nmax[4];
     int ace_lmax[4];
     double ace_lambda;
+    /* &input_ml ace_radial_poly (read_ml_file.F90:287, mld_radial_module.F90:92-104):
+     * 1 powTcheb, 2 expTcheb (the reference default; 0 also means 2) */
+    int ace_radial_poly;
 } mld_params;
 
 typedef struct mld_ctx mld_ctx;

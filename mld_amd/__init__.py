@@ -57,6 +57,7 @@ class mld_params(C.Structure):
         ("ace_nmax", C.c_int * 4),
         ("ace_lmax", C.c_int * 4),
         ("ace_lambda", C.c_double),
+        ("ace_radial_poly", C.c_int),
     ]
 
 
@@ -148,9 +149,10 @@ class Context:
                  weight_3ch: Optional[Sequence[float]] = None, r_cut_pair_in: Optional[np.ndarray] = None,
                  r_cut_width_in: float = 0.4, precision: int = PREC_FP64, device: int = 0,
                  descriptor: str = "bso4", ace_numax: int = 3, ace_nmax=(5, 3, 1), ace_lmax=(0, 4, 2),
-                 ace_lambda: float = 2.0):
+                 ace_lambda: float = 2.0, ace_radial_poly: int = 2):
         """descriptor "bso4" (SO(4) bispectrum) or "ace" (ace_* keys: body order - 1, radial
-        functions and l per nu, expTcheb lambda; weight adds the species-weighted channel)."""
+        functions and l per nu, radial lambda, radial family 1 powTcheb / 2 expTcheb; weight adds
+        the species-weighted channel)."""
         L = load()
         nch = 1 + (weight is not None) + (weight_3ch is not None)
         self._keep = []
@@ -161,6 +163,7 @@ class Context:
             p.ace_nmax[k] = int(ace_nmax[k]) if k < len(ace_nmax) else 0
             p.ace_lmax[k] = int(ace_lmax[k]) if k < len(ace_lmax) else 0
         p.ace_lambda = ace_lambda
+        p.ace_radial_poly = int(ace_radial_poly)
         p.j_max = j_max
         p.r_cut = r_cut
         p.inv_r0_input = default_inv_r0_input() if inv_r0_input is None else inv_r0_input

@@ -111,26 +111,51 @@ __device__ void stage_chunk(const AceDev& tb, AceSmem<T>& sm, int p0, int q0, in
     if (tid < nq) {
         const double4 d = pair_d[p0 + q0 + tid];
         const double r = d.w;
-        // radial_expTcheb with chebTU (g_1 .. g_NA)
-        const double pi = 3.14159265358979323846;
-        const double tmpr = r / rcut;
-        const double cost = 1.0 + cos(pi * tmpr), d_cost = -sin(pi * tmpr) * pi / rcut;
-        const double lambt = exp(tb.lam) - 1.0, expt = exp(-tb.lam * (tmpr - 1.0));
-        const double x = 1.0 - 2.0 * (expt - 1.0) / lambt, d_x = 2.0 * tb.lam / rcut * expt / lambt;
-        double Tm2 = 1.0, Tm1 = x, Um2 = 1.0, Um1 = 2.0 * x, dTm1 = 1.0;  // T_{n-2}, T_{n-1}, U..., dT_{n-1}
-        // n = 1
-        sm.g[tid][0] = (T)(0.5 * cost);
-        sm.dg[tid][0] = (T)(0.5 * d_cost);
-        // n >= 2 uses T_{n-1}; T_1 = x for n = 2
-        for (int n = 2; n <= tb.NA; ++n) {
-            const double Tn1 = Tm1, dTn1 = dTm1;
-            sm.g[tid][n - 1] = (T)(0.25 * (1.0 - Tn1) * cost);
-            sm.dg[tid][n - 1] = (T)(0.25 * (1.0 - Tn1) * d_cost - 0.25 * d_x * dTn1 * cost);
-            // advance to T_n, U_n, dT_n (chebTU)
-            const double Tn = x * Tm1 - (1.0 - x * x) * Um2;
-            const double Un = x * Um1 + Tn;
-            const double dTn = (double)n * Um1;
-            Tm2 = Tm1; Tm1 = Tn; Um2 = Um1; Um1 = Un; dTm1 = dTn;
+        double x, d_x;     // Chebyshev argument and its r-derivative
+        double cost = 0.0, d_cost = 0.0;
+        if (tb.radial == 1) {
+            // radial_powTcheb (mld_radial_module.F90:244-272): x = 1 - 2 (1 - r/rc)^lambda,
+            // g_n = (1 - T_n(x)) / 2, n = 1 .. NA
+            const double tmp = 1.0 - r / rcut, dtmp = pow(tmp, tb.lam - 1.0);
+            x = 1.0 - 2.0 * dtmp * tmp;
+            d_x = 2.0 * tb.lam / rcut * dtmp;
+        } else {
+            // radial_expTcheb (mld_radial_module.F90:275-307): g_1 = (1 + cos(pi r/rc)) / 2,
+            // g_n = (1 - T_{n-1}(x)) (1 + cos(pi r/rc)) / 4, n = 2 .. NA
+            const double pi = 3.14159265358979323846;
+            const double tmpr = r / rcut;
+            cost = 1.0 + cos(pi * tmpr);
+            d_cost = -sin(pi * tmpr) * pi / rcut;
+            const double lambt = exp(tb.lam) - 1.0, expt = exp(-tb.lam * (tmpr - 1.0));
+            x = 1.0 - 2.0 * (expt - 1.0) / lambt;
+            d_x = 2.0 * tb.lam / rcut * expt / lambt;
+        }
+        // chebTU (mld_radial_module.F90:186-211): T_n = x T_{n-1} - (1 - x^2) U_{n-2},
+        // U_n = x U_{n-1} + T_n, dT_n/dx = n U_{n-1}
+        double Tm1 = x, Um2 = 1.0, Um1 = 2.0 * x, dTm1 = 1.0;  // T_1, U_0, U_1, dT_1
+        if (tb.radial == 1) {
+            for (int n = 1; n <= tb.NA; ++n) {
+                sm.g[tid][n - 1] = (T)(0.5 - 0.5 * Tm1);
+                sm.dg[tid][n - 1] = (T)(-0.5 * d_x * dTm1);
+                const double Tn = x * Tm1 - (1.0 - x * x) * Um2;
+                const double Un = x * Um1 + Tn;
+                const double dTn = (double)(n + 1) * Um1;
+                Um2 = Um1; Tm1 = Tn; Um1 = Un; dTm1 = dTn;
+            }
+        } else {
+            sm.g[tid][0] = (T)(0.5 * cost);
+            sm.dg[tid][0] = (T)(0.5 * d_cost);
+            // n >= 2 uses T_{n-1}; T_1 = x for n = 2
+            for (int n = 2; n <= tb.NA; ++n) {
+                const double Tn1 = Tm1, dTn1 = dTm1;
+                sm.g[tid][n - 1] = (T)(0.25 * (1.0 - Tn1) * cost);
+                sm.dg[tid][n - 1] = (T)(0.25 * (1.0 - Tn1) * d_cost - 0.25 * d_x * dTn1 * cost);
+                // advance to T_n, U_n, dT_n (chebTU)
+                const double Tn = x * Tm1 - (1.0 - x * x) * Um2;
+                const double Un = x * Um1 + Tn;
+                const double dTn = (double)n * Um1;
+                Tm1 = Tn; Um2 = Um1; Um1 = Un; dTm1 = dTn;
+            }
         }
         sm.rh[tid][0] = (T)(d.x / r);
         sm.rh[tid][1] = (T)(d.y / r);

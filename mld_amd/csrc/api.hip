@@ -95,6 +95,11 @@ int mld_create(const mld_params* p, mld_ctx** out)
     if (is_ace) {
         // basis tables (compute_ace_nlmu_tools.F90's role; this build's enumeration) and the
         // species weights of the second chemical channel
+        if (p->ace_radial_poly != 0 && p->ace_radial_poly != 1 && p->ace_radial_poly != 2) {
+            fail(MLD_E_UNSUPPORTED, "ace_radial_poly: 1 (powTcheb) and 2 (expTcheb) are built");
+            mld_destroy(ctx);
+            return rc;
+        }
         int trc = mld::build_ace_tables(p->nchannel, p->ace_numax, p->ace_nmax, p->ace_lmax, p->ace_lambda, ctx->ace_h);
         if (trc) {
             fail(trc == 2 ? MLD_E_UNSUPPORTED : MLD_E_ARG,
@@ -147,6 +152,7 @@ int mld_create(const mld_params* p, mld_ctx** out)
         ad.LA2 = ctx->ace_h.LA2;
         ad.nA = ctx->ace_h.nA;
         ad.lam = ctx->ace_h.lam;
+        ad.radial = p->ace_radial_poly == 0 ? 2 : p->ace_radial_poly;
         *out = ctx;
         return MLD_OK;
     }

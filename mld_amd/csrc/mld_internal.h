@@ -155,6 +155,7 @@ struct AceDev {
     const AceTerm* dterms;
     int D, nch, NA, LA, LA2, nA, nslot;
     double lam;
+    int radial;  // 1 powTcheb, 2 expTcheb
 };
 
 // basis enumeration (nu-major; k1 <= k2 <= ... over keys (c, n); then l1, l2, ...; then the

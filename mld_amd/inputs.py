@@ -44,6 +44,7 @@ DEFAULTS: Dict[str, Any] = {
     "ace_nmax_list": " 5 3 1",    # :283
     "ace_lmax_list": " 0 4 2",    # :284
     "ace_lambda_list": " 2.d0 2.d0 2.d0",  # :286
+    "ace_radial_poly": 2,         # :287 (1 powTcheb, 2 expTcheb)
 }
 
 DESCRIPTOR_BSO4, DESCRIPTOR_ACE = 9, 300  # ml_in_ndm_module.F90:1312, 1320
@@ -167,7 +168,7 @@ def context_kwargs(nml: Dict[str, Any], precision: int = 0) -> Dict[str, Any]:
         kw.update(descriptor="ace", ace_numax=numax,
                   ace_nmax=[int(x) for x in _floats(nml["ace_nmax_list"])][:numax],
                   ace_lmax=[int(x) for x in _floats(nml["ace_lmax_list"])][:numax],
-                  ace_lambda=_floats(nml["ace_lambda_list"])[0])
+                  ace_lambda=_floats(nml["ace_lambda_list"])[0], ace_radial_poly=int(nml["ace_radial_poly"]))
     else:
         raise NotImplementedError(f"descriptor_type {dt}: only 9 (SO(4) bispectrum) and 300 (ACE) are built")
     return kw

@@ -5,6 +5,8 @@ Restated from the reference's ACE path where it is readable:
   radial functions   radial_expTcheb + chebTU (mld_radial_module.F90:186-208, 270-306):
                      x = 1 - 2 (exp(-lambda (r/rc - 1)) - 1) / (exp(lambda) - 1),
                      g_1 = (1 + cos(pi r/rc)) / 2,  g_n = (1 - T_{n-1}(x)) (1 + cos(pi r/rc)) / 4
+                     (ace_radial_poly = 2), or radial_powTcheb (:244-272, ace_radial_poly = 1):
+                     x = 1 - 2 (1 - r/rc)^lambda,  g_n = (1 - T_n(x)) / 2
   Y_lm               spherical_harm / solid_harm (angular_functions.f90:277-336)
   A_{c n l m}        = sum_j w_c(j) g_{n+1}(r_j) Y_lm(r_j)  (compute_uniqueA_neigh_ja,
                      compute/compute_ace_AB.F90:2475-2563), dA = w (dg r^ Y + g grad Y)
@@ -31,15 +33,21 @@ import numpy as np
 
 
 # ----------------------------------------------------------------------------- radial
-def radial(r: float, rc: float, lam: float, nmax: int):
-    """g_1..g_nmax and derivatives (radial_expTcheb, mld_radial_module.F90:270-306)."""
+def radial(r: float, rc: float, lam: float, nmax: int, kind: int = 2):
+    """g_1..g_nmax and derivatives: kind 2 radial_expTcheb (mld_radial_module.F90:275-307),
+    kind 1 radial_powTcheb (:244-272); Chebyshev T and U by chebTU (:186-211)."""
     tmpr = r / rc
-    cost = 1.0 + np.cos(pi * tmpr)
-    d_cost = -np.sin(pi * tmpr) * pi / rc
-    lambt = np.exp(lam) - 1.0
-    expt = np.exp(-lam * (tmpr - 1.0))
-    x = 1.0 - 2.0 * (expt - 1.0) / lambt
-    d_x = 2.0 * lam / rc * expt / lambt
+    if kind == 1:
+        tmp = 1.0 - tmpr
+        x = 1.0 - 2.0 * tmp ** lam
+        d_x = 2.0 * lam / rc * tmp ** (lam - 1.0)
+    else:
+        cost = 1.0 + np.cos(pi * tmpr)
+        d_cost = -np.sin(pi * tmpr) * pi / rc
+        lambt = np.exp(lam) - 1.0
+        expt = np.exp(-lam * (tmpr - 1.0))
+        x = 1.0 - 2.0 * (expt - 1.0) / lambt
+        d_x = 2.0 * lam / rc * expt / lambt
     nc = max(nmax, 2)
     T = np.zeros(nc + 1)
     U = np.zeros(nc + 1)
@@ -50,6 +58,8 @@ def radial(r: float, rc: float, lam: float, nmax: int):
         T[n] = x * T[n - 1] - (1.0 - x * x) * U[n - 2]
         U[n] = x * U[n - 1] + T[n]
         dT[n] = n * U[n - 1]
+    if kind == 1:
+        return 0.5 - 0.5 * T[1:nmax + 1], -0.5 * d_x * dT[1:nmax + 1]
     g = np.zeros(nmax)
     dg = np.zeros(nmax)
     for n in range(1, nmax + 1):
@@ -210,7 +220,7 @@ def terms(b):
 
 # ----------------------------------------------------------------------------- descriptors
 def ace(pos, cell_rows, itype, rcut, nch=1, weight=None, numax=3, nmax=(5, 3, 1), lmax=(0, 4, 2), lam=2.0,
-        want_grad=True, nl=None):
+        want_grad=True, nl=None, radial_poly=2):
     """B (nat, D) and dB/dr in the compact slot layout ((P + nat) * 3, D) (slot(a, 0) = self,
     slot(a, k) = k-th neighbour), as the library stores them; neighbour lists as for the
     bispectrum (oracle.neighbours)."""
@@ -244,7 +254,7 @@ def ace(pos, cell_rows, itype, rcut, nch=1, weight=None, numax=3, nmax=(5, 3, 1)
             p = off[a] + q
             v = dxyz[p]
             r = rr[p]
-            Rn[q], dRn[q] = radial(r, rcut, lam, NA)
+            Rn[q], dRn[q] = radial(r, rcut, lam, NA, radial_poly)
             rh[q] = v / r
             for l in range(LA + 1):
                 for m in range(-l, l + 1):

@@ -67,6 +67,27 @@ def test_ace_body_order_4_matches_oracle():
         ctx.close()
 
 
+def test_ace_powtcheb_radial_matches_oracle():
+    """ace_radial_poly = 1 (radial_powTcheb) through the GPU against the oracle."""
+    import mld_amd as M
+
+    rng = np.random.default_rng(14)
+    pos, cell, it = synth.hea_bcc(rng, 2, a=3.1, nspecies=5, sigma=0.1)
+    w = [1.0, 1.3, 0.8, 1.6, 1.1]
+    Eo, Go, nl = A.ace(pos, cell, it, 5.0, nch=2, weight=w, lam=2.5, radial_poly=1)
+    Ee, _, _ = A.ace(pos, cell, it, 5.0, nch=2, weight=w, lam=2.5, want_grad=False)
+    assert rel(Ee, Eo) > 1e-3  # the two radial families give different descriptors
+    for prec, tb, tg in ((M.PREC_FP64, 1e-11, 1e-10), (M.PREC_FP32, 2e-5, 1e-4)):
+        ctx = M.Context(r_cut=5.0, nspecies=5, precision=prec, descriptor="ace", weight=w, ace_lambda=2.5,
+                        ace_radial_poly=1)
+        b, E, G = gpu_ace(ctx, [(pos, cell, it)])
+        assert rel(E, Eo) < tb, (prec, rel(E, Eo))
+        assert rel(G.reshape(Go.shape), Go) < tg, (prec, rel(G.reshape(Go.shape), Go))
+        ctx.close()
+    with pytest.raises(M.MldError):
+        M.Context(r_cut=5.0, nspecies=5, descriptor="ace", ace_radial_poly=3)
+
+
 def test_ace_batch_invariance_and_large_box():
     import mld_amd as M
 

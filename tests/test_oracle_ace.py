@@ -42,6 +42,25 @@ def test_radial_vanishes_at_cutoff_and_derivative():
     assert np.abs(g).max() < 1e-15 and np.abs(dg).max() < 1e-12
 
 
+def test_powtcheb_radial_against_chebyshev_cosine_form():
+    """ace_radial_poly = 1 (radial_powTcheb): g_n = (1 - T_n(x)) / 2 with x = 1 - 2 (1 - r/rc)^lambda,
+    checked against T_n(x) = cos(n arccos x) (independent of the recurrence), central
+    differences of g, g_n(rc) = 0, and g_n(0) = (1 - (-1)^n) / 2."""
+    rc, lam = 5.0, 2.5
+    for r in (0.7, 1.9, 3.3, 4.6):
+        g, dg = A.radial(r, rc, lam, 7, kind=1)
+        x = 1.0 - 2.0 * (1.0 - r / rc) ** lam
+        ref = np.array([0.5 - 0.5 * np.cos(n * np.arccos(x)) for n in range(1, 8)])
+        assert np.abs(g - ref).max() < 1e-13
+        h = 1e-6
+        fd = (A.radial(r + h, rc, lam, 7, kind=1)[0] - A.radial(r - h, rc, lam, 7, kind=1)[0]) / (2 * h)
+        assert np.abs(fd - dg).max() < 1e-7
+    g, dg = A.radial(rc, rc, lam, 7, kind=1)
+    assert np.abs(g).max() < 1e-15
+    g0, _ = A.radial(0.0, rc, lam, 7, kind=1)
+    assert np.allclose(g0, [(1 - (-1) ** n) / 2 for n in range(1, 8)], atol=1e-14)
+
+
 def test_single_neighbour_closed_forms():
     """One neighbour at r: nu = 2 gives g g (2l+1)/4pi (addition theorem); nu = 3 gives
     g g g sqrt(prod(2l+1)/(4pi)^3) (l1 l2 l3; 0 0 0) (rotate the bond onto z)."""
