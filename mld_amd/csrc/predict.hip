@@ -86,14 +86,15 @@ __global__ void __launch_bounds__(256) atom_kernel(int natoms, int D, const T* _
     }
 }
 
-// one wave per centre atom: s[slot*3 + x] = g_a . dB_a^(slot)_x
+// one block per centre atom, its four wavefronts taking interleaved groups of four rows:
+// s[slot*3 + x] = g_a . dB_a^(slot)_x
 template <typename T>
 __global__ void __launch_bounds__(256) slot_dot_kernel(int natoms, int D, const int* __restrict__ pair_off,
                                                        const double* __restrict__ g, const T* __restrict__ grad,
                                                        double* __restrict__ s)
 {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int a = blockIdx.x * 4 + wv;
+    const int a = blockIdx.x;
     if (a >= natoms) return;
     double gr[DMAX / 64];
 #pragma unroll
@@ -104,7 +105,7 @@ __global__ void __launch_bounds__(256) slot_dot_kernel(int natoms, int D, const 
     const size_t s0 = (size_t)pair_off[a] + a;
     const int nrow = 3 * (pair_off[a + 1] - pair_off[a] + 1);
     // four rows at a time: their loads are in flight together, their reductions interleave
-    for (int r = 0; r < nrow; r += 4) {
+    for (int r = 4 * wv; r < nrow; r += 16) {
         double v[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -341,10 +342,10 @@ int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev)
         hipLaunchKernelGGL(energy_kernel, dim3(nstruct), dim3(256), 0, st, ctx->d_atom_off, ps->w0, e_atom, E_dev);
     if (F_dev && natoms) {
         if (f32)
-            hipLaunchKernelGGL(slot_dot_kernel<float>, dim3(nblk), dim3(256), 0, st, natoms, D, ctx->d_pair_off, g,
+            hipLaunchKernelGGL(slot_dot_kernel<float>, dim3(natoms), dim3(256), 0, st, natoms, D, ctx->d_pair_off, g,
                                (const float*)ctx->last_grad, s);
         else
-            hipLaunchKernelGGL(slot_dot_kernel<double>, dim3(nblk), dim3(256), 0, st, natoms, D, ctx->d_pair_off, g,
+            hipLaunchKernelGGL(slot_dot_kernel<double>, dim3(natoms), dim3(256), 0, st, natoms, D, ctx->d_pair_off, g,
                                (const double*)ctx->last_grad, s);
         if (ps->order == 3) {
             const long long nrow = 3 * (P + natoms);
