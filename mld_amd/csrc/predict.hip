@@ -41,7 +41,8 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// one wave per atom: g_a, e_a (+ reference energy) and d2_a
+// one block per atom, thread t owning columns d = t and t + 256 (D <= 512): g_a, e_a
+// (+ reference energy) and d2_a
 template <typename T>
 __global__ void __launch_bounds__(256) atom_kernel(int natoms, int D, const T* __restrict__ energy,
                                                    const int* __restrict__ itype, const double* __restrict__ w,
@@ -50,39 +51,45 @@ __global__ void __launch_bounds__(256) atom_kernel(int natoms, int D, const T* _
                                                    double* __restrict__ g, double* __restrict__ e_atom,
                                                    double* __restrict__ d2)
 {
-    __shared__ double sB[4][DMAX];
-    __shared__ double sV[4][DMAX];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int a = blockIdx.x * 4 + wv;
-    if (a >= natoms) return;  // whole wave leaves together; no block barrier below
-    for (int d = lane; d < D; d += 64) {
-        double b = (double)energy[(size_t)a * D + d];
-        sB[wv][d] = b;
-        if (mu) sV[wv][d] = b - mu[d];
+    __shared__ double sB[DMAX];
+    __shared__ double sV[DMAX];
+    __shared__ double red[2][4];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int a = blockIdx.x;
+    if (a >= natoms) return;  // the whole block leaves together
+    for (int d = tid; d < D; d += 256) {
+        const double b = (double)energy[(size_t)a * D + d];
+        sB[d] = b;
+        if (mu) sV[d] = b - mu[d];
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __syncthreads();
     double epart = 0.0, dpart = 0.0;
-    for (int d = lane; d < D; d += 64) {
-        // lanes take consecutive d, so each step over q reads one contiguous row of U / Sigma^-1
+    for (int d = tid; d < D; d += 256) {
+        // threads take consecutive d, so each step over q reads one contiguous row of U / Sigma^-1
         double hd = 0.0;
         if (U)  // U symmetric: (U B)_d = sum_q U[q][d] B_q
-            for (int q = 0; q < D; ++q) hd += U[(size_t)q * D + d] * sB[wv][q];
+            for (int q = 0; q < D; ++q) hd += U[(size_t)q * D + d] * sB[q];
         const double gd = w[d] + hd;
         g[(size_t)a * D + d] = gd;
-        epart += sB[wv][d] * (w[d] + 0.5 * hd);
+        epart += sB[d] * (w[d] + 0.5 * hd);
         if (mu) {
             // d2 = sum_d v_d (Sigma^-1 v)_d = sum_d v_d sum_q v_q S[q][d] (S as given, any symmetry)
             double t = 0.0;
-            for (int q = 0; q < D; ++q) t += sV[wv][q] * sinv[(size_t)q * D + d];
-            dpart += t * sV[wv][d];
+            for (int q = 0; q < D; ++q) t += sV[q] * sinv[(size_t)q * D + d];
+            dpart += t * sV[d];
         }
     }
+    // wavefront sums, then the four wavefronts in order
     epart = wave_sum(epart);
-    if (mu) dpart = wave_sum(dpart);
+    dpart = wave_sum(dpart);
     if (lane == 0) {
-        e_atom[a] = epart + (ref ? ref[itype[a]] : 0.0);
-        if (d2) d2[a] = dpart;
+        red[0][wv] = epart;
+        red[1][wv] = dpart;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        e_atom[a] = red[0][0] + red[0][1] + red[0][2] + red[0][3] + (ref ? ref[itype[a]] : 0.0);
+        if (d2) d2[a] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
     }
 }
 
@@ -324,10 +331,10 @@ int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev)
     const int nblk = (natoms + 3) / 4;
     if (natoms) {
         if (f32)
-            hipLaunchKernelGGL(atom_kernel<float>, dim3(nblk), dim3(256), 0, st, natoms, D, (const float*)ctx->last_energy,
+            hipLaunchKernelGGL(atom_kernel<float>, dim3(natoms), dim3(256), 0, st, natoms, D, (const float*)ctx->last_energy,
                                ctx->d_itype, ps->d_w, ps->d_U, ps->d_ref, mu, ps->d_sinv, g, e_atom, d2_dev);
         else
-            hipLaunchKernelGGL(atom_kernel<double>, dim3(nblk), dim3(256), 0, st, natoms, D, (const double*)ctx->last_energy,
+            hipLaunchKernelGGL(atom_kernel<double>, dim3(natoms), dim3(256), 0, st, natoms, D, (const double*)ctx->last_energy,
                                ctx->d_itype, ps->d_w, ps->d_U, ps->d_ref, mu, ps->d_sinv, g, e_atom, d2_dev);
     }
     if (ps->order == 3 && natoms) {
