@@ -105,6 +105,9 @@ int mld_ace(mld_ctx* ctx, int nat, const double* pos, const double* cell, const 
  * Precision of energy_dev / grad_dev: double (FP64 context) or float (FP32). */
 int mld_batch_neighbours(mld_ctx* ctx, int nstruct, const int* atom_off, const double* pos_dev,
                          const double* cell_dev, const int* itype_dev, long long* npairs);
+/* CSR pair list of the last neighbour build.  The pointers below are owned by the
+ * context and stay valid until its next neighbour build (mld_batch_neighbours, or
+ * mld_bso4 / mld_ace / mld_max_neighbours / mld_gram_add_structures, which run one). */
 int mld_batch_pairs_dev(mld_ctx* ctx, const int** pair_off_dev, const int** pair_idx_dev);
 /* pair geometry of the last neighbour build: P x (dx, dy, dz, r), dx = r_j - r_i (Angstrom) */
 int mld_batch_pair_geometry_dev(mld_ctx* ctx, const double** pair_d_dev);
