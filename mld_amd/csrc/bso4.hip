@@ -578,8 +578,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
             for (int q = lane; q < nb * 3; q += 64) outw[q] = (T)0;
             cx<T>* vb = &sm.u.g.vb[w][0][0];
             // level 0: U = 1, dU = 0, V = dfcut rhat.  The level loop below is unrolled (level
-            // sizes become constants); the FP32 kernel is held to 168 VGPRs, three wavefronts
-            // per SIMD, which its LDS footprint allows anyway
+            // sizes become constants); the FP32 kernel is held to 128 VGPRs, four wavefronts
+            // per SIMD, which its LDS footprint (four workgroups per CU) allows anyway
             if (lane == 0) {
                 sm.u.g.ub[w][0][0] = cx<T>{(T)1, (T)0};
 #pragma unroll
