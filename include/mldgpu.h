@@ -36,8 +36,13 @@ enum {
     MLD_E_NEIGH = 3,      /* neighbour count above nn_max (neighbours.F90:352-358) */
     MLD_E_SINGULAR = 4,   /* factorisation failed: matrix not positive definite */
     MLD_E_UNSUPPORTED = 5,/* parameter combination not built */
-    MLD_E_STATE = 6       /* call order (e.g. solve before gram_begin) */
+    MLD_E_STATE = 6,      /* call order (e.g. solve before gram_begin) */
+    MLD_E_NOT_CONVERGED = 7 /* MLD_SOLVE_CHOL_IR stopped above MLD_REFINE_TOL: w and diag_out hold
+                               the best iterate */
 };
+
+/* relative residual |h - (G + Lambda) w| / |h| that MLD_SOLVE_CHOL_IR must reach */
+#define MLD_REFINE_TOL 1e-11
 
 enum { MLD_PREC_FP64 = 0, MLD_PREC_FP32 = 1 };
 enum { MLD_DESC_BSO4 = 9, MLD_DESC_ACE = 300 };  /* descriptor_bispectrum_so4, descriptor_ace */
@@ -164,7 +169,9 @@ int mld_gram_set_keep_rows(mld_ctx* ctx, int keep);
  * unique quadratic columns it returns the reference's full-layout minimum-norm solution
  * folded onto those columns.  diag_out then holds [Jacobi sweeps, relative residual,
  * smallest kept |eigenvalue|, largest |eigenvalue|].
- * diag_out (optional, 4 doubles): [iterations, final relative residual, min pivot, max pivot]. */
+ * diag_out (optional, 4 doubles): [iterations, final relative residual, min pivot, max pivot].
+ * MLD_SOLVE_CHOL_IR returns MLD_E_NOT_CONVERGED (w and diag_out filled) when the refinement
+ * stops with a relative residual above MLD_REFINE_TOL. */
 int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w, double* diag_out);
 /* The MLD_SOLVE_SVD solve with an explicit relative threshold: the &input_ml key svd_rcond
  * (read_ml_file.F90:353-354, passed to dgelsd by gram_reduce_and_solve,
@@ -208,11 +215,16 @@ int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev);
 /* ---- serving: continuous batching of prediction requests ----
  * No counterpart in the reference (which evaluates one configuration at a time,
  * mld.F90:1579-1772).  mld_serve_create takes the model and Mahalanobis state of
- * ctx as they are at the call (mld_model_set first) and starts a worker thread with
+ * ctx as they are at the call (mld_model_set first; linear and quadratic models, a kernel
+ * model gives MLD_E_UNSUPPORTED) and starts a worker thread with
  * two private contexts on ctx's device.  Waiting requests are admitted, oldest
  * first, into whichever batch slot is free, up to max_batch_structs structures,
  * max_batch_atoms atoms and the HBM budget not held by the batch in flight; the two
- * slots' batches overlap on the device.  A request is one structure: pos (3, nat)
+ * slots' batches overlap on the device.  Admission first takes an estimate of each
+ * request's bytes (0.12 atoms / A^3 inside r_cut); once the batch's neighbour lists are
+ * built it reserves the bytes of the actual pair counts and returns the requests that no
+ * longer fit to the head of the queue (a batch of one runs when nothing else is in flight).
+ * A request is one structure: pos (3, nat)
  * column-major, cell (3, 3) columns = lattice vectors, itype (nat) 0-based.
  * mld_serve_poll (wait = 1 blocks) returns E, F (3, nat), d2 (nat) (pointers may be
  * NULL) and releases the request; done = 0 while it is queued or running.

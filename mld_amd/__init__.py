@@ -17,7 +17,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libmldgpu.so")
 
-MLD_OK, MLD_E_ARG, MLD_E_DEVICE, MLD_E_NEIGH, MLD_E_SINGULAR, MLD_E_UNSUPPORTED, MLD_E_STATE = range(7)
+(MLD_OK, MLD_E_ARG, MLD_E_DEVICE, MLD_E_NEIGH, MLD_E_SINGULAR, MLD_E_UNSUPPORTED, MLD_E_STATE,
+ MLD_E_NOT_CONVERGED) = range(8)
 PREC_FP64, PREC_FP32 = 0, 1
 DESC_BSO4, DESC_ACE = 9, 300
 

@@ -50,6 +50,21 @@ struct DevBuf {
 
 }  // namespace
 
+// An entry point that builds a batch in its own temporary device buffers (the host-array calls)
+// clears, on every return path, the context's pointers into those buffers once the stream has
+// drained: a later _dev call on the context then finds no batch instead of freed memory.
+// Declared after the buffers, it runs before they are freed.
+struct BatchStateGuard {
+    mld_ctx* c;
+    ~BatchStateGuard()
+    {
+        (void)hipStreamSynchronize(c->stream);
+        c->d_itype = nullptr;
+        c->last_energy = c->last_grad = nullptr;
+        c->last_ek = c->last_fk = nullptr;
+    }
+};
+
 extern "C" {
 
 const char* mld_last_error(const mld_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -357,6 +372,7 @@ int mld_max_neighbours(mld_ctx* ctx, int nat, const double* pos, const double* c
 {
     if (!ctx || nat < 0 || (nat > 0 && (!pos || !cell)) || !nn_max) return MLD_E_ARG;
     DevBuf dpos, dcell, dtype;
+    BatchStateGuard guard{ctx};
     MLD_HIP(ctx, hipMalloc(&dpos.p, (size_t)3 * nat * sizeof(double) + 8));
     MLD_HIP(ctx, hipMalloc(&dcell.p, 9 * sizeof(double)));
     MLD_HIP(ctx, hipMalloc(&dtype.p, (size_t)nat * sizeof(int) + 8));
@@ -371,7 +387,6 @@ int mld_max_neighbours(mld_ctx* ctx, int nat, const double* pos, const double* c
     int m = 0;
     for (int a = 0; a < nat; ++a) m = std::max(m, po[a + 1] - po[a]);
     *nn_max = m;
-    ctx->d_itype = nullptr;
     return MLD_OK;
 }
 
@@ -385,6 +400,7 @@ int mld_bso4(mld_ctx* ctx, int nat, const double* pos, const double* cell, const
     for (int a = 0; a < nat; ++a)
         if (itype[a] < 0 || itype[a] >= ctx->nsp) return mld_fail(ctx, MLD_E_ARG, "itype out of range");
     DevBuf dpos, dcell, dtype, den, dgr, dforce, dkind, dnn, dwide;
+    BatchStateGuard guard{ctx};
     MLD_HIP(ctx, hipMalloc(&dpos.p, (size_t)3 * nat * sizeof(double) + 8));
     MLD_HIP(ctx, hipMalloc(&dcell.p, 9 * sizeof(double)));
     MLD_HIP(ctx, hipMalloc(&dtype.p, (size_t)nat * sizeof(int) + 8));
@@ -455,9 +471,6 @@ int mld_bso4(mld_ctx* ctx, int nat, const double* pos, const double* cell, const
             for (int k = 0; k < n_neigh[a]; ++k) kind_neigh[(size_t)a + (size_t)nat * k] = pidx[po[a] + k] + 1;
         }
     }
-    ctx->d_itype = nullptr;
-    ctx->last_energy = ctx->last_grad = nullptr;
-    ctx->last_ek = ctx->last_fk = nullptr;
     return MLD_OK;
 }
 
@@ -491,16 +504,7 @@ int mld_gram_add_structures(mld_ctx* ctx, int order, int nstruct, const int* nat
     if (nstruct == 0) return MLD_OK;
     DevBuf dpos, dcell, dtype, dF, den, dgr, dek, dfk;
     // the batch state points into the buffers above: synchronise and clear it before they are freed
-    struct StateGuard {
-        mld_ctx* c;
-        ~StateGuard()
-        {
-            (void)hipStreamSynchronize(c->stream);
-            c->d_itype = nullptr;
-            c->last_energy = c->last_grad = nullptr;
-            c->last_ek = c->last_fk = nullptr;
-        }
-    } guard{ctx};
+    BatchStateGuard guard{ctx};
     MLD_HIP(ctx, hipMalloc(&dpos.p, (size_t)3 * N * sizeof(double) + 8));
     MLD_HIP(ctx, hipMalloc(&dcell.p, (size_t)9 * nstruct * sizeof(double)));
     MLD_HIP(ctx, hipMalloc(&dtype.p, (size_t)N * sizeof(int) + 8));

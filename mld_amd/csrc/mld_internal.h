@@ -269,4 +269,6 @@ int device_ensure(mld_ctx* ctx, void** p, size_t* cap, size_t bytes);
 int pair_transpose(mld_ctx* ctx, const int** rev_off, const int** rev_pair, const int** pair_src);
 void predict_free(mld_ctx* ctx);
 int predict_clone(mld_ctx* dst, const mld_ctx* src);
+// order of the fitted model (1, 2, 3), 0 before mld_model_set
+int predict_order(const mld_ctx* ctx);
 void kept_rows_free(mld_ctx* ctx);

@@ -211,6 +211,12 @@ void predict_free(mld_ctx* ctx)
     ctx->predict_state = nullptr;
 }
 
+int predict_order(const mld_ctx* ctx)
+{
+    const PredictState* ps = reinterpret_cast<const PredictState*>(ctx->predict_state);
+    return ps ? ps->order : 0;
+}
+
 // Copy the model and Mahalanobis state of src to dst (same D, same device).
 int predict_clone(mld_ctx* dst, const mld_ctx* src)
 {

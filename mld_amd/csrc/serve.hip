@@ -53,6 +53,7 @@ struct Slot {
     std::vector<Request*> reqs;
     size_t bytes = 0;
     int natoms = 0, nstruct = 0, want_f = 0, want_d2 = 0;
+    long long npairs = 0;
     std::vector<int> atom_off;
     // pinned host staging and device buffers (grown geometrically)
     void* h_in = nullptr; size_t cap_h_in = 0;
@@ -68,7 +69,7 @@ struct mld_server {
     mld_ctx* parent = nullptr;
     size_t budget = 0;
     int max_structs = 0, max_atoms = 0;
-    size_t per_atom_bytes = 0;
+    size_t nn_est = 0;
     Slot slots[NSLOT];
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -84,6 +85,7 @@ struct mld_server {
     long long atoms_done = 0, batches = 0, structs_in_batches = 0;
     int max_batch = 0;
     size_t inflight = 0, max_inflight = 0;
+    bool hold = false;  // the queue front waits for the batch in flight to finish
 };
 
 namespace {
@@ -105,8 +107,23 @@ int grow_dev(mld_ctx* c, void** p, size_t* cap, size_t need)
     return device_ensure(c, p, cap, std::max(need, *cap * 2));
 }
 
-// Launch the batch held by slot s (requests already chosen); returns non-zero on error.
-int launch(mld_server* sv, Slot& s)
+// Device bytes a batch holds while it is in flight: inputs, pair list and per-pair geometry
+// record, descriptors (and gradients), outputs.  Linear in its arguments, so the bytes of a batch
+// are the sum of the bytes of its structures.
+size_t batch_bytes(const mld_ctx* c, size_t natoms, size_t nstruct, size_t npairs, int want_f)
+{
+    const size_t esz = c->p.precision == MLD_PREC_FP32 ? 4 : 8;
+    size_t b = natoms * (3 * sizeof(double) + sizeof(int)) + nstruct * 9 * sizeof(double);
+    b += (natoms + 1) * sizeof(int) + npairs * (sizeof(int) + 4 * sizeof(double) + 24 * esz);
+    b += natoms * (size_t)c->D * esz;
+    if (want_f) b += (npairs + natoms) * 3 * (size_t)c->D * esz;
+    b += (nstruct + 4 * natoms) * sizeof(double);
+    return b;
+}
+
+// First phase of a batch held by slot s: upload and neighbour lists; fills act[i] with the
+// device bytes request i needs at its actual pair count.
+int launch_neighbours(mld_server* sv, Slot& s, std::vector<size_t>& act)
 {
     mld_ctx* c = s.ctx;
     const int ns = (int)s.reqs.size();
@@ -140,6 +157,25 @@ int launch(mld_server* sv, Slot& s)
     long long P = 0;
     rc = mld_batch_neighbours(c, ns, s.atom_off.data(), d_pos, d_cell, d_it, &P);
     if (rc) return rc;
+    s.npairs = P;
+    // pair counts per structure from the CSR offsets (the neighbour build has completed)
+    std::vector<int> off(na + 1);
+    MLD_HIP(c, hipMemcpy(off.data(), c->d_pair_off, (size_t)(na + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    act.assign(ns, 0);
+    for (int i = 0; i < ns; ++i)
+        act[i] = batch_bytes(c, s.reqs[i]->nat, 1, (size_t)(off[s.atom_off[i + 1]] - off[s.atom_off[i]]),
+                             s.reqs[i]->want_f);
+    return MLD_OK;
+}
+
+// Second phase: descriptors, prediction and download on the slot's stream.
+int launch_rest(mld_server* sv, Slot& s)
+{
+    (void)sv;
+    mld_ctx* c = s.ctx;
+    const int ns = s.nstruct, na = s.natoms;
+    const long long P = s.npairs;
+    int rc = MLD_OK;
     const size_t esz = c->p.precision == MLD_PREC_FP32 ? 4 : 8;
     const size_t b_e = (size_t)na * c->D * esz;
     const size_t b_g = s.want_f ? (size_t)(P + na) * 3 * c->D * esz : 0;
@@ -204,6 +240,7 @@ void worker_main(mld_server* sv)
             if (q == hipSuccess) complete(sv, s, MLD_OK);
             else if (q != hipErrorNotReady) complete(sv, s, MLD_E_DEVICE);
             else any_busy = true;
+            if (!s.busy) sv->hold = false;
         }
         sv->cv_done.notify_all();
         if (sv->stop && sv->queue.empty() && !any_busy) break;
@@ -211,7 +248,7 @@ void worker_main(mld_server* sv)
         Slot* free_slot = nullptr;
         for (Slot& s : sv->slots)
             if (!s.busy) { free_slot = &s; break; }
-        if (free_slot && !sv->queue.empty()) {
+        if (free_slot && !sv->queue.empty() && !sv->hold) {
             Slot& s = *free_slot;
             int na = 0;
             size_t bytes = 0;
@@ -229,15 +266,55 @@ void worker_main(mld_server* sv)
                 s.busy = true;
                 s.bytes = bytes;
                 sv->inflight += bytes;
-                sv->max_inflight = std::max(sv->max_inflight, sv->inflight);
                 sv->batches += 1;
                 sv->structs_in_batches += (long long)s.reqs.size();
-                sv->max_batch = std::max(sv->max_batch, (int)s.reqs.size());
                 const double t = now_s();
                 for (Request* r : s.reqs) r->t_start = t;
+                // the admission above used the requests' estimated bytes; with the neighbour lists
+                // built, the batch reserves its actual bytes and gives back the requests that no
+                // longer fit (a batch keeps at least one request when nothing else is in flight)
+                std::vector<size_t> act;
                 lk.unlock();
-                int rc = launch(sv, s);
+                int rc = launch_neighbours(sv, s, act);
                 lk.lock();
+                if (!rc) {
+                    const size_t other = sv->inflight - s.bytes;
+                    size_t tot = 0;
+                    size_t k = 0;
+                    while (k < s.reqs.size()) {
+                        if (other + tot + act[k] > sv->budget && !(k == 0 && other == 0)) break;
+                        tot += act[k];
+                        ++k;
+                    }
+                    if (k < s.reqs.size()) {
+                        for (size_t i = s.reqs.size(); i-- > k;) sv->queue.push_front(s.reqs[i]);
+                        sv->structs_in_batches -= (long long)(s.reqs.size() - k);
+                        s.reqs.resize(k);
+                    }
+                    sv->inflight = other;
+                    s.bytes = 0;
+                    if (k == 0) {
+                        // the batch in flight must finish first
+                        s.busy = false;
+                        sv->batches -= 1;
+                        sv->hold = true;
+                        continue;
+                    }
+                    sv->max_batch = std::max(sv->max_batch, (int)k);
+                    if (act.size() != k) {  // rebuild the lists of the kept requests
+                        lk.unlock();
+                        rc = launch_neighbours(sv, s, act);
+                        lk.lock();
+                    }
+                    s.bytes = tot;
+                    sv->inflight += tot;
+                    sv->max_inflight = std::max(sv->max_inflight, sv->inflight);
+                    if (!rc) {
+                        lk.unlock();
+                        rc = launch_rest(sv, s);
+                        lk.lock();
+                    }
+                }
                 if (rc) {
                     sv->err = mld_last_error(s.ctx);
                     (void)hipStreamSynchronize(s.ctx->stream);
@@ -266,18 +343,17 @@ int mld_serve_create(mld_ctx* ctx, size_t hbm_budget_bytes, int max_batch_struct
     if (max_batch_structs < 1 || max_batch_atoms < 1 || hbm_budget_bytes == 0)
         return mld_fail(ctx, MLD_E_ARG, "mld_serve_create: limits must be positive");
     if (!ctx->predict_state) return mld_fail(ctx, MLD_E_STATE, "mld_serve_create: mld_model_set first");
+    if (predict_order(ctx) == 3)
+        return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_serve_create: kernel models (order 3) are not served");
     mld_server* sv = new mld_server();
     sv->parent = ctx;
     sv->budget = hbm_budget_bytes;
     sv->max_structs = max_batch_structs;
     sv->max_atoms = max_batch_atoms;
-    // per-atom device bytes of a request: descriptors + gradients for up to
-    // nn_est neighbours (number density 0.12 / A^3 inside r_cut), pair list,
-    // inputs and outputs
+    // admission estimate of a request's neighbours per atom (number density 0.12 / A^3 inside
+    // r_cut); a batch then reserves the bytes of its actual pair count (worker_main)
     const double rc = ctx->p.r_cut;
-    const size_t nn_est = (size_t)std::ceil(4.0 / 3.0 * 3.14159265358979323846 * rc * rc * rc * 0.12);
-    const size_t esz = ctx->p.precision == MLD_PREC_FP32 ? 4 : 8;
-    sv->per_atom_bytes = (size_t)ctx->D * esz * (1 + 3 * (nn_est + 1)) + nn_est * (4 + 32) + 64 + 32;
+    sv->nn_est = (size_t)std::ceil(4.0 / 3.0 * 3.14159265358979323846 * rc * rc * rc * 0.12);
     for (Slot& s : sv->slots) {
         int rc2 = mld_create(&ctx->p, &s.ctx);
         if (rc2 == MLD_OK) rc2 = predict_clone(s.ctx, ctx);
@@ -314,7 +390,7 @@ int mld_serve_submit(mld_server* sv, int nat, const double* pos, const double* c
         for (int x = 0; x < 3; ++x) r->pos[3 * (size_t)a + x] = pos[3 * (size_t)a + x];
     r->cell.assign(cell, cell + 9);
     r->itype.assign(itype, itype + nat);
-    r->bytes = (size_t)nat * sv->per_atom_bytes;
+    r->bytes = batch_bytes(c, (size_t)nat, 1, (size_t)nat * sv->nn_est, r->want_f);
     if (r->bytes > sv->budget) {
         delete r;
         return mld_fail(c, MLD_E_ARG, "mld_serve_submit: structure exceeds the HBM budget");

@@ -816,6 +816,9 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
         diag_out[2] = mn;
         diag_out[3] = mx;
     }
+    if (method == MLD_SOLVE_CHOL_IR && !(relres <= MLD_REFINE_TOL))
+        return mld_fail(ctx, MLD_E_NOT_CONVERGED, "refinement stopped at relative residual " + std::to_string(relres) +
+                                                      " after " + std::to_string(iters) + " passes");
     return MLD_OK;
 }
 

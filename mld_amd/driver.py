@@ -103,18 +103,20 @@ def fit(ml_path: str, out_dir: Optional[str] = None, precision: int = 0, method:
     order = inputs.fit_order(nml)
     if order == 3:
         raise NotImplementedError("kernel fits need the kernel-point selection of the reference (MCD / CUR)")
+    if order == 2 and inputs.quadratic_type(nml) != inputs.QUADRATIC_QML:
+        raise NotImplementedError(f"mld_type_quadratic {inputs.quadratic_type(nml)}: only the full quadratic fit "
+                                  "(mld_type_quadratic = 2) is built by this driver")
     base = os.path.dirname(os.path.abspath(ml_path))
     configs = select_configs(nml, base)
+    weights = inputs.model_weights(nml, configs)
     ctx = Context(**inputs.context_kwargs(nml, precision))
     try:
         gf = F.GramFit(ctx, order=order)
-        by_class: Dict[str, List] = {}
-        for c, cl in configs:
-            by_class.setdefault(cl.cls, []).append((c, cl))
-        for cls, group in by_class.items():
-            cl = group[0][1]
-            w_e = cl.weights[0] if cl.use_e else 0.0
-            w_f = cl.weights[1] if cl.use_f else 0.0
+        # one batch per class and model weight pair (loss_per gives every atom count its own weights)
+        by_class: Dict[Tuple[str, float, float], List] = {}
+        for (c, cl), (we, wf) in zip(configs, weights):
+            by_class.setdefault((cl.cls, we if cl.use_e else 0.0, wf if cl.use_f else 0.0), []).append((c, cl))
+        for (cls, w_e, w_f), group in by_class.items():
             if w_e == 0.0 and w_f == 0.0:
                 continue
             b = _batch(ctx, group, inputs.elements(nml))

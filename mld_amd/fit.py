@@ -20,6 +20,7 @@ from typing import Optional, Sequence, Tuple
 import numpy as np
 
 SOLVE_CHOL_IR, SOLVE_CHOL, SOLVE_SVD = 0, 1, 2
+E_NOT_CONVERGED = 7  # MLD_E_NOT_CONVERGED
 
 
 def ncols(D: int, order: int, nk: int = 0) -> int:
@@ -129,22 +130,25 @@ class GramFit:
                                                   h.ctypes.data_as(C.c_void_p)))
         return G, h
 
-    def solve(self, lam: float, method: int = SOLVE_CHOL_IR, rcond: float = -1.0):
+    def solve(self, lam: float, method: int = SOLVE_CHOL_IR, rcond: float = -1.0, allow_unconverged: bool = False):
         """(G + lambda) w = h; method SOLVE_SVD gives the minimum-norm solution with eigenvalues at
-        or below rcond * max dropped (rcond < 0: machine epsilon, the reference's svd_rcond = -1)."""
+        or below rcond * max dropped (rcond < 0: machine epsilon, the reference's svd_rcond = -1).
+        SOLVE_CHOL_IR raises MldError (code E_NOT_CONVERGED) when the refinement stops above the
+        library's residual tolerance, unless allow_unconverged: then the best iterate comes back
+        with diag["converged"] = False."""
         w = np.zeros(self.ncol)
         diag = np.zeros(4)
         if method == SOLVE_SVD:
             self.ctx.check(self.ctx.L.mld_gram_solve_svd(self.ctx.h, lam, rcond, w.ctypes.data_as(C.c_void_p),
                                                          diag.ctypes.data_as(C.c_void_p)))
-        else:
-            self.ctx.check(self.ctx.L.mld_gram_solve(self.ctx.h, lam, method, w.ctypes.data_as(C.c_void_p),
-                                                     diag.ctypes.data_as(C.c_void_p)))
-        if method == SOLVE_SVD:
             return w, {"sweeps": int(diag[0]), "rel_residual": float(diag[1]), "min_kept_eigenvalue": float(diag[2]),
                        "max_eigenvalue": float(diag[3])}
+        rc = self.ctx.L.mld_gram_solve(self.ctx.h, lam, method, w.ctypes.data_as(C.c_void_p),
+                                       diag.ctypes.data_as(C.c_void_p))
+        if not (rc == E_NOT_CONVERGED and allow_unconverged):
+            self.ctx.check(rc)
         return w, {"iterations": int(diag[0]), "rel_residual": float(diag[1]), "min_pivot": float(diag[2]),
-                   "max_pivot": float(diag[3])}
+                   "max_pivot": float(diag[3]), "converged": rc == 0}
 
     def ridge_vector(self, lam: float) -> np.ndarray:
         """The per-column ridge the solve applies (lambda, lambda/2 on off-diagonal quadratic columns)."""

@@ -40,6 +40,12 @@ DEFAULTS: Dict[str, Any] = {
     "weight_per_element": "1.d0",      # :510
     "weight_per_element_3ch": "1.d0",  # :511
     "snap_order": -777,           # :591
+    "mld_order": -777,            # :590 (same key as snap_order, fix_double_input_for_integer)
+    "mld_type_quadratic": -777,   # :568 (-777 both unset: 1 = QNML, :572)
+    "snap_type_quadratic": -777,  # :569
+    "type_of_loss": 1,            # :530 (1 loss_init, 2 loss_fair, 3 loss_per; ml_in_ndm_module.F90:1407-1409)
+    "factor_energy_error": 1.0,   # :391
+    "factor_force_error": 1.0,    # :390
     "ace_numax": 3,               # :282
     "ace_nmax_list": " 5 3 1",    # :283
     "ace_lmax_list": " 0 4 2",    # :284
@@ -174,12 +180,61 @@ def context_kwargs(nml: Dict[str, Any], precision: int = 0) -> Dict[str, Any]:
     return kw
 
 
+def _pair_key(nml: Dict[str, Any], true_key: str, fake_key: str, default: int) -> int:
+    """fix_double_input_for_integer (read_ml_file.F90:2555-2594): two names of one key; both
+    set to different values is an error, neither set gives the default."""
+    a, b = int(nml.get(true_key, -777)), int(nml.get(fake_key, -777))
+    if a != -777 and b != -777 and a != b:
+        raise ValueError(f"{true_key} and {fake_key} are incompatible ({a} != {b})")
+    if a == -777 and b == -777:
+        return default
+    return a if a != -777 else b
+
+
+QUADRATIC_QNML, QUADRATIC_QML = 1, 2  # mld_type_quadratic_qnml / _qml (ml_in_ndm_module.F90:1364-1365)
+
+
 def fit_order(nml: Dict[str, Any]) -> int:
-    """1 linear, 2 quadratic (snap_order), 3 kernel model (ml_type 1, read_ml_file.F90:846-851)."""
+    """1 linear, 2 quadratic (mld_order / snap_order), 3 kernel model (ml_type 1,
+    read_ml_file.F90:846-851)."""
     if int(nml["ml_type"]) == 1:
         return 3
-    so = int(nml["snap_order"])
+    so = _pair_key(nml, "mld_order", "snap_order", -777)
     return 2 if so == 2 else 1
+
+
+def quadratic_type(nml: Dict[str, Any]) -> int:
+    """mld_type_quadratic (alias snap_type_quadratic), default 1 = QNML (read_ml_file.F90:567-572,
+    666-669): 1 the two-stage fit on E - E_LML (parameters.F90:414-420), 2 the full quadratic fit."""
+    return _pair_key(nml, "mld_type_quadratic", "snap_type_quadratic", QUADRATIC_QNML)
+
+
+def model_weights(nml: Dict[str, Any], configs) -> List[Tuple[float, float]]:
+    """Per-configuration (w_e_model, w_f_model) of the loss (main_train_regularization.F90:311-372):
+    type_of_loss 1 multiplies the db_model.in class weights by factor_energy_error /
+    factor_force_error; 2 (loss_fair) also divides them by the share of energy / force data
+    points in the training set (no_ene = n_E / (n_E + n_F + n_S), n_F = 3 N per configuration
+    with forces, n_S = 6 per configuration with a stress, mld.F90:636-650); 3 (loss_per) divides
+    them by N and 3 N of the configuration.  configs: (io.Config, io.DbClass) pairs."""
+    lt = int(nml["type_of_loss"])
+    if lt not in (1, 2, 3):
+        raise ValueError(f"unknown loss type type_of_loss {lt}")
+    fe, ff = float(nml["factor_energy_error"]), float(nml["factor_force_error"])
+    forces_on = bool(nml["desc_forces"])
+    n_e = sum(1 for c, _ in configs if c.has_energy)
+    n_f = sum(3 * c.nat for c, _ in configs if c.has_force and forces_on)
+    n_s = sum(6 for c, _ in configs if c.has_stress and forces_on)
+    no_e, no_f, no_s = max(n_e, 1), max(n_f, 1), max(n_s, 1)
+    tot = no_e + no_f + no_s
+    out = []
+    for c, cl in configs:
+        we, wf = float(cl.weights[0]), float(cl.weights[1])
+        if n_e > 0:
+            we = we * fe if lt == 1 else (we / (no_e / tot) * fe if lt == 2 else we / c.nat * fe)
+        if n_f > 0:
+            wf = wf * ff if lt == 1 else (wf / (no_f / tot) * ff if lt == 2 else wf / (3 * c.nat) * ff)
+        out.append((we, wf))
+    return out
 
 
 def ridge(nml: Dict[str, Any]) -> float:

@@ -131,10 +131,12 @@ def test_ace_fit_predict_and_serve():
     sv = Server(ctx, hbm_budget_bytes=1 << 30, max_batch_structs=4)
     ids = [sv.submit(p, c, t) for p, c, t in structs]
     res = [sv.poll(i) for i in ids]
+    # each request alone (submitted after the previous one returned)
+    alone = [sv.poll(sv.submit(p, c, t)) for p, c, t in structs]
     sv.close()
     off = b.atom_off
-    for s, (Es, Fs, Qs, _) in enumerate(res):
-        assert Es == float(Ep[s])
-        assert np.array_equal(Fs, Fp.cpu().numpy()[off[s]:off[s + 1]])
-        assert np.array_equal(Qs, Q.cpu().numpy()[off[s]:off[s + 1]])
+    for s, ((Es, Fs, Qs, _), (Ea, Fa, Qa, _)) in enumerate(zip(res, alone)):
+        assert Es == float(Ep[s]) and Ea == Es
+        assert np.array_equal(Fs, Fp.cpu().numpy()[off[s]:off[s + 1]]) and np.array_equal(Fa, Fs)
+        assert np.array_equal(Qs, Q.cpu().numpy()[off[s]:off[s + 1]]) and np.array_equal(Qa, Qs)
     ctx.close()

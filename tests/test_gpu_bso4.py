@@ -169,3 +169,37 @@ def test_edge_cases(ctx64):
     b.bso4(energy, grad)
     ctx64.synchronize()
     assert energy.shape[0] == len(p2) + 10
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_inner_cut_type3_against_oracle(prec):
+    """Pair inner cutoff (r_cut_pair_in, fcut type 3 of width r_cut_width_in) with neighbours in
+    its taper: B and dB/dr equal the oracle's (FP64 1e-10, FP32 1e-5)."""
+    import mld_amd as M
+
+    rng = np.random.default_rng(12)
+    pos, cell = synth.bcc_supercell(rng, 2.855, 2, 0.1)
+    it = (np.arange(len(pos)) % 2).astype(np.int32)
+    rin = np.array([2.3, 2.3, 2.3, 0.0])
+    c = M.Context(j_max=4.0, r_cut=4.7, nspecies=2, r_cut_pair_in=rin, r_cut_width_in=0.4, precision=prec)
+    e, f, nn, _ = c.bso4(pos, cell, it)
+    c.close()
+    eo, go, nl = O.bso4(pos, cell, it, 8, 4.7, rin=rin, win=0.4)
+    tol = 1e-10 if prec == 0 else 1e-5
+    assert rel(e.T, eo) < tol and rel(compact(f, nn), go) < tol
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_lbso4_diag_against_oracle(prec):
+    import mld_amd as M
+
+    rng = np.random.default_rng(4)
+    pos, cell = synth.bcc_supercell(rng, 2.855, 2, 0.1)
+    it = np.zeros(len(pos), np.int32)
+    c = M.Context(j_max=4.0, r_cut=4.7, lbso4_diag=True, precision=prec)
+    assert c.D == 35
+    e, f, nn, _ = c.bso4(pos, cell, it)
+    c.close()
+    eo, go, _ = O.bso4(pos, cell, it, 8, 4.7, diag=True)
+    tol = 1e-10 if prec == 0 else 1e-5
+    assert rel(e.T, eo) < tol and rel(compact(f, nn), go) < tol

@@ -101,3 +101,47 @@ def test_admission_budget_and_rejection():
     assert st["peak_admitted_bytes"] <= budget
     assert st["max_batch"] <= 4
     ctx.close()
+
+
+def test_admission_by_actual_pair_count():
+    """Structures compressed to 0.16 atoms / A^3 have more neighbours than the admission estimate
+    (0.12 / A^3): the batches reserve the bytes of their actual pair counts, so the peak held
+    stays within a budget of 2.5 such structures, and every result equals the direct one."""
+    from mld_amd.serve import Server
+
+    ctx = make_ctx(0)
+    rng = np.random.default_rng(23)
+    reqs = []
+    for _ in range(10):
+        (p, c), = synth.bcc_vacancy_structures(rng, 1, 54, 54, a=3.1, sigma=0.05)
+        reqs.append((p * 0.75, c * 0.75, rng.integers(0, 5, len(p)).astype(np.int32)))
+    ref = direct(ctx, reqs)
+    probe = Server(ctx, hbm_budget_bytes=1 << 40, max_batch_structs=1, max_batch_atoms=4000)
+    probe.poll(probe.submit(*reqs[0]))
+    one = probe.stats()["peak_admitted_bytes"]
+    probe.close()
+    budget = int(one * 2.5)
+    sv = Server(ctx, hbm_budget_bytes=budget, max_batch_structs=8, max_batch_atoms=4000)
+    ids = [sv.submit(p, c, t) for p, c, t in reqs]
+    out = [sv.poll(i) for i in ids]
+    st = sv.stats()
+    sv.close()
+    for (E, F, Q, _), (Er, Fr, Qr) in zip(out, ref):
+        assert E == Er and np.array_equal(F, Fr) and np.array_equal(Q, Qr)
+    assert st["count"] == 10 and st["peak_admitted_bytes"] <= budget
+    ctx.close()
+
+
+def test_kernel_model_is_not_served():
+    import mld_amd as M
+    from mld_amd import fit
+    from mld_amd.serve import Server
+
+    ctx = M.Context(j_max=4.0, r_cut=5.0)
+    rng = np.random.default_rng(3)
+    fit.kernel_po_set(ctx, rng.standard_normal((4, ctx.D)), 1.0, 0.7, 2.0)
+    fit.model_set(ctx, 3, rng.standard_normal(1 + ctx.D + 4))
+    with pytest.raises(M.MldError) as ex:
+        Server(ctx, hbm_budget_bytes=1 << 30, max_batch_structs=4, max_batch_atoms=1000)
+    assert ex.value.code == M.MLD_E_UNSUPPORTED
+    ctx.close()

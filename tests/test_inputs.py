@@ -64,3 +64,39 @@ def test_select_fit17_configurations():
         assert np.array_equal(c.pos, gold["pos"][off[s]:off[s + 1]])
         assert np.array_equal(c.cell, gold["cell"][s])
     assert {cl.cls for _, cl in cfg} == {"01", "02", "03", "04", "05", "10", "12"}
+
+
+def test_quadratic_type_and_order_keys():
+    n = dict(inputs.DEFAULTS)
+    assert inputs.quadratic_type(n) == inputs.QUADRATIC_QNML          # both unset: default 1
+    assert inputs.quadratic_type({**n, "snap_type_quadratic": 2}) == 2
+    assert inputs.quadratic_type({**n, "mld_type_quadratic": 2, "snap_type_quadratic": 2}) == 2
+    assert inputs.fit_order({**n, "mld_order": 2}) == 2 and inputs.fit_order({**n, "snap_order": 2}) == 2
+    import pytest
+
+    with pytest.raises(ValueError):
+        inputs.quadratic_type({**n, "mld_type_quadratic": 1, "snap_type_quadratic": 2})
+
+
+def test_model_weights_loss_types():
+    """main_train_regularization.F90:311-372 on three configurations of two classes: one without
+    forces, one with a stress (n_E = 3, n_F = 3 (2 + 4) = 18, n_S = 6)."""
+    from mld_amd import io
+
+    def cfg(nat, f, s):
+        return io.Config(pos=np.zeros((nat, 3)), cell=np.eye(3), itype=np.zeros(nat, np.int32), species=["W"],
+                         has_force=f, has_stress=s)
+
+    ca = io.DbClass(cls="01", klm="111", n_total=1, n_selected=1, use_e=True, use_f=True, use_s=False,
+                    weights=(2.0, 0.5, 1.0))
+    cb = io.DbClass(cls="02", klm="111", n_total=1, n_selected=1, use_e=True, use_f=True, use_s=False,
+                    weights=(1.0, 0.25, 1.0))
+    configs = [(cfg(2, True, False), ca), (cfg(3, False, False), ca), (cfg(4, True, True), cb)]
+    base = {**inputs.DEFAULTS, "factor_energy_error": 3.0, "factor_force_error": 5.0}
+    w1 = inputs.model_weights({**base, "type_of_loss": 1}, configs)
+    assert w1 == [(6.0, 2.5), (6.0, 2.5), (3.0, 1.25)]
+    tot = 3 + 18 + 6
+    w2 = inputs.model_weights({**base, "type_of_loss": 2}, configs)
+    assert np.allclose(w2, [(2.0 / (3 / tot) * 3, 0.5 / (18 / tot) * 5)] * 2 + [(1.0 / (3 / tot) * 3, 0.25 / (18 / tot) * 5)])
+    w3 = inputs.model_weights({**base, "type_of_loss": 3}, configs)
+    assert np.allclose(w3, [(2.0 / 2 * 3, 0.5 / 6 * 5), (2.0 / 3 * 3, 0.5 / 9 * 5), (1.0 / 4 * 3, 0.25 / 12 * 5)])

@@ -226,3 +226,75 @@ def test_kernel_po_normalisation_and_fd_gradient():
             fd = (ep[a] - em[a]) / (2 * h)
             an = fk[off[a] + a + slot, x]
             assert np.abs(fd - an).max() < 1e-6 * max(1.0, np.abs(an).max()), (j, slot, x, fd, an)
+
+
+def test_lbso4_diag_components_by_permutation_symmetry():
+    """lbso4_diag = .true. (compute_bispectrum_so4.F90:761-781) keeps (l1, l1, l) for every even l
+    up to min(jj, 2 l1): 35 components at 2j_max = 8.  Those with l >= l1 are components of the
+    default set; for l < l1 the bispectrum's permutation symmetry B_{l1 l2 l} / (l + 1) =
+    B_{l l2 l1} / (l1 + 1) gives B_{l1 l1 l} = (l + 1) / (l1 + 1) B_{l1 l l1}, a component of the
+    default set."""
+    rng = np.random.default_rng(4)
+    pos, cell = synth.bcc_supercell(rng, 2.855, 2, 0.1)
+    it = np.zeros(len(pos), np.int32)
+    es, gs, nl = O.bso4(pos, cell, it, 8, 4.7)
+    ed, gd, _ = O.bso4(pos, cell, it, 8, 4.7, diag=True, nl=nl)
+    std = [tuple(int(v) for v in x) for x in O.enumerate_bso4(8)]
+    dg = [tuple(int(v) for v in x) for x in O.enumerate_bso4(8, diag=True)]
+    assert len(dg) == 35 and all(a == b for a, b, _ in dg)
+    for k, (l1, _, l) in enumerate(dg):
+        j, f = (std.index((l1, l1, l)), 1.0) if l >= l1 else (std.index((l1, l, l1)), (l + 1) / (l1 + 1))
+        assert rel(ed[:, k], f * es[:, j]) < 1e-12
+        assert rel(gd[:, :, k], f * gs[:, :, j]) < 1e-11
+
+
+def _drop_pairs(nl, keep):
+    off, nbr, d, r = nl
+    cnt = np.array([keep[off[a]:off[a + 1]].sum() for a in range(len(off) - 1)])
+    off2 = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
+    return off2, nbr[keep].copy(), d[keep].copy(), r[keep].copy()
+
+
+def test_inner_cut_type3_limits():
+    """Pair inner cutoff (fcut_rij type 3 at r_cut_pair_in + r_cut_width_in, neighbours.F90:598-645,
+    compute_bispectrum_so4.F90:356-377): fcut = (1 - f_in) f_out.  On a perfect bcc lattice
+    (shells at 2.47 and 2.855 A) an inner cut whose taper ends below the first shell changes
+    nothing, and one at 2.7 A with width 0.1 A for unlike species removes exactly the unlike
+    first-shell neighbours: B equals the plain descriptor on the list without them."""
+    rng = np.random.default_rng(9)
+    pos, cell = synth.bcc_supercell(rng, 2.855, 3, 0.0)
+    it = (np.arange(len(pos)) % 2).astype(np.int32)
+    e0, g0, nl = O.bso4(pos, cell, it, 8, 4.7)
+    e1, g1, _ = O.bso4(pos, cell, it, 8, 4.7, rin=np.full(4, 1.5), win=0.4, nl=nl)
+    assert np.array_equal(e0, e1) and np.array_equal(g0, g1)
+    rin = np.array([0.0, 2.7, 2.7, 0.0])
+    e2, _, _ = O.bso4(pos, cell, it, 8, 4.7, rin=rin, win=0.1, nl=nl, want_grad=False)
+    off, nbr, _, r = nl
+    centre = np.repeat(np.arange(len(pos)), np.diff(off))
+    keep = ~((r < 2.7) & (it[centre] != it[nbr]))
+    assert 0 < (~keep).sum() < len(keep)
+    e3, _, _ = O.bso4(pos, cell, it, 8, 4.7, nl=_drop_pairs(nl, keep), want_grad=False)
+    assert rel(e2, e3) < 1e-13
+    assert rel(e2, e0) > 1e-3
+
+
+def test_inner_cut_type3_gradient_central_difference():
+    """Neighbours inside the taper of the inner cut (first shell 2.47 +- 0.1 A, taper 2.3-2.7 A):
+    dB/dr against central differences of B."""
+    rng = np.random.default_rng(12)
+    pos, cell = synth.bcc_supercell(rng, 2.855, 2, 0.1)
+    it = (np.arange(len(pos)) % 2).astype(np.int32)
+    rin = np.array([2.3, 2.3, 2.3, 0.0])
+    e, g, nl = O.bso4(pos, cell, it, 8, 4.7, rin=rin, win=0.4)
+    r = nl[3]
+    assert ((r > 2.3) & (r < 2.7)).sum() > 10
+    h = 1e-5
+    for a in (0, 5):
+        an = _slot_sum(g, nl, len(pos), a)
+        for x in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[a, x] += h
+            pm[a, x] -= h
+            ep, _, _ = O.bso4(pp, cell, it, 8, 4.7, rin=rin, win=0.4, want_grad=False)
+            em, _, _ = O.bso4(pm, cell, it, 8, 4.7, rin=rin, win=0.4, want_grad=False)
+            assert rel(an[:, x, :], (ep - em) / (2 * h)) < 1e-7
