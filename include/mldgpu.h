@@ -128,6 +128,20 @@ int mld_gram_begin(mld_ctx* ctx, int D);
 int mld_gram_add_rows(mld_ctx* ctx, int D, int mc, const double* Ac, const double* y, const double* w);
 int mld_gram_add_rows_dev(mld_ctx* ctx, int D, int mc, const double* Ac_dev, const double* y_dev,
                           const double* w_dev);
+/* No counterpart in the reference: mld_gram_add_rows_dev for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It allocates nothing: the caller owns the normal equations, G_dev (D x D) and h_dev (D), and
+ * the entries G_dev[i*D + j] with i <= j (the upper triangle in row-major order, the lower one
+ * in Fortran's column-major order) and h_dev accumulate A^T W A and A^T W y, the same sums as
+ * mld_gram_add_rows_dev forms (the other entries are not touched).  The scaled rows and the
+ * partial tiles live in one library-owned device array of MLD_NOALLOC_SCRATCH_MIB MiB per device,
+ * held until the process ends: mc * ceil(D / 64) * 64 doubles of rows plus their partial tiles
+ * must fit (MLD_E_ARG otherwise; about 9,500 rows at D = 1,596, 258,000 at D = 56).  Every call on
+ * a device shares that device's array, so calls that overlap on one device (two contexts, two
+ * threads) race.  Work is enqueued on the context's stream; the context needs no mld_gram_begin. */
+#define MLD_NOALLOC_SCRATCH_MIB 128
+int mld_gram_add_rows_noalloc(mld_ctx* ctx, int D, int mc, const double* Ac_dev, const double* y_dev,
+                              const double* w_dev, double* G_dev, double* h_dev);
 /* rows from the last batch descriptors: linear (order 1) or quadratic (order 2)
  * energy + force rows (mld_energy.F90:91-266, mld_forces.F90:222-500), or the rows of
  * a kernel model (order 3: [N, B, k] from the last mld_kernel_po_dev features with

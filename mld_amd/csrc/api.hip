@@ -107,6 +107,11 @@ int mld_create(const mld_params* p, mld_ctx** out)
     };
     if (hipSetDevice(p->device) != hipSuccess) { delete ctx; return MLD_E_DEVICE; }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return MLD_E_DEVICE; }
+    if (noalloc_scratch_init(ctx) != MLD_OK) {
+        const int code = MLD_E_DEVICE;
+        mld_destroy(ctx);
+        return code;
+    }
     if (is_ace) {
         // basis tables (compute_ace_nlmu_tools.F90's role; this build's enumeration) and the
         // species weights of the second chemical channel

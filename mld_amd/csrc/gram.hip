@@ -20,6 +20,7 @@
 // Split-K over row slices writes per-slice partial tiles that a reduction kernel
 // adds in slice order: the result does not depend on scheduling.
 #include <hipcub/hipcub.hpp>
+#include <mutex>
 
 #include "mld_internal.h"
 
@@ -637,10 +638,11 @@ __global__ void gemv_t_kernel(int M, int ldA, const double* __restrict__ A, cons
     part[(size_t)blockIdx.y * ldA + c] = s;
 }
 
-__global__ void gemv_reduce_kernel(int ldA, int nslice, const double* __restrict__ part, double* __restrict__ h)
+__global__ void gemv_reduce_kernel(int ldA, int nh, int nslice, const double* __restrict__ part, double* __restrict__ h)
 {
+    // h[c] += sum of the slices for c < nh (nh <= ldA: entries past nh are padding)
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ldA) return;
+    if (c >= nh) return;
     double s = 0.0;
     for (int sl = 0; sl < nslice; ++sl) s += part[(size_t)sl * ldA + c];
     h[c] += s;
@@ -668,6 +670,77 @@ struct GramScratch {
     void* rev = nullptr; size_t cap_rev = 0;
 };
 
+// Launch shape of the FP64 SYRK and GEMV over M rows of ldA columns: 128 x 128 upper tiles over
+// the columns rounded up to 128 (reads past ldA are zero) times row slices, the slice count the
+// one closest to whole waves of 512 resident workgroups (two per CU at this kernel's register
+// use) with at least 256 rows a slice and at most ns_max slices.
+struct SyrkPlan {
+    int nt2 = 0, nup2 = 0, ns = 1, rps = 0, hs = 1, hrps = 0;
+    size_t part_bytes = 0, gpart_bytes = 0;
+};
+
+static SyrkPlan plan_syrk(int M, int ldA, int ns_max, int hs_max)
+{
+    SyrkPlan pl;
+    pl.nt2 = (ldA + T2 - 1) / T2;
+    pl.nup2 = pl.nt2 * (pl.nt2 + 1) / 2;
+    const int cap = 512;
+    const int ns_hi = std::max(1, std::min(ns_max, M / 256));
+    int ns2 = 1;
+    double best = -1.0;
+    for (int ns = 1; ns <= ns_hi; ++ns) {
+        const long long wg = (long long)pl.nup2 * ns;
+        const double eff = (double)wg / (double)(((wg + cap - 1) / cap) * cap);
+        if (eff > best + 1e-9) { best = eff; ns2 = ns; }
+    }
+    int rps2 = (M + ns2 - 1) / ns2;
+    rps2 = ((rps2 + 31) / 32) * 32;
+    pl.ns = (M + rps2 - 1) / rps2;
+    pl.rps = rps2;
+    pl.part_bytes = (size_t)pl.ns * pl.nup2 * T2 * T2 * sizeof(double);
+    int hs = std::max(1, std::min(hs_max, M / 256));
+    pl.hrps = (M + hs - 1) / hs;
+    pl.hs = (M + pl.hrps - 1) / pl.hrps;
+    pl.gpart_bytes = (size_t)pl.hs * ldA * sizeof(double);
+    return pl;
+}
+
+// G (ldG x ldG row-major, upper triangle) += A^T A and h (nh entries) += A^T y for the row-major
+// rows A (M x ldA) with the partial tiles and GEMV partials in part / gpart (plan sizes).
+static void launch_syrk(hipStream_t st, const SyrkPlan& pl, int M, int ldA, const double* A, const double* y, double* part,
+                 double* gpart, double* G, int ldG, double* h, int nh)
+{
+    hipLaunchKernelGGL(syrk128_f64_kernel, dim3(pl.nup2, pl.ns), dim3(256), 0, st, M, ldA, pl.nt2, A, pl.rps, part);
+    hipLaunchKernelGGL(syrk128_reduce_kernel, dim3(pl.nup2), dim3(256), 0, st, pl.nt2, pl.ns, ldG,
+                       (const double*)part, G);
+    hipLaunchKernelGGL(gemv_t_kernel, dim3((ldA + 255) / 256, pl.hs), dim3(256), 0, st, M, ldA, A, y, pl.hrps, gpart);
+    hipLaunchKernelGGL(gemv_reduce_kernel, dim3((ldA + 255) / 256), dim3(256), 0, st, ldA, nh, pl.hs,
+                       (const double*)gpart, h);
+}
+
+// ------------------------------------------------ allocation-free accumulation (scratch array)
+// Scratch of mld_gram_add_rows_noalloc: a file-scope device array, one copy per device, alive
+// until the process ends.  Host code never uses the array's name as an address: the device
+// address of each device's copy is taken once (noalloc_scratch_init, from mld_create with that
+// device current) and kept per device ordinal.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)MLD_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int noalloc_scratch_init(mld_ctx* ctx)
+{
+    const int dev = ctx->p.device;
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return mld_fail(ctx, MLD_E_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return MLD_OK;
+    void* ptr = nullptr;
+    MLD_HIP(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = ptr;
+    return MLD_OK;
+}
+
 // Gram of row-major rows A (M x ldA, ldA multiple of 64) into ctx->d_G / d_h.
 int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, int mixed)
 {
@@ -677,59 +750,42 @@ int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, 
     GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
     int rc = MLD_OK;
     if (!mixed) {
-        // 128 x 128 tiles over the columns rounded up to 128 (reads past ldA are zero)
-        const int nt2 = (ldA + T2 - 1) / T2, nup2 = nt2 * (nt2 + 1) / 2;
-        // row slices: the workgroup count closest to whole waves of 512 resident workgroups
-        // (two per CU at this kernel's register use), at least 256 rows a slice
-        const int cap = 512;
-        const int ns_hi = std::max(1, std::min(64, M / 256));
-        int ns2 = 1;
-        double best = -1.0;
-        for (int ns = 1; ns <= ns_hi; ++ns) {
-            const long long wg = (long long)nup2 * ns;
-            const double eff = (double)wg / (double)(((wg + cap - 1) / cap) * cap);
-            if (eff > best + 1e-9) { best = eff; ns2 = ns; }
-        }
-        int rps2 = (M + ns2 - 1) / ns2;
-        rps2 = ((rps2 + 31) / 32) * 32;
-        ns2 = (M + rps2 - 1) / rps2;
-        rc = device_ensure(ctx, &gs.part, &gs.cap_part, (size_t)ns2 * nup2 * T2 * T2 * sizeof(double));
+        const SyrkPlan pl = plan_syrk(M, ldA, 64, 512);
+        rc = device_ensure(ctx, &gs.part, &gs.cap_part, pl.part_bytes);
         if (rc) return rc;
-        hipLaunchKernelGGL(syrk128_f64_kernel, dim3(nup2, ns2), dim3(256), 0, st, M, ldA, nt2, A, rps2,
-                           (double*)gs.part);
+        rc = device_ensure(ctx, &gs.gpart, &gs.cap_gpart, pl.gpart_bytes);
+        if (rc) return rc;
+        launch_syrk(st, pl, M, ldA, A, y, (double*)gs.part, (double*)gs.gpart, ctx->d_G, ldA, ctx->d_h, ldA);
         MLD_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(syrk128_reduce_kernel, dim3(nup2), dim3(256), 0, st, nt2, ns2, ldA,
-                           (const double*)gs.part, ctx->d_G);
-    } else {
-        const int ntile = ldA / TILE;
-        const int nup = ntile * (ntile + 1) / 2;
-        int nslice = std::max(1, std::min(1024, (2048 + nup - 1) / nup));
-        int rps = (M + nslice - 1) / nslice;
-        rps = ((rps + 31) / 32) * 32;
-        nslice = (M + rps - 1) / rps;
-        rc = device_ensure(ctx, &gs.part, &gs.cap_part, (size_t)nslice * nup * TILE * TILE * sizeof(double));
-        if (rc) return rc;
-        size_t n = (size_t)M * ldA;
-        rc = device_ensure(ctx, &gs.split, &gs.cap_split, 3 * n * sizeof(__bf16));
-        if (rc) return rc;
-        __bf16* h = (__bf16*)gs.split;
-        hipLaunchKernelGGL(split_bf16_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, A, h, h + n, h + 2 * n);
-        hipLaunchKernelGGL(syrk_bf16_kernel<6>, dim3(nup, nslice), dim3(256), 0, st, M, ldA, ntile, h, h + n,
-                           h + 2 * n, rps, 512, (double*)gs.part);
-        MLD_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(syrk_reduce_kernel, dim3(nup), dim3(256), 0, st, ntile, nslice, ldA,
-                           (const double*)gs.part, ctx->d_G);
+        ctx->gram_rows += M;
+        return MLD_OK;
     }
+    const int ntile = ldA / TILE;
+    const int nup = ntile * (ntile + 1) / 2;
+    int nslice = std::max(1, std::min(1024, (2048 + nup - 1) / nup));
+    int rps = (M + nslice - 1) / nslice;
+    rps = ((rps + 31) / 32) * 32;
+    nslice = (M + rps - 1) / rps;
+    rc = device_ensure(ctx, &gs.part, &gs.cap_part, (size_t)nslice * nup * TILE * TILE * sizeof(double));
+    if (rc) return rc;
+    size_t n = (size_t)M * ldA;
+    rc = device_ensure(ctx, &gs.split, &gs.cap_split, 3 * n * sizeof(__bf16));
+    if (rc) return rc;
+    __bf16* hb = (__bf16*)gs.split;
+    hipLaunchKernelGGL(split_bf16_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, A, hb, hb + n, hb + 2 * n);
+    hipLaunchKernelGGL(syrk_bf16_kernel<6>, dim3(nup, nslice), dim3(256), 0, st, M, ldA, ntile, hb, hb + n,
+                       hb + 2 * n, rps, 512, (double*)gs.part);
+    MLD_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(syrk_reduce_kernel, dim3(nup), dim3(256), 0, st, ntile, nslice, ldA,
+                       (const double*)gs.part, ctx->d_G);
     MLD_HIP(ctx, hipGetLastError());
     // h
-    int hs = std::max(1, std::min(512, M / 256));
-    int hrps = (M + hs - 1) / hs;
-    hs = (M + hrps - 1) / hrps;
-    rc = device_ensure(ctx, &gs.gpart, &gs.cap_gpart, (size_t)hs * ldA * sizeof(double));
+    const SyrkPlan pl = plan_syrk(M, ldA, 64, 512);
+    rc = device_ensure(ctx, &gs.gpart, &gs.cap_gpart, pl.gpart_bytes);
     if (rc) return rc;
-    hipLaunchKernelGGL(gemv_t_kernel, dim3((ldA + 255) / 256, hs), dim3(256), 0, st, M, ldA, A, y, hrps,
+    hipLaunchKernelGGL(gemv_t_kernel, dim3((ldA + 255) / 256, pl.hs), dim3(256), 0, st, M, ldA, A, y, pl.hrps,
                        (double*)gs.gpart);
-    hipLaunchKernelGGL(gemv_reduce_kernel, dim3((ldA + 255) / 256), dim3(256), 0, st, ldA, hs,
+    hipLaunchKernelGGL(gemv_reduce_kernel, dim3((ldA + 255) / 256), dim3(256), 0, st, ldA, ldA, pl.hs,
                        (const double*)gs.gpart, ctx->d_h);
     MLD_HIP(ctx, hipGetLastError());
     ctx->gram_rows += M;
@@ -899,6 +955,40 @@ int mld_gram_add_rows_dev(mld_ctx* ctx, int D, int mc, const double* Ac_dev, con
                        (double*)gs.rows, (double*)gs.y);
     MLD_HIP(ctx, hipGetLastError());
     return gram_accumulate_rows(ctx, mc, (const double*)gs.rows, (const double*)gs.y, 0);
+}
+
+int mld_gram_add_rows_noalloc(mld_ctx* ctx, int D, int mc, const double* Ac_dev, const double* y_dev,
+                              const double* w_dev, double* G_dev, double* h_dev)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (D <= 0 || mc < 0 || !G_dev || !h_dev || (mc > 0 && (!Ac_dev || !y_dev || !w_dev)))
+        return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_rows_noalloc: D must be positive, G, h and the rows non-null");
+    if (mc == 0) return MLD_OK;
+    const int dev = ctx->p.device;
+    unsigned char* base = (dev >= 0 && dev < NOALLOC_MAX_DEVICES) ? (unsigned char*)g_scratch_ptr[dev] : nullptr;
+    if (!base) return mld_fail(ctx, MLD_E_STATE, "mld_gram_add_rows_noalloc: scratch of the device not initialised");
+    // the same row layout and launch shape as mld_gram_add_rows_dev (identical sums), with fewer
+    // row slices when the partial tiles would not fit beside the rows
+    const int ld = ((D + TILE - 1) / TILE) * TILE;
+    auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
+    const size_t b_rows = r256((size_t)mc * ld * sizeof(double)), b_y = r256((size_t)mc * sizeof(double));
+    SyrkPlan pl = plan_syrk(mc, ld, 64, 512);
+    auto need = [&](const SyrkPlan& q) { return b_rows + b_y + r256(q.part_bytes) + r256(q.gpart_bytes); };
+    for (int ns_max = pl.ns - 1; need(pl) > NOALLOC_SCRATCH_BYTES && ns_max >= 1; --ns_max)
+        pl = plan_syrk(mc, ld, ns_max, 1);
+    if (need(pl) > NOALLOC_SCRATCH_BYTES)
+        return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_rows_noalloc: " + std::to_string(mc) + " rows of " +
+                                            std::to_string(D) + " columns need " + std::to_string(need(pl) >> 20) +
+                                            " MiB of scratch, more than its " + std::to_string(MLD_NOALLOC_SCRATCH_MIB) +
+                                            " MiB");
+    double* A = (double*)base;
+    double* y = (double*)(base + b_rows);
+    double* part = (double*)(base + b_rows + b_y);
+    double* gpart = (double*)(base + b_rows + b_y + r256(pl.part_bytes));
+    hipLaunchKernelGGL(scale_rows_kernel, dim3(mc), dim3(64), 0, ctx->stream, mc, D, ld, Ac_dev, y_dev, w_dev, A, y);
+    launch_syrk(ctx->stream, pl, mc, ld, A, y, part, gpart, G_dev, D, h_dev, D);
+    MLD_HIP(ctx, hipGetLastError());
+    return MLD_OK;
 }
 
 int mld_gram_add_rows(mld_ctx* ctx, int D, int mc, const double* Ac, const double* y, const double* w)

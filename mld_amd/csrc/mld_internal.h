@@ -263,6 +263,8 @@ bool bso4_tables_fit(int jj, int ncg_level, int ncg, int wsize);
 int launch_to_reference_layout(mld_ctx* ctx, int nat, int nn_max, const void* grad, double* force_dev,
                                int* kind_dev, int* nneigh_dev);
 int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, int mixed);
+// device address of the scratch array of mld_gram_add_rows_noalloc on the context's device (once per device)
+int noalloc_scratch_init(mld_ctx* ctx);
 int keep_rows_copy(mld_ctx* ctx, int M, const double* A, const double* y);
 void gram_scratch_free(mld_ctx* ctx);
 int device_ensure(mld_ctx* ctx, void** p, size_t* cap, size_t bytes);
