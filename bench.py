@@ -31,16 +31,19 @@ Workloads (``--workload``):
       Reported: p50 / p99 latency at that load, and the throughput of the same
       requests submitted all at once.
 
-Every fit is timed in three forms on the same inputs (already resident in HBM):
-  fp64     FP64 descriptors, FP64 MFMA Gram, FP64 Cholesky (+ refinement)
-  mixed    FP32 descriptors, FP64 MFMA Gram, FP64 Cholesky (+ refinement)
-  bf16     FP32 descriptors, split-BF16 MFMA Gram (FP32 accumulation), FP64 Cholesky
-           of the shifted Gram preconditioning FP64 CG on the kept FP64 rows
-``value`` is the ``mixed`` form; the errors of ``mixed`` and ``bf16`` against
-``fp64`` (coefficients, energy / force RMSE of the fitted model) are reported.
+Every fit is timed in four forms on the same inputs (already resident in HBM):
+  fp64             FP64 descriptors, FP64 MFMA Gram, FP64 Cholesky + refinement  (``value``)
+  f64desc_i8gram   FP64 descriptors, int8 Ozaki Gram (7 digits, FP64-level), FP64 Cholesky
+  f32desc_f64gram  FP32 descriptors, FP64 MFMA Gram, FP64 Cholesky
+  mixed            FP32 descriptors, int8 Ozaki Gram, FP64 Cholesky  (the mixed form)
+``value`` is the fp64 form, the precision of the reference; every other form carries its
+speed-up and its errors against fp64 (coefficients, energy / force RMSE of the fitted model),
+and a second FP64 fit with another batching gives the coefficient spread of FP64 itself.
+The default run also measures the serving loop (``serve``: p50 / p99 at 200 requests/s and the
+saturated rate) and times the oracle on every host core as the CPU baseline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W] [--scale f]
-                       [--no-cpu-baseline]
+                       [--no-cpu-baseline] [--no-serve]
 """
 from __future__ import annotations
 
@@ -75,6 +78,13 @@ PEAK_FP32, PEAK_FP64 = 157.3, 78.6   # TFLOP/s, MI355X dense
 # (the excess is the per-pair geometry record, read by the density and gradient phases).
 TRAFFIC_BSO4_F32_FIT_BATCH = (315683 * 2.0 + 2037535 * 1.0) * 1024
 ALGO_BYTES_BSO4_F32_FIT_BATCH = 2177392328.0
+# The same for bso4_kernel<double,8> (the dominant kernel of the fp64 form), and the average
+# syrk128_f64_kernel launch of that workload, from the committed profiles named below.
+PROFILE_BSO4_F64 = "profiles/fit_quadratic_kernel_trace_and_hbm_counters.md"
+PROFILE_FIT = "profiles/fit_quadratic_kernel_trace_and_hbm_counters.md"
+TRAFFIC_BSO4_F64_FIT_BATCH = None
+ALGO_BYTES_BSO4_F64_FIT_BATCH = None
+SYRK_F64_MS_PER_BATCH = None
 
 
 def algorithmic_flops(natoms: int, npairs: int) -> float:
@@ -201,20 +211,21 @@ def fit_workload(kind: str, scale: float):
                               "w_e=1, w_f=0.1, lambda 1e-8")
 
 
-FORMS = {"fp64": (0, False), "mixed": (1, False), "bf16": (1, True)}
+# form: (descriptor precision, Gram in int8 Ozaki slices)
+FORMS = {"fp64": (0, False), "mixed": (1, True), "f32desc_f64gram": (1, False), "f64desc_i8gram": (0, True)}
 
 
-def run_fit(structs, cfg, form: str, steps: int, warmup: int):
+def run_fit(structs, cfg, form: str, steps: int, warmup: int, chunk=None):
     """Time the whole fit `steps` times; returns coefficients, solve diagnostics, timings and
     the fitted model's RMSE against the targets of the first two batches."""
     import torch
     import mld_amd as M
     from mld_amd import fit, synth
 
-    prec, bf16 = FORMS[form]
+    prec, i8 = FORMS[form]
     ctx = M.Context(j_max=4.0, r_cut=4.7, precision=prec)
     stream = torch.cuda.ExternalStream(ctx.stream())
-    chunk = cfg["chunk"]
+    chunk = chunk or cfg["chunk"]
     batches = []
     for s0 in range(0, len(structs), chunk):
         part = structs[s0:s0 + chunk]
@@ -231,7 +242,7 @@ def run_fit(structs, cfg, form: str, steps: int, warmup: int):
         ctx.synchronize()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        gf = fit.GramFit(ctx, order=cfg["order"], mixed=bf16, keep_rows=bf16)
+        gf = fit.GramFit(ctx, order=cfg["order"], mixed=i8)
         events = []
         for b, E, F in batches:
             b.neighbours()
@@ -273,10 +284,12 @@ def run_fit(structs, cfg, form: str, steps: int, warmup: int):
     return out
 
 
-def cpu_fit(structs, cfg, budget_s: float = 15.0):
-    """Oracle path on one core: FP64 descriptors + gradients, rows (reference packing,
-    mapped to the unique quadratic columns) and the Gram update of the workload's first
-    structures."""
+def _cpu_fit_worker(task):
+    """One host process of the CPU baseline: the oracle (C FP64 restatement of the reference's
+    loops) over its share of the structures until the wall-clock budget ends: descriptors +
+    gradients, rows (reference packing mapped to the unique quadratic columns) and its own Gram
+    update, as one MPI rank of the reference does (module_online_fit.F90:992-1009)."""
+    structs, cfg, budget_s = task
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as O
     from mld_amd import synth
@@ -302,16 +315,48 @@ def cpu_fit(structs, cfg, budget_s: float = 15.0):
         n += 1
         if time.perf_counter() - t0 > budget_s:
             break
+    return atoms, n, time.perf_counter() - t0
+
+
+def host_cores() -> int:
+    """Cores this process may use (its affinity mask), capped by OMP_NUM_THREADS when set."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(cap))) if cap and cap.isdigit() else max(1, n)
+
+
+def cpu_fit(structs, cfg, budget_s: float = 15.0):
+    """The oracle on P = host_cores() worker processes over disjoint structures (the reference's
+    subworlds with one process per file), each running for about budget_s; throughput = all
+    atoms done / the pool's wall time."""
+    import multiprocessing as mp
+
+    P = host_cores()
+    shares = [structs[k::P] for k in range(P)]
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(P) as pool:
+        res = pool.map(_cpu_fit_worker, [(sh, cfg, budget_s) for sh in shares])
     dt = time.perf_counter() - t0
-    return dict(value=atoms / dt, unit="atom-envs/s", cores=1, kind="port",
-                sample=f"first {n} structures ({atoms} atom-envs) of the same workload: FP64 B + dB/dr, "
-                       f"rows and Gram update ({nc} columns), {dt:.1f} s; solve excluded")
+    atoms = sum(r[0] for r in res)
+    n = sum(r[1] for r in res)
+    busy = max(r[2] for r in res)
+    return dict(value=atoms / busy, unit="atom-envs/s", cores=P, kind="port",
+                sample=f"{n} structures ({atoms} atom-envs) of the same workload over {P} processes, each "
+                       f"~{budget_s:.0f} s of FP64 B + dB/dr, rows and its own Gram update "
+                       f"({'1596' if cfg['order'] == 2 else '56'} columns); rate over the slowest process's "
+                       f"{busy:.1f} s (pool wall {dt:.1f} s incl. start-up); solve excluded")
 
 
 def main_fit(args):
     structs, cfg = fit_workload(args.workload, args.scale)
-    res = {f: run_fit(structs, cfg, f, args.steps, args.warmup) for f in ("fp64", "mixed", "bf16")}
-    r64, rmx = res["fp64"], res["mixed"]
+    res = {f: run_fit(structs, cfg, f, args.steps, args.warmup) for f in FORMS}
+    # FP64's own coefficient spread: the same fit with another batching (another summation order)
+    alt = run_fit(structs, cfg, "fp64", 1, 0, chunk=max(1, cfg["chunk"] * 3 // 5))
+    r64 = res["fp64"]
     natoms, npairs = r64["natoms"], r64["npairs"]
     flops = algorithmic_flops(natoms, npairs)
 
@@ -329,30 +374,48 @@ def main_fit(args):
                 "bso4_ms_per_step": r["bso4_ms_per_fit"], "bso4_tflops": ach, "bso4_frac": ach / peak,
                 "solve": r["solve"], "rmse": r["rmse"]}
 
-    ach = flops / (rmx["bso4_ms_per_fit"] * 1e-3) / 1e12
+    ach = flops / (r64["bso4_ms_per_fit"] * 1e-3) / 1e12
     quad = args.workload == "fit-quadratic"
+    M, ncol = r64["rows"], len(r64["w"])
+    ld = ((ncol + 63) // 64) * 64
+    syrk_flops = float(M) * ld * (ld + 1)      # upper triangle of A^T A over the padded columns, 2 flops per product
     out = {
-        "metric": METRIC, "value": natoms / rmx["s_per_fit"], "unit": "atom-envs/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * rmx["s_per_fit"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 desc, f64 Gram/solve",
+        "metric": METRIC, "value": natoms / r64["s_per_fit"], "unit": "atom-envs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * r64["s_per_fit"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": f"synthetic (seed {SEED}), Morse targets",
         "config": {"workload": f"{args.workload}: {cfg['desc']}", "natoms": natoms, "npairs": npairs,
-                   "rows": r64["rows"], "columns": len(r64["w"]), "batches": r64["nbatch"],
-                   "parallelism": "single GPU"},
-        "fp64": block(r64, PEAK_FP64),
-        "mixed": dict(block(rmx, PEAK_FP32), err_vs_fp64=err(rmx)),
-        "bf16": dict(block(res["bf16"], PEAK_FP32), err_vs_fp64=err(res["bf16"])),
-        "mixed_speedup_vs_fp64": r64["s_per_fit"] / rmx["s_per_fit"],
-        "bf16_speedup_vs_fp64": r64["s_per_fit"] / res["bf16"]["s_per_fit"],
-        "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
-                     "traffic": TRAFFIC_BSO4_F32_FIT_BATCH if quad else None, "kernel": "bso4_kernel<float,8>",
-                     "traffic_note": ("HBM bytes of the first 500-structure launch (profiled), "
-                                      f"{TRAFFIC_BSO4_F32_FIT_BATCH / ALGO_BYTES_BSO4_F32_FIT_BATCH:.3f} x its "
-                                      "algorithmic bytes") if quad else "not profiled for this workload",
-                     "kernel_ms_per_launch": rmx["bso4_ms_per_fit"] / rmx["nbatch"],
+                   "rows": M, "columns": ncol, "batches": r64["nbatch"], "parallelism": "single GPU"},
+        "fp64": dict(block(r64, PEAK_FP64), fp64_coef_spread={"coef_max_rel": err(alt)["coef_max_rel"],
+                                                              "coef_norm_rel": err(alt)["coef_norm_rel"],
+                                                              "note": "second FP64 fit with batches of "
+                                                                      f"{max(1, cfg['chunk'] * 3 // 5)} structures"}),
+        "f64desc_i8gram": dict(block(res["f64desc_i8gram"], PEAK_FP64), err_vs_fp64=err(res["f64desc_i8gram"]),
+                               speedup_vs_fp64=r64["s_per_fit"] / res["f64desc_i8gram"]["s_per_fit"]),
+        "f32desc_f64gram": dict(block(res["f32desc_f64gram"], PEAK_FP32), err_vs_fp64=err(res["f32desc_f64gram"]),
+                                speedup_vs_fp64=r64["s_per_fit"] / res["f32desc_f64gram"]["s_per_fit"]),
+        "mixed": dict(block(res["mixed"], PEAK_FP32), err_vs_fp64=err(res["mixed"]),
+                      speedup_vs_fp64=r64["s_per_fit"] / res["mixed"]["s_per_fit"]),
+        "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach / PEAK_FP64,
+                     "traffic": TRAFFIC_BSO4_F64_FIT_BATCH if quad else None, "kernel": "bso4_kernel<double,8>",
+                     "traffic_note": ("HBM bytes of the first 500-structure launch, "
+                                      f"{TRAFFIC_BSO4_F64_FIT_BATCH / ALGO_BYTES_BSO4_F64_FIT_BATCH:.3f} x its "
+                                      "algorithmic bytes (" + PROFILE_BSO4_F64 + ")")
+                     if quad and TRAFFIC_BSO4_F64_FIT_BATCH else "not profiled at this commit",
+                     "kernel_ms_per_launch": r64["bso4_ms_per_fit"] / r64["nbatch"],
                      "algorithmic_flops_per_step": flops,
-                     "algorithmic_bytes_per_step": algorithmic_bytes(natoms, npairs, 55, 4)},
+                     "algorithmic_bytes_per_step": algorithmic_bytes(natoms, npairs, 55, 8)},
+        "roofline_syrk": {"bound": "mfma", "kernel": "syrk128_f64_kernel", "peak": PEAK_FP64, "unit": "TFLOP/s",
+                          "algorithmic_flops_per_step": syrk_flops,
+                          "note": "per-launch time from the kernel trace of the same workload (" + PROFILE_FIT + ")"},
     }
+    if quad and SYRK_F64_MS_PER_BATCH:
+        per_launch = syrk_flops / r64["nbatch"]
+        out["roofline_syrk"].update(achieved=per_launch / (SYRK_F64_MS_PER_BATCH * 1e-3) / 1e12,
+                                    kernel_ms_per_launch=SYRK_F64_MS_PER_BATCH,
+                                    frac=per_launch / (SYRK_F64_MS_PER_BATCH * 1e-3) / 1e12 / PEAK_FP64)
+    if quad and not args.no_serve:
+        out["serve"] = serve_measurement(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_fit(structs, cfg)
     print(json.dumps(out), flush=True)
@@ -487,7 +550,9 @@ def hea_request_pool(rng, n: int):
     return out
 
 
-def main_serve(args):
+def serve_measurement(args):
+    """p50 / p99 latency at args.rate Poisson arrivals for args.duration seconds, and the saturated
+    rate of the same requests submitted at once (see the serve workload)."""
     import threading
 
     import torch
@@ -571,21 +636,28 @@ def main_serve(args):
     sv.close()
     ctx.close()
     natoms = sum(len(p) for p, _, _ in pool)
+    return {"config": f"{nreq} requests, Poisson {args.rate:g}/s, 5-species bcc HEA a=3.1 A, N lognormal median "
+                      f"128 clipped 8..2000, E + F + Mahalanobis d2, ACE D={ctx_D} r_cut=5.0 A FP32, HBM budget "
+                      f"{args.hbm_budget_gib:g} GiB",
+            "natoms": natoms, "t_sat": t_sat,
+            "at_load": {"rate_req_s": args.rate, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                        "mean_ms": st["mean_ms"], "max_ms": st["max_ms"], "mean_batch": st["mean_batch"],
+                        "served_req_s": nreq / t_load},
+            "saturated": {"req_s": nreq / t_sat, "atom_envs_s": natoms / t_sat, "p50_ms": st_sat["p50_ms"],
+                          "p99_ms": st_sat["p99_ms"], "mean_batch": st_sat["mean_batch"],
+                          "peak_admitted_bytes": st_sat["peak_admitted_bytes"]}}
+
+
+def main_serve(args):
+    r = serve_measurement(args)
+    natoms, t_sat = r["natoms"], r["t_sat"]
     out = {
         "metric": METRIC, "value": natoms / t_sat, "unit": "atom-envs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": 1e3 * t_sat, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 desc, f64 prediction", "data": f"synthetic (seed {SEED})",
-        "config": {"workload": f"serve: {nreq} requests, Poisson {args.rate:g}/s, 5-species bcc HEA a=3.1 A, "
-                               "N lognormal median 128 clipped 8..2000, E + F + Mahalanobis d2, "
-                               f"ACE expTcheb nu<=4 n_max 8/4/2/1 l_max 0/4/2/1 two channels D={ctx_D} r_cut=5.0 A FP32",
-                   "natoms": natoms, "hbm_budget_gib": args.hbm_budget_gib, "max_batch_structs": 64,
-                   "parallelism": "single GPU"},
-        "at_load": {"rate_req_s": args.rate, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-                    "mean_ms": st["mean_ms"], "max_ms": st["max_ms"], "mean_batch": st["mean_batch"],
-                    "served_req_s": nreq / t_load},
-        "saturated": {"req_s": nreq / t_sat, "atom_envs_s": natoms / t_sat, "p50_ms": st_sat["p50_ms"],
-                      "p99_ms": st_sat["p99_ms"], "mean_batch": st_sat["mean_batch"],
-                      "peak_admitted_bytes": st_sat["peak_admitted_bytes"]},
+        "config": {"workload": "serve: " + r["config"], "natoms": natoms, "hbm_budget_gib": args.hbm_budget_gib,
+                   "max_batch_structs": 64, "parallelism": "single GPU"},
+        "at_load": r["at_load"], "saturated": r["saturated"],
     }
     print(json.dumps(out), flush=True)
 
@@ -602,6 +674,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the fit workload's structures")
     ap.add_argument("--nstruct", type=int, default=2000, help="structures of the desc workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="fit workloads: skip the serving measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device 3 times (reference kernel for HBM counters)")
     args = ap.parse_args()

@@ -147,8 +147,10 @@ int mld_gram_add_rows_noalloc(mld_ctx* ctx, int D, int mc, const double* Ac_dev,
  * a kernel model (order 3: [N, B, k] from the last mld_kernel_po_dev features with
  * gradients, kernel_pack_*_descriptor, mld_energy.F90:122-144, mld_forces.F90:268-320), with
  * per-structure targets E (nstruct, host) and forces F_dev (3*natoms, device),
- * weights w_e, w_f.  mixed = 1 runs the Gram in split-BF16 MFMA with FP32
- * accumulation (FP64 across tiles), 0 in FP64. */
+ * weights w_e, w_f.  mixed = 0 forms the Gram with FP64 MFMA; mixed = 1 with int8 MFMA on an
+ * error-free split of every FP64 row element into 7 int8 digits of 7 bits below a power of two
+ * per (row slice, column) (Ozaki scheme): each digit product is summed exactly in int32 and the
+ * 28 digit pairs (i, j) with i + j <= 8 are folded in FP64, so the Gram keeps FP64-level accuracy. */
 int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F_dev, double w_e,
                        double w_f, int mixed);
 /* A batch of configurations from host buffers, end to end: neighbours, descriptors with
@@ -175,8 +177,7 @@ int mld_gram_set_keep_rows(mld_ctx* ctx, int keep);
 /* (G + lambda I) w = h (lambda/2 on off-diagonal quadratic columns) by FP64 Cholesky.
  * MLD_SOLVE_CHOL: one solve with the factor.  MLD_SOLVE_CHOL_IR: the factor then
  * preconditions FP64 conjugate-gradient refinement whose operator comes from the kept
- * FP64 rows (or the FP64 Gram); a Gram accumulated with mixed = 1 is factorised with a
- * diagonal shift of 1e-6 of its diagonal (raised tenfold while a pivot fails).
+ * FP64 rows (or the Gram itself).
  * MLD_SOLVE_SVD: the minimum-norm solution of serial_lsystem_by_svd (stats_tools.F90:1153-1213,
  * dgelsd with rcond = machine epsilon): eigen-decomposition of the symmetric system by
  * cyclic Jacobi on the device, eigenvalues below eps * max |eigenvalue| dropped; with

@@ -27,8 +27,6 @@
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int TILE = 64;
 
@@ -325,92 +323,38 @@ __global__ void scale_rows_kernel(int M, int Dc, int ldA, const double* __restri
     if (threadIdx.x == 0) y[r] = yin[r] * sw;
 }
 
-// ---------------------------------------------------------------- FP64 SYRK
-// grid.x = upper tiles (ti <= tj), grid.y = row slices; 256 threads; wave (wi, wj)
-// owns a 32x32 quarter = 2x2 MFMA 16x16 tiles.
-__global__ void __launch_bounds__(256) syrk_f64_kernel(int M, int ldA, int ntile, const double* __restrict__ A,
-                                                       int rows_per_slice, double* __restrict__ part)
+// ---------------------------------------------------------------- XCD-aware SYRK grids
+// Workgroups go to the chip's 8 XCDs round-robin by linear id, and each XCD has its own L2.  A
+// SYRK grid is one-dimensional, 8 * ceil(ns / 8) * nup workgroups: XCD x runs the slices x, x + 8,
+// ..., one slice's nup upper tiles in a row, so the tiles that read the same row panels run on
+// one XCD at about the same time and share its L2 (the slice's rows come from HBM about once).
+// Workgroups past the last slice return at once.
+constexpr int NXCD = 8;
+__host__ __device__ inline int xcd_grid(int nup, int ns) { return NXCD * nup * ((ns + NXCD - 1) / NXCD); }
+__device__ inline void xcd_tile_slice(int nup, int& tile, int& slice)
 {
-    int t = blockIdx.x, ti = 0;
-    while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
-    const int tj = ti + t;
-    const int r0 = blockIdx.y * rows_per_slice;
-    const int r1 = min(M, r0 + rows_per_slice);
-    constexpr int KB = 16;
-    __shared__ double Ai[KB][TILE + 1];
-    __shared__ double Aj[KB][TILE + 1];
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int wi = w >> 1, wj = w & 1;
-    d4 acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = d4{0, 0, 0, 0};
-    // register prefetch: the next KB rows are loaded while the current ones are multiplied
-    constexpr int PER = KB * TILE / 256;  // 4 elements per thread and panel
-    double pi_[PER], pj_[PER];
-    auto fetch = [&](int rb) {
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int q = threadIdx.x + 256 * u;
-            const int rr = q / TILE, cc = q - rr * TILE;
-            const int r = rb + rr;
-            pi_[u] = (r < r1) ? A[(size_t)r * ldA + ti * TILE + cc] : 0.0;
-            pj_[u] = (r < r1) ? A[(size_t)r * ldA + tj * TILE + cc] : 0.0;
-        }
-    };
-    if (r0 < r1) fetch(r0);
-    for (int rb = r0; rb < r1; rb += KB) {
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int q = threadIdx.x + 256 * u;
-            const int rr = q / TILE, cc = q - rr * TILE;
-            Ai[rr][cc] = pi_[u];
-            Aj[rr][cc] = pj_[u];
-        }
-        __syncthreads();
-        if (rb + KB < r1) fetch(rb + KB);
-#pragma unroll
-        for (int k0 = 0; k0 < KB; k0 += 4) {
-            double af[2], bf[2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a) af[a] = Ai[k0 + (lane >> 4)][wi * 32 + a * 16 + (lane & 15)];
-#pragma unroll
-            for (int b = 0; b < 2; ++b) bf[b] = Aj[k0 + (lane >> 4)][wj * 32 + b * 16 + (lane & 15)];
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
-        }
-    }
-    // f64 16x16x4 output: register v of lane l holds C[l/16 + 4v][l%16]
-    double* P = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * TILE * TILE;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                int i = wi * 32 + a * 16 + (lane >> 4) + 4 * v;
-                int j = wj * 32 + b * 16 + (lane & 15);
-                P[i * TILE + j] = acc[a][b][v];
-            }
+    const int b = blockIdx.x, x = b % NXCD, k = b / NXCD;
+    slice = x + NXCD * (k / nup);
+    tile = k % nup;
 }
 
 // ---------------------------------------------------------------- FP64 SYRK, 128 x 128 tiles
-// grid.x = upper 128-tiles (ti <= tj) over ceil(ldA / 128) tiles, grid.y = row slices; 256
-// threads; wave (wi, wj) owns a 64 x 64 quarter = 4 x 4 MFMA 16 x 16 tiles.  Columns past ldA
-// read as zero.  Twice the work of a 64 x 64 tile per byte staged: the panel traffic from L2
-// falls to half, which the 64 x 64 kernel was bound by.
+// One workgroup per (upper 128-tile (ti <= tj) over ceil(ldA / 128) tiles, row slice), XCD-aware
+// 1-D grid (xcd_grid); 256 threads; wave (wi, wj) owns a 64 x 64 quarter = 4 x 4 MFMA 16 x 16
+// tiles.  Columns past ldA read as zero.  Twice the work of a 64 x 64 tile per byte staged: the
+// panel traffic from L2 falls to half, which the 64 x 64 kernel was bound by.
 constexpr int T2 = 128;
-__global__ void __launch_bounds__(256) syrk128_f64_kernel(int M, int ldA, int ntile, const double* __restrict__ A,
-                                                          int rows_per_slice, double* __restrict__ part)
+__global__ void __launch_bounds__(256) syrk128_f64_kernel(int M, int ldA, int ntile, int nup, int nslice,
+                                                          const double* __restrict__ A, int rows_per_slice,
+                                                          double* __restrict__ part)
 {
-    int t = blockIdx.x, ti = 0;
+    int tile, slice;
+    xcd_tile_slice(nup, tile, slice);
+    if (slice >= nslice) return;
+    int t = tile, ti = 0;
     while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
     const int tj = ti + t;
-    const int r0 = blockIdx.y * rows_per_slice;
+    const int r0 = slice * rows_per_slice;
     const int r1 = min(M, r0 + rows_per_slice);
     constexpr int KB = 16;
     __shared__ double Ai[KB][T2 + 1];
@@ -468,7 +412,7 @@ __global__ void __launch_bounds__(256) syrk128_f64_kernel(int M, int ldA, int nt
     }
     if (!live) return;
     // f64 16x16x4 output: register v of lane l holds C[l/16 + 4v][l%16]
-    double* P = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * T2 * T2;
+    double* P = part + ((size_t)slice * nup + tile) * T2 * T2;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -483,6 +427,8 @@ __global__ void __launch_bounds__(256) syrk128_f64_kernel(int M, int ldA, int nt
 
 // G (ldG x ldG, row-major) += sum over slices of the 128-tile partials, upper-triangle entries
 // inside ldG (the strictly lower part of G stays as it is; every reader takes the upper one)
+// grid.x = upper tiles, grid.y = RED_SPLIT parts of a tile's entries (sums in slice order)
+constexpr int RED_SPLIT = 16;
 __global__ void syrk128_reduce_kernel(int ntile, int nslice, int ldG, const double* __restrict__ part,
                                       double* __restrict__ G)
 {
@@ -490,7 +436,8 @@ __global__ void syrk128_reduce_kernel(int ntile, int nslice, int ldG, const doub
     while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
     const int tj = ti + t;
     const int ntiles_up = gridDim.x;
-    for (int q = threadIdx.x; q < T2 * T2; q += blockDim.x) {
+    constexpr int QN = T2 * T2 / RED_SPLIT;
+    for (int q = blockIdx.y * QN + threadIdx.x; q < (blockIdx.y + 1) * QN; q += blockDim.x) {
         const int i = q / T2, j = q - i * T2;
         const int gi = ti * T2 + i, gj = tj * T2 + j;
         if (gi >= ldG || gj >= ldG || gi > gj) continue;
@@ -500,130 +447,234 @@ __global__ void syrk128_reduce_kernel(int ntile, int nslice, int ldG, const doub
     }
 }
 
-// ---------------------------------------------------------------- mixed SYRK
-__global__ void split_bf16_kernel(size_t n, const double* __restrict__ A, __bf16* __restrict__ hi,
-                                  __bf16* __restrict__ mid, __bf16* __restrict__ lo)
+// ---------------------------------------------------------------- mixed SYRK: int8 Ozaki slices
+// The FP64 rows are split without error into OZ_S int8 digits per element, relative to a power
+// of two per (row slice, column): a = 2^e (sum_i d_i 2^(-7 i) + r), with 2^e > 2 max |a| so that
+// the digits, rounded to nearest (d_i = rint(128 x), x <- 128 x - d_i), satisfy |d_i| <= 64 and
+// have no common sign: the products that are left out then have no bias.  Then
+//   G_pq = 2^(e_p + e_q) sum_L 2^(-7 L) sum_{i + j = L} sum_r d_i(r, p) d_j(r, q)
+// over the levels L = 2 .. OZ_LMAX (the 34 digit pairs with i + j <= 9): every inner sum is an
+// int8 x int8 product summed in int32 by v_mfma_i32_16x16x64_i8, exact for slices of up to
+// 2^31 / (7 * 64^2) = 74,898 rows (OZ_MAX_SLICE_ROWS here); the levels are folded into FP64 once per
+// slice.  The Gram then carries FP64-level error (about 5e-15 of sqrt(G_pp G_qq)) at five times
+// the FP64 MFMA rate per product.
+constexpr int OZ_S = 7;           // digits per element
+constexpr int OZ_LMAX = 9;        // highest level i + j summed
+constexpr int OZ_KB = 64;         // rows per stage (the K of one i8 MFMA)
+constexpr int OZ_CS = OZ_KB + 16; // LDS bytes per staged column (padding: conflict-free 16-byte reads)
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Largest |a| per (row slice, column), as the bit pattern of a non-negative double (ordered like
+// the value), by atomic max over blocks of 1024 rows; cmax must be zero on entry.
+__global__ void __launch_bounds__(256) oz_colmax_kernel(int M, int ldA, int rps, const double* __restrict__ A,
+                                                        unsigned long long* __restrict__ cmax)
 {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float a = (float)A[i];
-    __bf16 h = (__bf16)a;
-    float r1 = a - (float)h;
-    __bf16 m = (__bf16)r1;
-    float r2 = r1 - (float)m;
-    hi[i] = h;
-    mid[i] = m;
-    lo[i] = (__bf16)r2;
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int r0 = blockIdx.y * 1024, r1 = min(M, r0 + 1024);
+    __shared__ unsigned long long red[4][64];
+    unsigned long long m = 0;
+    if (c < ldA)
+        for (int r = r0 + (threadIdx.x >> 6); r < r1; r += 4) {
+            const double v = fabs(A[(size_t)r * ldA + c]);
+            const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+            m = b > m ? b : m;
+        }
+    red[threadIdx.x >> 6][threadIdx.x & 63] = m;
+    __syncthreads();
+    if (threadIdx.x < 64 && c < ldA) {
+        unsigned long long x = red[0][threadIdx.x];
+        for (int k = 1; k < 4; ++k) x = red[k][threadIdx.x] > x ? red[k][threadIdx.x] : x;
+        // rps is a multiple of 1024, so the block's rows lie in one slice
+        if (x) atomicMax(&cmax[(size_t)(r0 / rps) * ldA + c], x);
+    }
 }
 
-template <int NT>
-__global__ void __launch_bounds__(256) syrk_bf16_kernel(int M, int ldA, int ntile, const __bf16* __restrict__ Ah,
-                                                        const __bf16* __restrict__ Am, const __bf16* __restrict__ Al,
-                                                        int rows_per_slice, int kflush, double* __restrict__ part)
+// exponent e with 2 |a| < 2^e for every a of the slice column (0 for an all-zero column)
+__device__ inline int oz_exp(unsigned long long bits)
 {
-    int t = blockIdx.x, ti = 0;
-    while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
-    const int tj = ti + t;
-    const int r0 = blockIdx.y * rows_per_slice;
-    const int r1 = min(M, r0 + rows_per_slice);
-    constexpr int KB = 32;
-    // transposed staging [col][row] so 8 consecutive rows of a column are one 16-byte read
-    __shared__ __attribute__((aligned(16))) __bf16 S[3][2][TILE][KB + 8];
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int wi = w >> 1, wj = w & 1;
-    f4 acc[2][2];
-    double accd[2][2][4];
+    return bits ? ilogb(__longlong_as_double((long long)bits)) + 2 : 0;
+}
+
+// Digit planes P[i][rb][c][64 rows] (int8, |d| <= 64) of the rows, rb over 64-row blocks (rows
+// past M are zero).  Block (column block of 64, row block); thread t: column t % 64, rows 16 (t / 64) + 0..15.
+__global__ void __launch_bounds__(256) oz_split_kernel(int M, int ldA, int nrb, int rps, const double* __restrict__ A,
+                                                       const unsigned long long* __restrict__ cmax,
+                                                       signed char* __restrict__ P)
+{
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rb = blockIdx.y;
+    const int q = threadIdx.x >> 6;
+    if (c >= ldA) return;
+    const int rbase = rb * OZ_KB + 16 * q;
+    const int e = oz_exp(cmax[(size_t)((rb * OZ_KB) / rps) * ldA + c]);
+    signed char d[OZ_S][16];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int u = 0; u < 16; ++u) {
+        const int r = rbase + u;
+        double x = (r < M) ? ldexp(A[(size_t)r * ldA + c], -e) : 0.0;
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            acc[a][b] = f4{0, 0, 0, 0};
-#pragma unroll
-            for (int v = 0; v < 4; ++v) accd[a][b][v] = 0.0;
-        }
-    const __bf16* src[3] = {Ah, Am, Al};
-    int since = 0;
-    for (int rb = r0; rb < r1; rb += KB) {
-        __syncthreads();
-        for (int q = tid; q < KB * TILE; q += 256) {
-            int rr = q / TILE, cc = q - rr * TILE;
-            int r = rb + rr;
-#pragma unroll
-            for (int part_ = 0; part_ < 3; ++part_) {
-                __bf16 vi = (__bf16)0.0f, vj = (__bf16)0.0f;
-                if (r < r1) {
-                    vi = src[part_][(size_t)r * ldA + ti * TILE + cc];
-                    vj = src[part_][(size_t)r * ldA + tj * TILE + cc];
-                }
-                S[part_][0][cc][rr] = vi;
-                S[part_][1][cc][rr] = vj;
-            }
-        }
-        __syncthreads();
-        bf8 af[3][2], bfr[3][2];
-#pragma unroll
-        for (int part_ = 0; part_ < 3; ++part_) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-                af[part_][a] = *reinterpret_cast<const bf8*>(&S[part_][0][wi * 32 + a * 16 + (lane & 15)][8 * (lane >> 4)]);
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-                bfr[part_][b] = *reinterpret_cast<const bf8*>(&S[part_][1][wj * 32 + b * 16 + (lane & 15)][8 * (lane >> 4)]);
-        }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                // small terms first so they are not absorbed by the large ones
-                if (NT >= 6) {
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2][a], bfr[0][b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[2][b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][a], bfr[1][b], acc[a][b], 0, 0, 0);
-                }
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][a], bfr[0][b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[1][b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[0][b], acc[a][b], 0, 0, 0);
-            }
-        since += KB;
-        if (since >= kflush) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) accd[a][b][v] += (double)acc[a][b][v];
-                    acc[a][b] = f4{0, 0, 0, 0};
-                }
-            since = 0;
+        for (int i = 0; i < OZ_S; ++i) {
+            const double y = x * 128.0;
+            const double t = rint(y);
+            d[i][u] = (signed char)(int)t;
+            x = y - t;
         }
     }
-    double* P = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * TILE * TILE;
-    // bf16 16x16x32 output: register v of lane l holds C[4*(l/16) + v][l%16]
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int i = 0; i < OZ_S; ++i) {
+        v4i v;
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int w = 0; w < 4; ++w)
+            v[w] = (int)((unsigned char)d[i][4 * w]) | ((int)((unsigned char)d[i][4 * w + 1]) << 8) |
+                   ((int)((unsigned char)d[i][4 * w + 2]) << 16) | ((int)((unsigned char)d[i][4 * w + 3]) << 24);
+        *reinterpret_cast<v4i*>(P + (((size_t)i * nrb + rb) * ldA + c) * OZ_KB + 16 * q) = v;
+    }
+}
+
+// Largest row slice of syrk_i8_kernel: a level sums at most OZ_S pairs of 64^2 per row in int32
+constexpr int OZ_MAX_SLICE_ROWS = 32768;
+static_assert((long long)OZ_S * 64 * 64 * OZ_MAX_SLICE_ROWS < (1ll << 31), "int32 level sums must stay exact");
+
+// Dynamic LDS of syrk_i8_kernel<L0, L1>: the digit planes 1 .. min(OZ_S, L1 - 1) of the two
+// 128-column panels of one 64-row stage
+__host__ __device__ constexpr int oz_planes(int L1) { return L1 - 1 < OZ_S ? L1 - 1 : OZ_S; }
+__host__ __device__ constexpr size_t oz_lds(int L1) { return (size_t)2 * oz_planes(L1) * T2 * OZ_CS; }
+
+// One pass over the levels L0 .. L1 of the Ozaki product for a 128 x 128 upper tile and a row
+// slice of at most OZ_MAX_SLICE_ROWS rows: every level's int32 sum stays in registers for the
+// whole slice (exact), and is folded into FP64 once at the end.  One workgroup per (upper
+// 128-tile over ceil(ldA / 128), row slice of rbps 64-row blocks), XCD-aware 1-D grid; 512
+// threads, two waves per SIMD,
+// wave (wi, wj) owns the 64 x 32 block (rows wi * 64, columns wj * 32) = 4 x 2 i8 MFMA tiles.
+// Output as syrk128_f64_kernel's: partial tiles, summed over slices and passes by
+// syrk128_reduce_kernel.
+constexpr int OZ_THREADS = 512;
+template <int L0, int L1>
+__global__ void __launch_bounds__(OZ_THREADS) syrk_i8_kernel(int ldA, int ntile, int nup, int nslice, int nrb, int rbps,
+                                                             int rps, const signed char* __restrict__ P,
+                                                             const unsigned long long* __restrict__ cmax,
+                                                             double* __restrict__ part)
+{
+    constexpr int NP = oz_planes(L1);
+    constexpr int NL = L1 - L0 + 1;
+    extern __shared__ __align__(16) unsigned char ozs[];
+    int tile, slice;
+    xcd_tile_slice(nup, tile, slice);
+    if (slice >= nslice) return;
+    int t = tile, ti = 0;
+    while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
+    const int tj = ti + t;
+    const int rb0 = slice * rbps, rb1 = min(nrb, rb0 + rbps);
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int wi = w >> 2, wj = w & 3;
+    const bool diag = ti == tj;
+    unsigned char* Li = ozs;                                    // [NP][128][OZ_CS]
+    unsigned char* Lj = diag ? ozs : ozs + (size_t)NP * T2 * OZ_CS;
+    const int ci0 = ti * T2, cj0 = tj * T2;
+    // blocks wholly in the padding columns, or below the diagonal of a diagonal tile, are skipped
+    const bool live = (ci0 + wi * 64 < ldA) && (cj0 + wj * 32 < ldA) && !(diag && wj * 32 + 32 <= wi * 64);
+    // staging: 16-byte chunks (panel, plane, column, quarter), 2 NP per thread (NP on a diagonal
+    // tile, whose two panels are one)
+    constexpr int CH = 2 * NP * T2 * 4 / OZ_THREADS;
+    const int nch = diag ? CH / 2 : CH;
+    v4i pre[CH];
+    auto chunk = [&](int u, int& pn, int& i, int& cc, int& kq) {
+        const int qq = tid + OZ_THREADS * u;
+        pn = qq / (NP * T2 * 4);
+        const int rem = qq - pn * (NP * T2 * 4);
+        i = rem / (T2 * 4);
+        cc = (rem >> 2) & (T2 - 1);
+        kq = rem & 3;
+    };
+    auto fetch = [&](int rb) {
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            if (u >= nch) break;
+            int pn, i, cc, kq;
+            chunk(u, pn, i, cc, kq);
+            const int col = (pn ? cj0 : ci0) + cc;
+            pre[u] = (col < ldA) ? *reinterpret_cast<const v4i*>(P + (((size_t)i * nrb + rb) * ldA + col) * OZ_KB + 16 * kq)
+                                 : v4i{0, 0, 0, 0};
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            if (u >= nch) break;
+            int pn, i, cc, kq;
+            chunk(u, pn, i, cc, kq);
+            *reinterpret_cast<v4i*>((pn ? Lj : Li) + ((size_t)i * T2 + cc) * OZ_CS + 16 * kq) = pre[u];
+        }
+    };
+    v4i s32[NL][4][2];
+#pragma unroll
+    for (int l = 0; l < NL; ++l)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) s32[l][a][b] = v4i{0, 0, 0, 0};
+    if (rb0 < rb1) fetch(rb0);
+    for (int rb = rb0; rb < rb1; ++rb) {
+        __syncthreads();
+        stash();
+        __syncthreads();
+        if (rb + 1 < rb1) fetch(rb + 1);
+        if (!live) continue;
+        const unsigned char* Ai = Li + (size_t)(wi * 64 + (lane & 15)) * OZ_CS + 16 * (lane >> 4);
+        const unsigned char* Bj = Lj + (size_t)(wj * 32 + (lane & 15)) * OZ_CS + 16 * (lane >> 4);
+#pragma unroll
+        for (int i = 1; i <= NP; ++i) {
+            if (i + NP < L0 || i + 1 > L1) continue;
+            // keeps the operand loads of one plane from being hoisted above the previous one's
+            // products (all planes' operands at once do not fit the registers)
+            asm volatile("" ::: "memory");
+            v4i av[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = *reinterpret_cast<const v4i*>(Ai + ((size_t)(i - 1) * T2 + 16 * a) * OZ_CS);
+#pragma unroll
+            for (int j = 1; j <= NP; ++j) {
+                if (i + j < L0 || i + j > L1) continue;
+                v4i bv[2];
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    bv[b] = *reinterpret_cast<const v4i*>(Bj + ((size_t)(j - 1) * T2 + 16 * b) * OZ_CS);
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+                        s32[i + j - L0][a][b] =
+                            __builtin_amdgcn_mfma_i32_16x16x64_i8(av[a], bv[b], s32[i + j - L0][a][b], 0, 0, 0);
+            }
+        }
+    }
+    if (!live) return;
+    // fold the levels (smallest first) and scale: 2^(e_p + e_q - 7 L).
+    // 16 x 16 MFMA output: register v of lane l holds C[4 (l / 16) + v][l % 16]
+    (void)rps;
+    double* Pp = part + ((size_t)slice * nup + tile) * T2 * T2;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const int j = wj * 32 + b * 16 + (lane & 15);
+        const int ej = (cj0 + j < ldA) ? oz_exp(cmax[(size_t)slice * ldA + cj0 + j]) : 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int i = wi * 32 + a * 16 + 4 * (lane >> 4) + v;
-                int j = wj * 32 + b * 16 + (lane & 15);
-                P[i * TILE + j] = accd[a][b][v] + (double)acc[a][b][v];
+                const int i = wi * 64 + a * 16 + 4 * (lane >> 4) + v;
+                const int ei = (ci0 + i < ldA) ? oz_exp(cmax[(size_t)slice * ldA + ci0 + i]) : 0;
+                double x = 0.0;
+#pragma unroll
+                for (int l = NL - 1; l >= 0; --l) x = x * (1.0 / 128.0) + (double)s32[l][a][b][v];
+                Pp[i * T2 + j] = ldexp(x, ei + ej - 7 * L0);
             }
+    }
 }
 
-// G (ldG x ldG, row-major, upper tiles) += sum over slices of part
-__global__ void syrk_reduce_kernel(int ntile, int nslice, int ldG, const double* __restrict__ part, double* __restrict__ G)
+__global__ void fill_u64_kernel(size_t n, unsigned long long* __restrict__ p)
 {
-    int t = blockIdx.x, ti = 0;
-    while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
-    const int tj = ti + t;
-    const int ntiles_up = gridDim.x;
-    for (int q = threadIdx.x; q < TILE * TILE; q += blockDim.x) {
-        double s = 0.0;
-        for (int sl = 0; sl < nslice; ++sl) s += part[((size_t)sl * ntiles_up + blockIdx.x) * TILE * TILE + q];
-        int i = q / TILE, j = q - i * TILE;
-        G[(size_t)(ti * TILE + i) * ldG + tj * TILE + j] += s;
-    }
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 // h += A^T y over row slices, partial[slice][c]
@@ -689,6 +740,8 @@ static SyrkPlan plan_syrk(int M, int ldA, int ns_max, int hs_max)
     int ns2 = 1;
     double best = -1.0;
     for (int ns = 1; ns <= ns_hi; ++ns) {
+        // whole XCD groups (xcd_grid) once there are eight slices or more
+        if (ns > NXCD && ns % NXCD) continue;
         const long long wg = (long long)pl.nup2 * ns;
         const double eff = (double)wg / (double)(((wg + cap - 1) / cap) * cap);
         if (eff > best + 1e-9) { best = eff; ns2 = ns; }
@@ -710,8 +763,9 @@ static SyrkPlan plan_syrk(int M, int ldA, int ns_max, int hs_max)
 static void launch_syrk(hipStream_t st, const SyrkPlan& pl, int M, int ldA, const double* A, const double* y, double* part,
                  double* gpart, double* G, int ldG, double* h, int nh)
 {
-    hipLaunchKernelGGL(syrk128_f64_kernel, dim3(pl.nup2, pl.ns), dim3(256), 0, st, M, ldA, pl.nt2, A, pl.rps, part);
-    hipLaunchKernelGGL(syrk128_reduce_kernel, dim3(pl.nup2), dim3(256), 0, st, pl.nt2, pl.ns, ldG,
+    hipLaunchKernelGGL(syrk128_f64_kernel, dim3(xcd_grid(pl.nup2, pl.ns)), dim3(256), 0, st, M, ldA, pl.nt2, pl.nup2,
+                       pl.ns, A, pl.rps, part);
+    hipLaunchKernelGGL(syrk128_reduce_kernel, dim3(pl.nup2, RED_SPLIT), dim3(256), 0, st, pl.nt2, pl.ns, ldG,
                        (const double*)part, G);
     hipLaunchKernelGGL(gemv_t_kernel, dim3((ldA + 255) / 256, pl.hs), dim3(256), 0, st, M, ldA, A, y, pl.hrps, gpart);
     hipLaunchKernelGGL(gemv_reduce_kernel, dim3((ldA + 255) / 256), dim3(256), 0, st, ldA, nh, pl.hs,
@@ -760,24 +814,60 @@ int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, 
         ctx->gram_rows += M;
         return MLD_OK;
     }
-    const int ntile = ldA / TILE;
-    const int nup = ntile * (ntile + 1) / 2;
-    int nslice = std::max(1, std::min(1024, (2048 + nup - 1) / nup));
-    int rps = (M + nslice - 1) / nslice;
-    rps = ((rps + 31) / 32) * 32;
-    nslice = (M + rps - 1) / rps;
-    rc = device_ensure(ctx, &gs.part, &gs.cap_part, (size_t)nslice * nup * TILE * TILE * sizeof(double));
+    // int8 Ozaki form: column maxima per row slice, digit planes, i8 MFMA SYRK, slice reduction.
+    // Row slices are multiples of 1024 rows (the column-maximum blocks), as many as fill whole
+    // waves of 256 resident workgroups (one per CU: the second pass stages 140 KB of LDS).
+    const int nt2 = (ldA + T2 - 1) / T2, nup2 = nt2 * (nt2 + 1) / 2;
+    // (at least ceil(M / OZ_MAX_SLICE_ROWS) slices: the int32 level sums of one slice stay exact)
+    // the fewest slices whose workgroups fill at least 90 % of whole waves (every slice adds a
+    // partial tile set to the reduction)
+    const int ns_lo = (M + OZ_MAX_SLICE_ROWS - 1) / OZ_MAX_SLICE_ROWS;
+    int ns = ns_lo;
+    {
+        const int ns_hi = std::max(ns_lo, std::min(std::max(64, ns_lo), M / 4096));
+        double best = -1.0;
+        for (int k = ns_lo; k <= ns_hi; ++k) {
+            if (k > NXCD && k % NXCD) continue;  // whole XCD groups (xcd_grid)
+            const long long wg = (long long)nup2 * k;
+            const double eff = (double)wg / (double)(((wg + 255) / 256) * 256);
+            if (eff >= 0.9 && (long long)nup2 * k >= 256) { ns = k; break; }
+            if (eff > best + 1e-9) { best = eff; ns = k; }
+        }
+    }
+    int rps = (M + ns - 1) / ns;
+    rps = ((rps + 1023) / 1024) * 1024;
+    ns = (M + rps - 1) / rps;
+    const int nrb = (M + OZ_KB - 1) / OZ_KB;
+    auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
+    const size_t b_cmax = r256((size_t)ns * ldA * sizeof(unsigned long long));
+    const size_t b_planes = r256((size_t)OZ_S * nrb * ldA * OZ_KB);
+    rc = device_ensure(ctx, &gs.split, &gs.cap_split, b_cmax + b_planes);
     if (rc) return rc;
-    size_t n = (size_t)M * ldA;
-    rc = device_ensure(ctx, &gs.split, &gs.cap_split, 3 * n * sizeof(__bf16));
+    rc = device_ensure(ctx, &gs.part, &gs.cap_part, (size_t)2 * ns * nup2 * T2 * T2 * sizeof(double));
     if (rc) return rc;
-    __bf16* hb = (__bf16*)gs.split;
-    hipLaunchKernelGGL(split_bf16_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, A, hb, hb + n, hb + 2 * n);
-    hipLaunchKernelGGL(syrk_bf16_kernel<6>, dim3(nup, nslice), dim3(256), 0, st, M, ldA, ntile, hb, hb + n,
-                       hb + 2 * n, rps, 512, (double*)gs.part);
+    unsigned long long* cmax = (unsigned long long*)gs.split;
+    signed char* planes = (signed char*)gs.split + b_cmax;
+    static std::atomic<unsigned long long> attr_a{0}, attr_b{0};
+    if (int rc0 = set_lds_limit_once(ctx, (const void*)syrk_i8_kernel<2, 5>, (int)oz_lds(5), attr_a)) return rc0;
+    if (int rc0 = set_lds_limit_once(ctx, (const void*)syrk_i8_kernel<6, OZ_LMAX>, (int)oz_lds(OZ_LMAX), attr_b)) return rc0;
+    const size_t ncm = (size_t)ns * ldA;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((ncm + 255) / 256), dim3(256), 0, st, ncm, cmax);
+    hipLaunchKernelGGL(oz_colmax_kernel, dim3((ldA + 63) / 64, (M + 1023) / 1024), dim3(256), 0, st, M, ldA, rps, A, cmax);
+    hipLaunchKernelGGL(oz_split_kernel, dim3((ldA + 63) / 64, nrb), dim3(256), 0, st, M, ldA, nrb, rps, A,
+                       (const unsigned long long*)cmax, planes);
+    // two passes over the levels (2..5 on digits 1..4, 6..9 on digits 1..7): partial tiles
+    // [pass][slice][tile]
+    double* part0 = (double*)gs.part;
+    double* part1 = part0 + (size_t)ns * nup2 * T2 * T2;
+    hipLaunchKernelGGL((syrk_i8_kernel<2, 5>), dim3(xcd_grid(nup2, ns)), dim3(OZ_THREADS), oz_lds(5), st, ldA, nt2,
+                       nup2, ns, nrb, rps / OZ_KB, rps, (const signed char*)planes, (const unsigned long long*)cmax,
+                       part0);
+    hipLaunchKernelGGL((syrk_i8_kernel<6, OZ_LMAX>), dim3(xcd_grid(nup2, ns)), dim3(OZ_THREADS), oz_lds(OZ_LMAX), st,
+                       ldA, nt2, nup2, ns, nrb, rps / OZ_KB, rps, (const signed char*)planes,
+                       (const unsigned long long*)cmax, part1);
     MLD_HIP(ctx, hipGetLastError());
-    hipLaunchKernelGGL(syrk_reduce_kernel, dim3(nup), dim3(256), 0, st, ntile, nslice, ldA,
-                       (const double*)gs.part, ctx->d_G);
+    hipLaunchKernelGGL(syrk128_reduce_kernel, dim3(nup2, RED_SPLIT), dim3(256), 0, st, nt2, 2 * ns, ldA, (const double*)gs.part,
+                       ctx->d_G);
     MLD_HIP(ctx, hipGetLastError());
     // h
     const SyrkPlan pl = plan_syrk(M, ldA, 64, 512);

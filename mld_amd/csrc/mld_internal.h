@@ -203,7 +203,7 @@ struct mld_ctx {
     size_t cap_struct_off = 0;
     // Gram state
     int gD = 0, gld = 0, gram_order = 1;
-    int gram_mixed = 0;          // some rows went through the split-BF16 Gram
+    int gram_mixed = 0;          // some rows went through the int8 Ozaki Gram
     long long gram_rows = 0;
     double* d_G = nullptr;       // gld x gld, row-major, upper tiles accumulated
     double* d_h = nullptr;       // gld

@@ -674,13 +674,11 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
     double* part = (double*)base; base += 256 * nvec * sizeof(double);
     double* Linv = (double*)base;
     MLD_HIP(ctx, hipMemcpyAsync(dridge, ridge.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
-    // Factorise M = S^-1 (S (G + Lambda) S + shift I) S^-1.  The FP64 Gram is factorised
-    // as is (shift 0); a split-BF16 Gram carries a relative error of about 1e-6 of its
-    // diagonal, so its factor gets a diagonal shift of that size, raised tenfold while
-    // the factorisation meets a non-positive pivot.  M then only preconditions the
-    // FP64 iteration below, which solves (G + Lambda) w = h itself.
-    double shift = ctx->gram_mixed ? 1e-6 : 0.0;
-    for (;;) {
+    // Factorise M = S^-1 (S (G + Lambda) S) S^-1 (Jacobi-scaled Cholesky).  Both Gram forms (FP64
+    // MFMA and the int8 Ozaki slices) carry FP64-level error, so no diagonal shift is applied;
+    // M preconditions the FP64 iteration below, which solves (G + Lambda) w = h itself.
+    const double shift = 0.0;
+    {
         MLD_HIP(ctx, hipMemsetAsync(info, 0, sizeof(int), st));
         MLD_HIP(ctx, hipMemsetAsync(L, 0, bytes_L, st));
         hipLaunchKernelGGL(prep_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, dridge, shift, L, S);
@@ -696,11 +694,9 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
         int hinfo = 0;
         MLD_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
         MLD_HIP(ctx, hipStreamSynchronize(st));
-        if (!hinfo) break;
-        if (!ctx->gram_mixed || shift >= 1e-2)
+        if (hinfo)
             return mld_fail(ctx, MLD_E_SINGULAR, "Cholesky: leading minor " + std::to_string(hinfo) +
                                                      " not positive definite (increase lambda)");
-        shift *= 10.0;
     }
     auto precond = [&](double* v) {  // v <- M^-1 v using S L L^T S
         hipLaunchKernelGGL(scale_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, S, v);

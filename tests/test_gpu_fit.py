@@ -82,26 +82,32 @@ def test_linear_fit_matches_oracle():
 
 
 def test_quadratic_mixed_fit_refines_to_fp64():
-    """Quadratic (unique-column) fit: split-BF16 Gram + FP64 Cholesky + refinement against
-    the kept FP64 rows reaches the FP64 solution of the same rows."""
+    """Quadratic (unique-column) fit at the workload's ridge lambda = 1e-8 on FP64 descriptors.  The
+    int8 Ozaki Gram differs from the FP64 MFMA Gram by FP64 rounding (5e-15 of sqrt(G_pp G_qq));
+    with FP64 Cholesky + refinement against the kept FP64 rows it converges in at most three passes
+    to the same fitted energies and forces (1e-9) and to coefficients no further from the FP64
+    solution than a second FP64 solution with another summation order (different batching) is:
+    at this ridge the Jacobi-scaled system's condition (pivots down to 2e-5) puts FP64's own
+    coefficient spread at about 5e-6, above a fixed 1e-6 bound."""
     import mld_amd as M
 
-    pos, cells, types, E, F = workload(2, 16, n=3, sigma=0.1)   # 864 atoms, 2608 rows, 1596 columns
+    pos, cells, types, E, F = workload(2, 48, n=3, sigma=0.1)   # 2,592 atoms, 7,824 rows, 1,596 columns
     ctx = M.Context(j_max=4.0, r_cut=4.7)
-    g64 = gpu_gram(ctx, pos, cells, types, E, F, 2, 1.0, 0.1, keep_rows=False)
+    lam = 1e-8
+    g64 = gpu_gram(ctx, pos, cells, types, E, F, 2, 1.0, 0.1, keep_rows=True)
     G, h = g64.export()
-    lam = 1e-6 * float(np.trace(G)) / G.shape[0]
     w64, d64 = g64.solve(lam)
-    r = g64.ridge_vector(lam)
-    res64 = np.linalg.norm(G @ w64 + r * w64 - h) / np.linalg.norm(h)
-    assert res64 < 1e-10
-    # mixed Gram on the same FP64 descriptors, refined against the kept FP64 rows
+    g64b = gpu_gram(ctx, pos, cells, types, E, F, 2, 1.0, 0.1, keep_rows=True, chunk=7)
+    w64b, _ = g64b.solve(lam)
     gmx = gpu_gram(ctx, pos, cells, types, E, F, 2, 1.0, 0.1, mixed=True, keep_rows=True)
+    Gm, hm = gmx.export()
     wmx, dmx = gmx.solve(lam)
-    assert dmx["iterations"] >= 1
-    resmx = np.linalg.norm(G @ wmx + r * wmx - h) / np.linalg.norm(h)
-    print("mixed solve", dmx, "fp64 solve", d64, "coef rel", rel(wmx, w64))
-    assert resmx < 1e-8
-    assert rel(G @ wmx, G @ w64) < 1e-8
-    assert rel(wmx, w64) < 1e-6
+    nrm = np.sqrt(np.outer(np.diag(G), np.diag(G)))
+    spread = rel(w64b, w64)
+    print("mixed solve", dmx, "fp64 solve", d64, "coef rel", rel(wmx, w64), "fp64 spread", spread,
+          "Gram diff", float((np.abs(Gm - G) / nrm).max()))
+    assert float((np.abs(Gm - G) / nrm).max()) < 2e-14 and rel(hm, h) < 1e-14
+    assert d64["converged"] and dmx["converged"] and dmx["iterations"] <= 3
+    assert rel(G @ wmx, G @ w64) < 1e-9
+    assert rel(wmx, w64) <= max(1e-6, 3.0 * spread)
     ctx.close()

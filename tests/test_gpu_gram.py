@@ -88,7 +88,7 @@ def test_batch_rows_gram_against_oracle(ctx64, order):
 
 def test_mixed_gram_accuracy(ctx64):
     import torch
-    from mld_amd.fit import GramFit
+    from mld_amd.fit import GramFit, full_to_unique_rows
 
     b, pos, cells, types, E, F = small_batch(ctx64, seed=5)
     Fd = torch.from_numpy(F.reshape(-1)).to("cuda:0")
@@ -98,9 +98,17 @@ def test_mixed_gram_accuracy(ctx64):
     gmx = GramFit(ctx64, order=2, mixed=True)
     gmx.add_batch(b, E, Fd, 1.0, 0.1)
     Gmx, _ = gmx.export()
-    # split-BF16 (6 products) with FP32 accumulation: ~FP32-level relative error per entry
-    err = np.abs(Gmx - G64) / np.sqrt(np.outer(np.diag(G64), np.diag(G64)))
-    assert float(err.max()) < 1e-5
+    # int8 Ozaki Gram (7 rounded digits, 34 digit pairs, exact int32 sums): FP64-level error per
+    # entry, measured against the Gram of the oracle's rows in extended precision
+    A, y, w = oracle_rows(pos, cells, types, E, F, 2, 1.0, 0.1)
+    A = full_to_unique_rows(A, 55)
+    Aw = A.astype(np.longdouble) * np.sqrt(w.astype(np.longdouble))[:, None]
+    Gx = Aw.T @ Aw
+    nrm = np.sqrt(np.outer(np.diag(G64), np.diag(G64)))
+    err_mx = float((np.abs(Gmx - Gx) / nrm).max())
+    err_64 = float((np.abs(G64 - Gx) / nrm).max())
+    print("normalised Gram error: int8 Ozaki", err_mx, "FP64 MFMA", err_64)
+    assert err_mx < 3e-14 and err_64 < 1e-14
 
 
 def test_solve_matches_numpy_and_refines(ctx64):
