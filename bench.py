@@ -25,9 +25,9 @@ Workloads (``--workload``):
       random occupancy, sigma = 0.05 A), atom counts lognormal (median 128, clipped to
       8..2,000); each request asks for the energy, forces and per-atom Mahalanobis
       scores of a linear model fitted beforehand to species-dependent Morse targets.
-      Descriptor: ACE with the Chebyshev (expTcheb) radial basis, n_max = 8 / 4 / 2 / 1 and
-      l_max = 0 / 4 / 2 / 1 for nu = 1 / 2 / 3 / 4 (body orders 2 to 5), two chemical
-      channels (D = 394), r_cut = 5.0 A, FP32.
+      Descriptor: the reference's ACE (Ralf radial chemistry, RPI couplings) with the
+      Chebyshev (expTcheb) radial splines, n_max = 8 / 2 / 1 and l_max = 0 / 4 / 2 for
+      nu = 1 / 2 / 3 (body orders 2 to 4), 5 species, ace_chem = 0 (D = 615), r_cut = 5.0 A, FP32.
       Reported: p50 / p99 latency at that load, and the throughput of the same
       requests submitted all at once.
 
@@ -539,6 +539,12 @@ def main_krr(args):
 
 
 # ----------------------------------------------------------------------------- serving
+# The serving descriptor: the reference's ACE (Ralf radial chemistry, RPI) for 5 species, n_max 8 / 2 / 1
+# and l_max 0 / 4 / 2 for nu = 1 .. 3 (body order <= 4), expTcheb lambda 3, r_cut 5.0 A with the
+# 0.5 A outer and 0.4 A inner tapers from r_cut_in 1.2 A, ACE_CHEM_INCOMPLETE (ace_chem = 0): D = 615.
+# (ACE_CHEM_STANDARD would place it in one of five blocks, D = 3075, four fifths zeros.)
+SERVE_ACE = dict(descriptor="ace", nspecies=5, ace_numax=3, ace_nmax=(8, 2, 1), ace_lmax=(0, 4, 2), ace_lambda=3.0,
+                 ace_gencg=2, ace_chem=0, ace_r_cut_in=1.2, r_cut_width_in=0.4, r_cut_width=0.5)
 def hea_request_pool(rng, n: int):
     from mld_amd import synth
 
@@ -561,12 +567,11 @@ def serve_measurement(args):
     from mld_amd.serve import Server
 
     rng = np.random.default_rng(SEED + 5)
-    weights = [1.0, 1.25, 1.5, 1.75, 2.0]
-    ace = dict(descriptor="ace", ace_numax=4, ace_nmax=(8, 4, 2, 1), ace_lmax=(0, 4, 2, 1), ace_lambda=2.0)
+    ace = dict(SERVE_ACE)
     Dm, am, r0m = synth.morse_mixture_params(5)
     # fit the served model (linear, FP64) on 200 structures of the same kind
     train = hea_request_pool(rng, 200)
-    c64 = M.Context(r_cut=5.0, nspecies=5, weight=weights, **ace)
+    c64 = M.Context(r_cut=5.0, **ace)
     b = M.Batch(c64, [p for p, _, _ in train], [c for _, c, _ in train], [t for _, _, t in train])
     b.neighbours()
     po, pi, pd = b.pair_tensors()
@@ -575,15 +580,14 @@ def serve_measurement(args):
     b.bso4(e, g)
     gf = fit.GramFit(c64, order=1)
     gf.add_batch(b, E.cpu().numpy(), F.reshape(-1).contiguous(), 1.0, 0.1)
-    # minimum-norm solve (the reference's svd_rcond = -1 semantics): the nu = 4 couplings
-    # through different intermediate L are not all linearly independent
+    # minimum-norm solve (the reference's svd_rcond = -1 semantics)
     w, _ = gf.solve(1e-8, fit.SOLVE_SVD)
     Bh = e.cpu().numpy()
     mu = Bh.mean(0)
     cov = np.cov(Bh.T)
     cov += 1e-6 * np.trace(cov) / cov.shape[0] * np.eye(cov.shape[0])
     c64.close()
-    ctx = M.Context(r_cut=5.0, nspecies=5, weight=weights, precision=M.PREC_FP32, **ace)
+    ctx = M.Context(r_cut=5.0, precision=M.PREC_FP32, **ace)
     fit.model_set(ctx, 1, w)
     fit.mahalanobis_set(ctx, mu, np.linalg.inv(cov))
     nreq = int(round(args.rate * args.duration))

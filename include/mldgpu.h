@@ -62,17 +62,32 @@ typedef struct mld_params {
     double r_cut_width_in;      /* &input_ml r_cut_width_in */
     int precision;              /* MLD_PREC_FP64 or MLD_PREC_FP32 */
     int device;                 /* HIP device ordinal */
-    /* ACE (descriptor_type 300), module_ace_desc keys: ace_numax (body order - 1, 1..4),
-     * ace_nmax_list / ace_lmax_list per nu (defaults "5 3 1" / "0 4 2", read_ml_file.F90:282-286),
-     * ace_lambda (expTcheb radial, ace_radial_poly = 2).  Chemical channels: nchannel 1
-     * (every neighbour) or 2 (plus one weighted by weight[species]). */
+    /* ACE (descriptor_type 300, ace_radial_chem = 1), module_ace_desc keys (read_ml_file.F90:278-305):
+     *   ace_numax            highest body order nu, 1 .. 6
+     *   ace_active[nu-1]     l_ace_order(nu)
+     *   ace_nmax, ace_lmax   ace_nmax_list / ace_lmax_list per nu (nu = 1: n = 0 .. nmax, l = 0)
+     *   ace_lambda           ace_lambda_list(1), the lambda of the radial (compute_ace.F90:510-512)
+     *   ace_radial_poly      1 powTcheb, 2 expTcheb (0 also means 2)
+     *   ace_gencg            1 RI (GENCG_DRAFT), 2 RPI (GENCG_DUSORT; 0 also means 2)
+     *   ace_chem             0 ACE_CHEM_INCOMPLETE (D = ace_dim), 1 ACE_CHEM_STANDARD
+     *                        (D = ace_dim * nspecies, the centre species' block filled)
+     *   ace_npoints_spline   radial spline points (0 means the default 1000)
+     *   r_cut_width          &input_ml r_cut_width, outer taper width (0 means 0.5)
+     *   ace_r_cut_in         &input_ml r_cut_in: start of the radial spline grid and inner taper
+     *                        (the reference default is -1.2: no inner taper)
+     * r_cut_width_in above is the inner taper width.  itype indexes the species mu = 1 .. nspecies;
+     * nchannel must be 1. */
     int ace_numax;
-    int ace_nmax[4];
-    int ace_lmax[4];
+    int ace_active[6];
+    int ace_nmax[6];
+    int ace_lmax[6];
     double ace_lambda;
-    /* &input_ml ace_radial_poly (read_ml_file.F90:287, mld_radial_module.F90:92-104):
-     * 1 powTcheb, 2 expTcheb (the reference default; 0 also means 2) */
     int ace_radial_poly;
+    int ace_gencg;
+    int ace_chem;
+    int ace_npoints_spline;
+    double r_cut_width;
+    double ace_r_cut_in;
 } mld_params;
 
 typedef struct mld_ctx mld_ctx;
@@ -82,9 +97,15 @@ typedef struct mld_ctx mld_ctx;
  * gen_dimension_for_bispectrum_so4).  One context per host thread. */
 int mld_create(const mld_params* p, mld_ctx** out);
 int mld_destroy(mld_ctx* ctx);
+/* ctx NULL: the message of this thread's last failed mld_create or mld_ace_basis_review */
 const char* mld_last_error(const mld_ctx* ctx);
 /* D = bisso4_dim * nchannel (dim_xdesc) */
 int mld_descriptor_dim(const mld_ctx* ctx, int* D);
+/* ACE basis review without a device (the table print_ace_descriptor writes,
+ * compute/compute_ace_nlmu_tools.F90:2620-2680): for body order nu = 1 .. 6, review[(nu-1)*7 + k]
+ * = dim_mu, dim_n, dim_l0, dim_Lb, dim_Li, dim_nl0, dim_B (k = 0 .. 6), -1 for an inactive order;
+ * ace_dim = sum of dim_B, D the descriptor dimension.  p->descriptor_type must be MLD_DESC_ACE. */
+int mld_ace_basis_review(const mld_params* p, int review[42], int* ace_dim, int* D);
 
 /* Largest per-atom neighbour count of one configuration (compute_max_n_neigh,
  * first_neighbour_pass.F90:41-129): the caller sizes kind_neigh / force. */

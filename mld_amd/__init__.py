@@ -10,7 +10,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
-from typing import Optional, Sequence
+from typing import Any, Dict, Optional, Sequence
 
 import numpy as np
 
@@ -24,7 +24,7 @@ DESC_BSO4, DESC_ACE = 9, 300
 
 # Every symbol include/mldgpu.h declares.
 EXPORTS = [
-    "mld_create", "mld_destroy", "mld_last_error", "mld_descriptor_dim", "mld_max_neighbours", "mld_bso4", "mld_ace",
+    "mld_create", "mld_destroy", "mld_last_error", "mld_descriptor_dim", "mld_ace_basis_review", "mld_max_neighbours", "mld_bso4", "mld_ace",
     "mld_batch_neighbours", "mld_batch_pairs_dev", "mld_batch_pair_geometry_dev", "mld_batch_bso4_dev", "mld_gram_begin", "mld_gram_add_rows",
     "mld_gram_add_rows_dev", "mld_gram_add_rows_noalloc", "mld_gram_add_batch", "mld_gram_add_structures", "mld_gram_export", "mld_gram_import", "mld_gram_set_keep_rows", "mld_gram_solve", "mld_gram_solve_svd",
     "mld_kernel_po_set", "mld_kernel_po_dev", "mld_kernel_po", "mld_model_set", "mld_mahalanobis_set", "mld_predict_dev",
@@ -55,11 +55,69 @@ class mld_params(C.Structure):
         ("precision", C.c_int),
         ("device", C.c_int),
         ("ace_numax", C.c_int),
-        ("ace_nmax", C.c_int * 4),
-        ("ace_lmax", C.c_int * 4),
+        ("ace_active", C.c_int * 6),
+        ("ace_nmax", C.c_int * 6),
+        ("ace_lmax", C.c_int * 6),
         ("ace_lambda", C.c_double),
         ("ace_radial_poly", C.c_int),
+        ("ace_gencg", C.c_int),
+        ("ace_chem", C.c_int),
+        ("ace_npoints_spline", C.c_int),
+        ("r_cut_width", C.c_double),
+        ("ace_r_cut_in", C.c_double),
     ]
+
+
+ACE_NU_MAX = 6
+
+
+def _fill_ace(p: "mld_params", ace_numax: int, ace_nmax, ace_lmax, ace_active, ace_lambda: float,
+              ace_radial_poly: int, ace_gencg: int, ace_chem: int, ace_npoints_spline: int, r_cut_width: float,
+              ace_r_cut_in: float) -> None:
+    """ACE keys into mld_params: ace_active[nu - 1] is l_ace_order(nu) (default: every order up to
+    ace_numax)."""
+    if not 1 <= ace_numax <= ACE_NU_MAX:
+        raise ValueError(f"ace_numax {ace_numax}: 1 .. {ACE_NU_MAX}")
+    act = [True] * ace_numax if ace_active is None else list(ace_active)
+    p.ace_numax = ace_numax
+    for k in range(ACE_NU_MAX):
+        p.ace_active[k] = int(bool(act[k])) if k < min(len(act), ace_numax) else 0
+        p.ace_nmax[k] = int(ace_nmax[k]) if k < len(ace_nmax) else 0
+        p.ace_lmax[k] = int(ace_lmax[k]) if k < len(ace_lmax) else 0
+    p.ace_lambda = ace_lambda
+    p.ace_radial_poly = int(ace_radial_poly)
+    p.ace_gencg = int(ace_gencg)
+    p.ace_chem = int(ace_chem)
+    p.ace_npoints_spline = int(ace_npoints_spline)
+    p.r_cut_width = float(r_cut_width)
+    p.ace_r_cut_in = float(ace_r_cut_in)
+
+
+def ace_basis_review(nspecies: int = 1, ace_numax: int = 3, ace_nmax=(5, 3, 1), ace_lmax=(0, 4, 2), ace_active=None,
+                     ace_gencg: int = 2, ace_chem: int = 1, r_cut: float = 5.0) -> Dict[str, Any]:
+    """The reference's ACE basis review table (print_ace_descriptor,
+    compute_ace_nlmu_tools.F90:2620-2680) from the library's own enumeration, without a device:
+    {"orders": {nu: {dim_mu, dim_n, dim_l0, dim_Lb, dim_Li, dim_nl0, dim_B}}, "ace_dim", "D"}."""
+    L = load()
+    p = mld_params()
+    p.descriptor_type = DESC_ACE
+    p.r_cut = r_cut
+    p.nspecies = nspecies
+    p.nchannel = 1
+    p.r_cut_width_in = 0.4
+    _fill_ace(p, ace_numax, ace_nmax, ace_lmax, ace_active, 3.0, 2, ace_gencg, ace_chem, 0, 0.5, -1.2)
+    rv = (C.c_int * 42)()
+    ad, D = C.c_int(), C.c_int()
+    rc = L.mld_ace_basis_review(C.byref(p), rv, C.byref(ad), C.byref(D))
+    if rc != 0:
+        raise MldError(rc, L.mld_last_error(None).decode())
+    keys = ("dim_mu", "dim_n", "dim_l0", "dim_Lb", "dim_Li", "dim_nl0", "dim_B")
+    orders = {}
+    for nu in range(1, ACE_NU_MAX + 1):
+        row = [rv[(nu - 1) * 7 + k] for k in range(7)]
+        if row[0] >= 0:
+            orders[nu] = dict(zip(keys, row))
+    return {"orders": orders, "ace_dim": ad.value, "D": D.value}
 
 
 _lib = None
@@ -86,6 +144,7 @@ def load() -> C.CDLL:
         "mld_destroy": [vp],
         "mld_last_error": [vp],
         "mld_descriptor_dim": [vp, ip],
+        "mld_ace_basis_review": [C.POINTER(mld_params), ip, ip, ip],
         "mld_max_neighbours": [vp, i, vp, vp, ip],
         "mld_bso4": [vp, i, vp, vp, vp, i, vp, vp, vp, vp, i],
         "mld_ace": [vp, i, vp, vp, vp, i, vp, vp, vp, vp, i],
@@ -151,21 +210,21 @@ class Context:
                  weight_3ch: Optional[Sequence[float]] = None, r_cut_pair_in: Optional[np.ndarray] = None,
                  r_cut_width_in: float = 0.4, precision: int = PREC_FP64, device: int = 0,
                  descriptor: str = "bso4", ace_numax: int = 3, ace_nmax=(5, 3, 1), ace_lmax=(0, 4, 2),
-                 ace_lambda: float = 2.0, ace_radial_poly: int = 2):
-        """descriptor "bso4" (SO(4) bispectrum) or "ace" (ace_* keys: body order - 1, radial
-        functions and l per nu, radial lambda, radial family 1 powTcheb / 2 expTcheb; weight adds
-        the species-weighted channel)."""
+                 ace_lambda: float = 2.0, ace_radial_poly: int = 2, ace_active=None, ace_gencg: int = 2,
+                 ace_chem: int = 1, ace_npoints_spline: int = 1000, r_cut_width: float = 0.5,
+                 ace_r_cut_in: float = -1.2):
+        """descriptor "bso4" (SO(4) bispectrum) or "ace" (the reference's ACE with the &input_ml
+        keys ace_numax, l_ace_order (ace_active), ace_nmax_list / ace_lmax_list, ace_lambda_list(1),
+        ace_radial_poly, ace_gencg, ace_chem, ace_npoints_spline, r_cut_width, r_cut_in and
+        r_cut_width_in; species are the chemical channels)."""
         L = load()
         nch = 1 + (weight is not None) + (weight_3ch is not None)
         self._keep = []
         p = mld_params()
         p.descriptor_type = DESC_ACE if descriptor == "ace" else DESC_BSO4
-        p.ace_numax = ace_numax
-        for k in range(4):
-            p.ace_nmax[k] = int(ace_nmax[k]) if k < len(ace_nmax) else 0
-            p.ace_lmax[k] = int(ace_lmax[k]) if k < len(ace_lmax) else 0
-        p.ace_lambda = ace_lambda
-        p.ace_radial_poly = int(ace_radial_poly)
+        if descriptor == "ace":
+            _fill_ace(p, ace_numax, ace_nmax, ace_lmax, ace_active, ace_lambda, ace_radial_poly, ace_gencg, ace_chem,
+                      ace_npoints_spline, r_cut_width, ace_r_cut_in)
         p.j_max = j_max
         p.r_cut = r_cut
         p.inv_r0_input = default_inv_r0_input() if inv_r0_input is None else inv_r0_input
@@ -183,7 +242,7 @@ class Context:
         h = C.c_void_p()
         rc = L.mld_create(C.byref(p), C.byref(h))
         if rc != 0:
-            raise MldError(rc, "mld_create failed (check j_max, r_cut, inv_r0_input, weights)")
+            raise MldError(rc, "mld_create failed: " + L.mld_last_error(None).decode())
         self.h = h
         self.L = L
         self.precision = precision
@@ -282,4 +341,4 @@ class Context:
 
 from .batch import Batch  # noqa: E402  (device-resident batch API)
 
-__all__ = ["Context", "Batch", "load", "MldError", "PREC_FP64", "PREC_FP32", "EXPORTS"]
+__all__ = ["Context", "Batch", "load", "MldError", "PREC_FP64", "PREC_FP32", "EXPORTS", "ace_basis_review"]

@@ -1,18 +1,24 @@
-// ACE (atomic cluster expansion) descriptors and their position gradients on gfx950.
+// ACE descriptors of the reference (descriptor_type 300, ace_radial_chem = 1) and their
+// position gradients on gfx950.
 //
-// One 256-thread workgroup per atom (compute_ace, compute/compute_ace.F90:40-341):
-//   1. per chunk of 32 neighbours: radial functions g_n and dg_n (radial_expTcheb,
-//      mld_radial_module.F90:186-208, 270-306), Y_lm and grad Y_lm (solid harmonics,
-//      angular_functions.f90:277-336) into LDS, computed in FP64;
-//   2. A_{c n l m} = sum_j w_c(j) g_n(r_j) Y_lm(r_j)  (compute_uniqueA_neigh_ja,
-//      compute/compute_ace_AB.F90:2475-2563), accumulated in neighbour order;
-//   3. B_b = Re sum_terms coef prod_i A_{e_i}  (one thread per basis function);
-//   4. G_s = dB_b/dA_e for the distinct A entries e of each b (sum over terms and factors of
-//      coef * prod of the other factors), once per atom in LDS; then
-//      dB_b/dr_j = Re sum_s G_s dA_e(j), dA_e(j) = w_c(j) (dg_n r^ Y_lm + g_n grad Y_lm),
-//      one work item per (neighbour, b); the self slot is -sum_j (:329-333).
-// Channels: c = 0 every neighbour (weight 1), c = 1 weighted by the neighbour's species
-// weight.  Precision T = double (FP64 form) or float (FP32 form, geometry in FP64).
+// One 256-thread workgroup per atom (compute_ace, compute/compute_ace.F90:141-341):
+//   1. per chunk of ACE_NBC neighbours, into LDS: R_n and dR_n/dr from the clamped cubic
+//      splines (evaluateSpline / derivativeSpline, mld_cubic_spline.F90:189-240: 0 outside
+//      [r_cut_in, r_cut], |value| < 100 eps set to 0), Y_lm and grad Y_lm for m >= 0 (solid
+//      harmonics, angular_functions.f90:277-336) and Y_l,-m = (-1)^m conj(Y_lm)
+//      (compute_ylm_for_neigh_ja, compute/compute_ace_AB.F90:1772-1846), evaluated in FP64;
+//   2. A_{mu n l m} = sum over the neighbours j of species mu of R_n(r_j) Y_lm(r_j)
+//      (compute_uniqueA_neigh_ja, :2475-2563);
+//   3. B_b = Re sum_terms coef prod_i A_{e_i} (compute_B_basis_atom_adjoint, :2023-2340;
+//      nu = 1: A_{mu n 0 0}, compute_AB_body_one_neigh_ja, :2566-2689), written into the block
+//      of the centre species for ACE_CHEM_STANDARD (compute_ace.F90:145-147), zeros elsewhere;
+//   4. gradients, per tile of functions: G_s = dB_b/dA_e for the slots of the tile into LDS, then
+//      for every neighbour j of species mu
+//        dB_b/dr_j = Re sum over the slots of b with species mu of G_s (dR_n r^_j Y_lm + R_n grad Y_lm);
+//      the self slot is -sum_j (compute_ace.F90:329-333), with no sign flip (descriptors.F90:293-294).
+// Precision T = double (FP64 form) or float (FP32 form; geometry, splines and Y_lm in FP64).
+#include <cfloat>
+
 #include "mld_internal.h"
 
 namespace {
@@ -20,8 +26,8 @@ namespace {
 using mld::AceDev;
 using mld::AceTerm;
 
-constexpr int NBC = 32;  // neighbours per LDS chunk
-constexpr int NTH = 256;
+constexpr int NBC = mld::ACE_NBC;
+constexpr int NTH = mld::ACE_NTH;
 
 template <typename T>
 struct cx2 {
@@ -91,87 +97,90 @@ __device__ __forceinline__ void ylm_dev(int l, int m, double x, double y, double
     }
 }
 
+
 template <typename T>
-struct AceSmem {
-    cx2<T> A[mld::ACE_NCH * mld::ACE_NMAX * mld::ACE_LM];
-    T g[NBC][mld::ACE_NMAX], dg[NBC][mld::ACE_NMAX];
-    cx2<T> Y[NBC][mld::ACE_LM];
-    cx2<T> dY[NBC][mld::ACE_LM][3];
-    T w1[NBC];
-    T rh[NBC][3];
+struct AceLds {
+    cx2<T>* A;   // [nA]
+    cx2<T>* Y;   // [NBC][LA2]
+    cx2<T>* dY;  // [NBC][LA2][3]
+    cx2<T>* G;   // [slot_cap]
+    T* R;        // [NBC][NR]
+    T* dR;       // [NBC][NR]
+    T* rh;       // [NBC][3]
+    T* self;     // [NTH][3]
+    int* sp;     // [NBC]
+    int* code;   // [slot_cap]
 };
+
+template <typename T>
+__device__ AceLds<T> carve(const AceDev& tb, unsigned char* base)
+{
+    AceLds<T> L;
+    L.A = reinterpret_cast<cx2<T>*>(base);
+    L.Y = L.A + tb.nA;
+    L.dY = L.Y + NBC * tb.LA2;
+    L.G = L.dY + NBC * tb.LA2 * 3;
+    L.R = reinterpret_cast<T*>(L.G + tb.slot_cap);
+    L.dR = L.R + NBC * tb.NR;
+    L.rh = L.dR + NBC * tb.NR;
+    L.self = L.rh + NBC * 3;
+    L.sp = reinterpret_cast<int*>(L.self + NTH * 3);
+    L.code = L.sp + NBC;
+    return L;
+}
 
 // per-neighbour factors of the neighbours [q0, q0 + nq) of the atom into LDS
 template <typename T>
-__device__ void stage_chunk(const AceDev& tb, AceSmem<T>& sm, int p0, int q0, int nq, const double4* __restrict__ pair_d,
-                            const int* __restrict__ pair_idx, const int* __restrict__ itype,
-                            const double* __restrict__ wfac, int nsp, double rcut)
+__device__ void stage_chunk(const AceDev& tb, const AceLds<T>& L, int p0, int q0, int nq,
+                            const double4* __restrict__ pair_d, const int* __restrict__ pair_idx,
+                            const int* __restrict__ itype)
 {
     const int tid = threadIdx.x;
+    const int NR = tb.NR;
+    for (int it = tid; it < nq * NR; it += NTH) {
+        const int q = it / NR, n = it - q * NR;
+        const double r = pair_d[p0 + q0 + q].w;
+        double v = 0.0, dv = 0.0;
+        if (r >= tb.rin && r <= tb.rc) {  // NaN fails both
+            const int i = min((int)floor((r - tb.rin) / tb.delta), tb.npts - 2);
+            const double h = r - tb.xg[i];
+            const double* c = tb.spl + ((size_t)n * tb.npts + i) * 4;
+            v = c[0] + h * (c[1] + h * (c[2] + c[3] * h));
+            dv = c[1] + 2.0 * c[2] * h + 3.0 * c[3] * h * h;
+            if (fabs(v) < 100.0 * DBL_EPSILON) v = 0.0;
+            if (fabs(dv) < 100.0 * DBL_EPSILON) dv = 0.0;
+        }
+        L.R[q * NR + n] = (T)v;
+        L.dR[q * NR + n] = (T)dv;
+    }
     if (tid < nq) {
         const double4 d = pair_d[p0 + q0 + tid];
-        const double r = d.w;
-        double x, d_x;     // Chebyshev argument and its r-derivative
-        double cost = 0.0, d_cost = 0.0;
-        if (tb.radial == 1) {
-            // radial_powTcheb (mld_radial_module.F90:244-272): x = 1 - 2 (1 - r/rc)^lambda,
-            // g_n = (1 - T_n(x)) / 2, n = 1 .. NA
-            const double tmp = 1.0 - r / rcut, dtmp = pow(tmp, tb.lam - 1.0);
-            x = 1.0 - 2.0 * dtmp * tmp;
-            d_x = 2.0 * tb.lam / rcut * dtmp;
-        } else {
-            // radial_expTcheb (mld_radial_module.F90:275-307): g_1 = (1 + cos(pi r/rc)) / 2,
-            // g_n = (1 - T_{n-1}(x)) (1 + cos(pi r/rc)) / 4, n = 2 .. NA
-            const double pi = 3.14159265358979323846;
-            const double tmpr = r / rcut;
-            cost = 1.0 + cos(pi * tmpr);
-            d_cost = -sin(pi * tmpr) * pi / rcut;
-            const double lambt = exp(tb.lam) - 1.0, expt = exp(-tb.lam * (tmpr - 1.0));
-            x = 1.0 - 2.0 * (expt - 1.0) / lambt;
-            d_x = 2.0 * tb.lam / rcut * expt / lambt;
-        }
-        // chebTU (mld_radial_module.F90:186-211): T_n = x T_{n-1} - (1 - x^2) U_{n-2},
-        // U_n = x U_{n-1} + T_n, dT_n/dx = n U_{n-1}
-        double Tm1 = x, Um2 = 1.0, Um1 = 2.0 * x, dTm1 = 1.0;  // T_1, U_0, U_1, dT_1
-        if (tb.radial == 1) {
-            for (int n = 1; n <= tb.NA; ++n) {
-                sm.g[tid][n - 1] = (T)(0.5 - 0.5 * Tm1);
-                sm.dg[tid][n - 1] = (T)(-0.5 * d_x * dTm1);
-                const double Tn = x * Tm1 - (1.0 - x * x) * Um2;
-                const double Un = x * Um1 + Tn;
-                const double dTn = (double)(n + 1) * Um1;
-                Um2 = Um1; Tm1 = Tn; Um1 = Un; dTm1 = dTn;
-            }
-        } else {
-            sm.g[tid][0] = (T)(0.5 * cost);
-            sm.dg[tid][0] = (T)(0.5 * d_cost);
-            // n >= 2 uses T_{n-1}; T_1 = x for n = 2
-            for (int n = 2; n <= tb.NA; ++n) {
-                const double Tn1 = Tm1, dTn1 = dTm1;
-                sm.g[tid][n - 1] = (T)(0.25 * (1.0 - Tn1) * cost);
-                sm.dg[tid][n - 1] = (T)(0.25 * (1.0 - Tn1) * d_cost - 0.25 * d_x * dTn1 * cost);
-                // advance to T_n, U_n, dT_n (chebTU)
-                const double Tn = x * Tm1 - (1.0 - x * x) * Um2;
-                const double Un = x * Um1 + Tn;
-                const double dTn = (double)n * Um1;
-                Tm1 = Tn; Um2 = Um1; Um1 = Un; dTm1 = dTn;
-            }
-        }
-        sm.rh[tid][0] = (T)(d.x / r);
-        sm.rh[tid][1] = (T)(d.y / r);
-        sm.rh[tid][2] = (T)(d.z / r);
-        sm.w1[tid] = (tb.nch > 1) ? (T)wfac[nsp + itype[pair_idx[p0 + q0 + tid]]] : (T)0;
+        L.rh[tid * 3 + 0] = (T)(d.x / d.w);
+        L.rh[tid * 3 + 1] = (T)(d.y / d.w);
+        L.rh[tid * 3 + 2] = (T)(d.z / d.w);
+        L.sp[tid] = itype[pair_idx[p0 + q0 + tid]];
     }
-    for (int it = tid; it < nq * tb.LA2; it += NTH) {
-        const int q = it / tb.LA2, lm = it - q * tb.LA2;
-        int l = 0;
-        while ((l + 1) * (l + 1) <= lm) ++l;
-        const int m = lm - l * l - l;
+    const int nlm = (tb.LA + 1) * (tb.LA + 2) / 2;  // (l, m >= 0)
+    for (int it = tid; it < nq * nlm; it += NTH) {
+        const int q = it / nlm;
+        int k = it - q * nlm, l = 0;
+        while (k > l) {
+            k -= l + 1;
+            ++l;
+        }
+        const int m = k;
         const double4 d = pair_d[p0 + q0 + q];
         dcx Y, dY[3];
         ylm_dev(l, m, d.x, d.y, d.z, Y, dY);
-        sm.Y[q][lm] = cx2<T>{(T)Y.r, (T)Y.i};
-        for (int a = 0; a < 3; ++a) sm.dY[q][lm][a] = cx2<T>{(T)dY[a].r, (T)dY[a].i};
+        const int lm = l * l + l;
+        L.Y[q * tb.LA2 + lm + m] = cx2<T>{(T)Y.r, (T)Y.i};
+        for (int a = 0; a < 3; ++a) L.dY[(q * tb.LA2 + lm + m) * 3 + a] = cx2<T>{(T)dY[a].r, (T)dY[a].i};
+        if (m > 0) {
+            const double s = (m & 1) ? -1.0 : 1.0;
+            L.Y[q * tb.LA2 + lm - m] = cx2<T>{(T)(s * Y.r), (T)(-s * Y.i)};
+            for (int a = 0; a < 3; ++a)
+                L.dY[(q * tb.LA2 + lm - m) * 3 + a] = cx2<T>{(T)(s * dY[a].r), (T)(-s * dY[a].i)};
+        }
     }
 }
 
@@ -179,146 +188,175 @@ template <typename T>
 __device__ inline cx2<T> cmulT(cx2<T> a, cx2<T> b) { return cx2<T>{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
 
 template <typename T>
+__device__ inline cx2<T> term_product(const AceTerm& t, const cx2<T>* A)
+{
+    cx2<T> pr{(T)t.coef, (T)0};
+#pragma unroll
+    for (int i = 0; i < mld::ACE_NU_MAX; ++i)
+        if (t.e[i] >= 0) pr = cmulT(pr, A[t.e[i]]);
+    return pr;
+}
+
+template <typename T>
 __global__ void __launch_bounds__(NTH) ace_kernel(AceDev tb, int natoms, const int* __restrict__ pair_off,
                                                   const int* __restrict__ pair_idx, const double4* __restrict__ pair_d,
-                                                  const int* __restrict__ itype, const double* __restrict__ wfac, int nsp,
-                                                  double rcut, int want_grad, T* __restrict__ energy, T* __restrict__ grad)
+                                                  const int* __restrict__ itype, int want_grad, T* __restrict__ energy,
+                                                  T* __restrict__ grad)
 {
-    __shared__ AceSmem<T> sm;
+    extern __shared__ __align__(16) unsigned char ace_lds[];
+    const AceLds<T> L = carve<T>(tb, ace_lds);
     const int a = blockIdx.x;
     if (a >= natoms) return;
     const int tid = threadIdx.x;
     const int p0 = pair_off[a];
     const int nn = pair_off[a + 1] - p0;
-    const int D = tb.D;
-    const size_t base = (size_t)p0 + a;
-    if (nn == 0) {
-        for (int b = tid; b < D; b += NTH) energy[(size_t)a * D + b] = (T)0;
-        if (want_grad)
-            for (int q = tid; q < 3 * D; q += NTH) grad[base * 3 * D + q] = (T)0;
-        return;
+    const int D = tb.D, AD = tb.ace_dim, nsp = tb.nsp;
+    const int off = (tb.chem == 1) ? itype[a] * AD : 0;
+    const size_t base = (size_t)p0 + a;  // self slot
+    T* erow = energy + (size_t)a * D;
+    // zeros outside the centre species' block (everywhere for an atom without neighbours)
+    for (int c = tid; c < D; c += NTH)
+        if (nn == 0 || c < off || c >= off + AD) erow[c] = (T)0;
+    if (want_grad) {
+        const size_t nrow = (size_t)(nn + 1) * 3;
+        const int W = (nn == 0) ? D : D - AD;
+        T* g0 = grad + base * 3 * (size_t)D;
+        for (size_t it = tid; it < nrow * W; it += NTH) {
+            const size_t row = it / W;
+            const int k = (int)(it - row * W);
+            const int c = (nn == 0 || k < off) ? k : k + AD;
+            g0[row * D + c] = (T)0;
+        }
     }
+    if (nn == 0) return;
     // ---- A
-    for (int e = tid; e < tb.nA; e += NTH) sm.A[e] = cx2<T>{(T)0, (T)0};
+    for (int e = tid; e < tb.nA; e += NTH) L.A[e] = cx2<T>{(T)0, (T)0};
     for (int q0 = 0; q0 < nn; q0 += NBC) {
         const int nq = min(NBC, nn - q0);
         __syncthreads();
-        stage_chunk<T>(tb, sm, p0, q0, nq, pair_d, pair_idx, itype, wfac, nsp, rcut);
+        stage_chunk<T>(tb, L, p0, q0, nq, pair_d, pair_idx, itype);
         __syncthreads();
         for (int e = tid; e < tb.nA; e += NTH) {
-            const int lm = e % tb.LA2, cn = e / tb.LA2, n = cn % tb.NA, c = cn / tb.NA;
-            cx2<T> s = sm.A[e];
+            const int lm = e % tb.LA2, mn = e / tb.LA2, n = mn % tb.NR, mu = mn / tb.NR;
+            cx2<T> s = L.A[e];
             for (int q = 0; q < nq; ++q) {
-                const T f = (c == 0 ? (T)1 : sm.w1[q]) * sm.g[q][n];
-                s.r += f * sm.Y[q][lm].r;
-                s.i += f * sm.Y[q][lm].i;
+                if (L.sp[q] != mu) continue;
+                const T f = L.R[q * tb.NR + n];
+                const cx2<T> y = L.Y[q * tb.LA2 + lm];
+                s.r += f * y.r;
+                s.i += f * y.i;
             }
-            sm.A[e] = s;
+            L.A[e] = s;
         }
     }
     __syncthreads();
     // ---- B
-    for (int b = tid; b < D; b += NTH) {
-        cx2<T> s{(T)0, (T)0};
-        for (int t = tb.toff[b]; t < tb.toff[b + 1]; ++t) {
-            const AceTerm term = tb.terms[t];
-            cx2<T> pr = sm.A[term.e[0]];
-            if (term.e[1] >= 0) pr = cmulT(pr, sm.A[term.e[1]]);
-            if (term.e[2] >= 0) pr = cmulT(pr, sm.A[term.e[2]]);
-            if (term.e[3] >= 0) pr = cmulT(pr, sm.A[term.e[3]]);
-            s.r += (T)term.coef * pr.r;
-            s.i += (T)term.coef * pr.i;
-        }
-        energy[(size_t)a * D + b] = s.r;
+    for (int b = tid; b < AD; b += NTH) {
+        T s = (T)0;
+        for (int t = tb.toff[b]; t < tb.toff[b + 1]; ++t) s += term_product<T>(tb.terms[t], L.A).r;
+        erow[off + b] = s;
     }
     if (!want_grad) return;
-    // ---- dB/dr: G_s = dB_b / dA_e for the slots s (distinct A entries) of every b, once per
-    // atom; then per neighbour dB_b/dr_j = Re sum_s G_s dA_e(j), work items (neighbour, b)
-    extern __shared__ __align__(16) unsigned char ace_dyn[];
-    cx2<T>* Gs = reinterpret_cast<cx2<T>*>(ace_dyn);
-    int* sc = reinterpret_cast<int*>(ace_dyn + (size_t)tb.nslot * sizeof(cx2<T>));
-    for (int s = tid; s < tb.nslot; s += NTH) {
-        cx2<T> gsum{(T)0, (T)0};
-        for (int t = tb.dtoff[s]; t < tb.dtoff[s + 1]; ++t) {
-            const AceTerm dt = tb.dterms[t];
-            cx2<T> pr{(T)dt.coef, (T)0};
-            for (int i = 0; i < 3 && dt.e[i] >= 0; ++i) pr = cmulT(pr, sm.A[dt.e[i]]);
-            gsum.r += pr.r;
-            gsum.i += pr.i;
-        }
-        Gs[s] = gsum;
-        sc[s] = tb.scode[s];
-    }
-    for (int q0 = 0; q0 < nn; q0 += NBC) {
-        const int nq = min(NBC, nn - q0);
-        __syncthreads();
-        stage_chunk<T>(tb, sm, p0, q0, nq, pair_d, pair_idx, itype, wfac, nsp, rcut);
-        __syncthreads();
-        for (int it = tid; it < nq * D; it += NTH) {
-            const int q = it / D, b = it - q * D;
-            const T w1q = sm.w1[q];
-            const T r0 = sm.rh[q][0], r1 = sm.rh[q][1], r2 = sm.rh[q][2];
-            T d0 = (T)0, d1 = (T)0, d2 = (T)0;
-            for (int s = tb.soff[b]; s < tb.soff[b + 1]; ++s) {
-                const int code = sc[s];
-                const int c = code >> 16, n = (code >> 8) & 255, lm = code & 255;
-                const T wq = (c == 0) ? (T)1 : w1q;
-                const T gq = wq * sm.g[q][n], dgq = wq * sm.dg[q][n];
-                const cx2<T> G = Gs[s], Y = sm.Y[q][lm];
-                const cx2<T> dY0 = sm.dY[q][lm][0], dY1 = sm.dY[q][lm][1], dY2 = sm.dY[q][lm][2];
-                // Re(G dA_x), dA_x = dg rhat_x Y + g dY_x
-                const T gy = dgq * (G.r * Y.r - G.i * Y.i);
-                d0 += gy * r0 + gq * (G.r * dY0.r - G.i * dY0.i);
-                d1 += gy * r1 + gq * (G.r * dY1.r - G.i * dY1.i);
-                d2 += gy * r2 + gq * (G.r * dY2.r - G.i * dY2.i);
+    // ---- dB/dr, one tile of functions at a time
+    for (int tile = 0; tile < tb.ntile; ++tile) {
+        const int f0 = tb.tile_off[tile], nf = tb.tile_off[tile + 1] - f0;
+        const int s0 = tb.soff[(size_t)f0 * nsp], s1 = tb.soff[(size_t)(f0 + nf) * nsp];
+        __syncthreads();  // the previous tile's readers of G, code and self are done
+        for (int s = s0 + tid; s < s1; s += NTH) {
+            cx2<T> g{(T)0, (T)0};
+            for (int t = tb.dtoff[s]; t < tb.dtoff[s + 1]; ++t) {
+                const cx2<T> pr = term_product<T>(tb.dterms[t], L.A);
+                g.r += pr.r;
+                g.i += pr.i;
             }
-            T* out = grad + (base + 1 + q0 + q) * 3 * (size_t)D + b;
-            out[0] = d0;
-            out[D] = d1;
-            out[2 * (size_t)D] = d2;
+            L.G[s - s0] = g;
+            L.code[s - s0] = tb.scode[s];
+        }
+        // thread (grp, bl): function f0 + bl, neighbours grp, grp + ngrp, ... of every chunk
+        const int ngrp = NTH / nf;
+        const int grp = tid / nf, bl = tid - grp * nf;
+        const bool act = grp < ngrp;
+        const int b = f0 + bl;
+        T self0 = (T)0, self1 = (T)0, self2 = (T)0;
+        for (int q0 = 0; q0 < nn; q0 += NBC) {
+            const int nq = min(NBC, nn - q0);
+            __syncthreads();
+            stage_chunk<T>(tb, L, p0, q0, nq, pair_d, pair_idx, itype);
+            __syncthreads();
+            if (!act) continue;
+            for (int q = grp; q < nq; q += ngrp) {
+                const int mu = L.sp[q];
+                const int sb = tb.soff[(size_t)b * nsp + mu] - s0, se = tb.soff[(size_t)b * nsp + mu + 1] - s0;
+                const T r0 = L.rh[q * 3 + 0], r1 = L.rh[q * 3 + 1], r2 = L.rh[q * 3 + 2];
+                T d0 = (T)0, d1 = (T)0, d2 = (T)0;
+                for (int s = sb; s < se; ++s) {
+                    const int code = L.code[s];
+                    const int n = code >> 8, lm = code & 255;
+                    const T gq = L.R[q * tb.NR + n], dgq = L.dR[q * tb.NR + n];
+                    const cx2<T> G = L.G[s], Y = L.Y[q * tb.LA2 + lm];
+                    const cx2<T>* dY = L.dY + (q * tb.LA2 + lm) * 3;
+                    // Re(G dA_x), dA_x = dR r^_x Y + R dY_x
+                    const T gy = dgq * (G.r * Y.r - G.i * Y.i);
+                    d0 += gy * r0 + gq * (G.r * dY[0].r - G.i * dY[0].i);
+                    d1 += gy * r1 + gq * (G.r * dY[1].r - G.i * dY[1].i);
+                    d2 += gy * r2 + gq * (G.r * dY[2].r - G.i * dY[2].i);
+                }
+                T* out = grad + (base + 1 + q0 + q) * 3 * (size_t)D + off + b;
+                out[0] = d0;
+                out[D] = d1;
+                out[2 * (size_t)D] = d2;
+                self0 += d0;
+                self1 += d1;
+                self2 += d2;
+            }
+        }
+        if (act) {
+            L.self[(grp * nf + bl) * 3 + 0] = self0;
+            L.self[(grp * nf + bl) * 3 + 1] = self1;
+            L.self[(grp * nf + bl) * 3 + 2] = self2;
+        }
+        __syncthreads();
+        if (tid < nf) {
+            T acc0 = (T)0, acc1 = (T)0, acc2 = (T)0;
+            for (int gg = 0; gg < ngrp; ++gg) {
+                acc0 += L.self[(gg * nf + tid) * 3 + 0];
+                acc1 += L.self[(gg * nf + tid) * 3 + 1];
+                acc2 += L.self[(gg * nf + tid) * 3 + 2];
+            }
+            T* out = grad + base * 3 * (size_t)D + off + f0 + tid;
+            out[0] = -acc0;
+            out[D] = -acc1;
+            out[2 * (size_t)D] = -acc2;
         }
     }
-    // self slot = -sum of the neighbour slots, in neighbour order (the slots written above by
-    // this workgroup are visible after the barrier)
-    __syncthreads();
-    for (int b = tid; b < D; b += NTH)
-        for (int x = 0; x < 3; ++x) {
-            T acc = (T)0;
-            for (int q = 0; q < nn; ++q) acc -= grad[((base + 1 + q) * 3 + x) * (size_t)D + b];
-            grad[(base * 3 + x) * (size_t)D + b] = acc;
-        }
 }
 
-size_t ace_dyn_bytes(int nslot, size_t cxsize)
-{
-    return (size_t)nslot * (cxsize + sizeof(int)) + 16;
-}
+std::atomic<unsigned long long> g_lds_set_f64{0}, g_lds_set_f32{0};
 
 }  // namespace
 
-bool ace_tables_fit(int nslot)
+static size_t ace_lds_bytes(const mld_ctx* ctx)
 {
-    // FP64 workgroup: static tables + slot values and codes within the 160 KB of LDS of a CU
-    return sizeof(AceSmem<double>) + ace_dyn_bytes(nslot, sizeof(cx2<double>)) <= 160 * 1024;
+    const size_t rb = (ctx->p.precision == MLD_PREC_FP32) ? sizeof(float) : sizeof(double);
+    return mld::ace_lds_fixed(ctx->ace_h, rb) + (size_t)ctx->ace_d.slot_cap * mld::ace_lds_per_slot(rb);
 }
 
 int launch_ace(mld_ctx* ctx, int want_grad, void* energy, void* grad)
 {
     if (ctx->natoms == 0) return MLD_OK;
+    const int dyn = (int)ace_lds_bytes(ctx);
     if (ctx->p.precision == MLD_PREC_FP32) {
-        const size_t dyn = ace_dyn_bytes(ctx->ace_d.nslot, sizeof(cx2<float>));
-        MLD_HIP(ctx, hipFuncSetAttribute((const void*)ace_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)dyn));
+        int rc = set_lds_limit_once(ctx, (const void*)ace_kernel<float>, dyn, g_lds_set_f32);
+        if (rc != MLD_OK) return rc;
         hipLaunchKernelGGL(ace_kernel<float>, dim3(ctx->natoms), dim3(NTH), dyn, ctx->stream, ctx->ace_d, ctx->natoms,
-                           ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_itype, ctx->d_wfac, ctx->nsp,
-                           ctx->p.r_cut, want_grad, (float*)energy, (float*)grad);
+                           ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_itype, want_grad, (float*)energy,
+                           (float*)grad);
     } else {
-        const size_t dyn = ace_dyn_bytes(ctx->ace_d.nslot, sizeof(cx2<double>));
-        MLD_HIP(ctx, hipFuncSetAttribute((const void*)ace_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)dyn));
+        int rc = set_lds_limit_once(ctx, (const void*)ace_kernel<double>, dyn, g_lds_set_f64);
+        if (rc != MLD_OK) return rc;
         hipLaunchKernelGGL(ace_kernel<double>, dim3(ctx->natoms), dim3(NTH), dyn, ctx->stream, ctx->ace_d, ctx->natoms,
-                           ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_itype, ctx->d_wfac, ctx->nsp,
-                           ctx->p.r_cut, want_grad, (double*)energy, (double*)grad);
+                           ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_itype, want_grad, (double*)energy,
+                           (double*)grad);
     }
     MLD_HIP(ctx, hipGetLastError());
     return MLD_OK;

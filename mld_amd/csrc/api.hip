@@ -65,14 +65,48 @@ struct BatchStateGuard {
     }
 };
 
+namespace {
+
+// &input_ml ACE keys of the parameters; zeros select the reference's defaults where the C
+// struct cannot say "unset" (ace_radial_poly, ace_gencg, ace_npoints_spline, r_cut_width)
+void ace_config(const mld_params* p, mld::AceConfig& c)
+{
+    c.numax = p->ace_numax;
+    for (int v = 0; v < mld::ACE_NU_MAX; ++v) {
+        c.active[v] = p->ace_active[v] ? 1 : 0;
+        c.nmax[v] = p->ace_nmax[v];
+        c.lmax[v] = p->ace_lmax[v];
+    }
+    c.nsp = p->nspecies;
+    c.gencg = p->ace_gencg == 0 ? 2 : p->ace_gencg;
+    c.chem = p->ace_chem;
+    c.radial_poly = p->ace_radial_poly == 0 ? 2 : p->ace_radial_poly;
+    c.lam = p->ace_lambda;
+    c.rc = p->r_cut;
+    c.w_out = p->r_cut_width == 0.0 ? 0.5 : p->r_cut_width;
+    c.rin = p->ace_r_cut_in;
+    c.w_in = p->r_cut_width_in;
+    c.npoints = p->ace_npoints_spline == 0 ? 1000 : p->ace_npoints_spline;
+}
+
+}  // namespace
+
 extern "C" {
 
-const char* mld_last_error(const mld_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+// message of the last mld_create / mld_ace_basis_review failure of this thread (no context to hold it)
+static thread_local std::string t_setup_err;
+
+const char* mld_last_error(const mld_ctx* ctx)
+{
+    if (ctx) return ctx->err.c_str();
+    return t_setup_err.empty() ? "null context" : t_setup_err.c_str();
+}
 
 int mld_create(const mld_params* p, mld_ctx** out)
 {
     if (!p || !out) return MLD_E_ARG;
     *out = nullptr;
+    t_setup_err.clear();
     mld_ctx* ctx = new mld_ctx();
     ctx->p = *p;
     const bool is_ace = p->descriptor_type == MLD_DESC_ACE;
@@ -82,7 +116,7 @@ int mld_create(const mld_params* p, mld_ctx** out)
     }
     ctx->jj = is_ace ? 0 : (int)(2.0 * p->j_max);
     const bool bso4_ok = is_ace || (ctx->jj >= 1 && ctx->jj <= mld::JJ_MAX && p->inv_r0_input > 0.0 && p->inv_r0_input < 1.0);
-    const int nch_max = is_ace ? mld::ACE_NCH : mld::NCH_MAX;
+    const int nch_max = is_ace ? 1 : mld::NCH_MAX;
     if (!bso4_ok || !(p->r_cut > 0.0) || p->nchannel < 1 || p->nchannel > nch_max || p->nspecies < 1 ||
         (p->nchannel >= 2 && !p->weight) || (p->nchannel >= 3 && !p->weight_3ch)) {
         delete ctx;
@@ -102,6 +136,7 @@ int mld_create(const mld_params* p, mld_ctx** out)
     int rc = MLD_OK;
     auto fail = [&](int code, const std::string& m) {
         mld_fail(ctx, code, m);
+        t_setup_err = m;
         rc = code;
         return code;
     };
@@ -113,66 +148,90 @@ int mld_create(const mld_params* p, mld_ctx** out)
         return code;
     }
     if (is_ace) {
-        // basis tables (compute_ace_nlmu_tools.F90's role; this build's enumeration) and the
-        // species weights of the second chemical channel
-        if (p->ace_radial_poly != 0 && p->ace_radial_poly != 1 && p->ace_radial_poly != 2) {
-            fail(MLD_E_UNSUPPORTED, "ace_radial_poly: 1 (powTcheb) and 2 (expTcheb) are built");
+        // the reference's basis (base_cnlm, compute_ace_nlmu_tools.F90) and radial splines
+        // (build_ace_radial, compute_ace_AB.F90:240-330)
+        mld::AceConfig cfg;
+        std::string err;
+        ace_config(p, cfg);
+        int trc = mld::build_ace_tables(cfg, ctx->ace_h, err);
+        if (trc != MLD_OK) {
+            fail(trc, err);
             mld_destroy(ctx);
             return rc;
         }
-        int trc = mld::build_ace_tables(p->nchannel, p->ace_numax, p->ace_nmax, p->ace_lmax, p->ace_lambda, ctx->ace_h);
-        if (trc) {
-            fail(trc == 2 ? MLD_E_UNSUPPORTED : MLD_E_ARG,
-                 trc == 2 ? "ACE basis larger than 512 functions" : "ACE parameters out of range (numax 1..4, nmax 1..8, lmax 0..4)");
+        mld::AceHost& ah = ctx->ace_h;
+        ctx->D = ah.D;
+        // LDS: the fixed part plus the slots of the largest function tile, within 64 KB (two
+        // workgroups per CU) when the basis allows, else the 160 KB of a CU
+        const size_t rbytes = (p->precision == MLD_PREC_FP32) ? sizeof(float) : sizeof(double);
+        const size_t fixed = mld::ace_lds_fixed(ah, rbytes), per = mld::ace_lds_per_slot(rbytes);
+        int fn_slots = 0;
+        for (int b = 0; b < ah.ace_dim; ++b)
+            fn_slots = std::max(fn_slots, ah.soff[(size_t)(b + 1) * ah.nsp] - ah.soff[(size_t)b * ah.nsp]);
+        int cap = -1;
+        for (size_t budget : {(size_t)64 * 1024, (size_t)160 * 1024})
+            if (fixed < budget && (int)((budget - fixed) / per) >= fn_slots) {
+                cap = (int)((budget - fixed) / per);
+                break;
+            }
+        if (cap < 0) {
+            fail(MLD_E_UNSUPPORTED, "ACE basis too large for the descriptor kernel's LDS (" + std::to_string(fn_slots) +
+                                        " slots in one function, " + std::to_string(fixed) + " fixed bytes)");
             mld_destroy(ctx);
             return rc;
         }
-        ctx->D = ctx->ace_h.D;
-        if (!ace_tables_fit((int)ctx->ace_h.scode.size())) {
-            fail(MLD_E_UNSUPPORTED, "ACE basis too large for the gradient tables (" +
-                                        std::to_string(ctx->ace_h.scode.size()) + " A slots)");
+        mld::ace_make_tiles(ah, cap);
+        // one device allocation: terms, dterms, knots, splines (8-byte items), then the int tables
+        std::vector<std::pair<const void*, size_t>> parts = {
+            {ah.terms.data(), ah.terms.size() * sizeof(mld::AceTerm)},
+            {ah.dterms.data(), ah.dterms.size() * sizeof(mld::AceTerm)},
+            {ah.xg.data(), ah.xg.size() * sizeof(double)},
+            {ah.spl.data(), ah.spl.size() * sizeof(double)},
+            {ah.toff.data(), ah.toff.size() * sizeof(int)},
+            {ah.soff.data(), ah.soff.size() * sizeof(int)},
+            {ah.scode.data(), ah.scode.size() * sizeof(int)},
+            {ah.dtoff.data(), ah.dtoff.size() * sizeof(int)},
+            {ah.tile_off.data(), ah.tile_off.size() * sizeof(int)}};
+        std::vector<size_t> offs;
+        size_t total = 0;
+        for (auto& pt : parts) {
+            offs.push_back(total);
+            total += (pt.second + 255) & ~(size_t)255;
+        }
+        std::vector<char> pack(total, 0);
+        for (size_t i = 0; i < parts.size(); ++i)
+            if (parts[i].second) std::memcpy(pack.data() + offs[i], parts[i].first, parts[i].second);
+        if (hipMalloc(&ctx->d_ace_tables, total) != hipSuccess ||
+            hipMemcpy(ctx->d_ace_tables, pack.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
+            fail(MLD_E_DEVICE, "device allocation of ACE tables failed");
             mld_destroy(ctx);
             return rc;
         }
-        auto upa = [&](void** dst, const void* src, size_t bytes) -> bool {
-            if (hipMalloc(dst, bytes ? bytes : 8) != hipSuccess) return false;
-            if (bytes && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return false;
-            return true;
-        };
-        std::vector<double> wf((size_t)2 * ctx->nsp, 1.0);
-        if (ctx->nch >= 2)
-            for (int s2 = 0; s2 < ctx->nsp; ++s2) wf[ctx->nsp + s2] = ctx->h_weight[s2];
-        const mld::AceHost& ah = ctx->ace_h;
-        // gradient tables packed: dterms (8-byte aligned) first, then soff, scode, dtoff
-        std::vector<char> gpack(ah.dterms.size() * sizeof(mld::AceTerm) +
-                                (ah.soff.size() + ah.scode.size() + ah.dtoff.size()) * sizeof(int));
-        size_t o_dt = 0, o_so = o_dt + ah.dterms.size() * sizeof(mld::AceTerm);
-        size_t o_sc = o_so + ah.soff.size() * sizeof(int), o_do = o_sc + ah.scode.size() * sizeof(int);
-        std::memcpy(gpack.data() + o_dt, ah.dterms.data(), ah.dterms.size() * sizeof(mld::AceTerm));
-        std::memcpy(gpack.data() + o_so, ah.soff.data(), ah.soff.size() * sizeof(int));
-        std::memcpy(gpack.data() + o_sc, ah.scode.data(), ah.scode.size() * sizeof(int));
-        std::memcpy(gpack.data() + o_do, ah.dtoff.data(), ah.dtoff.size() * sizeof(int));
-        bool ok = upa(&ctx->d_ace_terms, ctx->ace_h.terms.data(), ctx->ace_h.terms.size() * sizeof(mld::AceTerm)) &&
-                  upa(&ctx->d_ace_toff, ctx->ace_h.toff.data(), ctx->ace_h.toff.size() * sizeof(int)) &&
-                  upa(&ctx->d_ace_grad, gpack.data(), gpack.size()) &&
-                  upa((void**)&ctx->d_wfac, wf.data(), wf.size() * sizeof(double));
-        if (!ok) { fail(MLD_E_DEVICE, "device allocation of ACE tables failed"); mld_destroy(ctx); return rc; }
+        const char* base = (const char*)ctx->d_ace_tables;
         mld::AceDev& ad = ctx->ace_d;
-        ad.terms = (const mld::AceTerm*)ctx->d_ace_terms;
-        ad.toff = (const int*)ctx->d_ace_toff;
-        ad.dterms = (const mld::AceTerm*)((const char*)ctx->d_ace_grad + o_dt);
-        ad.soff = (const int*)((const char*)ctx->d_ace_grad + o_so);
-        ad.scode = (const int*)((const char*)ctx->d_ace_grad + o_sc);
-        ad.dtoff = (const int*)((const char*)ctx->d_ace_grad + o_do);
-        ad.nslot = (int)ah.scode.size();
-        ad.D = ctx->ace_h.D;
-        ad.nch = ctx->ace_h.nch;
-        ad.NA = ctx->ace_h.NA;
-        ad.LA = ctx->ace_h.LA;
-        ad.LA2 = ctx->ace_h.LA2;
-        ad.nA = ctx->ace_h.nA;
-        ad.lam = ctx->ace_h.lam;
-        ad.radial = p->ace_radial_poly == 0 ? 2 : p->ace_radial_poly;
+        ad.terms = (const mld::AceTerm*)(base + offs[0]);
+        ad.dterms = (const mld::AceTerm*)(base + offs[1]);
+        ad.xg = (const double*)(base + offs[2]);
+        ad.spl = (const double*)(base + offs[3]);
+        ad.toff = (const int*)(base + offs[4]);
+        ad.soff = (const int*)(base + offs[5]);
+        ad.scode = (const int*)(base + offs[6]);
+        ad.dtoff = (const int*)(base + offs[7]);
+        ad.tile_off = (const int*)(base + offs[8]);
+        ad.nsp = ah.nsp;
+        ad.NR = ah.NR;
+        ad.LA = ah.LA;
+        ad.LA2 = ah.LA2;
+        ad.nA = ah.nA;
+        ad.ace_dim = ah.ace_dim;
+        ad.D = ah.D;
+        ad.chem = ah.chem;
+        ad.npts = ah.npts;
+        ad.ntile = (int)ah.tile_off.size() - 1;
+        ad.slot_cap = ah.max_tile_slots;
+        ad.rin = ah.rin;
+        ad.rc = ah.rc;
+        ad.delta = ah.delta;
         *out = ctx;
         return MLD_OK;
     }
@@ -244,7 +303,7 @@ int mld_destroy(mld_ctx* ctx)
     gram_scratch_free(ctx);
     kept_rows_free(ctx);
     predict_free(ctx);
-    void* ptrs[] = {ctx->d_ace_terms, ctx->d_ace_toff, ctx->d_ace_grad, ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
+    void* ptrs[] = {ctx->d_ace_tables, ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
                     ctx->d_geo, ctx->d_count, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_scan_tmp,
                     ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_pgeo, ctx->d_etgt, ctx->d_zref, ctx->d_zz, ctx->d_zref32, ctx->d_kiz};
     for (void* q : ptrs)
@@ -258,6 +317,26 @@ int mld_descriptor_dim(const mld_ctx* ctx, int* D)
 {
     if (!ctx || !D) return MLD_E_ARG;
     *D = ctx->D;
+    return MLD_OK;
+}
+
+int mld_ace_basis_review(const mld_params* p, int review[42], int* ace_dim, int* D)
+{
+    if (!p || !review || p->descriptor_type != MLD_DESC_ACE) return MLD_E_ARG;
+    t_setup_err.clear();
+    mld::AceConfig cfg;
+    ace_config(p, cfg);
+    mld::AceHost t;
+    std::string err;
+    const int rc = mld::build_ace_tables(cfg, t, err);
+    if (rc != MLD_OK) {
+        t_setup_err = err;
+        return rc;
+    }
+    for (int v = 0; v < mld::ACE_NU_MAX; ++v)
+        for (int k = 0; k < mld::ACE_REVIEW_COLS; ++k) review[v * mld::ACE_REVIEW_COLS + k] = t.review[v][k];
+    if (ace_dim) *ace_dim = t.ace_dim;
+    if (D) *D = t.D;
     return MLD_OK;
 }
 

@@ -100,6 +100,8 @@ __host__ __device__ inline int rows_force_ch(int D) { return D > 192 ? 4 : 16; }
 
 // threads of rows_force_kernel: one 4 x 4 quadratic block each (3 * 136 blocks at D = 62)
 constexpr int RF_THREADS = 512;
+// widest linear descriptor of rows_force_kernel: its 3 D linear sums sit in registers, NLIN per thread
+constexpr int RF_LINEAR_MAX = 512;
 
 // LDS of rows_force_kernel: one staged output row (doubles), then the chunk's gradients and
 // descriptors in the descriptor precision (tsize bytes)
@@ -145,8 +147,8 @@ __global__ void __launch_bounds__(RF_THREADS) rows_force_kernel(int order, int D
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-    // linear items: (x, d) = tid + RF_THREADS v over 3 * D (D <= 512)
-    constexpr int NLIN = 3 * 512 / RF_THREADS;
+    // linear items: (x, d) = tid + RF_THREADS v over 3 * D (D <= RF_LINEAR_MAX)
+    constexpr int NLIN = 3 * RF_LINEAR_MAX / RF_THREADS;
     double lin[NLIN];
 #pragma unroll
     for (int v = 0; v < NLIN; ++v) lin[v] = 0.0;
@@ -974,7 +976,18 @@ int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_de
         *M_out = M;
         return MLD_OK;
     }
-    if (natoms) {
+    if (natoms && order == 1 && D > RF_LINEAR_MAX) {
+        // wide linear rows (e.g. ACE_CHEM_STANDARD bases): the kernel-model row kernel with no
+        // kernel features sums the same slots in the same order, straight from the gradients
+        if (f32)
+            hipLaunchKernelGGL(rows_force_krr_kernel<float>, dim3(natoms), dim3(256), 0, st, D, 0, ldA, ctx->d_atom_struct,
+                               ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src, (const float*)ctx->last_grad,
+                               (const float*)nullptr, F_dev, swf, A, y);
+        else
+            hipLaunchKernelGGL(rows_force_krr_kernel<double>, dim3(natoms), dim3(256), 0, st, D, 0, ldA, ctx->d_atom_struct,
+                               ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src, (const double*)ctx->last_grad,
+                               (const double*)nullptr, F_dev, swf, A, y);
+    } else if (natoms) {
         const size_t fsm = rows_force_smem(D, ldA, f32 ? sizeof(float) : sizeof(double));
         // the largest layout (D = 192 linear, D = 62 quadratic) is < 100 KB
         static std::atomic<unsigned long long> attr_f32{0}, attr_f64{0};
@@ -1115,7 +1128,6 @@ int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F
     if (Dc != ctx->gD)
         return mld_fail(ctx, MLD_E_ARG, "mld_gram_begin D must be 1 + D (+ D(D+1)/2 for order 2, + nk for order 3)");
     if (order == 2 && Dc > 2048) return mld_fail(ctx, MLD_E_UNSUPPORTED, "quadratic rows are built for at most 2048 columns");
-    if (order == 1 && D > 512) return mld_fail(ctx, MLD_E_UNSUPPORTED, "linear rows are built for descriptors of at most 512 components");
     ctx->gram_order = order;
     if (!ctx->last_energy || !ctx->last_grad) return mld_fail(ctx, MLD_E_STATE, "mld_batch_bso4_dev with gradients first");
     if (!E || (ctx->natoms && !F_dev)) return mld_fail(ctx, MLD_E_ARG, "targets missing");

@@ -113,37 +113,59 @@ struct StructGeo {
 };
 void struct_geometry(const double* cell_colmajor, double rcut, StructGeo& g);
 
-// Wigner 3j symbol (j1 j2 j3; m1 m2 m3) with integer arguments (0 outside the selection rules)
-double wigner3j(int j1, int m1, int j2, int m2, int j3, int m3);
 
-// ---- ACE (atomic cluster expansion) tables ----
-constexpr int ACE_NMAX = 8;     // radial functions per nu
-constexpr int ACE_LMAX = 4;     // l of Y_lm
-constexpr int ACE_LM = (ACE_LMAX + 1) * (ACE_LMAX + 1);
-constexpr int ACE_NCH = 2;      // chemical channels: every neighbour / species-weighted
-constexpr int ACE_DMAX = 512;   // basis functions per atom (two per thread of a 256-thread block)
+// ---- ACE (atomic cluster expansion, descriptor_type 300) tables ----
+constexpr int ACE_NU_MAX = 6;   // body orders nu = 1 .. 6
+constexpr int ACE_LMAX = 6;     // l of Y_lm
+constexpr int ACE_NR_MAX = 32;  // radial functions n = 0 .. ACE_NR_MAX - 1
+constexpr int ACE_NBC = 16;     // neighbours per LDS chunk of the descriptor kernel
+constexpr int ACE_NTH = 256;    // threads of the descriptor kernel's workgroup (one per atom)
+constexpr int ACE_FTILE = 256;  // basis functions per gradient tile (at most one per thread)
 
-// one product term of a basis function: coef * prod_i A[e_i] (e = -1: unused); the A index is
-// e = (c * NA + n) * LA2 + l * l + l + m
+// one product term: coef * prod_i A[e_i] (e = -1: unused), A index
+// e = (mu * NR + n) * LA2 + l * l + l + m
 struct AceTerm {
     double coef;
-    int e[4];
+    int e[ACE_NU_MAX];
 };
 
-// Gradient tables: basis function b depends on the distinct A entries ("slots")
-// soff[b] .. soff[b+1]; slot s has code (c << 16) | (n << 8) | lm and the terms of
-// dB_b / dA_e (coef * prod of the other factors, unused factors -1) dtoff[s] .. dtoff[s+1].
+// &input_ml keys of the ACE descriptor (module_ace_desc, read_ml_file.F90:278-290)
+struct AceConfig {
+    int numax = 3;
+    int active[ACE_NU_MAX] = {0, 0, 0, 0, 0, 0};  // l_ace_order(nu)
+    int nmax[ACE_NU_MAX] = {0, 0, 0, 0, 0, 0};
+    int lmax[ACE_NU_MAX] = {0, 0, 0, 0, 0, 0};
+    int nsp = 1;          // mu_max: chemical species
+    int gencg = 2;        // 1 RI (GENCG_DRAFT), 2 RPI (GENCG_DUSORT)
+    int chem = 1;         // 0 ACE_CHEM_INCOMPLETE, 1 ACE_CHEM_STANDARD
+    int radial_poly = 2;  // 1 powTcheb, 2 expTcheb
+    double lam = 3.0;     // ace_lambda_list(1)
+    double rc = 5.0, w_out = 0.5, rin = -1.2, w_in = 0.4;
+    int npoints = 1000;   // ace_npoints_spline
+};
+
+// Per-order review: dim_mu, dim_n, dim_l0, dim_Lb, dim_Li, dim_nl0, dim_B (print_ace_descriptor,
+// compute_ace_nlmu_tools.F90:2620-2680); -1 for an inactive order.
+constexpr int ACE_REVIEW_COLS = 7;
+
+// Basis function b (0 .. ace_dim - 1) has the terms toff[b] .. toff[b+1].  Gradient tables:
+// the distinct A entries ("slots") of b with species mu are soff[b * nsp + mu] .. soff[b * nsp + mu + 1];
+// slot s has code (n << 8) | lm and the terms of dB_b / dA_e (coef * prod of the other factors)
+// dtoff[s] .. dtoff[s+1].  tile_off: function tiles whose slots fit the kernel's LDS.
 struct AceHost {
-    int nch = 1, numax = 3, NA = 0, LA = 0, LA2 = 0, nA = 0, D = 0;
-    int nmax[4] = {0, 0, 0, 0}, lmax[4] = {0, 0, 0, 0};
-    double lam = 2.0;
-    std::vector<int> toff;   // [D + 1]
-    std::vector<int> nu;     // [D]
+    int nsp = 1, NR = 0, LA = 0, LA2 = 1, nA = 0, ace_dim = 0, D = 0, chem = 1, npts = 0;
+    double rin = 0.0, rc = 0.0, delta = 0.0;
+    std::vector<double> xg;   // [npts] spline knots
+    std::vector<double> spl;  // [NR][npts][4] clamped cubic spline coefficients a, b, c, d
+    std::vector<int> toff;    // [ace_dim + 1]
     std::vector<AceTerm> terms;
-    std::vector<int> soff;   // [D + 1]
-    std::vector<int> scode;  // [nslot]
-    std::vector<int> dtoff;  // [nslot + 1]
+    std::vector<int> soff;    // [ace_dim * nsp + 1]
+    std::vector<int> scode;   // [nslot]
+    std::vector<int> dtoff;   // [nslot + 1]
     std::vector<AceTerm> dterms;
+    std::vector<int> tile_off;  // [ntile + 1] function boundaries
+    int max_tile_slots = 0;
+    int review[ACE_NU_MAX][ACE_REVIEW_COLS];
 };
 
 struct AceDev {
@@ -153,14 +175,21 @@ struct AceDev {
     const int* scode;
     const int* dtoff;
     const AceTerm* dterms;
-    int D, nch, NA, LA, LA2, nA, nslot;
-    double lam;
-    int radial;  // 1 powTcheb, 2 expTcheb
+    const int* tile_off;
+    const double* xg;
+    const double* spl;
+    int nsp, NR, LA, LA2, nA, ace_dim, D, chem, npts, ntile, slot_cap;
+    double rin, rc, delta;
 };
 
-// basis enumeration (nu-major; k1 <= k2 <= ... over keys (c, n); then l1, l2, ...; then the
-// intermediate L of nu = 4)
-int build_ace_tables(int nch, int numax, const int* nmax, const int* lmax, double lam, AceHost& t);
+// Builds the reference's basis (enumeration, RI / RPI coefficients) and the radial splines.
+// Returns MLD_OK or an MLD_E_* code with err set.
+int build_ace_tables(const AceConfig& cfg, AceHost& t, std::string& err);
+// Cuts the functions into tiles whose slots fit slot_cap (and at most ACE_FTILE functions).
+void ace_make_tiles(AceHost& t, int slot_cap);
+// LDS bytes of the descriptor kernel (fixed part, per slot) for the element size of its precision
+size_t ace_lds_fixed(const AceHost& t, size_t real_bytes);
+size_t ace_lds_per_slot(size_t real_bytes);
 
 }  // namespace mld
 
@@ -228,9 +257,7 @@ struct mld_ctx {
     // ACE contexts (descriptor_type MLD_DESC_ACE)
     mld::AceHost ace_h;
     mld::AceDev ace_d{};
-    void* d_ace_terms = nullptr;
-    void* d_ace_toff = nullptr;
-    void* d_ace_grad = nullptr;  // soff, scode, dtoff, dterms in one allocation
+    void* d_ace_tables = nullptr;  // terms, dterms, splines and the int tables in one allocation
 };
 
 int mld_fail(mld_ctx* ctx, int code, const std::string& msg);
@@ -256,8 +283,6 @@ inline int set_lds_limit_once(mld_ctx* ctx, const void* fn, int bytes, std::atom
 int launch_neighbours_pos(mld_ctx* ctx, const double* pos);
 int launch_bso4(mld_ctx* ctx, int want_grad, void* energy, void* grad);
 int launch_ace(mld_ctx* ctx, int want_grad, void* energy, void* grad);
-// the ACE gradient tables (slots) fit the LDS of one workgroup
-bool ace_tables_fit(int nslot);
 // ncg_level: largest CG1 range of one output level (cgbase[j+1] - cgbase[j]); ncg: all tables
 bool bso4_tables_fit(int jj, int ncg_level, int ncg, int wsize);
 int launch_to_reference_layout(mld_ctx* ctx, int nat, int nn_max, const void* grad, double* force_dev,

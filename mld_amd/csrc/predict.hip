@@ -19,7 +19,8 @@
 
 namespace {
 
-constexpr int DMAX = mld::ACE_DMAX;  // 512 = 8 values per lane
+constexpr int DREG = 512;    // slot_dot_kernel: g_a held in registers, 8 values per lane
+constexpr int DMAX = 8192;   // atom_kernel: B_a and B_a - mu in 2 D doubles of LDS (128 KB)
 
 struct PredictState {
     int order = 0, ncol = 0, D = 0;
@@ -41,8 +42,8 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// one block per atom, thread t owning columns d = t and t + 256 (D <= 512): g_a, e_a
-// (+ reference energy) and d2_a
+// one block per atom, thread t owning columns d = t, t + 256, ...: g_a, e_a (+ reference
+// energy) and d2_a; B_a and B_a - mu in dynamic LDS (2 D doubles)
 template <typename T>
 __global__ void __launch_bounds__(256) atom_kernel(int natoms, int D, const T* __restrict__ energy,
                                                    const int* __restrict__ itype, const double* __restrict__ w,
@@ -51,8 +52,9 @@ __global__ void __launch_bounds__(256) atom_kernel(int natoms, int D, const T* _
                                                    double* __restrict__ g, double* __restrict__ e_atom,
                                                    double* __restrict__ d2)
 {
-    __shared__ double sB[DMAX];
-    __shared__ double sV[DMAX];
+    extern __shared__ double atom_lds[];
+    double* sB = atom_lds;
+    double* sV = atom_lds + D;
     __shared__ double red[2][4];
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int a = blockIdx.x;
@@ -103,9 +105,9 @@ __global__ void __launch_bounds__(256) slot_dot_kernel(int natoms, int D, const 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int a = blockIdx.x;
     if (a >= natoms) return;
-    double gr[DMAX / 64];
+    double gr[DREG / 64];
 #pragma unroll
-    for (int u = 0; u < DMAX / 64; ++u) {
+    for (int u = 0; u < DREG / 64; ++u) {
         int d = lane + 64 * u;
         gr[u] = d < D ? g[(size_t)a * D + d] : 0.0;
     }
@@ -119,7 +121,7 @@ __global__ void __launch_bounds__(256) slot_dot_kernel(int natoms, int D, const 
             if (r + k < nrow) {
                 const T* src = grad + (s0 * 3 + r + k) * D;
 #pragma unroll
-                for (int u = 0; u < DMAX / 64; ++u) {
+                for (int u = 0; u < DREG / 64; ++u) {
                     const int d = lane + 64 * u;
                     if (d < D) v[k] += gr[u] * (double)src[d];
                 }
@@ -132,6 +134,39 @@ __global__ void __launch_bounds__(256) slot_dot_kernel(int natoms, int D, const 
         if (lane < 4 && r + lane < nrow) s[s0 * 3 + r + lane] = lane == 0 ? v[0] : lane == 1 ? v[1] : lane == 2 ? v[2] : v[3];
     }
 }
+
+// slot_dot_kernel for D > 512: g_a in dynamic LDS (D doubles), four rows per wavefront at a time
+template <typename T>
+__global__ void __launch_bounds__(256) slot_dot_wide_kernel(int natoms, int D, const int* __restrict__ pair_off,
+                                                            const double* __restrict__ g, const T* __restrict__ grad,
+                                                            double* __restrict__ s)
+{
+    extern __shared__ double sg[];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int a = blockIdx.x;
+    if (a >= natoms) return;
+    for (int d = threadIdx.x; d < D; d += 256) sg[d] = g[(size_t)a * D + d];
+    __syncthreads();
+    const size_t s0 = (size_t)pair_off[a] + a;
+    const int nrow = 3 * (pair_off[a + 1] - pair_off[a] + 1);
+    for (int r = 4 * wv; r < nrow; r += 16) {
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        const int nk = min(4, nrow - r);
+        for (int d = lane; d < D; d += 64) {
+            const double gd = sg[d];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k < nk) v[k] += gd * (double)grad[(s0 * 3 + r + k) * D + d];
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] += __shfl_xor(v[k], o, 64);
+        if (lane < nk) s[s0 * 3 + r + lane] = lane == 0 ? v[0] : lane == 1 ? v[1] : lane == 2 ? v[2] : v[3];
+    }
+}
+
+std::atomic<unsigned long long> g_atom_lds_f64{0}, g_atom_lds_f32{0}, g_wide_lds_f64{0}, g_wide_lds_f32{0};
 
 // kernel part of a kernel model: v[r] += wk . feat[r] for rows r of nk features (one wave per row)
 template <typename T>
@@ -318,7 +353,7 @@ int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev)
         return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: forces need descriptors with gradients");
     if (d2_dev && !ps->d_mu) return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: mld_mahalanobis_set first");
     if (ps->D != ctx->D) return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: model dimension changed");
-    if (ctx->D > DMAX) return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_predict_dev: descriptor dimension above 512");
+    if (ctx->D > DMAX) return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_predict_dev: descriptor dimension above 8192");
     if (ps->order == 3 && (ps->nk != ctx->nk || !ctx->last_ek || (F_dev && !ctx->last_fk)))
         return mld_fail(ctx, MLD_E_STATE, "mld_predict_dev: kernel model needs mld_kernel_po_dev features of this batch");
     const int natoms = ctx->natoms, nstruct = ctx->nstruct, D = ctx->D;
@@ -335,13 +370,19 @@ int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev)
     const bool f32 = ctx->p.precision == MLD_PREC_FP32;
     const double* mu = d2_dev ? ps->d_mu : nullptr;
     const int nblk = (natoms + 3) / 4;
+    const int atom_lds = (int)(2 * (size_t)D * sizeof(double));
     if (natoms) {
-        if (f32)
-            hipLaunchKernelGGL(atom_kernel<float>, dim3(natoms), dim3(256), 0, st, natoms, D, (const float*)ctx->last_energy,
-                               ctx->d_itype, ps->d_w, ps->d_U, ps->d_ref, mu, ps->d_sinv, g, e_atom, d2_dev);
-        else
-            hipLaunchKernelGGL(atom_kernel<double>, dim3(natoms), dim3(256), 0, st, natoms, D, (const double*)ctx->last_energy,
-                               ctx->d_itype, ps->d_w, ps->d_U, ps->d_ref, mu, ps->d_sinv, g, e_atom, d2_dev);
+        if (f32) {
+            if ((rc = set_lds_limit_once(ctx, (const void*)atom_kernel<float>, atom_lds, g_atom_lds_f32))) return rc;
+            hipLaunchKernelGGL(atom_kernel<float>, dim3(natoms), dim3(256), atom_lds, st, natoms, D,
+                               (const float*)ctx->last_energy, ctx->d_itype, ps->d_w, ps->d_U, ps->d_ref, mu, ps->d_sinv,
+                               g, e_atom, d2_dev);
+        } else {
+            if ((rc = set_lds_limit_once(ctx, (const void*)atom_kernel<double>, atom_lds, g_atom_lds_f64))) return rc;
+            hipLaunchKernelGGL(atom_kernel<double>, dim3(natoms), dim3(256), atom_lds, st, natoms, D,
+                               (const double*)ctx->last_energy, ctx->d_itype, ps->d_w, ps->d_U, ps->d_ref, mu, ps->d_sinv,
+                               g, e_atom, d2_dev);
+        }
     }
     if (ps->order == 3 && natoms) {
         if (f32)
@@ -354,12 +395,24 @@ int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev)
     if (E_dev && nstruct)
         hipLaunchKernelGGL(energy_kernel, dim3(nstruct), dim3(256), 0, st, ctx->d_atom_off, ps->w0, e_atom, E_dev);
     if (F_dev && natoms) {
-        if (f32)
+        const int wide_lds = (int)((size_t)D * sizeof(double));
+        if (D <= DREG && f32)
             hipLaunchKernelGGL(slot_dot_kernel<float>, dim3(natoms), dim3(256), 0, st, natoms, D, ctx->d_pair_off, g,
                                (const float*)ctx->last_grad, s);
-        else
+        else if (D <= DREG)
             hipLaunchKernelGGL(slot_dot_kernel<double>, dim3(natoms), dim3(256), 0, st, natoms, D, ctx->d_pair_off, g,
                                (const double*)ctx->last_grad, s);
+        else if (f32) {
+            if ((rc = set_lds_limit_once(ctx, (const void*)slot_dot_wide_kernel<float>, wide_lds, g_wide_lds_f32)))
+                return rc;
+            hipLaunchKernelGGL(slot_dot_wide_kernel<float>, dim3(natoms), dim3(256), wide_lds, st, natoms, D,
+                               ctx->d_pair_off, g, (const float*)ctx->last_grad, s);
+        } else {
+            if ((rc = set_lds_limit_once(ctx, (const void*)slot_dot_wide_kernel<double>, wide_lds, g_wide_lds_f64)))
+                return rc;
+            hipLaunchKernelGGL(slot_dot_wide_kernel<double>, dim3(natoms), dim3(256), wide_lds, st, natoms, D,
+                               ctx->d_pair_off, g, (const double*)ctx->last_grad, s);
+        }
         if (ps->order == 3) {
             const long long nrow = 3 * (P + natoms);
             const unsigned nb2 = (unsigned)((nrow + 3) / 4);

@@ -176,167 +176,6 @@ void build_tables(int jj, int diag, HostTables& t)
     for (int j = jj + 1; j <= JJ_MAX + 1; ++j) t.tbase[j] = (int)t.tasks.size();
 }
 
-double wigner3j(int j1, int m1, int j2, int m2, int j3, int m3)
-{
-    if (m1 + m2 + m3 != 0) return 0.0;
-    if (j3 > j1 + j2 || j3 < std::abs(j1 - j2)) return 0.0;
-    if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(m3) > j3) return 0.0;
-    return w3j(2 * j1, 2 * m1, 2 * j2, 2 * m2, 2 * j3, 2 * m3);
-}
-
-int build_ace_tables(int nch, int numax, const int* nmax, const int* lmax, double lam, AceHost& t)
-{
-    t = AceHost();
-    if (nch < 1 || nch > ACE_NCH || numax < 1 || numax > 4) return 1;
-    t.nch = nch;
-    t.numax = numax;
-    t.lam = lam;
-    for (int v = 0; v < numax; ++v) {
-        if (nmax[v] < 1 || nmax[v] > ACE_NMAX || lmax[v] < 0 || lmax[v] > ACE_LMAX) return 1;
-        t.nmax[v] = nmax[v];
-        t.lmax[v] = (v == 0) ? 0 : lmax[v];
-        t.NA = std::max(t.NA, nmax[v]);
-        t.LA = std::max(t.LA, t.lmax[v]);
-    }
-    t.LA2 = (t.LA + 1) * (t.LA + 1);
-    t.nA = nch * t.NA * t.LA2;
-    auto eidx = [&](int c, int n, int l, int m) { return (c * t.NA + n) * t.LA2 + l * l + l + m; };
-    auto keys = [&](int v) {
-        std::vector<std::pair<int, int>> k;
-        for (int c = 0; c < nch; ++c)
-            for (int n = 0; n < t.nmax[v]; ++n) k.push_back({c, n});
-        return k;
-    };
-    t.toff.push_back(0);
-    auto close = [&](int nu) {
-        t.nu.push_back(nu);
-        t.toff.push_back((int)t.terms.size());
-    };
-    if (numax >= 1)
-        for (auto k : keys(0)) {
-            t.terms.push_back(AceTerm{1.0, {eidx(k.first, k.second, 0, 0), -1, -1, -1}});
-            close(1);
-        }
-    if (numax >= 2) {
-        auto K = keys(1);
-        for (size_t i1 = 0; i1 < K.size(); ++i1)
-            for (size_t i2 = i1; i2 < K.size(); ++i2)
-                for (int l = 0; l <= t.lmax[1]; ++l) {
-                    for (int m = -l; m <= l; ++m)
-                        t.terms.push_back(AceTerm{(m & 1) ? -1.0 : 1.0,
-                                                  {eidx(K[i1].first, K[i1].second, l, m), eidx(K[i2].first, K[i2].second, l, -m), -1, -1}});
-                    close(2);
-                }
-    }
-    if (numax >= 3) {
-        auto K = keys(2);
-        const int L = t.lmax[2];
-        const int nk = (int)K.size();
-        for (int i1 = 0; i1 < nk; ++i1)
-            for (int i2 = i1; i2 < nk; ++i2)
-                for (int i3 = i2; i3 < nk; ++i3)
-                    for (int l1 = 0; l1 <= L; ++l1)
-                        for (int l2 = 0; l2 <= L; ++l2)
-                            for (int l3 = std::abs(l1 - l2); l3 <= std::min(L, l1 + l2); ++l3) {
-                                if ((l1 + l2 + l3) & 1) continue;
-                                // (i1, l1) <= (i2, l2) <= (i3, l3) lexicographically
-                                auto le = [](int a, int la, int b, int lb) { return a < b || (a == b && la <= lb); };
-                                if (!le(i1, l1, i2, l2) || !le(i2, l2, i3, l3)) continue;
-                                for (int m1 = -l1; m1 <= l1; ++m1)
-                                    for (int m2 = -l2; m2 <= l2; ++m2) {
-                                        const int m3 = -m1 - m2;
-                                        if (std::abs(m3) > l3) continue;
-                                        const double w = wigner3j(l1, m1, l2, m2, l3, m3);
-                                        if (w == 0.0) continue;
-                                        t.terms.push_back(AceTerm{w, {eidx(K[i1].first, K[i1].second, l1, m1),
-                                                                      eidx(K[i2].first, K[i2].second, l2, m2),
-                                                                      eidx(K[i3].first, K[i3].second, l3, m3), -1}});
-                                    }
-                                close(3);
-                            }
-    }
-    if (numax >= 4) {
-        // pairs (k1 l1)(k2 l2) and (k3 l3)(k4 l4) coupled to L:
-        // sum_M (-1)^(L-M) (l1 l2 L; m1 m2 -M) (l3 l4 L; m3 m4 M) A1 A2 A3 A4
-        auto K = keys(3);
-        const int L = t.lmax[3];
-        const int nk = (int)K.size();
-        auto lt2 = [](int a, int la, int b, int lb) { return a < b || (a == b && la <= lb); };
-        for (int i1 = 0; i1 < nk; ++i1)
-            for (int i2 = i1; i2 < nk; ++i2)
-                for (int i3 = i1; i3 < nk; ++i3)
-                    for (int i4 = i3; i4 < nk; ++i4)
-                        for (int l1 = 0; l1 <= L; ++l1)
-                            for (int l2 = 0; l2 <= L; ++l2)
-                                for (int l3 = 0; l3 <= L; ++l3)
-                                    for (int l4 = 0; l4 <= L; ++l4) {
-                                        if ((l1 + l2 + l3 + l4) & 1) continue;
-                                        if (!lt2(i1, l1, i2, l2) || !lt2(i3, l3, i4, l4)) continue;
-                                        // (i1, l1, i2, l2) <= (i3, l3, i4, l4) lexicographically
-                                        const int ka[4] = {i1, l1, i2, l2}, kb[4] = {i3, l3, i4, l4};
-                                        int cmp = 0;
-                                        for (int z = 0; z < 4 && !cmp; ++z) cmp = (ka[z] < kb[z]) ? -1 : (ka[z] > kb[z] ? 1 : 0);
-                                        if (cmp > 0) continue;
-                                        const int lo = std::max(std::abs(l1 - l2), std::abs(l3 - l4));
-                                        const int hi = std::min(l1 + l2, l3 + l4);
-                                        for (int Lc = lo; Lc <= hi; ++Lc) {
-                                            // a coupled pair of equal (k, l) has no odd-L part
-                                            if (i1 == i2 && l1 == l2 && (Lc & 1)) continue;
-                                            if (i3 == i4 && l3 == l4 && (Lc & 1)) continue;
-                                            for (int M = -Lc; M <= Lc; ++M)
-                                                for (int m1 = -l1; m1 <= l1; ++m1) {
-                                                    const int m2 = M - m1;
-                                                    if (std::abs(m2) > l2) continue;
-                                                    const double w12 = wigner3j(l1, m1, l2, m2, Lc, -M);
-                                                    if (w12 == 0.0) continue;
-                                                    for (int m3 = -l3; m3 <= l3; ++m3) {
-                                                        const int m4 = -M - m3;
-                                                        if (std::abs(m4) > l4) continue;
-                                                        const double w34 = wigner3j(l3, m3, l4, m4, Lc, M);
-                                                        if (w34 == 0.0) continue;
-                                                        const double sg = ((Lc - M) & 1) ? -1.0 : 1.0;
-                                                        t.terms.push_back(AceTerm{sg * w12 * w34,
-                                                                                  {eidx(K[i1].first, K[i1].second, l1, m1),
-                                                                                   eidx(K[i2].first, K[i2].second, l2, m2),
-                                                                                   eidx(K[i3].first, K[i3].second, l3, m3),
-                                                                                   eidx(K[i4].first, K[i4].second, l4, m4)}});
-                                                    }
-                                                }
-                                            close(4);
-                                        }
-                                    }
-    }
-    t.D = (int)t.nu.size();
-    // slots and derivative terms: dB_b/dA_e = sum over terms and factors i with e_i = e of
-    // coef * prod_{j != i} A_{e_j}
-    t.soff.assign(1, 0);
-    t.dtoff.assign(1, 0);
-    for (int b = 0; b < t.D; ++b) {
-        std::vector<int> es;
-        for (int q = t.toff[b]; q < t.toff[b + 1]; ++q)
-            for (int i = 0; i < 4 && t.terms[q].e[i] >= 0; ++i)
-                if (std::find(es.begin(), es.end(), t.terms[q].e[i]) == es.end()) es.push_back(t.terms[q].e[i]);
-        for (int e : es) {
-            const int lm = e % t.LA2, cn = e / t.LA2, n = cn % t.NA, c = cn / t.NA;
-            t.scode.push_back((c << 16) | (n << 8) | lm);
-            for (int q = t.toff[b]; q < t.toff[b + 1]; ++q) {
-                const AceTerm& tm = t.terms[q];
-                for (int i = 0; i < 4 && tm.e[i] >= 0; ++i) {
-                    if (tm.e[i] != e) continue;
-                    AceTerm d{tm.coef, {-1, -1, -1, -1}};
-                    int k = 0;
-                    for (int j = 0; j < 4 && tm.e[j] >= 0; ++j)
-                        if (j != i) d.e[k++] = tm.e[j];
-                    t.dterms.push_back(d);
-                }
-            }
-            t.dtoff.push_back((int)t.dterms.size());
-        }
-        t.soff.push_back((int)t.scode.size());
-    }
-    return (t.D > ACE_DMAX) ? 2 : 0;
-}
-
 void struct_geometry(const double* cell, double rcut, StructGeo& g)
 {
     for (int i = 0; i < 9; ++i) g.cell[i] = cell[i];

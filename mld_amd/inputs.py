@@ -46,12 +46,21 @@ DEFAULTS: Dict[str, Any] = {
     "type_of_loss": 1,            # :530 (1 loss_init, 2 loss_fair, 3 loss_per; ml_in_ndm_module.F90:1407-1409)
     "factor_energy_error": 1.0,   # :391
     "factor_force_error": 1.0,    # :390
+    "r_cut_width": 0.5,           # :429
+    "r_cut_in": -1.2,             # :442
+    "zetaace_order": 1,           # :279
     "ace_numax": 3,               # :282
     "ace_nmax_list": " 5 3 1",    # :283
     "ace_lmax_list": " 0 4 2",    # :284
     "ace_lambda_list": " 2.d0 2.d0 2.d0",  # :286
     "ace_radial_poly": 2,         # :287 (1 powTcheb, 2 expTcheb)
+    "ace_chem": 1,                # :288 (0 incomplete, 1 standard)
+    "ace_radial_chem": 1,         # :289 (1 Ralf)
+    "ace_npoints_spline": 1000,   # :290
+    "ace_gencg": 2,               # :304 (1 RI, 2 RPI)
+    "l_ace_set_rcut": False,      # :300
 }
+ACE_NU_LIMIT = 6  # l_ace_order(1 .. 6), all .false. by default (:280-281)
 
 DESCRIPTOR_BSO4, DESCRIPTOR_ACE = 9, 300  # ml_in_ndm_module.F90:1312, 1320
 
@@ -108,9 +117,10 @@ def parse_namelist(text: str, group: str = "input_ml") -> Dict[str, Any]:
     joined = " ".join(body)
     out: Dict[str, Any] = {}
     # split at "key =" boundaries outside quotes
-    parts = re.split(r"""(?:^|[\s,])([A-Za-z_][A-Za-z0-9_%]*)\s*=(?=(?:[^'"]|'[^']*'|"[^"]*")*$)""", joined)
+    parts = re.split(r"""(?:^|[\s,])([A-Za-z_][A-Za-z0-9_%]*(?:\(\s*\d+\s*\))?)\s*=(?=(?:[^'"]|'[^']*'|"[^"]*")*$)""",
+                     joined)
     for k in range(1, len(parts) - 1, 2):
-        out[parts[k].lower()] = _value(parts[k + 1])
+        out[re.sub(r"\s+", "", parts[k]).lower()] = _value(parts[k + 1])  # "l_ace_order(1)" keys keep the index
     return out
 
 
@@ -157,6 +167,31 @@ def species_weights(nml: Dict[str, Any]) -> Tuple[Optional[np.ndarray], Optional
     return w1, w3
 
 
+def ace_kwargs(nml: Dict[str, Any]) -> Dict[str, Any]:
+    """The ACE keys (descriptor_type 300) as mld_amd.Context arguments: ace_numax, l_ace_order(nu),
+    ace_nmax_list, ace_lmax_list, ace_lambda_list(1) (the radial's lambda, compute_ace.F90:510-512),
+    ace_radial_poly, ace_gencg, ace_chem, ace_npoints_spline and the global radial cutoffs r_cut_in,
+    r_cut_width_in, r_cut_width.  Raises for the options that are not built: ace_radial_chem other
+    than 1 (Ralf), zetaace_order > 1 and per-order cutoffs (l_ace_set_rcut)."""
+    if int(nml["ace_radial_chem"]) != 1:
+        raise NotImplementedError(f"ace_radial_chem {nml['ace_radial_chem']}: only 1 (Ralf) is built")
+    if int(nml["zetaace_order"]) != 1:
+        raise NotImplementedError("zetaace_order > 1 is not built")
+    if bool(nml["l_ace_set_rcut"]):
+        raise NotImplementedError("l_ace_set_rcut (per-order cutoffs) is not built")
+    numax = int(nml["ace_numax"])
+    nmax = [int(x) for x in _floats(nml["ace_nmax_list"])]
+    lmax = [int(x) for x in _floats(nml["ace_lmax_list"])]
+    active = [bool(nml.get(f"l_ace_order({k})", False)) for k in range(1, ACE_NU_LIMIT + 1)][:numax]
+    if len(nmax) < numax or len(lmax) < numax:
+        raise ValueError("ace_nmax_list / ace_lmax_list have fewer items than ace_numax")
+    return dict(descriptor="ace", ace_numax=numax, ace_active=active, ace_nmax=nmax[:numax], ace_lmax=lmax[:numax],
+                ace_lambda=_floats(nml["ace_lambda_list"])[0], ace_radial_poly=int(nml["ace_radial_poly"]),
+                ace_gencg=int(nml["ace_gencg"]), ace_chem=int(nml["ace_chem"]),
+                ace_npoints_spline=int(nml["ace_npoints_spline"]), ace_r_cut_in=float(nml["r_cut_in"]),
+                r_cut_width_in=float(nml["r_cut_width_in"]), r_cut_width=float(nml["r_cut_width"]))
+
+
 def context_kwargs(nml: Dict[str, Any], precision: int = 0) -> Dict[str, Any]:
     """Arguments of mld_amd.Context for the run's descriptor (SO(4) bispectrum or ACE)."""
     dt = int(nml["descriptor_type"])
@@ -170,11 +205,9 @@ def context_kwargs(nml: Dict[str, Any], precision: int = 0) -> Dict[str, Any]:
         kw.update(j_max=float(nml["j_max"]), inv_r0_input=float(nml["inv_r0_input"]),
                   lbso4_diag=bool(nml["lbso4_diag"]))
     elif dt == DESCRIPTOR_ACE:
-        numax = int(nml["ace_numax"])
-        kw.update(descriptor="ace", ace_numax=numax,
-                  ace_nmax=[int(x) for x in _floats(nml["ace_nmax_list"])][:numax],
-                  ace_lmax=[int(x) for x in _floats(nml["ace_lmax_list"])][:numax],
-                  ace_lambda=_floats(nml["ace_lambda_list"])[0], ace_radial_poly=int(nml["ace_radial_poly"]))
+        kw.update(ace_kwargs(nml))
+        kw.pop("weight", None)  # the chemical channels of ACE are the species themselves
+        kw.pop("weight_3ch", None)
     else:
         raise NotImplementedError(f"descriptor_type {dt}: only 9 (SO(4) bispectrum) and 300 (ACE) are built")
     return kw

@@ -1,5 +1,9 @@
-"""ACE descriptors and gradients on the GPU against the ACE oracle (oracle/ace.py), batch
-invariance, and the fit / prediction / serving paths on an ACE context."""
+"""The reference's ACE (descriptor_type 300) on the GPU against the ACE oracle (oracle/ace.py):
+the bases of the reference's own ACE tests fit21 (W, nu <= 5, RPI) and fit24 (Ta Ti V W,
+ACE_CHEM_STANDARD, D = 3576), the five-species serving basis, and nu = 6 with the RI
+coefficients, powTcheb radials and ACE_CHEM_INCOMPLETE; batch invariance; and the fit /
+prediction / serving paths on an ACE context.  FP64 to 1e-10 of the largest value, FP32 to
+2e-5 (B) and 1e-4 (dB/dr)."""
 import numpy as np
 import pytest
 
@@ -8,9 +12,24 @@ from mld_amd import synth
 
 pytestmark = pytest.mark.gpu
 
+FIT21 = dict(nspecies=1, ace_numax=5, ace_nmax=(14, 4, 3, 2, 1), ace_lmax=(0, 3, 2, 2, 1), ace_lambda=3.0,
+             ace_r_cut_in=1.2, r_cut_width_in=0.4, r_cut_width=0.5)
+FIT24 = dict(nspecies=4, ace_numax=5, ace_nmax=(14, 2, 1, 1, 1), ace_lmax=(0, 2, 1, 1, 1), ace_lambda=3.0,
+             ace_r_cut_in=1.7, r_cut_width_in=0.4, r_cut_width=0.5)
+SERVE = dict(nspecies=5, ace_numax=3, ace_nmax=(8, 2, 1), ace_lmax=(0, 4, 2), ace_lambda=3.0, ace_r_cut_in=1.2,
+             r_cut_width_in=0.4, r_cut_width=0.5)
+NU6 = dict(nspecies=2, ace_numax=6, ace_nmax=(3, 2, 2, 1, 1, 1), ace_lmax=(0, 2, 2, 1, 1, 1), ace_lambda=2.5,
+           ace_r_cut_in=1.2, r_cut_width_in=0.4, r_cut_width=0.5, ace_gencg=1, ace_radial_poly=1, ace_chem=0)
+
 
 def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def oracle_model(kw, rcut=5.0):
+    return A.AceRef(kw["ace_numax"], kw["ace_nmax"], kw["ace_lmax"], kw["nspecies"], kw.get("ace_active"),
+                    kw.get("ace_gencg", 2), kw.get("ace_chem", 1), rcut, kw["r_cut_width"], kw["ace_r_cut_in"],
+                    kw["r_cut_width_in"], kw["ace_lambda"], kw.get("ace_radial_poly", 2))
 
 
 def gpu_ace(ctx, structs, want_grad=True):
@@ -23,69 +42,33 @@ def gpu_ace(ctx, structs, want_grad=True):
     return b, e.double().cpu().numpy(), None if g is None else g.double().cpu().numpy()
 
 
-CASES = [
-    # (cells per edge, channels): 2x2x2 and 3x3x3 cells are thinner than 2 r_cut (image replicas)
-    (2, 2),
-    (3, 1),
-]
+CASES = [("fit21", FIT21, 3.165), ("fit24", FIT24, 3.2), ("serve", SERVE, 3.1), ("nu6", NU6, 3.1)]
 
 
-@pytest.mark.parametrize("n,nch", CASES)
-def test_ace_matches_oracle(n, nch):
+@pytest.mark.parametrize("name,kw,a", CASES, ids=[c[0] for c in CASES])
+def test_ace_matches_oracle(name, kw, a):
     import mld_amd as M
 
-    rng = np.random.default_rng(10 + n)
-    pos, cell, it = synth.hea_bcc(rng, n, a=3.1, nspecies=5, sigma=0.1)
-    w = [1.0, 1.3, 0.8, 1.6, 1.1]
-    kw = dict(weight=w) if nch == 2 else {}
-    Eo, Go, nl = A.ace(pos, cell, it, 5.0, nch=nch, weight=w if nch == 2 else None)
-    for prec, tb, tg in ((M.PREC_FP64, 1e-11, 1e-10), (M.PREC_FP32, 2e-5, 1e-4)):
-        ctx = M.Context(r_cut=5.0, nspecies=5, precision=prec, descriptor="ace", **kw)
-        assert ctx.D == Eo.shape[1]
+    rng = np.random.default_rng(10 + len(name))
+    pos, cell, it = synth.hea_bcc(rng, 2, a=a, nspecies=kw["nspecies"], sigma=0.1)
+    model = oracle_model(kw)
+    Eo, Go, nl = A.ace(pos, cell, it, 5.0, model=model)
+    print(name, "oracle D", Eo.shape[1], flush=True)
+    for prec, tb, tg in ((M.PREC_FP64, 1e-10, 1e-10), (M.PREC_FP32, 2e-5, 1e-4)):
+        ctx = M.Context(r_cut=5.0, precision=prec, descriptor="ace", **kw)
+        assert ctx.D == Eo.shape[1] == model.D
         b, E, G = gpu_ace(ctx, [(pos, cell, it)])
-        assert rel(E, Eo) < tb, (prec, rel(E, Eo))
-        assert rel(G.reshape(Go.shape), Go) < tg, (prec, rel(G.reshape(Go.shape), Go))
+        assert rel(E, Eo) < tb, (name, prec, rel(E, Eo))
+        assert rel(G.reshape(Go.shape), Go) < tg, (name, prec, rel(G.reshape(Go.shape), Go))
         ctx.close()
 
 
-def test_ace_body_order_4_matches_oracle():
-    """ace_numax = 4: the four-A couplings through an intermediate L (oracle/ace.py) on the GPU."""
+def test_ace_keys_out_of_scope_are_rejected():
     import mld_amd as M
 
-    rng = np.random.default_rng(12)
-    pos, cell, it = synth.hea_bcc(rng, 2, a=3.1, nspecies=5, sigma=0.1)
-    w = [1.0, 1.3, 0.8, 1.6, 1.1]
-    nu4 = dict(numax=4, nmax=(3, 2, 1, 1), lmax=(0, 2, 1, 2))
-    Eo, Go, nl = A.ace(pos, cell, it, 5.0, nch=2, weight=w, **nu4)
-    for prec, tb, tg in ((M.PREC_FP64, 1e-11, 1e-10), (M.PREC_FP32, 2e-5, 1e-4)):
-        ctx = M.Context(r_cut=5.0, nspecies=5, precision=prec, descriptor="ace", weight=w, ace_numax=4,
-                        ace_nmax=nu4["nmax"], ace_lmax=nu4["lmax"])
-        assert ctx.D == Eo.shape[1]
-        b, E, G = gpu_ace(ctx, [(pos, cell, it)])
-        assert rel(E, Eo) < tb, (prec, rel(E, Eo))
-        assert rel(G.reshape(Go.shape), Go) < tg, (prec, rel(G.reshape(Go.shape), Go))
-        ctx.close()
-
-
-def test_ace_powtcheb_radial_matches_oracle():
-    """ace_radial_poly = 1 (radial_powTcheb) through the GPU against the oracle."""
-    import mld_amd as M
-
-    rng = np.random.default_rng(14)
-    pos, cell, it = synth.hea_bcc(rng, 2, a=3.1, nspecies=5, sigma=0.1)
-    w = [1.0, 1.3, 0.8, 1.6, 1.1]
-    Eo, Go, nl = A.ace(pos, cell, it, 5.0, nch=2, weight=w, lam=2.5, radial_poly=1)
-    Ee, _, _ = A.ace(pos, cell, it, 5.0, nch=2, weight=w, lam=2.5, want_grad=False)
-    assert rel(Ee, Eo) > 1e-3  # the two radial families give different descriptors
-    for prec, tb, tg in ((M.PREC_FP64, 1e-11, 1e-10), (M.PREC_FP32, 2e-5, 1e-4)):
-        ctx = M.Context(r_cut=5.0, nspecies=5, precision=prec, descriptor="ace", weight=w, ace_lambda=2.5,
-                        ace_radial_poly=1)
-        b, E, G = gpu_ace(ctx, [(pos, cell, it)])
-        assert rel(E, Eo) < tb, (prec, rel(E, Eo))
-        assert rel(G.reshape(Go.shape), Go) < tg, (prec, rel(G.reshape(Go.shape), Go))
-        ctx.close()
-    with pytest.raises(M.MldError):
-        M.Context(r_cut=5.0, nspecies=5, descriptor="ace", ace_radial_poly=3)
+    for bad in (dict(ace_chem=2), dict(ace_radial_poly=3), dict(ace_gencg=3)):
+        with pytest.raises(M.MldError):
+            M.Context(r_cut=5.0, descriptor="ace", **{**SERVE, **bad})
 
 
 def test_ace_batch_invariance_and_large_box():
@@ -94,7 +77,7 @@ def test_ace_batch_invariance_and_large_box():
     rng = np.random.default_rng(3)
     structs = [synth.hea_bcc(rng, 4, a=3.1, nspecies=5, sigma=0.05), synth.hea_bcc(rng, 2, a=3.1, nspecies=5, sigma=0.1),
                synth.hea_bcc(rng, 3, a=3.1, nspecies=5, sigma=0.08)]
-    ctx = M.Context(r_cut=5.0, nspecies=5, descriptor="ace", weight=[1.0, 1.3, 0.8, 1.6, 1.1])
+    ctx = M.Context(r_cut=5.0, descriptor="ace", **SERVE)
     b, E, G = gpu_ace(ctx, structs)
     off = b.atom_off
     for s, st in enumerate(structs):
@@ -108,15 +91,17 @@ def test_ace_batch_invariance_and_large_box():
 
 
 def test_ace_fit_predict_and_serve():
-    """An ACE context drives the same rows / Gram / prediction / serving paths."""
-    import torch
+    """An ACE context drives the same rows / Gram / prediction / serving paths; each served result
+    is the same alone and in the mix."""
     import mld_amd as M
     from mld_amd import fit
     from mld_amd.serve import Server
 
     rng = np.random.default_rng(4)
+    kw = dict(nspecies=5, ace_numax=3, ace_nmax=(4, 1, 1), ace_lmax=(0, 2, 1), ace_lambda=3.0, ace_r_cut_in=1.2,
+              r_cut_width_in=0.4, r_cut_width=0.5)
     structs = [synth.hea_bcc(rng, 2, a=3.1, nspecies=5, sigma=0.1) for _ in range(6)]
-    ctx = M.Context(r_cut=5.0, nspecies=5, descriptor="ace")
+    ctx = M.Context(r_cut=5.0, descriptor="ace", **kw)
     b, E, G = gpu_ace(ctx, structs)
     po, pi, pd = b.pair_tensors()
     Et, Ft = synth.morse_from_pairs(po, pi, pd, b.atom_off, rc=5.0)
@@ -139,4 +124,36 @@ def test_ace_fit_predict_and_serve():
         assert Es == float(Ep[s]) and Ea == Es
         assert np.array_equal(Fs, Fp.cpu().numpy()[off[s]:off[s + 1]]) and np.array_equal(Fa, Fs)
         assert np.array_equal(Qs, Q.cpu().numpy()[off[s]:off[s + 1]]) and np.array_equal(Qa, Qs)
+    ctx.close()
+
+
+def test_wide_descriptor_prediction_matches_rows():
+    """fit24's D = 3576 (above the 512 columns held in registers): energies and forces of a linear
+    model against w . rows on the host."""
+    import torch
+    import mld_amd as M
+    from mld_amd import fit
+
+    rng = np.random.default_rng(8)
+    structs = [synth.hea_bcc(rng, 2, a=3.2, nspecies=4, sigma=0.1) for _ in range(2)]
+    ctx = M.Context(r_cut=5.0, descriptor="ace", **FIT24)
+    b, E, G = gpu_ace(ctx, structs)
+    w = rng.normal(size=1 + ctx.D)  # [w0, w_1 .. w_D]
+    fit.model_set(ctx, 1, w)
+    Ep, Fp, _ = fit.predict(ctx, b)
+    off = b.atom_off
+    po = b.pair_tensors()[0].cpu().numpy()
+    pidx = b.pair_tensors()[1].cpu().numpy()
+    e_atom = E @ w[1:] + w[0]
+    for s in range(len(structs)):
+        assert abs(float(Ep[s]) - e_atom[off[s]:off[s + 1]].sum()) < 1e-10 * max(1.0, abs(e_atom).sum())
+    # F_k = -(dE_k/dr_k + sum over i with k in nb(i) of dE_i/dr_k)
+    Gs = G.reshape(-1, 3, ctx.D) @ w[1:]
+    F = np.zeros((len(E), 3))
+    for a in range(len(E)):
+        s0 = po[a] + a
+        F[a] -= Gs[s0]
+        for k in range(po[a + 1] - po[a]):
+            F[pidx[po[a] + k]] -= Gs[s0 + 1 + k]
+    assert rel(Fp.cpu().numpy(), F) < 1e-10
     ctx.close()

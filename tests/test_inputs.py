@@ -4,6 +4,7 @@ import math
 import os
 
 import numpy as np
+import pytest
 
 from mld_amd import driver, inputs
 
@@ -100,3 +101,27 @@ def test_model_weights_loss_types():
     assert np.allclose(w2, [(2.0 / (3 / tot) * 3, 0.5 / (18 / tot) * 5)] * 2 + [(1.0 / (3 / tot) * 3, 0.25 / (18 / tot) * 5)])
     w3 = inputs.model_weights({**base, "type_of_loss": 3}, configs)
     assert np.allclose(w3, [(2.0 / 2 * 3, 0.5 / 6 * 5), (2.0 / 3 * 3, 0.5 / 9 * 5), (1.0 / 4 * 3, 0.25 / 12 * 5)])
+
+
+def test_ace_keys_of_fit21_and_fit24():
+    """The ACE keys of the reference's fit21 / fit24 inputs (l_ace_order(nu) indexed keys included)
+    as Context arguments, and the basis they give: ace_dim 207 and 894 (D 207 and 3576, REF/ndm.out)."""
+    import mld_amd as M
+
+    want = {"fit21": (1, [14, 4, 3, 2, 1, 1], [0, 3, 2, 2, 1, 1], 1.2, 207, 207),
+            "fit24": (4, [14, 2, 1, 1, 1, 1], [0, 2, 1, 1, 1, 1], 1.7, 894, 3576)}
+    for name, (nsp, nmax, lmax, rin, ace_dim, D) in want.items():
+        nml = inputs.read_input_ml(os.path.join(G, f"{name}_vacancy.ml"))
+        kw = inputs.context_kwargs(nml)
+        assert kw["descriptor"] == "ace" and kw["nspecies"] == nsp
+        assert kw["ace_active"] == [True, True, True, True, True, False]
+        assert kw["ace_nmax"] == nmax and kw["ace_lmax"] == lmax
+        assert (kw["ace_gencg"], kw["ace_chem"], kw["ace_radial_poly"], kw["ace_lambda"]) == (2, 1, 2, 3.0)
+        assert (kw["ace_r_cut_in"], kw["r_cut_width_in"], kw["r_cut_width"], kw["r_cut"]) == (rin, 0.4, 0.5, 5.0)
+        r = M.ace_basis_review(nspecies=nsp, ace_numax=kw["ace_numax"], ace_nmax=nmax, ace_lmax=lmax,
+                               ace_active=kw["ace_active"], ace_gencg=2, ace_chem=1)
+        assert (r["ace_dim"], r["D"]) == (ace_dim, D)
+    nml = inputs.read_input_ml(os.path.join(G, "fit21_vacancy.ml"))
+    for key, val in (("ace_radial_chem", 3), ("zetaace_order", 2), ("l_ace_set_rcut", True)):
+        with pytest.raises(NotImplementedError):
+            inputs.context_kwargs({**nml, key: val})
