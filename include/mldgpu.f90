@@ -110,6 +110,15 @@ module mldgpu_c
       import; type(c_ptr), value :: ctx; real(c_double), value :: lambda, rcond
       real(c_double), intent(out) :: w(*), diag(4)
     end function
+    ! the six stress rows of every stressed structure of the last batch (linear model), after
+    ! mld_gram_add_batch: s(6, nstruct) Voigt targets, w_s(nstruct) weights (0: no stress rows)
+    integer(c_int) function mld_gram_add_stress(ctx, order, s, w_s, sign_stress, sign_stress_big_box) bind(c)
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: order
+      real(c_double), intent(in) :: s(*), w_s(*); real(c_double), value :: sign_stress, sign_stress_big_box
+    end function
+    integer(c_int) function mld_predict_stress_dev(ctx, sign_stress, sign_stress_big_box, s_dev) bind(c)
+      import; type(c_ptr), value :: ctx, s_dev; real(c_double), value :: sign_stress, sign_stress_big_box
+    end function
     integer(c_int) function mld_model_set(ctx, order, ncol, w, ref_energy) bind(c)
       import; type(c_ptr), value :: ctx; integer(c_int), value :: order, ncol
       real(c_double), intent(in) :: w(*); type(c_ptr), value :: ref_energy

@@ -174,6 +174,15 @@ int mld_gram_add_rows_noalloc(mld_ctx* ctx, int D, int mc, const double* Ac_dev,
  * 28 digit pairs (i, j) with i + j <= 8 are folded in FP64, so the Gram keeps FP64-level accuracy. */
 int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F_dev, double w_e,
                        double w_f, int mixed);
+/* The six stress rows of every structure of the last batch with w_s[s] > 0 (train_fill_Amat_with_stress
+ * and para_pack_stress_descriptor, mld_stress.F90:181-268, 272-756), linear model (order 1):
+ * row_c[1 + d] = sign_stress * 3 / V * f * sum over the atoms a and neighbours k of
+ * dB_a/dr_k (x_c, d) * u_k(y_c) / r_k, c = (x, y) = xx, yy, zz, yz, xz, xy; u = r_k - r_a, f = 1 in a
+ * small cell, -sign_stress_big_box in a large one (neighbours.F90:823-835); V = |det(cell)|.
+ * S (6 * nstruct): the Voigt targets xx yy zz yz xz xy per structure as the poscar holds them;
+ * w_s (nstruct): w_s_model per structure, 0 for none.  Needs the batch's gradients. */
+int mld_gram_add_stress(mld_ctx* ctx, int order, const double* S, const double* w_s, double sign_stress,
+                        double sign_stress_big_box);
 /* A batch of configurations from host buffers, end to end: neighbours, descriptors with
  * gradients (the context's descriptor and precision), (order 3: kernel_po features,) energy
  * and force rows and the FP64 Gram update -- the per-configuration loop of
@@ -200,8 +209,8 @@ int mld_gram_set_keep_rows(mld_ctx* ctx, int keep);
  * preconditions FP64 conjugate-gradient refinement whose operator comes from the kept
  * FP64 rows (or the Gram itself).
  * MLD_SOLVE_SVD: the minimum-norm solution of serial_lsystem_by_svd (stats_tools.F90:1153-1213,
- * dgelsd with rcond = machine epsilon): eigen-decomposition of the symmetric system by
- * cyclic Jacobi on the device, eigenvalues below eps * max |eigenvalue| dropped; with
+ * dgelsd; rcond < 0 is LAPACK's machine precision dlamch('E') = 2^-53): eigen-decomposition of the symmetric system by
+ * cyclic Jacobi on the device, eigenvalues at or below 2^-53 * max |eigenvalue| dropped; with
  * unique quadratic columns it returns the reference's full-layout minimum-norm solution
  * folded onto those columns.  diag_out then holds [Jacobi sweeps, relative residual,
  * smallest kept |eigenvalue|, largest |eigenvalue|].
@@ -211,7 +220,7 @@ int mld_gram_set_keep_rows(mld_ctx* ctx, int keep);
 int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w, double* diag_out);
 /* The MLD_SOLVE_SVD solve with an explicit relative threshold: the &input_ml key svd_rcond
  * (read_ml_file.F90:353-354, passed to dgelsd by gram_reduce_and_solve,
- * module_online_fit.F90:580-583); rcond < 0 means machine epsilon, as in dgelsd. */
+ * module_online_fit.F90:580-583); rcond < 0 means dlamch('E') = 2^-53, as in dgelsd. */
 int mld_gram_solve_svd(mld_ctx* ctx, double lambda, double rcond, double* w, double* diag_out);
 
 /* ---- kernel_po features (compute/compute_kernel.F90:91-150, 476-866) ----
@@ -247,6 +256,9 @@ int mld_kernel_po(mld_ctx* ctx, int nat, const double* energy, const double* for
 int mld_model_set(mld_ctx* ctx, int order, int ncol, const double* w, const double* ref_energy);
 int mld_mahalanobis_set(mld_ctx* ctx, const double* mean, const double* sigma_inv);
 int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev);
+/* Stresses of a linear model on the last batch (md_mld_compute_stress, mld.F90:1777-1867):
+ * S_dev (6 * nstruct) = w . [0, stress row] with the rows of mld_gram_add_stress. */
+int mld_predict_stress_dev(mld_ctx* ctx, double sign_stress, double sign_stress_big_box, double* S_dev);
 
 /* ---- serving: continuous batching of prediction requests ----
  * No counterpart in the reference (which evaluates one configuration at a time,

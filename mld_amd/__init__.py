@@ -28,6 +28,7 @@ EXPORTS = [
     "mld_batch_neighbours", "mld_batch_pairs_dev", "mld_batch_pair_geometry_dev", "mld_batch_bso4_dev", "mld_gram_begin", "mld_gram_add_rows",
     "mld_gram_add_rows_dev", "mld_gram_add_rows_noalloc", "mld_gram_add_batch", "mld_gram_add_structures", "mld_gram_export", "mld_gram_import", "mld_gram_set_keep_rows", "mld_gram_solve", "mld_gram_solve_svd",
     "mld_kernel_po_set", "mld_kernel_po_dev", "mld_kernel_po", "mld_model_set", "mld_mahalanobis_set", "mld_predict_dev",
+    "mld_gram_add_stress", "mld_predict_stress_dev",
     "mld_serve_create", "mld_serve_submit", "mld_serve_poll", "mld_serve_stats", "mld_serve_destroy",
     "mld_get_stream", "mld_synchronize",
 ]
@@ -169,6 +170,8 @@ def load() -> C.CDLL:
         "mld_model_set": [vp, i, i, vp, vp],
         "mld_mahalanobis_set": [vp, vp, vp],
         "mld_predict_dev": [vp, vp, vp, vp],
+        "mld_gram_add_stress": [vp, i, vp, vp, d, d],
+        "mld_predict_stress_dev": [vp, d, d, vp],
         "mld_serve_create": [vp, C.c_size_t, i, i, vpp],
         "mld_serve_submit": [vp, i, vp, vp, vp, i, i, i64p],
         "mld_serve_poll": [vp, C.c_longlong, i, vp, vp, vp, vp, ip],

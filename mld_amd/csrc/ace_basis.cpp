@@ -21,9 +21,9 @@
 //                   reference cuts at 1e-14 s_max (stats_tools.F90:2060-2068); these Grams are
 //                   multiples of projectors (kept values equal s_max, the rest are zeros that
 //                   rounding lifts to 3.6e-15 s_max at nu = 6), so the cut here is 1e-8 s_max.
-//                   Singular vectors of one singular value are a basis of its space that the
-//                   library routine chooses; here it is fixed canonically: Gram-Schmidt of the
-//                   space's projector applied to e_1, e_2, ... (same span, so the same fits).
+//                   The singular vectors are those of svd_enforce_signs_by_ref (stats_tools.F90:
+//                   2114-2253): rotation of equal singular values by V^T diag(1..n) V, then the
+//                   sign rule; both depend only on the singular spaces (canonical_basis).
 //   B order         entries, then their columns (build_Bfull, nlmu_tools:1899-1930); orders
 //                   nu = 1, 2, ... (compute_ace.F90:178-296); nu = 1: (mu, n), n = 0 .. n_max(1)
 //                   (compute_AB_body_one_neigh_ja, compute_ace_AB.F90:2566-2689)
@@ -271,9 +271,13 @@ void jacobi_eigen(std::vector<double> A, int n, std::vector<double>& w, std::vec
     for (int i = 0; i < n; ++i) w[i] = A[(size_t)i * n + i];
 }
 
-// singular values s > 1e-8 s_max of the symmetric PSD G (n x n) with a canonical orthonormal
-// basis of every singular space (values within 1e-9 s_max grouped): Gram-Schmidt of the
-// space's projector applied to e_1, e_2, ...
+// Singular values s > 1e-8 s_max of the symmetric PSD G (n x n) with the singular vectors that
+// svd_enforce_signs_by_ref (stats_tools.F90:2114-2253) fixes, whatever basis the SVD returned:
+// within every run of values within 1e-12 s_max of the run's first, the run's vectors V_b are
+// rotated by the eigenvectors of V_b^T diag(1 .. n) V_b in ascending order of eigenvalue; then each
+// vector's sign makes the sum of its entries positive, or, with that sum within 1e-14 sqrt(n) of
+// zero, its first largest-magnitude entry.  V_b^T diag V_b depends only on the run's space, so the
+// result does too when its eigenvalues are distinct (they are integers here, at least 1 apart).
 std::vector<std::pair<double, std::vector<double>>> canonical_basis(const std::vector<double>& G, int n)
 {
     std::vector<double> S(G);
@@ -292,37 +296,49 @@ std::vector<std::pair<double, std::vector<double>>> canonical_basis(const std::v
         if (std::fabs(w[k]) > 1e-8 * smax) keep.push_back(k);
     size_t k = 0;
     while (k < keep.size()) {
-        size_t j = k;
-        while (j + 1 < keep.size() && std::fabs(w[keep[j + 1]] - w[keep[k]]) <= 1e-9 * smax) ++j;
-        const int nc = (int)(j - k + 1);
-        std::vector<double> P((size_t)n * n, 0.0);
-        double lam = 0.0;
-        for (size_t q = k; q <= j; ++q) {
-            const int c = keep[q];
-            lam += w[c];
-            for (int a = 0; a < n; ++a)
-                for (int b = 0; b < n; ++b) P[(size_t)a * n + b] += V[(size_t)a * n + c] * V[(size_t)b * n + c];
+        size_t j = k + 1;
+        while (j < keep.size() && std::fabs(w[keep[j]] - w[keep[k]]) < 1e-12 * smax) ++j;
+        const int d = (int)(j - k);
+        std::vector<std::vector<double>> run(d, std::vector<double>(n));
+        for (int q = 0; q < d; ++q)
+            for (int a = 0; a < n; ++a) run[q][a] = V[(size_t)a * n + keep[k + q]];
+        if (d > 1) {
+            std::vector<double> H((size_t)d * d, 0.0), eh, E;
+            for (int q1 = 0; q1 < d; ++q1)
+                for (int q2 = 0; q2 < d; ++q2) {
+                    double t = 0.0;
+                    for (int a = 0; a < n; ++a) t += run[q1][a] * (double)(a + 1) * run[q2][a];
+                    H[(size_t)q1 * d + q2] = t;
+                }
+            jacobi_eigen(H, d, eh, E);
+            std::vector<int> up(d);
+            for (int q = 0; q < d; ++q) up[q] = q;
+            std::stable_sort(up.begin(), up.end(), [&](int x, int y) { return eh[x] < eh[y]; });
+            std::vector<std::vector<double>> rot(d, std::vector<double>(n, 0.0));
+            for (int q = 0; q < d; ++q)
+                for (int q2 = 0; q2 < d; ++q2) {
+                    const double e = E[(size_t)q2 * d + up[q]];
+                    for (int a = 0; a < n; ++a) rot[q][a] += e * run[q2][a];
+                }
+            run.swap(rot);
         }
-        lam /= nc;
-        std::vector<std::vector<double>> basis;
-        for (int e = 0; e < n && (int)basis.size() < nc; ++e) {
-            std::vector<double> v(n);
-            for (int a = 0; a < n; ++a) v[a] = P[(size_t)a * n + e];
-            for (const auto& bv : basis) {
-                double d = 0.0;
-                for (int a = 0; a < n; ++a) d += bv[a] * v[a];
-                for (int a = 0; a < n; ++a) v[a] -= d * bv[a];
+        for (int q = 0; q < d; ++q) {
+            std::vector<double>& v = run[q];
+            double proj = 0.0;
+            for (double x : v) proj += x;
+            proj /= std::sqrt((double)n);
+            bool flip = proj < 0.0;
+            if (!flip && std::fabs(proj) <= 1e-14) {
+                int idx = 0;
+                for (int a = 1; a < n; ++a)
+                    if (std::fabs(v[a]) > std::fabs(v[idx])) idx = a;
+                flip = v[idx] < 0.0;
             }
-            double nv = 0.0;
-            for (double x : v) nv += x * x;
-            nv = std::sqrt(nv);
-            if (nv > 1e-3) {
-                for (double& x : v) x /= nv;
-                basis.push_back(v);
-            }
+            if (flip)
+                for (double& x : v) x = -x;
+            out.push_back({w[keep[k + q]], v});
         }
-        for (auto& bv : basis) out.push_back({lam, bv});
-        k = j + 1;
+        k = j;
     }
     return out;
 }

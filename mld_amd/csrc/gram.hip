@@ -315,6 +315,59 @@ __global__ void __launch_bounds__(256) rows_force_krr_kernel(int D, int nk, int 
     if (threadIdx.x < 3) y[row0 + threadIdx.x] = F[3 * (size_t)k + threadIdx.x] * swf;
 }
 
+// Linear stress rows (para_pack_stress_descriptor, mld_stress.F90:272-756, placed by
+// train_fill_Amat_with_stress, :181-268): for stressed structure s and the Voigt components
+// c = (x, y) = (0 0) (1 1) (2 2) (1 2) (0 2) (0 1),
+//   row[c][1 + d] = sign_stress * 3 / V * f * sum over the atoms a of s and their neighbours k of
+//                   g_{a,k}(x, d) * u_k(y) / r_k,
+// g the gradient slot dB_a/dr_k, V = |det(cell)| (calc_volume, neighbours.F90:1331-1368) and
+// u = r_k - r_a; f = 1 in a small cell (u_ij, neighbours.F90:1273-1301) and -sign_stress_big_box in
+// a large one (there the reference takes u = r_a - r_k and scales the force); column 0 is 0.  Rows
+// and targets carry sqrt(w_s).  One block per stressed structure; thread t owns the columns t,
+// t + 256, ... and sums the structure's pairs in order.
+template <typename T>
+__global__ void __launch_bounds__(256) rows_stress_kernel(int D, int ldA, const int* __restrict__ sidx,
+                                                          const double* __restrict__ sw, const double* __restrict__ S,
+                                                          const int* __restrict__ atom_off, const int* __restrict__ pair_off,
+                                                          const double4* __restrict__ pair_d,
+                                                          const mld::StructGeo* __restrict__ geo, const T* __restrict__ grad,
+                                                          double sign_stress, double sign_big, double* __restrict__ A,
+                                                          double* __restrict__ y)
+{
+    const int r = blockIdx.x;
+    const int s = sidx[r];
+    const double* c = geo[s].cell;  // columns a1, a2, a3
+    const double det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[3] * (c[1] * c[8] - c[2] * c[7]) +
+                       c[6] * (c[1] * c[5] - c[2] * c[4]);
+    const double f = sign_stress * 3.0 / fabs(det) * (geo[s].small ? 1.0 : -sign_big) * sw[r];
+    const int a0 = atom_off[s], a1 = atom_off[s + 1];
+    double* Ar = A + (size_t)6 * r * ldA;
+    for (int col = threadIdx.x; col < ldA; col += 256) {
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (col >= 1 && col <= D) {
+            const int d = col - 1;
+            for (int a = a0; a < a1; ++a)
+                for (int p = pair_off[a]; p < pair_off[a + 1]; ++p) {
+                    const double4 pd = pair_d[p];
+                    const double ux = pd.x / pd.w, uy = pd.y / pd.w, uz = pd.z / pd.w;
+                    const size_t slot = (size_t)p + a + 1;
+                    const double g0 = (double)grad[(slot * 3 + 0) * D + d];
+                    const double g1 = (double)grad[(slot * 3 + 1) * D + d];
+                    const double g2 = (double)grad[(slot * 3 + 2) * D + d];
+                    acc[0] += g0 * ux;
+                    acc[1] += g1 * uy;
+                    acc[2] += g2 * uz;
+                    acc[3] += g1 * uz;
+                    acc[4] += g0 * uz;
+                    acc[5] += g0 * uy;
+                }
+        }
+#pragma unroll
+        for (int cc = 0; cc < 6; ++cc) Ar[(size_t)cc * ldA + col] = f * acc[cc];
+    }
+    if (threadIdx.x < 6) y[6 * (size_t)r + threadIdx.x] = S[6 * (size_t)s + threadIdx.x] * sw[r];
+}
+
 __global__ void scale_rows_kernel(int M, int Dc, int ldA, const double* __restrict__ Ain, const double* __restrict__ yin,
                                   const double* __restrict__ w, double* __restrict__ A, double* __restrict__ y)
 {
@@ -721,6 +774,7 @@ struct GramScratch {
     void* y = nullptr; size_t cap_y = 0;
     void* split = nullptr; size_t cap_split = 0;
     void* rev = nullptr; size_t cap_rev = 0;
+    void* stress_in = nullptr; size_t cap_stress_in = 0;  // stressed-structure list, sqrt(w_s), targets
 };
 
 // Launch shape of the FP64 SYRK and GEMV over M rows of ldA columns: 128 x 128 upper tiles over
@@ -1013,7 +1067,7 @@ void gram_scratch_free(mld_ctx* ctx)
 {
     if (!ctx->gram_scratch) return;
     GramScratch* gs = reinterpret_cast<GramScratch*>(ctx->gram_scratch);
-    void* ps[] = {gs->part, gs->gpart, gs->rows, gs->y, gs->split, gs->rev};
+    void* ps[] = {gs->part, gs->gpart, gs->rows, gs->y, gs->split, gs->rev, gs->stress_in};
     for (void* q : ps)
         if (q) (void)hipFree(q);
     delete gs;
@@ -1141,6 +1195,61 @@ int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F
     int rc = launch_rows(ctx, order, dE, F_dev, w_e, w_f, &A, &y, &M);
     if (mixed) ctx->gram_mixed = 1;
     if (!rc) rc = gram_accumulate_rows(ctx, M, A, y, mixed);
+    if (!rc && ctx->keep_rows) rc = keep_rows_copy(ctx, M, A, y);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (rc) return rc;
+    if (e != hipSuccess) return mld_fail(ctx, MLD_E_DEVICE, hipGetErrorString(e));
+    return MLD_OK;
+}
+
+int mld_gram_add_stress(mld_ctx* ctx, int order, const double* S, const double* w_s, double sign_stress,
+                        double sign_stress_big_box)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
+    if (order != 1)
+        return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_gram_add_stress: stress rows are built for the linear model");
+    const int D = ctx->D;
+    if (1 + D != ctx->gD) return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_stress: the Gram is not 1 + D wide");
+    if (!ctx->last_grad) return mld_fail(ctx, MLD_E_STATE, "mld_batch_bso4_dev with gradients first");
+    const int nstruct = ctx->nstruct;
+    if (nstruct && (!S || !w_s)) return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_stress: S and w_s needed");
+    std::vector<int> sidx;
+    std::vector<double> sw;
+    for (int s = 0; s < nstruct; ++s) {
+        if (!(w_s[s] >= 0.0)) return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_stress: weights must be non-negative");
+        if (w_s[s] > 0.0) {
+            sidx.push_back(s);
+            sw.push_back(std::sqrt(w_s[s]));
+        }
+    }
+    const int ns = (int)sidx.size();
+    if (ns == 0) return MLD_OK;
+    ctx->gram_order = order;
+    GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
+    const size_t b_sw = sizeof(double) * ns, b_S = sizeof(double) * 6 * (size_t)nstruct, b_i = sizeof(int) * ns;
+    if (int rc = device_ensure(ctx, &gs.stress_in, &gs.cap_stress_in, b_sw + b_S + b_i + 64)) return rc;
+    double* d_sw = (double*)gs.stress_in;
+    double* d_S = d_sw + ns;
+    int* d_idx = (int*)(d_S + 6 * (size_t)nstruct);
+    MLD_HIP(ctx, hipMemcpyAsync(d_sw, sw.data(), b_sw, hipMemcpyHostToDevice, ctx->stream));
+    MLD_HIP(ctx, hipMemcpyAsync(d_S, S, b_S, hipMemcpyHostToDevice, ctx->stream));
+    MLD_HIP(ctx, hipMemcpyAsync(d_idx, sidx.data(), b_i, hipMemcpyHostToDevice, ctx->stream));
+    const int M = 6 * ns, ldA = ctx->gld;
+    if (int rc = device_ensure(ctx, &gs.rows, &gs.cap_rows, (size_t)M * ldA * sizeof(double) + 64)) return rc;
+    if (int rc = device_ensure(ctx, &gs.y, &gs.cap_y, (size_t)M * sizeof(double) + 64)) return rc;
+    double* A = (double*)gs.rows;
+    double* y = (double*)gs.y;
+    if (ctx->p.precision == MLD_PREC_FP32)
+        hipLaunchKernelGGL(rows_stress_kernel<float>, dim3(ns), dim3(256), 0, ctx->stream, D, ldA, d_idx, d_sw, d_S,
+                           ctx->d_atom_off, ctx->d_pair_off, ctx->d_pair_d, ctx->d_geo, (const float*)ctx->last_grad,
+                           sign_stress, sign_stress_big_box, A, y);
+    else
+        hipLaunchKernelGGL(rows_stress_kernel<double>, dim3(ns), dim3(256), 0, ctx->stream, D, ldA, d_idx, d_sw, d_S,
+                           ctx->d_atom_off, ctx->d_pair_off, ctx->d_pair_d, ctx->d_geo, (const double*)ctx->last_grad,
+                           sign_stress, sign_stress_big_box, A, y);
+    MLD_HIP(ctx, hipGetLastError());
+    int rc = gram_accumulate_rows(ctx, M, A, y, 0);
     if (!rc && ctx->keep_rows) rc = keep_rows_copy(ctx, M, A, y);
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (rc) return rc;

@@ -33,6 +33,7 @@ struct PredictState {
     double* d_wk = nullptr;    // [nk] kernel coefficients (order 3)
     int nk = 0;
     void* work = nullptr; size_t cap_work = 0;
+    void* swork = nullptr; size_t cap_swork = 0;  // per-slot w . dB of the stress prediction
 };
 
 __device__ inline double wave_sum(double v)
@@ -183,6 +184,63 @@ __global__ void __launch_bounds__(256) krr_dot_kernel(long long nrow, int nk, co
     if (lane == 0) v[r] += s;
 }
 
+// t[r] = w . feat[r] for rows r of D values (one wave per row)
+template <typename T>
+__global__ void __launch_bounds__(256) row_dot_kernel(long long nrow, int D, const double* __restrict__ w,
+                                                      const T* __restrict__ feat, double* __restrict__ t)
+{
+    const long long r = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= nrow) return;
+    const T* f = feat + (size_t)r * D;
+    double v = 0.0;
+    for (int c = lane; c < D; c += 64) v += w[c] * (double)f[c];
+    v = wave_sum(v);
+    if (lane == 0) t[r] = v;
+}
+
+// Stress of a linear model (md_mld_compute_stress, mld.F90:1777-1867: w . [0, pack_stress_linear]):
+//   S_c(s) = sign_stress * 3 / V * f * sum over the pairs p of s of t(slot(p), x_c) u_p(y_c) / r_p,
+// t = w . dB per slot and direction, (x, y) of c and f as in the stress rows (mld_stress.F90:272-756);
+// one block per structure, pairs strided over the threads, then a fixed-order tree sum.
+__global__ void __launch_bounds__(256) stress_kernel(const int* __restrict__ atom_off, const int* __restrict__ pair_off,
+                                                     const int* __restrict__ pair_src, const double4* __restrict__ pair_d,
+                                                     const mld::StructGeo* __restrict__ geo, const double* __restrict__ t,
+                                                     double sign_stress, double sign_big, double* __restrict__ S)
+{
+    __shared__ double red[6][256];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const int p0 = pair_off[atom_off[s]], p1 = pair_off[atom_off[s + 1]];
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int p = p0 + tid; p < p1; p += 256) {
+        const size_t slot = (size_t)p + pair_src[p] + 1;
+        const double4 pd = pair_d[p];
+        const double ux = pd.x / pd.w, uy = pd.y / pd.w, uz = pd.z / pd.w;
+        const double t0 = t[slot * 3 + 0], t1 = t[slot * 3 + 1], t2 = t[slot * 3 + 2];
+        acc[0] += t0 * ux;
+        acc[1] += t1 * uy;
+        acc[2] += t2 * uz;
+        acc[3] += t1 * uz;
+        acc[4] += t0 * uz;
+        acc[5] += t0 * uy;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) red[c][tid] = acc[c];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) red[c][tid] += red[c][tid + o];
+        __syncthreads();
+    }
+    if (tid < 6) {
+        const double* cl = geo[s].cell;
+        const double det = cl[0] * (cl[4] * cl[8] - cl[5] * cl[7]) - cl[3] * (cl[1] * cl[8] - cl[2] * cl[7]) +
+                           cl[6] * (cl[1] * cl[5] - cl[2] * cl[4]);
+        S[6 * (size_t)s + tid] = sign_stress * 3.0 / fabs(det) * (geo[s].small ? 1.0 : -sign_big) * red[tid][0];
+    }
+}
+
 // one block per structure: E_s = w0 N_s + sum_a e_a (fixed tree order)
 __global__ void __launch_bounds__(256) energy_kernel(const int* __restrict__ atom_off, double w0,
                                                      const double* __restrict__ e_atom, double* __restrict__ E)
@@ -239,7 +297,7 @@ void predict_free(mld_ctx* ctx)
 {
     if (!ctx->predict_state) return;
     PredictState* ps = reinterpret_cast<PredictState*>(ctx->predict_state);
-    void* q[] = {ps->d_w, ps->d_U, ps->d_ref, ps->d_mu, ps->d_sinv, ps->d_wk, ps->work};
+    void* q[] = {ps->d_w, ps->d_U, ps->d_ref, ps->d_mu, ps->d_sinv, ps->d_wk, ps->work, ps->swork};
     for (void* p : q)
         if (p) (void)hipFree(p);
     delete ps;
@@ -429,6 +487,39 @@ int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev)
         hipLaunchKernelGGL(force_kernel, dim3((3 * natoms + 255) / 256), dim3(256), 0, st, natoms, ctx->d_pair_off,
                            rev_off, rev_pair, pair_src, s, F_dev);
     }
+    MLD_HIP(ctx, hipGetLastError());
+    return MLD_OK;
+}
+
+int mld_predict_stress_dev(mld_ctx* ctx, double sign_stress, double sign_stress_big_box, double* S_dev)
+{
+    if (!ctx || !S_dev) return MLD_E_ARG;
+    PredictState* ps = reinterpret_cast<PredictState*>(ctx->predict_state);
+    if (!ps || !ps->d_w) return mld_fail(ctx, MLD_E_STATE, "mld_predict_stress_dev: mld_model_set first");
+    if (ps->order != 1) return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_predict_stress_dev: stresses of linear models only");
+    if (ps->D != ctx->D) return mld_fail(ctx, MLD_E_STATE, "mld_predict_stress_dev: model dimension changed");
+    if (!ctx->last_grad) return mld_fail(ctx, MLD_E_STATE, "mld_predict_stress_dev: descriptors with gradients first");
+    const int natoms = ctx->natoms, nstruct = ctx->nstruct, D = ctx->D;
+    if (!nstruct) return MLD_OK;
+    const long long nrow = 3 * (ctx->npairs + natoms);
+    int rc = device_ensure(ctx, &ps->swork, &ps->cap_swork, (size_t)nrow * sizeof(double) + 64);
+    if (rc) return rc;
+    double* t = (double*)ps->swork;
+    hipStream_t st = ctx->stream;
+    if (nrow) {
+        const unsigned nb = (unsigned)((nrow + 3) / 4);
+        if (ctx->p.precision == MLD_PREC_FP32)
+            hipLaunchKernelGGL(row_dot_kernel<float>, dim3(nb), dim3(256), 0, st, nrow, D, ps->d_w,
+                               (const float*)ctx->last_grad, t);
+        else
+            hipLaunchKernelGGL(row_dot_kernel<double>, dim3(nb), dim3(256), 0, st, nrow, D, ps->d_w,
+                               (const double*)ctx->last_grad, t);
+    }
+    const int *rev_off, *rev_pair, *pair_src;
+    rc = pair_transpose(ctx, &rev_off, &rev_pair, &pair_src);
+    if (rc) return rc;
+    hipLaunchKernelGGL(stress_kernel, dim3(nstruct), dim3(256), 0, st, ctx->d_atom_off, ctx->d_pair_off, pair_src,
+                       ctx->d_pair_d, ctx->d_geo, t, sign_stress, sign_stress_big_box, S_dev);
     MLD_HIP(ctx, hipGetLastError());
     return MLD_OK;
 }

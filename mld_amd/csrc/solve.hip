@@ -475,8 +475,9 @@ extern "C" int mld_gram_set_keep_rows(mld_ctx* ctx, int keep)
 }
 
 // Minimum-norm solution of (G + Lambda) w = h in the sense of the reference's SVD
-// solve (serial_lsystem_by_svd, stats_tools.F90:1153-1213, LAPACK dgelsd with rcond =
-// machine epsilon for rcond < 0): eigenvalues at or below rcond * max |eigenvalue| are treated as zero.  For
+// solve (serial_lsystem_by_svd, stats_tools.F90:1153-1213, LAPACK dgelsd; rcond < 0 means LAPACK's
+// machine precision dlamch('E') = 2^-53): eigenvalues at or below rcond * max |eigenvalue| are
+// treated as zero.  For
 // the unique quadratic columns the solve runs on S (G + Lambda) S with S = sqrt(2) on the
 // off-diagonal columns, which is the full-layout system restricted to its range: the
 // result equals the reference's minimum-norm solution on the D^2 layout folded back.
@@ -553,7 +554,7 @@ static int solve_min_norm(mld_ctx* ctx, const std::vector<double>& ridge, double
     MLD_HIP(ctx, hipStreamSynchronize(st));
     double emax = 0.0;
     for (double e : ev) emax = std::max(emax, std::fabs(e));
-    const double thr = (rcond < 0.0 ? DBL_EPSILON : rcond) * emax;
+    const double thr = (rcond < 0.0 ? 0.5 * DBL_EPSILON : rcond) * emax;  // dlamch('E') = 2^-53
     double kept_min = emax;
     for (int i = 0; i < n2; ++i) {
         if (std::fabs(ev[i]) > thr) {

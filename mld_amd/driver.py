@@ -90,53 +90,126 @@ def descriptors(ml_path: str, out_dir: Optional[str] = None, precision: int = 0,
 
 
 def fit(ml_path: str, out_dir: Optional[str] = None, precision: int = 0, method: Optional[int] = None,
-        overrides: Optional[Dict[str, Any]] = None, export: bool = False) -> Tuple[np.ndarray, Dict[str, Any]]:
-    """Linear (snap_order 1) or quadratic (2) fit of the selected configurations: energy rows
-    weighted by w_e, force rows by w_f of their class (type_of_loss = 1), rows of classes whose
-    E / F flags are off left out; lambda_krr; the minimum-norm solve of the reference by default.
-    Writes <descriptor>_snap1_params.pot and lammps_<...> to out_dir when given; export returns the
-    accumulated G and h in diag as well."""
+        overrides: Optional[Dict[str, Any]] = None, export: bool = False,
+        predict: bool = False) -> Tuple[np.ndarray, Dict[str, Any]]:
+    """The fit of an &input_ml file and its db_model.in selection (fit_configs)."""
+    nml = {**inputs.read_input_ml(ml_path), **(overrides or {})}
+    base = os.path.dirname(os.path.abspath(ml_path))
+    return fit_configs(nml, select_configs(nml, base), out_dir, precision, method, export, predict)
+
+
+def fit_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]], out_dir: Optional[str] = None,
+                precision: int = 0, method: Optional[int] = None, export: bool = False,
+                predict: bool = False) -> Tuple[np.ndarray, Dict[str, Any]]:
+    """Linear (mld_order 1) or quadratic (2) fit of the configurations (mld.F90:279-420): energy rows
+    with the header energy iread_energy less ref_energy_per_element, force rows, and for the linear
+    model the stress rows of the classes whose S flag is on (train_fill_Amat_with_stress,
+    mld_stress.F90:181-268), each weighted by the loss's w_e / w_f / w_s (model_weights); rows of
+    classes whose flag is off are left out; lambda_krr; the minimum-norm solve of the reference by
+    default.  Writes <descriptor>_snap1_params.pot and lammps_<...> to out_dir when given; export
+    returns the accumulated G and h in diag; predict returns the fitted model's energies, forces and
+    stresses of every configuration in diag["predictions"] (config order: (E, F (nat, 3), S (6) or None))."""
     import torch
     from . import Context, fit as F
 
-    nml = {**inputs.read_input_ml(ml_path), **(overrides or {})}
     order = inputs.fit_order(nml)
     if order == 3:
         raise NotImplementedError("kernel fits need the kernel-point selection of the reference (MCD / CUR)")
     if order == 2 and inputs.quadratic_type(nml) != inputs.QUADRATIC_QML:
         raise NotImplementedError(f"mld_type_quadratic {inputs.quadratic_type(nml)}: only the full quadratic fit "
                                   "(mld_type_quadratic = 2) is built by this driver")
-    base = os.path.dirname(os.path.abspath(ml_path))
-    configs = select_configs(nml, base)
     weights = inputs.model_weights(nml, configs)
+    els = inputs.elements(nml)
+    eref = inputs.reference_energies(nml)
+    iread = int(nml["iread_energy"])
+    forces_on = bool(nml["desc_forces"])
+    sgn, sgn_big = float(nml["sign_stress"]), float(nml["sign_stress_big_box"])
     ctx = Context(**inputs.context_kwargs(nml, precision))
     try:
         gf = F.GramFit(ctx, order=order)
-        # one batch per class and model weight pair (loss_per gives every atom count its own weights)
-        by_class: Dict[Tuple[str, float, float], List] = {}
-        for (c, cl), (we, wf) in zip(configs, weights):
-            by_class.setdefault((cl.cls, we if cl.use_e else 0.0, wf if cl.use_f else 0.0), []).append((c, cl))
-        for (cls, w_e, w_f), group in by_class.items():
-            if w_e == 0.0 and w_f == 0.0:
+        # one batch per class and energy / force weight pair (loss_per gives every atom count its own
+        # weights); stress weights go per structure
+        groups: Dict[Tuple[str, float, float], List[int]] = {}
+        for k, ((c, cl), (we, wf, ws)) in enumerate(zip(configs, weights)):
+            groups.setdefault((cl.cls, we if cl.use_e else 0.0, wf if (cl.use_f and forces_on) else 0.0), []).append(k)
+        wsel = [ws if (cl.use_s and c.has_stress and forces_on) else 0.0 for (c, cl), (_, _, ws) in zip(configs, weights)]
+        if order != 1 and any(wsel):
+            raise NotImplementedError("stress rows are built for the linear model")
+        batches = []
+        for (cls, w_e, w_f), idx in groups.items():
+            if w_e == 0.0 and w_f == 0.0 and not any(wsel[k] for k in idx):
                 continue
-            b = _batch(ctx, group, inputs.elements(nml))
+            group = [configs[k] for k in idx]
+            b = _batch(ctx, group, els)
             e, g = b.alloc_outputs(True)
             b.bso4(e, g)
             for c, _ in group:
                 if (w_e and not c.has_energy) or (w_f and (not c.has_force or c.forces is None)):
                     raise ValueError(f"{c.name}: class {cls} fits {'energies' if w_e else ''} "
                                      f"{'forces' if w_f else ''} that the file does not hold")
-            E = np.array([c.energy for c, _ in group])
+            E = np.array([c.header_energy(iread) - eref[_species_index(c, els)].sum() for c, _ in group])
             Fh = np.concatenate([c.forces if w_f else np.zeros((len(c.pos), 3)) for c, _ in group]).reshape(-1)
             gf.add_batch(b, E, torch.from_numpy(Fh).to(b.device), w_e, w_f)
+            ws = np.array([wsel[k] for k in idx])
+            if ws.any():
+                S = np.array([configs[k][0].stress if wsel[k] else np.zeros(6) for k in idx])
+                gf.add_stress(b, S, ws, sgn, sgn_big)
+            batches.append(idx)
         m = F.SOLVE_SVD if method is None else method
         w, diag = gf.solve(inputs.ridge(nml), m, rcond=float(nml["svd_rcond"]))
         if export:  # the normal equations as accumulated (diag["G"], diag["h"])
             diag["G"], diag["h"] = gf.export()
+        if predict:
+            diag["predictions"] = _predict(ctx, order, w, configs, [configs[k] for idx in batches for k in idx],
+                                           [k for idx in batches for k in idx], els, sgn, sgn_big)
         D = ctx.D
     finally:
         ctx.close()
     if out_dir:
-        F.write_coefficients(out_dir, w, D, order, inputs.elements(nml), float(nml["j_max"]), float(nml["r_cut"]),
+        F.write_coefficients(out_dir, w, D, order, els, float(nml["j_max"]), float(nml["r_cut"]),
                              lbso4_diag=bool(nml["lbso4_diag"]))
     return w, diag
+
+
+def _predict(ctx, order: int, w: np.ndarray, configs, group, index, els: List[str], sgn: float, sgn_big: float):
+    """Energies, forces and (linear model) stresses of the configurations group (positions index in
+    configs) with the coefficients w, in batches of up to 4,096 atoms."""
+    from . import fit as F
+
+    F.model_set(ctx, order, np.asarray(w, np.float64))
+    preds: List[Any] = [None] * len(configs)
+    start = 0
+    while start < len(group):
+        stop, nat = start, 0
+        while stop < len(group) and (stop == start or nat + group[stop][0].nat <= 4096):
+            nat += group[stop][0].nat
+            stop += 1
+        b = _batch(ctx, group[start:stop], els)
+        e, g = b.alloc_outputs(True)
+        b.bso4(e, g)
+        Ep, Fp, _ = F.predict(ctx, b)
+        Sp = F.predict_stress(ctx, b, sgn, sgn_big).cpu().numpy() if order == 1 else None
+        Ep, Fp = Ep.cpu().numpy(), Fp.cpu().numpy()
+        for j in range(stop - start):
+            preds[index[start + j]] = (float(Ep[j]), Fp[b.atom_off[j]:b.atom_off[j + 1]].copy(),
+                                       None if Sp is None else Sp[j].copy())
+        start = stop
+    return preds
+
+
+def predict_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]], w: np.ndarray,
+                    precision: int = 0) -> List[Tuple[float, np.ndarray, Optional[np.ndarray]]]:
+    """Energies, forces and stresses (linear model) of the configurations with the coefficients w: the
+    evaluation the reference runs for its train / test sets (train_snap_compute_energy_force_stress,
+    mld.F90:944-1020; main_test_mld, :2506-2743).  One (E, F (nat, 3), S (6) or None) per configuration."""
+    from . import Context
+
+    order = inputs.fit_order(nml)
+    if order == 3:
+        raise NotImplementedError("kernel models are evaluated through the library's kernel features")
+    ctx = Context(**inputs.context_kwargs(nml, precision))
+    try:
+        return _predict(ctx, order, w, configs, configs, list(range(len(configs))), inputs.elements(nml),
+                        float(nml["sign_stress"]), float(nml["sign_stress_big_box"]))
+    finally:
+        ctx.close()

@@ -78,6 +78,18 @@ class GramFit:
         self.ctx.check(self.ctx.L.mld_gram_add_batch(self.ctx.h, self.order, E.ctypes.data_as(C.c_void_p),
                                                      C.c_void_p(F_dev.data_ptr()), w_e, w_f, int(self.mixed)))
 
+    def add_stress(self, batch, S: np.ndarray, w_s: np.ndarray, sign_stress: float = 1.0,
+                   sign_stress_big_box: float = 1.0):
+        """The six stress rows of every structure of the batch with w_s[s] > 0 (linear model):
+        S (nstruct, 6) Voigt targets xx yy zz yz xz xy, w_s (nstruct) weights (mld_gram_add_stress)."""
+        S = np.ascontiguousarray(S, np.float64).reshape(-1, 6)
+        w_s = np.ascontiguousarray(w_s, np.float64).reshape(-1)
+        if S.shape[0] != batch.nstruct or w_s.shape[0] != batch.nstruct:
+            raise ValueError("add_stress: S (nstruct, 6) and w_s (nstruct) of the batch")
+        self.ctx.check(self.ctx.L.mld_gram_add_stress(self.ctx.h, self.order, S.ctypes.data_as(C.c_void_p),
+                                                      w_s.ctypes.data_as(C.c_void_p), float(sign_stress),
+                                                      float(sign_stress_big_box)))
+
     def add_structures(self, pos: Sequence[np.ndarray], cells: Sequence[np.ndarray], types: Sequence[np.ndarray],
                        E: np.ndarray, F: Sequence[np.ndarray], w_e: float = 1.0, w_f: float = 1.0):
         """Configurations from host arrays, end to end (mld_gram_add_structures): pos[s] (nat, 3),
@@ -132,7 +144,7 @@ class GramFit:
 
     def solve(self, lam: float, method: int = SOLVE_CHOL_IR, rcond: float = -1.0, allow_unconverged: bool = False):
         """(G + lambda) w = h; method SOLVE_SVD gives the minimum-norm solution with eigenvalues at
-        or below rcond * max dropped (rcond < 0: machine epsilon, the reference's svd_rcond = -1).
+        or below rcond * max dropped (rcond < 0: LAPACK's machine precision 2^-53, the reference's svd_rcond = -1).
         SOLVE_CHOL_IR raises MldError (code E_NOT_CONVERGED) when the refinement stops above the
         library's residual tolerance, unless allow_unconverged: then the best iterate comes back
         with diag["converged"] = False."""
@@ -224,6 +236,19 @@ def predict(ctx, batch, energies: bool = True, forces: bool = True, d2: bool = F
     ctx.check(ctx.L.mld_predict_dev(ctx.h, ptr(E), ptr(F), ptr(D2)))
     ctx.synchronize()
     return E, F, D2
+
+
+def predict_stress(ctx, batch, sign_stress: float = 1.0, sign_stress_big_box: float = 1.0):
+    """Stresses (nstruct, 6) of a linear model on the batch's last descriptors (mld_predict_stress_dev)."""
+    import torch
+
+    from .batch import torch_ready
+
+    S = torch.empty((batch.nstruct, 6), dtype=torch.float64, device=batch.device)
+    torch_ready(batch.device)
+    ctx.check(ctx.L.mld_predict_stress_dev(ctx.h, float(sign_stress), float(sign_stress_big_box), C.c_void_p(S.data_ptr())))
+    ctx.synchronize()
+    return S
 
 
 def write_coefficients(directory: str, w: np.ndarray, D: int, order: int, elements, j_max: float, r_cut: float,

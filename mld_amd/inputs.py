@@ -46,6 +46,11 @@ DEFAULTS: Dict[str, Any] = {
     "type_of_loss": 1,            # :530 (1 loss_init, 2 loss_fair, 3 loss_per; ml_in_ndm_module.F90:1407-1409)
     "factor_energy_error": 1.0,   # :391
     "factor_force_error": 1.0,    # :390
+    "factor_stress_error": 1.0,   # :392
+    "sign_stress": 1.0,           # :374
+    "sign_stress_big_box": 1.0,   # :375
+    "iread_energy": 2,            # :330 (1 E_total, 2 E_fit1, 3 E_fit2 of the poscar header)
+    "ref_energy_per_element": "0.d0",  # :512
     "r_cut_width": 0.5,           # :429
     "r_cut_in": -1.2,             # :442
     "zetaace_order": 1,           # :279
@@ -242,17 +247,18 @@ def quadratic_type(nml: Dict[str, Any]) -> int:
     return _pair_key(nml, "mld_type_quadratic", "snap_type_quadratic", QUADRATIC_QNML)
 
 
-def model_weights(nml: Dict[str, Any], configs) -> List[Tuple[float, float]]:
-    """Per-configuration (w_e_model, w_f_model) of the loss (main_train_regularization.F90:311-372):
-    type_of_loss 1 multiplies the db_model.in class weights by factor_energy_error /
-    factor_force_error; 2 (loss_fair) also divides them by the share of energy / force data
-    points in the training set (no_ene = n_E / (n_E + n_F + n_S), n_F = 3 N per configuration
-    with forces, n_S = 6 per configuration with a stress, mld.F90:636-650); 3 (loss_per) divides
-    them by N and 3 N of the configuration.  configs: (io.Config, io.DbClass) pairs."""
+def model_weights(nml: Dict[str, Any], configs) -> List[Tuple[float, float, float]]:
+    """Per-configuration (w_e_model, w_f_model, w_s_model) of the loss
+    (main_train_regularization.F90:311-400): type_of_loss 1 multiplies the db_model.in class
+    weights w_e, w_f, w_s by factor_energy_error / factor_force_error / factor_stress_error;
+    2 (loss_fair) also divides them by the share of energy / force / stress data points in the
+    training set (no_ene = n_E / (n_E + n_F + n_S), n_F = 3 N per configuration with forces,
+    n_S = 6 per configuration with a stress, mld.F90:636-650); 3 (loss_per) divides them by N,
+    3 N and 3 N of the configuration.  configs: (io.Config, io.DbClass) pairs."""
     lt = int(nml["type_of_loss"])
     if lt not in (1, 2, 3):
         raise ValueError(f"unknown loss type type_of_loss {lt}")
-    fe, ff = float(nml["factor_energy_error"]), float(nml["factor_force_error"])
+    fe, ff, fs = float(nml["factor_energy_error"]), float(nml["factor_force_error"]), float(nml["factor_stress_error"])
     forces_on = bool(nml["desc_forces"])
     n_e = sum(1 for c, _ in configs if c.has_energy)
     n_f = sum(3 * c.nat for c, _ in configs if c.has_force and forces_on)
@@ -261,12 +267,24 @@ def model_weights(nml: Dict[str, Any], configs) -> List[Tuple[float, float]]:
     tot = no_e + no_f + no_s
     out = []
     for c, cl in configs:
-        we, wf = float(cl.weights[0]), float(cl.weights[1])
+        we, wf, ws = float(cl.weights[0]), float(cl.weights[1]), float(cl.weights[2])
         if n_e > 0:
             we = we * fe if lt == 1 else (we / (no_e / tot) * fe if lt == 2 else we / c.nat * fe)
         if n_f > 0:
             wf = wf * ff if lt == 1 else (wf / (no_f / tot) * ff if lt == 2 else wf / (3 * c.nat) * ff)
-        out.append((we, wf))
+        if n_s > 0:
+            ws = ws * fs if lt == 1 else (ws / (no_s / tot) * fs if lt == 2 else ws / (3 * c.nat) * fs)
+        out.append((we, wf, ws))
+    return out
+
+
+def reference_energies(nml: Dict[str, Any]) -> np.ndarray:
+    """ref_energy_per_element per chemical element (read_ml_file.F90:512, 1519-1535): missing
+    items are 0."""
+    els = elements(nml)
+    vals = _floats(nml["ref_energy_per_element"])
+    out = np.zeros(len(els))
+    out[:min(len(els), len(vals))] = vals[:len(els)]
     return out
 
 

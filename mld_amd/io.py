@@ -61,7 +61,9 @@ class Config:
     species: List[str]
     energy: float = 0.0      # E_total as read
     forces: Optional[np.ndarray] = None  # (nat, 3)
-    stress: Optional[np.ndarray] = None
+    stress: Optional[np.ndarray] = None  # (6,) Voigt xx yy zz yz xz xy as read
+    e_fit1: float = 0.0      # the header's E_fit1 and E_fit2
+    e_fit2: float = 0.0
     has_energy: bool = True
     has_force: bool = False
     has_stress: bool = False
@@ -70,6 +72,11 @@ class Config:
     @property
     def nat(self) -> int:
         return int(self.pos.shape[0])
+
+    def header_energy(self, iread_energy: int = 2) -> float:
+        """E_total, E_fit1 or E_fit2 of the header for iread_energy 1, 2, 3 (config_real%energy,
+        build_subdata.F90:2905-2913, before the reference energies of the atoms are taken off)."""
+        return {1: self.energy, 2: self.e_fit1, 3: self.e_fit2}[int(iread_energy)]
 
 
 def _has_number(line: str) -> bool:
@@ -92,6 +99,8 @@ def read_poscar(path: str, read_forces: bool = True) -> Config:
     species = [head[2 + 2 * i] for i in range(nsp)]
     rest = head[2 + 2 * nsp:]
     e_total = float(rest[0]) if rest else 0.0
+    e_fit1 = float(rest[1]) if len(rest) > 1 else e_total
+    e_fit2 = float(rest[2]) if len(rest) > 2 else e_fit1
     alat = float(lines[1].split()[0])
     box = np.array([[float(t) for t in lines[2 + i].split()[:3]] for i in range(3)]) * alat
     li = 5
@@ -118,7 +127,7 @@ def read_poscar(path: str, read_forces: bool = True) -> Config:
                 stress = np.array([float(t) for t in lines[li2 + 1].split()[:6]])
         except (ValueError, IndexError):
             forces = None
-    return Config(pos=pos, cell=box, itype=itype, species=species, energy=e_total,
+    return Config(pos=pos, cell=box, itype=itype, species=species, energy=e_total, e_fit1=e_fit1, e_fit2=e_fit2,
                   forces=forces, stress=stress, has_energy=efs[0] == "1",
                   has_force=len(efs) > 1 and efs[1] == "1" and forces is not None,
                   has_stress=len(efs) > 2 and efs[2] == "1" and stress is not None,

@@ -25,10 +25,9 @@ n_max, l_max of that order and mu_max species:
   RPI         ace_gencg = 2 and nu > 3: the Gram G_{L L'} = sum over the permutations sigma that keep
               the (n, l, mu) tuples, sum_m cg(m, L) cg(sigma m, L'); its singular values above the
               rank cut (_canonical_basis) keep rpicg = cg u / sqrt(s) (base_cnlm_compute_rpi_gencg,
-              :1278-1439; module_svd_small_matrix, stats_tools.F90:2032-2080).  The singular vectors of equal
-              singular values are any orthonormal basis of their space (dgesvd's choice); this
-              restatement takes a canonical one: Gram-Schmidt of the space's projector applied to
-              e_1, e_2, ... (the span, and so every fit, is the same);
+              :1278-1439; module_svd_small_matrix, stats_tools.F90:2032-2080), u the singular
+              vectors after svd_enforce_signs_by_ref's rotation of equal singular values and its
+              sign rule (:2114-2253), which depend on the singular spaces only;
   B           Re sum_m c(m) prod_i A(mu_i, n_i, l_i, m_i) (compute_B_basis_atom_adjoint,
               compute_ace_AB.F90:2023-2340), one per column of c; order: the (mu, n, l) entries, then
               the columns.
@@ -179,40 +178,46 @@ def _canonical(lbold, n, mu):
 
 
 def _canonical_basis(G: np.ndarray, tol: float = 1e-8):
-    """Singular values s > tol s_max of the symmetric PSD G and a canonical orthonormal basis of each
-    singular space (eigenvalues within 1e-9 s_max of each other grouped): Gram-Schmidt of the
-    space's projector applied to e_1, e_2, ...; returns (s_k, u_k) with s descending.
-    The reference keeps s > 1e-14 s_max (stats_tools.F90:2060-2068).  These Grams are multiples of
-    projectors: every kept value equals s_max to rounding and the rest are zeros that rounding lifts
-    to 3.6e-15 s_max when 720 permutations are summed (nu = 6), so 1e-8 keeps the same functions
+    """Singular values s > tol s_max of the symmetric PSD G with the singular vectors that
+    svd_enforce_signs_by_ref (stats_tools.F90:2114-2253) fixes; returns (s_k, u_k), s descending.
+    Runs of values within 1e-12 s_max of the run's first are rotated by the eigenvectors (ascending)
+    of V_b^T diag(1..n) V_b; each vector's sign then makes its entry sum positive, or, with that sum
+    within 1e-14 sqrt(n) of zero, its first largest-magnitude entry.  The result depends only on
+    each run's space (not on the eigen-solver's basis) while V_b^T diag V_b has distinct eigenvalues.
+    The reference keeps s > 1e-14 s_max (:2060-2068).  These Grams are multiples of projectors:
+    every kept value equals s_max to rounding and the rest are zeros that rounding lifts to
+    3.6e-15 s_max when 720 permutations are summed (nu = 6), so 1e-8 keeps the same functions
     wherever the reference's cut is not decided by rounding."""
     w, V = np.linalg.eigh((G + G.T) / 2)
-    order = np.argsort(-w)
-    w, V = w[order], V[:, order]
-    smax = max(abs(w[0]), 1e-300) if len(w) else 0.0
-    keep = [k for k in range(len(w)) if abs(w[k]) > tol * smax]
+    order = np.argsort(-w, kind="stable")
+    return _fix_singular_vectors(w[order], V[:, order], tol)
+
+
+def _fix_singular_vectors(w: np.ndarray, V: np.ndarray, tol: float = 1e-8):
+    """_canonical_basis on a given eigen-decomposition: w descending, V's columns orthonormal."""
+    n = V.shape[0]
+    if n == 0:
+        return []
+    smax = max(abs(w[0]), 1e-300)
+    r = int(np.sum(np.abs(w) > tol * smax))
     out = []
     k = 0
-    while k < len(keep):
-        j = k
-        while j + 1 < len(keep) and abs(w[keep[j + 1]] - w[keep[k]]) <= 1e-9 * smax:
+    while k < r:
+        j = k + 1
+        while j < r and abs(w[j] - w[k]) < 1e-12 * smax:
             j += 1
-        cols = [keep[q] for q in range(k, j + 1)]
-        P = V[:, cols] @ V[:, cols].T
-        lam = float(np.mean(w[cols]))
-        basis: List[np.ndarray] = []
-        for e in range(G.shape[0]):
-            v = P[:, e].copy()
-            for b in basis:
-                v -= (b @ v) * b
-            nv = np.linalg.norm(v)
-            if nv > 1e-3:
-                basis.append(v / nv)
-            if len(basis) == len(cols):
-                break
-        for b in basis:
-            out.append((lam, b))
-        k = j + 1
+        Vb = V[:, k:j]
+        if j - k > 1:
+            H = Vb.T @ (np.arange(1, n + 1, dtype=np.float64)[:, None] * Vb)
+            eh, E = np.linalg.eigh((H + H.T) / 2)
+            Vb = Vb @ E[:, np.argsort(eh, kind="stable")]
+        for q in range(j - k):
+            u = Vb[:, q].copy()
+            proj = u.sum() / sqrt(n)
+            if proj < 0.0 or (abs(proj) <= 1e-14 and u[int(np.argmax(np.abs(u)))] < 0.0):
+                u = -u
+            out.append((float(w[k + q]), u))
+        k = j
     return out
 
 

@@ -23,6 +23,7 @@
  */
 #include <complex.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -34,7 +35,7 @@ typedef double complex cplx;
  * (spherical_4d recursion, Z packing, zaxpy contraction, Gram).  Counts
  * complex multiply-adds (cFMA); one cFMA = 8 real flops.
  * ------------------------------------------------------------------------ */
-static uint64_t g_cfma = 0;
+static uint64_t g_cfma = 0; /* exact for single-threaded use (the flop-model test) */
 uint64_t orc_cfma_count(void) { return g_cfma; }
 void orc_cfma_reset(void) { g_cfma = 0; }
 
@@ -99,9 +100,16 @@ static inline size_t cgti(int jj, int j1, int m1, int j2, int m2, int j)
     size_t n = (size_t)jj + 1, s = 2 * (size_t)jj + 1;
     return ((((size_t)j1 * s + (m1 + jj)) * n + j2) * s + (m2 + jj)) * n + j;
 }
+static pthread_mutex_t g_cgmu = PTHREAD_MUTEX_INITIALIZER;
+/* Builds the table for jj once; callers on several threads must all use one jj (the table of
+ * another jj replaces it). */
 static void cg_table(int jj)
 {
-    if (g_cgjj == jj) return;
+    pthread_mutex_lock(&g_cgmu);
+    if (g_cgjj == jj) {
+        pthread_mutex_unlock(&g_cgmu);
+        return;
+    }
     free(g_cgtab);
     size_t n = (size_t)jj + 1, s = 2 * (size_t)jj + 1;
     g_cgtab = calloc(n * s * n * s * n, sizeof(double));
@@ -112,6 +120,7 @@ static void cg_table(int jj)
                     for (int j = 0; j <= jj; ++j)
                         g_cgtab[cgti(jj, j1, m1, j2, m2, j)] = orc_cg(j1, m1, j2, m2, j, m1 + m2);
     g_cgjj = jj;
+    pthread_mutex_unlock(&g_cgmu);
 }
 #define CGV(j1, m1, j2, m2, j) g_cgtab[cgti(jj, j1, m1, j2, m2, j)]
 

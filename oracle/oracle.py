@@ -158,6 +158,34 @@ def rows(energy, grad, nl, order=1, want_force=True):
     return out
 
 
+VOIGT = ((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1))  # (force component x, vector component y)
+
+
+def stress_rows(grad, nl, cell_rows, D, small, sign_stress=1.0, sign_big=1.0):
+    """Linear stress rows (6, 1 + D) of one configuration: para_pack_stress_descriptor
+    (mld_stress.F90:272-756) for the linear model, placed by train_fill_Amat_with_stress (:181-268).
+    For the Voigt components c = (x, y) in VOIGT,
+        row[c][1 + d] = sign_stress * 3 / V * f * sum over atoms a and neighbours k >= 1 of
+                        g_{a,k}(x, d) * u_k(y) / r_k,
+    g the gradient slot dB_a/dr_k, V = |det(cell)| (calc_volume, neighbours.F90:1331-1368);
+    in a small cell u_k = r_k - r_a (u_ij, neighbours.F90:1273-1301) and f = 1, in a large cell
+    u_k = r_a - r_k with the force scaled by sign_stress_big_box, f = -sign_big here with
+    u = r_k - r_a; column 0 is 0."""
+    off, nbr, dxyz, rr = nl
+    nat = len(off) - 1
+    V = abs(float(np.linalg.det(np.asarray(cell_rows, np.float64).reshape(3, 3))))
+    f = 1.0 if small else -sign_big
+    out = np.zeros((6, 1 + D))
+    if grad is None or len(nbr) == 0:
+        return out
+    slots = np.concatenate([np.arange(off[a] + a + 1, off[a + 1] + a + 1) for a in range(nat)]).astype(np.int64)
+    g = grad[slots]                      # (P, 3, D), pair order
+    unit = dxyz / rr[:, None]            # (P, 3)
+    for c, (x, y) in enumerate(VOIGT):
+        out[c, 1:] = sign_stress * 3.0 / V * f * (g[:, x, :] * unit[:, y:y + 1]).sum(axis=0)
+    return out
+
+
 def gram_accumulate(G, h, rows_, y, w):
     mc, D = rows_.shape
     lib().orc_gram_accumulate(D, mc, np.ascontiguousarray(rows_), np.ascontiguousarray(y, np.float64),
@@ -185,12 +213,13 @@ def standardise_kernel_points(draft: np.ndarray) -> np.ndarray:
 
 
 def solve_svd(G: np.ndarray, h: np.ndarray, lam: float, rcond: float = -1.0) -> np.ndarray:
-    """gram_reduce_and_solve (module_online_fit.F90:550-590): symmetrise, +lambda I, dgelsd."""
+    """gram_reduce_and_solve (module_online_fit.F90:550-590): symmetrise, +lambda I, dgelsd; rcond < 0
+    is LAPACK's machine precision dlamch('E') = 2^-53 (half of NumPy's eps)."""
     import scipy.linalg as sl
 
     Gs = np.triu(G) + np.triu(G, 1).T
     Gs = Gs + max(lam, 0.0) * np.eye(G.shape[0])
-    cond = None if rcond < 0 else rcond
+    cond = 2.0 ** -53 if rcond < 0 else rcond
     w, *_ = sl.lstsq(Gs, h, cond=cond, lapack_driver="gelsd")
     return w
 

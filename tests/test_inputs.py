@@ -80,7 +80,7 @@ def test_quadratic_type_and_order_keys():
 
 
 def test_model_weights_loss_types():
-    """main_train_regularization.F90:311-372 on three configurations of two classes: one without
+    """main_train_regularization.F90:311-400 on three configurations of two classes: one without
     forces, one with a stress (n_E = 3, n_F = 3 (2 + 4) = 18, n_S = 6)."""
     from mld_amd import io
 
@@ -91,16 +91,30 @@ def test_model_weights_loss_types():
     ca = io.DbClass(cls="01", klm="111", n_total=1, n_selected=1, use_e=True, use_f=True, use_s=False,
                     weights=(2.0, 0.5, 1.0))
     cb = io.DbClass(cls="02", klm="111", n_total=1, n_selected=1, use_e=True, use_f=True, use_s=False,
-                    weights=(1.0, 0.25, 1.0))
+                    weights=(1.0, 0.25, 4.0))
     configs = [(cfg(2, True, False), ca), (cfg(3, False, False), ca), (cfg(4, True, True), cb)]
-    base = {**inputs.DEFAULTS, "factor_energy_error": 3.0, "factor_force_error": 5.0}
+    base = {**inputs.DEFAULTS, "factor_energy_error": 3.0, "factor_force_error": 5.0, "factor_stress_error": 7.0}
     w1 = inputs.model_weights({**base, "type_of_loss": 1}, configs)
-    assert w1 == [(6.0, 2.5), (6.0, 2.5), (3.0, 1.25)]
+    assert w1 == [(6.0, 2.5, 7.0), (6.0, 2.5, 7.0), (3.0, 1.25, 28.0)]
     tot = 3 + 18 + 6
     w2 = inputs.model_weights({**base, "type_of_loss": 2}, configs)
-    assert np.allclose(w2, [(2.0 / (3 / tot) * 3, 0.5 / (18 / tot) * 5)] * 2 + [(1.0 / (3 / tot) * 3, 0.25 / (18 / tot) * 5)])
+    assert np.allclose(w2, [(2.0 / (3 / tot) * 3, 0.5 / (18 / tot) * 5, 1.0 / (6 / tot) * 7)] * 2 +
+                       [(1.0 / (3 / tot) * 3, 0.25 / (18 / tot) * 5, 4.0 / (6 / tot) * 7)])
     w3 = inputs.model_weights({**base, "type_of_loss": 3}, configs)
-    assert np.allclose(w3, [(2.0 / 2 * 3, 0.5 / 6 * 5), (2.0 / 3 * 3, 0.5 / 9 * 5), (1.0 / 4 * 3, 0.25 / 12 * 5)])
+    assert np.allclose(w3, [(2.0 / 2 * 3, 0.5 / 6 * 5, 1.0 / 6 * 7), (2.0 / 3 * 3, 0.5 / 9 * 5, 1.0 / 9 * 7),
+                            (1.0 / 4 * 3, 0.25 / 12 * 5, 4.0 / 12 * 7)])
+
+
+def test_header_energies_and_reference_energies():
+    """The fitted energy is E_fit1 of the poscar header (iread_energy = 2) less ref_energy_per_element of
+    the atoms (build_subdata.F90:2905-2913): fit17's first file has E_total -594.5425001, E_fit1 14.2547..."""
+    from mld_amd import io
+
+    c = io.read_poscar(os.path.join(G, "fit17_DB", "01_111_000001.poscar"))
+    assert c.energy == -594.5425001 and c.e_fit1 == 14.254739149999978 and c.header_energy() == c.e_fit1
+    assert c.header_energy(1) == c.energy and c.header_energy(3) == 14.254739149999978
+    nml = {**inputs.DEFAULTS, "chemical_elements": "Ta W", "ref_energy_per_element": "-1.5d0"}
+    assert list(inputs.reference_energies(nml)) == [-1.5, 0.0]
 
 
 def test_ace_keys_of_fit21_and_fit24():
@@ -111,7 +125,7 @@ def test_ace_keys_of_fit21_and_fit24():
     want = {"fit21": (1, [14, 4, 3, 2, 1, 1], [0, 3, 2, 2, 1, 1], 1.2, 207, 207),
             "fit24": (4, [14, 2, 1, 1, 1, 1], [0, 2, 1, 1, 1, 1], 1.7, 894, 3576)}
     for name, (nsp, nmax, lmax, rin, ace_dim, D) in want.items():
-        nml = inputs.read_input_ml(os.path.join(G, f"{name}_vacancy.ml"))
+        nml = inputs.read_input_ml(os.path.join(G, f"{name}_ace_input.ml"))
         kw = inputs.context_kwargs(nml)
         assert kw["descriptor"] == "ace" and kw["nspecies"] == nsp
         assert kw["ace_active"] == [True, True, True, True, True, False]
@@ -121,7 +135,7 @@ def test_ace_keys_of_fit21_and_fit24():
         r = M.ace_basis_review(nspecies=nsp, ace_numax=kw["ace_numax"], ace_nmax=nmax, ace_lmax=lmax,
                                ace_active=kw["ace_active"], ace_gencg=2, ace_chem=1)
         assert (r["ace_dim"], r["D"]) == (ace_dim, D)
-    nml = inputs.read_input_ml(os.path.join(G, "fit21_vacancy.ml"))
+    nml = inputs.read_input_ml(os.path.join(G, "fit21_ace_input.ml"))
     for key, val in (("ace_radial_chem", 3), ("zetaace_order", 2), ("l_ace_set_rcut", True)):
         with pytest.raises(NotImplementedError):
             inputs.context_kwargs({**nml, key: val})

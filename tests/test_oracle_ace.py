@@ -3,6 +3,7 @@
 SciPy, the Chebyshev cosine form, the nu = 2 power spectrum, single-bond products), rotation and
 permutation invariance, central differences, and the span of the RPI basis against the RI basis.
 The library's own enumeration (mld_ace_basis_review, host only) is checked against both."""
+import itertools
 import math
 
 import numpy as np
@@ -284,6 +285,49 @@ def test_rpi_spans_the_ri_space():
     assert np.abs(X @ coef - Y).max() < 1e-9 * np.abs(Y).max()
     s = np.linalg.svd(X, compute_uv=False)
     assert int((s > 1e-9 * s[0]).sum()) == rpi.ace_dim
+
+
+def test_rpi_singular_vectors_are_fixed_by_the_space():
+    """svd_enforce_signs_by_ref's rule picks the vectors from the singular space alone: known answers
+    on small spaces, and the same vectors whatever orthonormal basis of each space the eigen-solver
+    returned (a random rotation within the space), on the RPI Grams of fit24's nu = 4 and 5."""
+    e = np.eye(3)
+    # span{e1 + e2, e3}: V^T diag(1, 2, 3) V = diag(1.5, 3) -> (e1 + e2)/sqrt2 first; sums positive
+    V = np.stack([-e[2], (e[0] + e[1]) / math.sqrt(2)], axis=1)
+    got = A._fix_singular_vectors(np.array([2.0, 2.0]), V)
+    assert np.allclose(got[0][1], (e[0] + e[1]) / math.sqrt(2), atol=1e-15)
+    assert np.allclose(got[1][1], e[2], atol=1e-15)
+    # span{e1 - e2}: entry sum 0 -> the first largest-magnitude entry is made positive
+    got = A._fix_singular_vectors(np.array([1.0]), ((e[1] - e[0]) / math.sqrt(2))[:, None])
+    assert np.allclose(got[0][1], (e[0] - e[1]) / math.sqrt(2), atol=1e-15)
+    rng = np.random.default_rng(8)
+    checked = 0
+    for nu in (4, 5):
+        ob = A.OrderBasis(nu, 1, 1, 4, 2)
+        for mu, n, lb, L in ob.entries:
+            M0 = A.m0_space(lb)
+            C = np.array([[A.tree_cg(ob.tree, Lr, m) for Lr in L] for m in M0])
+            trip = list(zip(n, lb, mu))
+            where = {tuple(m): im for im, m in enumerate(M0)}
+            G = np.zeros((len(L), len(L)))
+            for p in itertools.permutations(range(nu)):
+                if all(trip[p[i]] == trip[i] for i in range(nu)):
+                    G += C.T @ C[[where[tuple(m[p[i]] for i in range(nu))] for m in M0]]
+            w, V = np.linalg.eigh(G)
+            o = np.argsort(-w)
+            w, V = w[o], V[:, o]
+            r = int((w > 1e-8 * w[0]).sum())
+            if r < 2:
+                continue
+            Q, _ = np.linalg.qr(rng.normal(size=(r, r)))
+            V2 = V.copy()
+            V2[:, :r] = V[:, :r] @ Q
+            a, b = A._fix_singular_vectors(w, V), A._fix_singular_vectors(w, V2)
+            assert len(a) == len(b) == r
+            for (_, u1), (_, u2) in zip(a, b):
+                assert np.abs(u1 - u2).max() < 1e-12
+            checked += 1
+    assert checked >= 20
 
 
 def test_structure_blocks_and_self_slot():
