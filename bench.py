@@ -74,17 +74,21 @@ PEAK_FP32, PEAK_FP64 = 157.3, 78.6   # TFLOP/s, MI355X dense
 # HBM bytes of one bso4_kernel<float,8> launch on the first 500-structure batch of the
 # quadratic-fit workload (96,582 atoms, 3,020,350 pairs), from rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE passes scaled by a 1 GiB copy (factors 2.0 and 1.0):
-# (315,683 x 2.0 + 2,037,535 x 1.0) KiB; the launch's algorithmic bytes are 2,177,392,328
+# (315,677 x 2.0 + 2,037,552 x 1.0) KiB; the launch's algorithmic bytes are 2,177,392,328
 # (the excess is the per-pair geometry record, read by the density and gradient phases).
-TRAFFIC_BSO4_F32_FIT_BATCH = (315683 * 2.0 + 2037535 * 1.0) * 1024
+TRAFFIC_BSO4_F32_FIT_BATCH = (315677 * 2.0 + 2037552 * 1.0) * 1024
 ALGO_BYTES_BSO4_F32_FIT_BATCH = 2177392328.0
 # The same for bso4_kernel<double,8> (the dominant kernel of the fp64 form), and the average
-# syrk128_f64_kernel launch of that workload, from the committed profiles named below.
+# syrk128_f64_kernel launch of that workload, from the committed profile named below:
+# (596,317 x 2.0 + 4,066,581 x 1.0) KiB against 4,255,815,488 algorithmic bytes; the FP64 SYRK's
+# launch averages 14.768 ms in the kernel trace (1,000 structures) and moves (6,471,472 x 2.0 +
+# 582,400) KiB per launch.
 PROFILE_BSO4_F64 = "profiles/fit_quadratic_kernel_trace_and_hbm_counters.md"
 PROFILE_FIT = "profiles/fit_quadratic_kernel_trace_and_hbm_counters.md"
-TRAFFIC_BSO4_F64_FIT_BATCH = None
-ALGO_BYTES_BSO4_F64_FIT_BATCH = None
-SYRK_F64_MS_PER_BATCH = None
+TRAFFIC_BSO4_F64_FIT_BATCH = (596317 * 2.0 + 4066581 * 1.0) * 1024
+ALGO_BYTES_BSO4_F64_FIT_BATCH = 4255815488.0
+SYRK_F64_MS_PER_BATCH = 14.768
+TRAFFIC_SYRK_F64_FIT_BATCH = (6471472 * 2.0 + 582400 * 1.0) * 1024
 
 
 def algorithmic_flops(natoms: int, npairs: int) -> float:
@@ -183,7 +187,7 @@ def main_desc(args):
                  "kernel_ms": r64["kernel_ms"], "achieved_tflops": ach64, "frac": ach64 / PEAK_FP64},
         "mixed_speedup_vs_fp64": r64["s_per_step"] / r32["s_per_step"],
         "err_fp32_vs_fp64": {"B_max_rel": err_b, "dBdr_max_rel": err_g},
-        "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
+        "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
                      "traffic": None, "kernel": "bso4_kernel<float,8>", "kernel_ms": r32["kernel_ms"],
                      "algorithmic_flops_per_launch": flops,
                      "algorithmic_bytes_per_launch": algorithmic_bytes(natoms, npairs, D, 4)},
@@ -353,9 +357,12 @@ def cpu_fit(structs, cfg, budget_s: float = 15.0):
 
 def main_fit(args):
     structs, cfg = fit_workload(args.workload, args.scale)
-    res = {f: run_fit(structs, cfg, f, args.steps, args.warmup) for f in FORMS}
+    forms = [f for f in FORMS if f in args.forms.split(",")]
+    if "fp64" not in forms:
+        forms.insert(0, "fp64")
+    res = {f: run_fit(structs, cfg, f, args.steps, args.warmup) for f in forms}
     # FP64's own coefficient spread: the same fit with another batching (another summation order)
-    alt = run_fit(structs, cfg, "fp64", 1, 0, chunk=max(1, cfg["chunk"] * 3 // 5))
+    alt = run_fit(structs, cfg, "fp64", 1, 0, chunk=max(1, cfg["chunk"] * 3 // 5)) if args.spread else None
     r64 = res["fp64"]
     natoms, npairs = r64["natoms"], r64["npairs"]
     flops = algorithmic_flops(natoms, npairs)
@@ -386,16 +393,7 @@ def main_fit(args):
         "data": f"synthetic (seed {SEED}), Morse targets",
         "config": {"workload": f"{args.workload}: {cfg['desc']}", "natoms": natoms, "npairs": npairs,
                    "rows": M, "columns": ncol, "batches": r64["nbatch"], "parallelism": "single GPU"},
-        "fp64": dict(block(r64, PEAK_FP64), fp64_coef_spread={"coef_max_rel": err(alt)["coef_max_rel"],
-                                                              "coef_norm_rel": err(alt)["coef_norm_rel"],
-                                                              "note": "second FP64 fit with batches of "
-                                                                      f"{max(1, cfg['chunk'] * 3 // 5)} structures"}),
-        "f64desc_i8gram": dict(block(res["f64desc_i8gram"], PEAK_FP64), err_vs_fp64=err(res["f64desc_i8gram"]),
-                               speedup_vs_fp64=r64["s_per_fit"] / res["f64desc_i8gram"]["s_per_fit"]),
-        "f32desc_f64gram": dict(block(res["f32desc_f64gram"], PEAK_FP32), err_vs_fp64=err(res["f32desc_f64gram"]),
-                                speedup_vs_fp64=r64["s_per_fit"] / res["f32desc_f64gram"]["s_per_fit"]),
-        "mixed": dict(block(res["mixed"], PEAK_FP32), err_vs_fp64=err(res["mixed"]),
-                      speedup_vs_fp64=r64["s_per_fit"] / res["mixed"]["s_per_fit"]),
+        "fp64": block(r64, PEAK_FP64),
         "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach / PEAK_FP64,
                      "traffic": TRAFFIC_BSO4_F64_FIT_BATCH if quad else None, "kernel": "bso4_kernel<double,8>",
                      "traffic_note": ("HBM bytes of the first 500-structure launch, "
@@ -409,11 +407,22 @@ def main_fit(args):
                           "algorithmic_flops_per_step": syrk_flops,
                           "note": "per-launch time from the kernel trace of the same workload (" + PROFILE_FIT + ")"},
     }
+    if alt is not None:
+        out["fp64"]["fp64_coef_spread"] = {"coef_max_rel": err(alt)["coef_max_rel"],
+                                           "coef_norm_rel": err(alt)["coef_norm_rel"],
+                                           "note": f"second FP64 fit with batches of {max(1, cfg['chunk'] * 3 // 5)} "
+                                                   "structures"}
+    for f in forms[1:]:
+        out[f] = dict(block(res[f], PEAK_FP64 if FORMS[f][0] == 0 else PEAK_FP32), err_vs_fp64=err(res[f]),
+                      speedup_vs_fp64=r64["s_per_fit"] / res[f]["s_per_fit"])
     if quad and SYRK_F64_MS_PER_BATCH:
-        per_launch = syrk_flops / r64["nbatch"]
+        # the first batch's launch (290,246 rows), as profiled
+        per_launch = 290246.0 * ld * (ld + 1)
         out["roofline_syrk"].update(achieved=per_launch / (SYRK_F64_MS_PER_BATCH * 1e-3) / 1e12,
                                     kernel_ms_per_launch=SYRK_F64_MS_PER_BATCH,
-                                    frac=per_launch / (SYRK_F64_MS_PER_BATCH * 1e-3) / 1e12 / PEAK_FP64)
+                                    frac=per_launch / (SYRK_F64_MS_PER_BATCH * 1e-3) / 1e12 / PEAK_FP64,
+                                    traffic=TRAFFIC_SYRK_F64_FIT_BATCH,
+                                    algorithmic_flops_per_launch=per_launch)
     if quad and not args.no_serve:
         out["serve"] = serve_measurement(args)
     if not args.no_cpu_baseline:
@@ -678,6 +687,9 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the fit workload's structures")
     ap.add_argument("--nstruct", type=int, default=2000, help="structures of the desc workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default=",".join(FORMS), help="fit workloads: precision forms to time (fp64 always)")
+    ap.add_argument("--no-spread", dest="spread", action="store_false",
+                    help="fit workloads: skip the second FP64 fit that measures FP64's own coefficient spread")
     ap.add_argument("--no-serve", action="store_true", help="fit workloads: skip the serving measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device 3 times (reference kernel for HBM counters)")

@@ -110,7 +110,13 @@ module mldgpu_c
       import; type(c_ptr), value :: ctx; real(c_double), value :: lambda, rcond
       real(c_double), intent(out) :: w(*), diag(4)
     end function
-    ! the six stress rows of every stressed structure of the last batch (linear model), after
+    ! two-stage quadratic fit (mld_type_quadratic = 1): linear columns first, then the quadratic
+    ! columns on the residual targets; diag(1:4) linear stage, diag(5:8) quadratic stage
+    integer(c_int) function mld_gram_solve_qnml(ctx, lambda, method, rcond, w, diag) bind(c)
+      import; type(c_ptr), value :: ctx; real(c_double), value :: lambda, rcond
+      integer(c_int), value :: method; real(c_double), intent(out) :: w(*), diag(8)
+    end function
+    ! the six stress rows of every stressed structure of the last batch (linear or quadratic model), after
     ! mld_gram_add_batch: s(6, nstruct) Voigt targets, w_s(nstruct) weights (0: no stress rows)
     integer(c_int) function mld_gram_add_stress(ctx, order, s, w_s, sign_stress, sign_stress_big_box) bind(c)
       import; type(c_ptr), value :: ctx; integer(c_int), value :: order

@@ -179,6 +179,9 @@ int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F
  * row_c[1 + d] = sign_stress * 3 / V * f * sum over the atoms a and neighbours k of
  * dB_a/dr_k (x_c, d) * u_k(y_c) / r_k, c = (x, y) = xx, yy, zz, yz, xz, xy; u = r_k - r_a, f = 1 in a
  * small cell, -sign_stress_big_box in a large one (neighbours.F90:823-835); V = |det(cell)|.
+ * Quadratic model (order 2), unique column (p <= q): the same sum of
+ * dB_a/dr_k (x_c, p) B_q(k) + dB_a/dr_k (x_c, q) B_p(k) with B of the neighbour k
+ * (force_ij_quadratic, mld_stress.F90:480-488, 584-600).
  * S (6 * nstruct): the Voigt targets xx yy zz yz xz xy per structure as the poscar holds them;
  * w_s (nstruct): w_s_model per structure, 0 for none.  Needs the batch's gradients. */
 int mld_gram_add_stress(mld_ctx* ctx, int order, const double* S, const double* w_s, double sign_stress,
@@ -222,6 +225,16 @@ int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w, double* d
  * (read_ml_file.F90:353-354, passed to dgelsd by gram_reduce_and_solve,
  * module_online_fit.F90:580-583); rcond < 0 means dlamch('E') = 2^-53, as in dgelsd. */
 int mld_gram_solve_svd(mld_ctx* ctx, double lambda, double rcond, double* w, double* diag_out);
+/* The two-stage quadratic fit (mld_type_quadratic = 1, QNML, the reference's default for
+ * mld_order = 2): train_snap_get_parameters_precondition_inversion_matrix (parameters.F90:832-1015)
+ * fits the linear columns [N, B] first, (G_LL + lambda) w_L = h_L, then the quadratic columns to
+ * the residual targets, (G_QQ + Lambda_Q) w_Q = h_Q - G_LQ^T w_L, each stage with the dense solve
+ * of train_snap_get_parameters_linear (:1018-1212) on its block of the accumulated Gram, whose
+ * columns are the quadratic unique layout [N, B, (B_p B_q)_{p<=q}] (gD = 1 + D + D(D+1)/2).
+ * method MLD_SOLVE_SVD (rcond as in mld_gram_solve_svd), MLD_SOLVE_CHOL or MLD_SOLVE_CHOL_IR (the
+ * refinement then iterates on the Gram blocks).  w = [w_L, w_Q] (gD values); diag_out (optional,
+ * 8 doubles): the four diagnostics of the linear stage, then those of the quadratic stage. */
+int mld_gram_solve_qnml(mld_ctx* ctx, double lambda, int method, double rcond, double* w, double* diag_out);
 
 /* ---- kernel_po features (compute/compute_kernel.F90:91-150, 476-866) ----
  * zref (D, nk) column-major: the standardised reference points (global_kernel).
@@ -256,8 +269,9 @@ int mld_kernel_po(mld_ctx* ctx, int nat, const double* energy, const double* for
 int mld_model_set(mld_ctx* ctx, int order, int ncol, const double* w, const double* ref_energy);
 int mld_mahalanobis_set(mld_ctx* ctx, const double* mean, const double* sigma_inv);
 int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev);
-/* Stresses of a linear model on the last batch (md_mld_compute_stress, mld.F90:1777-1867):
- * S_dev (6 * nstruct) = w . [0, stress row] with the rows of mld_gram_add_stress. */
+/* Stresses of a linear or quadratic model on the last batch (md_mld_compute_stress,
+ * mld.F90:1777-1867): S_dev (6 * nstruct) = w . [0, stress row] with the rows of
+ * mld_gram_add_stress. */
 int mld_predict_stress_dev(mld_ctx* ctx, double sign_stress, double sign_stress_big_box, double* S_dev);
 
 /* ---- serving: continuous batching of prediction requests ----

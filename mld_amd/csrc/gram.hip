@@ -315,24 +315,29 @@ __global__ void __launch_bounds__(256) rows_force_krr_kernel(int D, int nk, int 
     if (threadIdx.x < 3) y[row0 + threadIdx.x] = F[3 * (size_t)k + threadIdx.x] * swf;
 }
 
-// Linear stress rows (para_pack_stress_descriptor, mld_stress.F90:272-756, placed by
+// Stress rows (para_pack_stress_descriptor, mld_stress.F90:272-756, placed by
 // train_fill_Amat_with_stress, :181-268): for stressed structure s and the Voigt components
 // c = (x, y) = (0 0) (1 1) (2 2) (1 2) (0 2) (0 1),
 //   row[c][1 + d] = sign_stress * 3 / V * f * sum over the atoms a of s and their neighbours k of
 //                   g_{a,k}(x, d) * u_k(y) / r_k,
 // g the gradient slot dB_a/dr_k, V = |det(cell)| (calc_volume, neighbours.F90:1331-1368) and
 // u = r_k - r_a; f = 1 in a small cell (u_ij, neighbours.F90:1273-1301) and -sign_stress_big_box in
-// a large one (there the reference takes u = r_a - r_k and scales the force); column 0 is 0.  Rows
-// and targets carry sqrt(w_s).  One block per stressed structure; thread t owns the columns t,
-// t + 256, ... and sums the structure's pairs in order.
+// a large one (there the reference takes u = r_a - r_k and scales the force); column 0 is 0.
+// Quadratic model (order 2), unique column (p <= q) of the block:
+//   row[c][(p, q)] = sign_stress * 3 / V * f * sum of (g_{a,k}(x, p) B_q(k) + g_{a,k}(x, q) B_p(k)) u_k(y) / r_k
+// with B of the neighbour k (force_ij_quadratic, mld_stress.F90:480-488, 584-600: energy(:, ik),
+// ik = kind_neigh(ia, iw)), the reference's full-layout entry (p, q).  Rows and targets carry
+// sqrt(w_s).  One block per stressed structure; thread t owns the columns t, t + 256, ... and
+// sums the structure's pairs in order.
 template <typename T>
-__global__ void __launch_bounds__(256) rows_stress_kernel(int D, int ldA, const int* __restrict__ sidx,
+__global__ void __launch_bounds__(256) rows_stress_kernel(int D, int ncol, int ldA, const int* __restrict__ sidx,
                                                           const double* __restrict__ sw, const double* __restrict__ S,
                                                           const int* __restrict__ atom_off, const int* __restrict__ pair_off,
+                                                          const int* __restrict__ pair_idx,
                                                           const double4* __restrict__ pair_d,
-                                                          const mld::StructGeo* __restrict__ geo, const T* __restrict__ grad,
-                                                          double sign_stress, double sign_big, double* __restrict__ A,
-                                                          double* __restrict__ y)
+                                                          const mld::StructGeo* __restrict__ geo, const T* __restrict__ energy,
+                                                          const T* __restrict__ grad, double sign_stress, double sign_big,
+                                                          double* __restrict__ A, double* __restrict__ y)
 {
     const int r = blockIdx.x;
     const int s = sidx[r];
@@ -360,6 +365,31 @@ __global__ void __launch_bounds__(256) rows_stress_kernel(int D, int ldA, const 
                     acc[3] += g1 * uz;
                     acc[4] += g0 * uz;
                     acc[5] += g0 * uy;
+                }
+        } else if (col > D && col < ncol) {
+            // unique quadratic column -> (p, q), p <= q, p-major
+            int q = col - 1 - D, p = 0;
+            while (q >= D - p) {
+                q -= D - p;
+                ++p;
+            }
+            q += p;
+            for (int a = a0; a < a1; ++a)
+                for (int pp = pair_off[a]; pp < pair_off[a + 1]; ++pp) {
+                    const double4 pd = pair_d[pp];
+                    const double ux = pd.x / pd.w, uy = pd.y / pd.w, uz = pd.z / pd.w;
+                    const size_t slot = (size_t)pp + a + 1;
+                    const T* Bk = energy + (size_t)pair_idx[pp] * D;
+                    const double bp = (double)Bk[p], bq = (double)Bk[q];
+                    const double h0 = (double)grad[(slot * 3 + 0) * D + p] * bq + (double)grad[(slot * 3 + 0) * D + q] * bp;
+                    const double h1 = (double)grad[(slot * 3 + 1) * D + p] * bq + (double)grad[(slot * 3 + 1) * D + q] * bp;
+                    const double h2 = (double)grad[(slot * 3 + 2) * D + p] * bq + (double)grad[(slot * 3 + 2) * D + q] * bp;
+                    acc[0] += h0 * ux;
+                    acc[1] += h1 * uy;
+                    acc[2] += h2 * uz;
+                    acc[3] += h1 * uz;
+                    acc[4] += h0 * uz;
+                    acc[5] += h0 * uy;
                 }
         }
 #pragma unroll
@@ -1207,11 +1237,12 @@ int mld_gram_add_stress(mld_ctx* ctx, int order, const double* S, const double* 
 {
     if (!ctx) return MLD_E_ARG;
     if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
-    if (order != 1)
-        return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_gram_add_stress: stress rows are built for the linear model");
+    if (order != 1 && order != 2)
+        return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_gram_add_stress: stress rows of the linear and quadratic models only");
     const int D = ctx->D;
-    if (1 + D != ctx->gD) return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_stress: the Gram is not 1 + D wide");
-    if (!ctx->last_grad) return mld_fail(ctx, MLD_E_STATE, "mld_batch_bso4_dev with gradients first");
+    const int ncol = 1 + D + (order == 2 ? D * (D + 1) / 2 : 0);
+    if (ncol != ctx->gD) return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_stress: the Gram is not as wide as the order's rows");
+    if (!ctx->last_grad || !ctx->last_energy) return mld_fail(ctx, MLD_E_STATE, "mld_batch_bso4_dev with gradients first");
     const int nstruct = ctx->nstruct;
     if (nstruct && (!S || !w_s)) return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_stress: S and w_s needed");
     std::vector<int> sidx;
@@ -1241,13 +1272,15 @@ int mld_gram_add_stress(mld_ctx* ctx, int order, const double* S, const double* 
     double* A = (double*)gs.rows;
     double* y = (double*)gs.y;
     if (ctx->p.precision == MLD_PREC_FP32)
-        hipLaunchKernelGGL(rows_stress_kernel<float>, dim3(ns), dim3(256), 0, ctx->stream, D, ldA, d_idx, d_sw, d_S,
-                           ctx->d_atom_off, ctx->d_pair_off, ctx->d_pair_d, ctx->d_geo, (const float*)ctx->last_grad,
-                           sign_stress, sign_stress_big_box, A, y);
+        hipLaunchKernelGGL(rows_stress_kernel<float>, dim3(ns), dim3(256), 0, ctx->stream, D, ncol, ldA, d_idx, d_sw, d_S,
+                           ctx->d_atom_off, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_geo,
+                           (const float*)ctx->last_energy, (const float*)ctx->last_grad, sign_stress,
+                           sign_stress_big_box, A, y);
     else
-        hipLaunchKernelGGL(rows_stress_kernel<double>, dim3(ns), dim3(256), 0, ctx->stream, D, ldA, d_idx, d_sw, d_S,
-                           ctx->d_atom_off, ctx->d_pair_off, ctx->d_pair_d, ctx->d_geo, (const double*)ctx->last_grad,
-                           sign_stress, sign_stress_big_box, A, y);
+        hipLaunchKernelGGL(rows_stress_kernel<double>, dim3(ns), dim3(256), 0, ctx->stream, D, ncol, ldA, d_idx, d_sw, d_S,
+                           ctx->d_atom_off, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_geo,
+                           (const double*)ctx->last_energy, (const double*)ctx->last_grad, sign_stress,
+                           sign_stress_big_box, A, y);
     MLD_HIP(ctx, hipGetLastError());
     int rc = gram_accumulate_rows(ctx, M, A, y, 0);
     if (!rc && ctx->keep_rows) rc = keep_rows_copy(ctx, M, A, y);

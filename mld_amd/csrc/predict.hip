@@ -199,9 +199,40 @@ __global__ void __launch_bounds__(256) row_dot_kernel(long long nrow, int D, con
     if (lane == 0) t[r] = v;
 }
 
-// Stress of a linear model (md_mld_compute_stress, mld.F90:1777-1867: w . [0, pack_stress_linear]):
+// Quadratic model stress terms: t[slot(p) * 3 + x] = g_k . dB_a^(k)_x for every pair p = (a, k), g_k =
+// w_lin + U B_k of the NEIGHBOUR k: the quadratic stress row holds g_p(x) B_q(k) + g_q(x) B_p(k)
+// (force_ij_quadratic, mld_stress.F90:480-488, 584-600, energy(:, ik)), whose product with the
+// coefficients is dB . (w_lin + U B_k).  One wavefront per pair, four per block.
+template <typename T>
+__global__ void __launch_bounds__(256) pair_gdot_kernel(long long npairs, int D, const int* __restrict__ pair_src,
+                                                        const int* __restrict__ pair_idx, const double* __restrict__ g,
+                                                        const T* __restrict__ grad, double* __restrict__ t)
+{
+    const int lane = threadIdx.x & 63;
+    const long long p = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= npairs) return;
+    const size_t slot = (size_t)p + pair_src[p] + 1;
+    const double* gk = g + (size_t)pair_idx[p] * D;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    for (int d = lane; d < D; d += 64) {
+        const double gd = gk[d];
+        v0 += gd * (double)grad[(slot * 3 + 0) * D + d];
+        v1 += gd * (double)grad[(slot * 3 + 1) * D + d];
+        v2 += gd * (double)grad[(slot * 3 + 2) * D + d];
+    }
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    v2 = wave_sum(v2);
+    if (lane == 0) {
+        t[slot * 3 + 0] = v0;
+        t[slot * 3 + 1] = v1;
+        t[slot * 3 + 2] = v2;
+    }
+}
+
+// Stress of a linear or quadratic model (md_mld_compute_stress, mld.F90:1777-1867: w . [0, pack_stress_*]):
 //   S_c(s) = sign_stress * 3 / V * f * sum over the pairs p of s of t(slot(p), x_c) u_p(y_c) / r_p,
-// t = w . dB per slot and direction, (x, y) of c and f as in the stress rows (mld_stress.F90:272-756);
+// t = w . dB per slot and direction (linear) or the neighbour's g . dB (quadratic, pair_gdot_kernel), (x, y) of c and f as in the stress rows (mld_stress.F90:272-756);
 // one block per structure, pairs strided over the threads, then a fixed-order tree sum.
 __global__ void __launch_bounds__(256) stress_kernel(const int* __restrict__ atom_off, const int* __restrict__ pair_off,
                                                      const int* __restrict__ pair_src, const double4* __restrict__ pair_d,
@@ -496,28 +527,63 @@ int mld_predict_stress_dev(mld_ctx* ctx, double sign_stress, double sign_stress_
     if (!ctx || !S_dev) return MLD_E_ARG;
     PredictState* ps = reinterpret_cast<PredictState*>(ctx->predict_state);
     if (!ps || !ps->d_w) return mld_fail(ctx, MLD_E_STATE, "mld_predict_stress_dev: mld_model_set first");
-    if (ps->order != 1) return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_predict_stress_dev: stresses of linear models only");
+    if (ps->order != 1 && ps->order != 2)
+        return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_predict_stress_dev: stresses of linear and quadratic models only");
     if (ps->D != ctx->D) return mld_fail(ctx, MLD_E_STATE, "mld_predict_stress_dev: model dimension changed");
-    if (!ctx->last_grad) return mld_fail(ctx, MLD_E_STATE, "mld_predict_stress_dev: descriptors with gradients first");
+    if (!ctx->last_grad || !ctx->last_energy)
+        return mld_fail(ctx, MLD_E_STATE, "mld_predict_stress_dev: descriptors with gradients first");
+    if (ctx->D > DMAX) return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_predict_stress_dev: descriptor dimension above 8192");
     const int natoms = ctx->natoms, nstruct = ctx->nstruct, D = ctx->D;
     if (!nstruct) return MLD_OK;
     const long long nrow = 3 * (ctx->npairs + natoms);
-    int rc = device_ensure(ctx, &ps->swork, &ps->cap_swork, (size_t)nrow * sizeof(double) + 64);
+    const bool f32 = ctx->p.precision == MLD_PREC_FP32;
+    auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
+    const size_t bt = r256((size_t)nrow * sizeof(double));
+    const size_t bg = ps->order == 2 ? r256((size_t)natoms * D * sizeof(double)) : 0;
+    const size_t be = ps->order == 2 ? r256((size_t)natoms * sizeof(double)) : 0;
+    int rc = device_ensure(ctx, &ps->swork, &ps->cap_swork, bt + bg + be + 64);
     if (rc) return rc;
     double* t = (double*)ps->swork;
     hipStream_t st = ctx->stream;
-    if (nrow) {
+    const int *rev_off, *rev_pair, *pair_src;
+    rc = pair_transpose(ctx, &rev_off, &rev_pair, &pair_src);
+    if (rc) return rc;
+    if (ps->order == 2) {
+        double* g = (double*)((char*)ps->swork + bt);
+        double* e_atom = (double*)((char*)ps->swork + bt + bg);
+        const int atom_lds = (int)(2 * (size_t)D * sizeof(double));
+        const long long P = ctx->npairs;
+        if (natoms) {
+            if (f32) {
+                if ((rc = set_lds_limit_once(ctx, (const void*)atom_kernel<float>, atom_lds, g_atom_lds_f32))) return rc;
+                hipLaunchKernelGGL(atom_kernel<float>, dim3(natoms), dim3(256), atom_lds, st, natoms, D,
+                                   (const float*)ctx->last_energy, ctx->d_itype, ps->d_w, ps->d_U, nullptr, nullptr,
+                                   nullptr, g, e_atom, nullptr);
+            } else {
+                if ((rc = set_lds_limit_once(ctx, (const void*)atom_kernel<double>, atom_lds, g_atom_lds_f64))) return rc;
+                hipLaunchKernelGGL(atom_kernel<double>, dim3(natoms), dim3(256), atom_lds, st, natoms, D,
+                                   (const double*)ctx->last_energy, ctx->d_itype, ps->d_w, ps->d_U, nullptr, nullptr,
+                                   nullptr, g, e_atom, nullptr);
+            }
+        }
+        if (P) {
+            const unsigned nb = (unsigned)((P + 3) / 4);
+            if (f32)
+                hipLaunchKernelGGL(pair_gdot_kernel<float>, dim3(nb), dim3(256), 0, st, P, D, pair_src, ctx->d_pair_idx, g,
+                                   (const float*)ctx->last_grad, t);
+            else
+                hipLaunchKernelGGL(pair_gdot_kernel<double>, dim3(nb), dim3(256), 0, st, P, D, pair_src, ctx->d_pair_idx, g,
+                                   (const double*)ctx->last_grad, t);
+        }
+    } else if (nrow) {
         const unsigned nb = (unsigned)((nrow + 3) / 4);
-        if (ctx->p.precision == MLD_PREC_FP32)
+        if (f32)
             hipLaunchKernelGGL(row_dot_kernel<float>, dim3(nb), dim3(256), 0, st, nrow, D, ps->d_w,
                                (const float*)ctx->last_grad, t);
         else
             hipLaunchKernelGGL(row_dot_kernel<double>, dim3(nb), dim3(256), 0, st, nrow, D, ps->d_w,
                                (const double*)ctx->last_grad, t);
     }
-    const int *rev_off, *rev_pair, *pair_src;
-    rc = pair_transpose(ctx, &rev_off, &rev_pair, &pair_src);
-    if (rc) return rc;
     hipLaunchKernelGGL(stress_kernel, dim3(nstruct), dim3(256), 0, st, ctx->d_atom_off, ctx->d_pair_off, pair_src,
                        ctx->d_pair_d, ctx->d_geo, t, sign_stress, sign_stress_big_box, S_dev);
     MLD_HIP(ctx, hipGetLastError());

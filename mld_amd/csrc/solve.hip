@@ -436,16 +436,48 @@ void kept_rows_free(mld_ctx* ctx)
     ctx->kept_M.clear();
 }
 
+// A square block of the accumulated system: columns col0 .. col0 + n - 1 of the Gram (the upper
+// triangle from G, leading dimension ctx->gld) and the right-hand side h of those columns.
+struct SysView {
+    const double* G;
+    const double* h;
+    int n;
+    int col0;
+};
+
+static SysView full_view(const mld_ctx* ctx) { return SysView{ctx->d_G, ctx->d_h, ctx->gD, 0}; }
+
+// quadratic unique column c (full layout index) -> true when it pairs two different components
+static bool off_diagonal_quadratic(const mld_ctx* ctx, int c)
+{
+    if (ctx->gram_order != 2 || c < 1 + ctx->D) return false;
+    int q = c - 1 - ctx->D, p = 0;
+    while (q >= ctx->D - p) { q -= ctx->D - p; ++p; }
+    return q != 0;
+}
+
+// qnml_rhs: h2[j] = h[nL + j] - sum_{i < nL} G[i][nL + j] w[i] (the right-hand side of the
+// quadratic block after the linear stage: Q^T W (y - L w_L) = h_Q - G_LQ^T w_L)
+__global__ void qnml_rhs_kernel(int n, int nL, int ld, const double* __restrict__ G, const double* __restrict__ h,
+                                const double* __restrict__ wL, double* __restrict__ h2)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n - nL) return;
+    double acc = h[nL + j];
+    for (int i = 0; i < nL; ++i) acc -= G[(size_t)i * ld + nL + j] * wL[i];
+    h2[j] = acc;
+}
+
 // r = h - (G + Lambda) w, or -(G + Lambda) w when with_rhs == 0; from the kept FP64
 // rows (r = A^T (y - A w) - Lambda w) when use_rows, else from the accumulated Gram
-static int residual(mld_ctx* ctx, const double* ridge, const double* w, double* r, double* tbuf, double* part,
-                    int use_rows, int with_rhs = 1)
+static int residual(mld_ctx* ctx, const SysView& v, const double* ridge, const double* w, double* r, double* tbuf,
+                    double* part, int use_rows, int with_rhs = 1)
 {
     hipStream_t st = ctx->stream;
-    const int n = ctx->gD, ld = ctx->gld;
+    const int n = v.n, ld = ctx->gld;
     if (!use_rows) {
-        hipLaunchKernelGGL(resid_gram_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, ridge,
-                           with_rhs ? ctx->d_h : nullptr, w, r);
+        hipLaunchKernelGGL(resid_gram_kernel, dim3(n), dim3(256), 0, st, n, ld, v.G, ridge, with_rhs ? v.h : nullptr,
+                           w, r);
         MLD_HIP(ctx, hipGetLastError());
         return MLD_OK;
     }
@@ -481,19 +513,14 @@ extern "C" int mld_gram_set_keep_rows(mld_ctx* ctx, int keep)
 // the unique quadratic columns the solve runs on S (G + Lambda) S with S = sqrt(2) on the
 // off-diagonal columns, which is the full-layout system restricted to its range: the
 // result equals the reference's minimum-norm solution on the D^2 layout folded back.
-static int solve_min_norm(mld_ctx* ctx, const std::vector<double>& ridge, double rcond, double* w_out,
-                          double* diag_out)
+static int solve_min_norm(mld_ctx* ctx, const SysView& v, const std::vector<double>& ridge, double rcond,
+                          double* w_out, double* diag_out)
 {
     hipStream_t st = ctx->stream;
-    const int n = ctx->gD, ld = ctx->gld, n2 = n + (n & 1), h2 = n2 / 2;
+    const int n = v.n, ld = ctx->gld, n2 = n + (n & 1), h2 = n2 / 2;
     std::vector<double> sc(n, 1.0);
-    if (ctx->gram_order == 2)
-        for (int c = 1 + ctx->D; c < n; ++c) {
-            int q = c - 1 - ctx->D, p = 0;
-            while (q >= ctx->D - p) { q -= ctx->D - p; ++p; }
-            q += p;
-            if (p != q) sc[c] = std::sqrt(2.0);
-        }
+    for (int c = 0; c < n; ++c)
+        if (off_diagonal_quadratic(ctx, v.col0 + c)) sc[c] = std::sqrt(2.0);
     const size_t nn = (size_t)n2 * n2;
     double *A = nullptr, *V = nullptr, *vec = nullptr;
     MLD_HIP(ctx, hipMalloc(&A, nn * sizeof(double)));
@@ -514,7 +541,7 @@ static int solve_min_norm(mld_ctx* ctx, const std::vector<double>& ridge, double
     int2* pq = (int2*)(cs + h2);               // h2
     MLD_HIP(ctx, hipMemcpyAsync(dr, ridge.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
     MLD_HIP(ctx, hipMemcpyAsync(dsc, sc.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(mn_build_kernel, dim3(n2), dim3(256), 0, st, n, n2, ld, ctx->d_G, dr, dsc, ctx->d_h, A, V, b);
+    hipLaunchKernelGGL(mn_build_kernel, dim3(n2), dim3(256), 0, st, n, n2, ld, v.G, dr, dsc, v.h, A, V, b);
     MLD_HIP(ctx, hipGetLastError());
     const long long nthr = (long long)h2 * h2 + (long long)n2 * h2;
     const unsigned nblk = (unsigned)((nthr + 255) / 256);
@@ -568,7 +595,7 @@ static int solve_min_norm(mld_ctx* ctx, const std::vector<double>& ridge, double
     hipLaunchKernelGGL(mn_v_kernel, dim3((n2 + 3) / 4), dim3(256), 0, st, n2, V, y, x);
     std::vector<double> hx(n2), hr(n), hh(n);
     MLD_HIP(ctx, hipMemcpyAsync(hx.data(), x, (size_t)n2 * sizeof(double), hipMemcpyDeviceToHost, st));
-    MLD_HIP(ctx, hipMemcpyAsync(hh.data(), ctx->d_h, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    MLD_HIP(ctx, hipMemcpyAsync(hh.data(), v.h, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
     MLD_HIP(ctx, hipStreamSynchronize(st));
     long double hn = 0.0L;
     for (int i = 0; i < n; ++i) hn += (long double)hh[i] * hh[i];
@@ -576,7 +603,7 @@ static int solve_min_norm(mld_ctx* ctx, const std::vector<double>& ridge, double
     auto residual_of = [&](std::vector<double>& wv) -> int {
         for (int i = 0; i < n; ++i) wv[i] = sc[i] * hx[i];
         MLD_HIP(ctx, hipMemcpyAsync(x, wv.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(resid_gram_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, dr, ctx->d_h, x, r);
+        hipLaunchKernelGGL(resid_gram_kernel, dim3(n), dim3(256), 0, st, n, ld, v.G, dr, v.h, x, r);
         MLD_HIP(ctx, hipMemcpyAsync(hr.data(), r, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
         MLD_HIP(ctx, hipStreamSynchronize(st));
         return MLD_OK;
@@ -626,7 +653,8 @@ static int solve_min_norm(mld_ctx* ctx, const std::vector<double>& ridge, double
     return MLD_OK;
 }
 
-static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond, double* w_out, double* diag_out)
+static int gram_solve(mld_ctx* ctx, const SysView& v, double lambda, int method, double svd_rcond, double* w_out,
+                      double* diag_out)
 {
     if (!ctx) return MLD_E_ARG;
     if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
@@ -634,20 +662,13 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
     if (method != MLD_SOLVE_CHOL_IR && method != MLD_SOLVE_CHOL && method != MLD_SOLVE_SVD)
         return mld_fail(ctx, MLD_E_ARG, "unknown method");
     hipStream_t st = ctx->stream;
-    const int n = ctx->gD, ld = ctx->gld;
+    const int n = v.n, ld = ctx->gld;
     if (lambda < 0.0) lambda = 0.0;  // lambda_krr < 0 -> 0 (module_online_fit.F90:571)
     // ridge per column
     std::vector<double> ridge(n, lambda);
-    if (ctx->gram_order == 2) {
-        int D = ctx->D;
-        for (int c = 1 + D; c < n; ++c) {
-            int q = c - 1 - D, p = 0;
-            while (q >= D - p) { q -= D - p; ++p; }
-            q += p;
-            if (p != q) ridge[c] = 0.5 * lambda;
-        }
-    }
-    if (method == MLD_SOLVE_SVD) return solve_min_norm(ctx, ridge, svd_rcond, w_out, diag_out);
+    for (int c = 0; c < n; ++c)
+        if (off_diagonal_quadratic(ctx, v.col0 + c)) ridge[c] = 0.5 * lambda;
+    if (method == MLD_SOLVE_SVD) return solve_min_norm(ctx, v, ridge, svd_rcond, w_out, diag_out);
     size_t nb = (size_t)((n + NB - 1) / NB);
     size_t bytes_L = (size_t)ld * ld * sizeof(double);
     size_t nvec = (size_t)ld;
@@ -682,7 +703,7 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
     {
         MLD_HIP(ctx, hipMemsetAsync(info, 0, sizeof(int), st));
         MLD_HIP(ctx, hipMemsetAsync(L, 0, bytes_L, st));
-        hipLaunchKernelGGL(prep_kernel, dim3(n), dim3(256), 0, st, n, ld, ctx->d_G, dridge, shift, L, S);
+        hipLaunchKernelGGL(prep_kernel, dim3(n), dim3(256), 0, st, n, ld, v.G, dridge, shift, L, S);
         for (size_t k = 0; k < nb; ++k) {
             hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, st, n, ld, L, (int)k, info, Linv);
             int nrem = (int)(nb - k - 1);
@@ -721,10 +742,11 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
         for (int i = 0; i < n; ++i) s += (long double)a[i] * b[i];
         return (double)s;
     };
-    const int use_rows = ctx->keep_rows && !ctx->kept_A.empty();
+    // the kept rows give the residual of the whole system only
+    const int use_rows = ctx->keep_rows && !ctx->kept_A.empty() && v.col0 == 0 && v.n == ctx->gD;
     // b = h (from the rows when kept), w0 = M^-1 b
     MLD_HIP(ctx, hipMemsetAsync(w, 0, (size_t)n * sizeof(double), st));
-    int rc = residual(ctx, dridge, w, r, tbuf, part, use_rows);
+    int rc = residual(ctx, v, dridge, w, r, tbuf, part, use_rows);
     if (rc) return rc;
     std::vector<double> hb(n), hx(n), hr(n), hz(n), hp(n), hq(n), best(n);
     if ((rc = to_host(hb, r))) return rc;
@@ -734,7 +756,7 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
     MLD_HIP(ctx, hipMemcpyAsync(w, d, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
     if ((rc = to_host(hx, w))) return rc;
     int iters = 1;
-    rc = residual(ctx, dridge, w, r, tbuf, part, use_rows);
+    rc = residual(ctx, v, dridge, w, r, tbuf, part, use_rows);
     if (rc) return rc;
     if ((rc = to_host(hr, r))) return rc;
     double relres = bnorm > 0 ? std::sqrt(dot(hr, hr)) / bnorm : 0.0;
@@ -752,7 +774,7 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
         int since_best = 0;
         for (int it = 0; it < maxit; ++it) {
             if ((rc = to_dev(d, hp))) return rc;
-            rc = residual(ctx, dridge, d, r, tbuf, part, use_rows, 0);  // r = -(G + Lambda) p
+            rc = residual(ctx, v, dridge, d, r, tbuf, part, use_rows, 0);  // r = -(G + Lambda) p
             if (rc) return rc;
             if ((rc = to_host(hq, r))) return rc;
             for (double& v : hq) v = -v;
@@ -767,7 +789,7 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
             // true residual every 8 steps (and at the end) keeps the recurrence honest
             if ((it & 7) == 7) {
                 if ((rc = to_dev(w, hx))) return rc;
-                rc = residual(ctx, dridge, w, r, tbuf, part, use_rows);
+                rc = residual(ctx, v, dridge, w, r, tbuf, part, use_rows);
                 if (rc) return rc;
                 if ((rc = to_host(hr, r))) return rc;
             }
@@ -790,7 +812,7 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
         }
         // report the true residual of the returned vector
         if ((rc = to_dev(w, best))) return rc;
-        rc = residual(ctx, dridge, w, r, tbuf, part, use_rows);
+        rc = residual(ctx, v, dridge, w, r, tbuf, part, use_rows);
         if (rc) return rc;
         if ((rc = to_host(hr, r))) return rc;
         relres = bnorm > 0 ? std::sqrt(dot(hr, hr)) / bnorm : 0.0;
@@ -821,10 +843,57 @@ static int gram_solve(mld_ctx* ctx, double lambda, int method, double svd_rcond,
 
 extern "C" int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w_out, double* diag_out)
 {
-    return gram_solve(ctx, lambda, method, -1.0, w_out, diag_out);
+    if (!ctx) return MLD_E_ARG;
+    return gram_solve(ctx, full_view(ctx), lambda, method, -1.0, w_out, diag_out);
 }
 
 extern "C" int mld_gram_solve_svd(mld_ctx* ctx, double lambda, double rcond, double* w_out, double* diag_out)
 {
-    return gram_solve(ctx, lambda, MLD_SOLVE_SVD, rcond, w_out, diag_out);
+    if (!ctx) return MLD_E_ARG;
+    return gram_solve(ctx, full_view(ctx), lambda, MLD_SOLVE_SVD, rcond, w_out, diag_out);
+}
+
+// Two-stage quadratic fit (mld_type_quadratic = 1, QNML): train_snap_get_parameters_precondition_
+// inversion_matrix (parameters.F90:832-1015) splits the rows into the linear block L = [N, B] and the
+// quadratic block Q (split_Amat_in_Lmat_Qmat, :476-587), fits L to y (precond_linear_from_Lmat,
+// :591-674), then Q to the residual y - L w_L (precond_nonlinear_from_Qmat, :677-829), each with the
+// dense solve of train_snap_get_parameters_linear (:1018-1212): (L W L^T + lambda) w_L = L W y,
+// (Q W Q^T + lambda) w_Q = Q W (y - L^T w_L) = h_Q - G_LQ^T w_L.  Both stages read blocks of the
+// one accumulated Gram.  w = [w_L, w_Q]; diag_out (8 doubles) holds the four diagnostics of each
+// stage in order.
+extern "C" int mld_gram_solve_qnml(mld_ctx* ctx, double lambda, int method, double rcond, double* w_out,
+                                   double* diag_out)
+{
+    if (!ctx) return MLD_E_ARG;
+    if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
+    if (!w_out) return mld_fail(ctx, MLD_E_ARG, "mld_gram_solve_qnml: null w");
+    const int D = ctx->D, nL = 1 + D, n = ctx->gD;
+    if (n != nL + D * (D + 1) / 2)
+        return mld_fail(ctx, MLD_E_ARG, "mld_gram_solve_qnml: the Gram is not 1 + D + D(D+1)/2 columns wide");
+    ctx->gram_order = 2;  // the columns are the quadratic unique layout (also when packed by the caller)
+    double d1[4] = {0, 0, 0, 0}, d2[4] = {0, 0, 0, 0};
+    int rc = gram_solve(ctx, SysView{ctx->d_G, ctx->d_h, nL, 0}, lambda, method, rcond, w_out, d1);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    double *dwL = nullptr, *dh2 = nullptr;
+    MLD_HIP(ctx, hipMalloc(&dwL, (size_t)nL * sizeof(double)));
+    struct Free {
+        double** a;
+        double** b;
+        ~Free() { (void)hipFree(*a); (void)hipFree(*b); }
+    } fr{&dwL, &dh2};
+    MLD_HIP(ctx, hipMalloc(&dh2, (size_t)(n - nL) * sizeof(double)));
+    MLD_HIP(ctx, hipMemcpyAsync(dwL, w_out, (size_t)nL * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(qnml_rhs_kernel, dim3((n - nL + 255) / 256), dim3(256), 0, st, n, nL, ctx->gld, ctx->d_G,
+                       ctx->d_h, dwL, dh2);
+    MLD_HIP(ctx, hipGetLastError());
+    rc = gram_solve(ctx, SysView{ctx->d_G + (size_t)nL * ctx->gld + nL, dh2, n - nL, nL}, lambda, method, rcond,
+                    w_out + nL, d2);
+    MLD_HIP(ctx, hipStreamSynchronize(st));
+    if (diag_out)
+        for (int i = 0; i < 4; ++i) {
+            diag_out[i] = d1[i];
+            diag_out[4 + i] = d2[i];
+        }
+    return rc;
 }

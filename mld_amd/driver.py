@@ -102,22 +102,27 @@ def fit_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]]
                 precision: int = 0, method: Optional[int] = None, export: bool = False,
                 predict: bool = False) -> Tuple[np.ndarray, Dict[str, Any]]:
     """Linear (mld_order 1) or quadratic (2) fit of the configurations (mld.F90:279-420): energy rows
-    with the header energy iread_energy less ref_energy_per_element, force rows, and for the linear
-    model the stress rows of the classes whose S flag is on (train_fill_Amat_with_stress,
-    mld_stress.F90:181-268), each weighted by the loss's w_e / w_f / w_s (model_weights); rows of
-    classes whose flag is off are left out; lambda_krr; the minimum-norm solve of the reference by
-    default.  Writes <descriptor>_snap1_params.pot and lammps_<...> to out_dir when given; export
-    returns the accumulated G and h in diag; predict returns the fitted model's energies, forces and
-    stresses of every configuration in diag["predictions"] (config order: (E, F (nat, 3), S (6) or None))."""
+    with the header energy iread_energy less ref_energy_per_element, force rows, and the stress rows
+    of the classes whose S flag is on (train_fill_Amat_with_stress, mld_stress.F90:181-268), each
+    weighted by the loss's w_e / w_f / w_s (model_weights); rows of classes whose flag is off are
+    left out; lambda_krr; the minimum-norm solve of the reference by default.  The quadratic model
+    is fitted as mld_type_quadratic says: 1 (the default) the two-stage QNML fit, linear columns
+    first and the quadratic ones on the residual (parameters.F90:414-420, 832-1015), 2 the full
+    quadratic fit in one solve (:406-412).  Writes <descriptor>_<snap1|qnml1>_params.pot,
+    lammps_<...>, train_{energy,force,stress}.out and, for the two-stage fit, the
+    {energy,force,stress}_deviation_{linear,quadratic}.milady files to out_dir when given; export returns the accumulated G and h in diag; predict returns the fitted model's energies, forces and
+    stresses of every configuration in diag["predictions"] (config order: (E, F (nat, 3), S (6) or None));
+    a QNML fit adds those of its linear stage alone in diag["predictions_linear"]."""
     import torch
     from . import Context, fit as F
 
     order = inputs.fit_order(nml)
     if order == 3:
         raise NotImplementedError("kernel fits need the kernel-point selection of the reference (MCD / CUR)")
-    if order == 2 and inputs.quadratic_type(nml) != inputs.QUADRATIC_QML:
-        raise NotImplementedError(f"mld_type_quadratic {inputs.quadratic_type(nml)}: only the full quadratic fit "
-                                  "(mld_type_quadratic = 2) is built by this driver")
+    qtype = inputs.quadratic_type(nml) if order == 2 else None
+    if order == 2 and qtype not in (inputs.QUADRATIC_QNML, inputs.QUADRATIC_QML):
+        raise NotImplementedError(f"mld_type_quadratic {qtype}: the two-stage (1) and the full (2) quadratic fits "
+                                  "are built by this driver")
     weights = inputs.model_weights(nml, configs)
     els = inputs.elements(nml)
     eref = inputs.reference_energies(nml)
@@ -133,8 +138,6 @@ def fit_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]]
         for k, ((c, cl), (we, wf, ws)) in enumerate(zip(configs, weights)):
             groups.setdefault((cl.cls, we if cl.use_e else 0.0, wf if (cl.use_f and forces_on) else 0.0), []).append(k)
         wsel = [ws if (cl.use_s and c.has_stress and forces_on) else 0.0 for (c, cl), (_, _, ws) in zip(configs, weights)]
-        if order != 1 and any(wsel):
-            raise NotImplementedError("stress rows are built for the linear model")
         batches = []
         for (cls, w_e, w_f), idx in groups.items():
             if w_e == 0.0 and w_f == 0.0 and not any(wsel[k] for k in idx):
@@ -156,23 +159,62 @@ def fit_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]]
                 gf.add_stress(b, S, ws, sgn, sgn_big)
             batches.append(idx)
         m = F.SOLVE_SVD if method is None else method
-        w, diag = gf.solve(inputs.ridge(nml), m, rcond=float(nml["svd_rcond"]))
+        if qtype == inputs.QUADRATIC_QNML:
+            w, diag = gf.solve_qnml(inputs.ridge(nml), m, rcond=float(nml["svd_rcond"]))
+        else:
+            w, diag = gf.solve(inputs.ridge(nml), m, rcond=float(nml["svd_rcond"]))
         if export:  # the normal equations as accumulated (diag["G"], diag["h"])
             diag["G"], diag["h"] = gf.export()
-        if predict:
-            diag["predictions"] = _predict(ctx, order, w, configs, [configs[k] for idx in batches for k in idx],
-                                           [k for idx in batches for k in idx], els, sgn, sgn_big)
+        if predict or out_dir:
+            group = [configs[k] for idx in batches for k in idx]
+            index = [k for idx in batches for k in idx]
+            diag["predictions"] = _predict(ctx, order, w, configs, group, index, els, sgn, sgn_big)
+            if qtype == inputs.QUADRATIC_QNML:  # the linear stage alone (*_deviation_linear.milady)
+                diag["predictions_linear"] = _predict(ctx, 1, w[:1 + ctx.D], configs, group, index, els, sgn,
+                                                      sgn_big)
         D = ctx.D
     finally:
         ctx.close()
     if out_dir:
         F.write_coefficients(out_dir, w, D, order, els, float(nml["j_max"]), float(nml["r_cut"]),
                              lbso4_diag=bool(nml["lbso4_diag"]))
+        recs = fit_records(nml, configs, weights, diag["predictions"])
+        io.write_train_outputs(out_dir, recs)
+        if "predictions_linear" in diag:
+            lin = fit_records(nml, configs, weights, diag["predictions_linear"])
+            io.write_deviation_files(out_dir, lin, "linear")
+            io.write_deviation_files(out_dir, recs, "quadratic", base=lin)
+        if not predict:
+            diag.pop("predictions", None)
+            diag.pop("predictions_linear", None)
     return w, diag
 
 
+def fit_records(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]], weights, preds) -> List[io.FitRecord]:
+    """The rows of the fit per configuration in the reference's order (selection order; energy row,
+    force rows, stress rows), with their targets and the predictions preds (fit_configs' order):
+    what train_*.out and the deviation files list."""
+    els = inputs.elements(nml)
+    eref = inputs.reference_energies(nml)
+    iread = int(nml["iread_energy"])
+    forces_on = bool(nml["desc_forces"])
+    out = []
+    for k, ((c, cl), (we, wf, ws)) in enumerate(zip(configs, weights)):
+        if preds[k] is None:
+            continue
+        e_pred, f_pred, s_pred = preds[k]
+        er = float(eref[_species_index(c, els)].sum())
+        use_f = cl.use_f and forces_on
+        use_s = cl.use_s and c.has_stress and forces_on
+        out.append(io.FitRecord(cls=cl.cls, filename=str(nml["db_path"]) + c.name,
+                                e_target=c.header_energy(iread) - er if cl.use_e else None, e_ref=er,
+                                f_target=c.forces if use_f else None, s_target=np.asarray(c.stress) if use_s else None,
+                                e_pred=e_pred, f_pred=f_pred if use_f else None, s_pred=s_pred if use_s else None))
+    return out
+
+
 def _predict(ctx, order: int, w: np.ndarray, configs, group, index, els: List[str], sgn: float, sgn_big: float):
-    """Energies, forces and (linear model) stresses of the configurations group (positions index in
+    """Energies, forces and stresses of the configurations group (positions index in
     configs) with the coefficients w, in batches of up to 4,096 atoms."""
     from . import fit as F
 
@@ -188,7 +230,7 @@ def _predict(ctx, order: int, w: np.ndarray, configs, group, index, els: List[st
         e, g = b.alloc_outputs(True)
         b.bso4(e, g)
         Ep, Fp, _ = F.predict(ctx, b)
-        Sp = F.predict_stress(ctx, b, sgn, sgn_big).cpu().numpy() if order == 1 else None
+        Sp = F.predict_stress(ctx, b, sgn, sgn_big).cpu().numpy() if order in (1, 2) else None
         Ep, Fp = Ep.cpu().numpy(), Fp.cpu().numpy()
         for j in range(stop - start):
             preds[index[start + j]] = (float(Ep[j]), Fp[b.atom_off[j]:b.atom_off[j + 1]].copy(),
@@ -199,9 +241,10 @@ def _predict(ctx, order: int, w: np.ndarray, configs, group, index, els: List[st
 
 def predict_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]], w: np.ndarray,
                     precision: int = 0) -> List[Tuple[float, np.ndarray, Optional[np.ndarray]]]:
-    """Energies, forces and stresses (linear model) of the configurations with the coefficients w: the
-    evaluation the reference runs for its train / test sets (train_snap_compute_energy_force_stress,
-    mld.F90:944-1020; main_test_mld, :2506-2743).  One (E, F (nat, 3), S (6) or None) per configuration."""
+    """Energies, forces and stresses (linear and quadratic models) of the configurations with the
+    coefficients w: the evaluation the reference runs for its train / test sets
+    (train_snap_compute_energy_force_stress, mld.F90:944-1020; main_test_mld, :2506-2743).
+    One (E, F (nat, 3), S (6) or None) per configuration."""
     from . import Context
 
     order = inputs.fit_order(nml)

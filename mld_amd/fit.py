@@ -80,7 +80,7 @@ class GramFit:
 
     def add_stress(self, batch, S: np.ndarray, w_s: np.ndarray, sign_stress: float = 1.0,
                    sign_stress_big_box: float = 1.0):
-        """The six stress rows of every structure of the batch with w_s[s] > 0 (linear model):
+        """The six stress rows of every structure of the batch with w_s[s] > 0 (linear or quadratic model):
         S (nstruct, 6) Voigt targets xx yy zz yz xz xy, w_s (nstruct) weights (mld_gram_add_stress)."""
         S = np.ascontiguousarray(S, np.float64).reshape(-1, 6)
         w_s = np.ascontiguousarray(w_s, np.float64).reshape(-1)
@@ -162,6 +162,18 @@ class GramFit:
         return w, {"iterations": int(diag[0]), "rel_residual": float(diag[1]), "min_pivot": float(diag[2]),
                    "max_pivot": float(diag[3]), "converged": rc == 0}
 
+    def solve_qnml(self, lam: float, method: int = SOLVE_SVD, rcond: float = -1.0):
+        """Two-stage quadratic fit (mld_type_quadratic = 1, QNML; mld_gram_solve_qnml): the linear
+        columns [N, B] on their own, then the quadratic columns on the residual targets.  Returns
+        w = [w_L, w_Q] and the diagnostics of both stages."""
+        if self.order != 2:
+            raise ValueError("solve_qnml: the two-stage fit needs the quadratic rows (order 2)")
+        w = np.zeros(self.ncol)
+        diag = np.zeros(8)
+        self.ctx.check(self.ctx.L.mld_gram_solve_qnml(self.ctx.h, lam, method, rcond, w.ctypes.data_as(C.c_void_p),
+                                                      diag.ctypes.data_as(C.c_void_p)))
+        return w, {"linear_stage": diag[:4].tolist(), "quadratic_stage": diag[4:].tolist(), "nlin": 1 + self.D}
+
     def ridge_vector(self, lam: float) -> np.ndarray:
         """The per-column ridge the solve applies (lambda, lambda/2 on off-diagonal quadratic columns)."""
         r = np.full(self.ncol, max(lam, 0.0))
@@ -239,7 +251,8 @@ def predict(ctx, batch, energies: bool = True, forces: bool = True, d2: bool = F
 
 
 def predict_stress(ctx, batch, sign_stress: float = 1.0, sign_stress_big_box: float = 1.0):
-    """Stresses (nstruct, 6) of a linear model on the batch's last descriptors (mld_predict_stress_dev)."""
+    """Stresses (nstruct, 6) of a linear or quadratic model on the batch's last descriptors
+    (mld_predict_stress_dev)."""
     import torch
 
     from .batch import torch_ready

@@ -265,3 +265,87 @@ def read_params_pot(path: str) -> np.ndarray:
         n = int(fh.readline())
         w = np.array([float(fh.readline()) for _ in range(n)])
     return w
+
+
+# ----------------------------------------------------------------------------- fit outputs
+def _es2(v: float, w: int = 20, d: int = 10) -> str:
+    """Fortran ESw.d (no exponent width): d.ddd...E+ee, and d.ddd...+eee past two exponent digits."""
+    if v == 0.0:
+        s = "0." + "0" * d + "E+00"
+    else:
+        m, e = f"{v:.{d}E}".split("E")
+        ei = int(e)
+        s = f"{m}E{ei:+03d}" if abs(ei) <= 99 else f"{m}{ei:+04d}"
+    return s.rjust(w)
+
+
+@dataclasses.dataclass
+class FitRecord:
+    """One configuration's rows in a fit, as the reference lists them: class label, file name as the
+    run knows it (db_path + name), energy target (header energy less the atoms' reference energies),
+    the sum of those reference energies, force targets (nat, 3) or None when no force rows, stress
+    targets (6) or None, and the model's predictions of the same rows (w . row, no reference energy)."""
+    cls: str
+    filename: str
+    e_target: Optional[float]
+    e_ref: float
+    f_target: Optional[np.ndarray]
+    s_target: Optional[np.ndarray]
+    e_pred: Optional[float] = None
+    f_pred: Optional[np.ndarray] = None
+    s_pred: Optional[np.ndarray] = None
+
+
+def write_train_outputs(directory: str, records: List[FitRecord], prefix: str = "train") -> List[str]:
+    """<prefix>_energy.out, <prefix>_force.out, <prefix>_stress.out: per row the prediction and the
+    target, '(2es20.10," ",(a)," ",(a))' with the class and the 80-character file name
+    (train_snap_compute_energy_force_stress, mld.F90:944-1020: energies carry the reference energies
+    of the atoms, ene_snap + local_ref_energy and yfunc_train + local_ref_energy)."""
+    paths = [os.path.join(directory, f"{prefix}_{k}.out") for k in ("energy", "force", "stress")]
+    with open(paths[0], "w") as fe, open(paths[1], "w") as ff, open(paths[2], "w") as fs:
+        for r in records:
+            tail = f" {r.cls} {r.filename.ljust(80)}\n"
+            if r.e_target is not None:
+                fe.write(_es2(r.e_pred + r.e_ref) + _es2(r.e_target + r.e_ref) + tail)
+            if r.f_target is not None:
+                for p, t in zip(np.asarray(r.f_pred).reshape(-1), np.asarray(r.f_target).reshape(-1)):
+                    ff.write(_es2(float(p)) + _es2(float(t)) + tail)
+            if r.s_target is not None:
+                for p, t in zip(np.asarray(r.s_pred).reshape(-1), np.asarray(r.s_target).reshape(-1)):
+                    fs.write(_es2(float(p)) + _es2(float(t)) + tail)
+    return paths
+
+
+def write_deviation_files(directory: str, records: List[FitRecord], stage: str,
+                          base: Optional[List[FitRecord]] = None) -> List[str]:
+    """energy_, force_, stress_deviation_<stage>.milady: per row its index in the design matrix
+    (energy row, force rows, stress rows of each configuration in turn), y - w.row, w.row and y,
+    '(i9," ",3e20.10," ",(a)," ",(a))' with the class and the trimmed file name
+    (precond_linear_from_Lmat, parameters.F90:591-674, and precond_nonlinear_from_Qmat, :677-829).
+    For the second stage of a two-stage fit, base holds the first stage's predictions: y is then
+    the first stage's residual and w.row the second stage's share, the final prediction less the
+    first stage's."""
+    paths = [os.path.join(directory, f"{k}_deviation_{stage}.milady") for k in ("energy", "force", "stress")]
+    it = 0
+    with open(paths[0], "w") as fe, open(paths[1], "w") as ff, open(paths[2], "w") as fs:
+        for k, r in enumerate(records):
+            tail = f" {r.cls} {r.filename.strip()}\n"
+            b = base[k] if base is not None else None
+
+            def line(fh, y, p, y0=0.0, p0=0.0):
+                nonlocal it
+                it += 1
+                yy, pp = y - y0, p - p0
+                fh.write(f"{it:9d} " + _fe(yy - pp) + _fe(pp) + _fe(yy) + tail)
+
+            if r.e_target is not None:
+                line(fe, r.e_target, r.e_pred, b.e_pred if b else 0.0, b.e_pred if b else 0.0)
+            if r.f_target is not None:
+                fb = np.asarray(b.f_pred).reshape(-1) if b else np.zeros(3 * len(r.f_target))
+                for t, p, q in zip(np.asarray(r.f_target).reshape(-1), np.asarray(r.f_pred).reshape(-1), fb):
+                    line(ff, float(t), float(p), float(q), float(q))
+            if r.s_target is not None:
+                sb = np.asarray(b.s_pred).reshape(-1) if b else np.zeros(6)
+                for t, p, q in zip(np.asarray(r.s_target).reshape(-1), np.asarray(r.s_pred).reshape(-1), sb):
+                    line(fs, float(t), float(p), float(q), float(q))
+    return paths

@@ -161,28 +161,40 @@ def rows(energy, grad, nl, order=1, want_force=True):
 VOIGT = ((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1))  # (force component x, vector component y)
 
 
-def stress_rows(grad, nl, cell_rows, D, small, sign_stress=1.0, sign_big=1.0):
-    """Linear stress rows (6, 1 + D) of one configuration: para_pack_stress_descriptor
-    (mld_stress.F90:272-756) for the linear model, placed by train_fill_Amat_with_stress (:181-268).
-    For the Voigt components c = (x, y) in VOIGT,
+def stress_rows(grad, nl, cell_rows, D, small, sign_stress=1.0, sign_big=1.0, energy=None, order=1):
+    """Stress rows (6, 1 + D) of one configuration for the linear model, (6, 1 + D + D^2) for the
+    quadratic one (energy needed): para_pack_stress_descriptor (mld_stress.F90:272-756), placed by
+    train_fill_Amat_with_stress (:181-268).  For the Voigt components c = (x, y) in VOIGT,
         row[c][1 + d] = sign_stress * 3 / V * f * sum over atoms a and neighbours k >= 1 of
                         g_{a,k}(x, d) * u_k(y) / r_k,
     g the gradient slot dB_a/dr_k, V = |det(cell)| (calc_volume, neighbours.F90:1331-1368);
     in a small cell u_k = r_k - r_a (u_ij, neighbours.F90:1273-1301) and f = 1, in a large cell
     u_k = r_a - r_k with the force scaled by sign_stress_big_box, f = -sign_big here with
-    u = r_k - r_a; column 0 is 0."""
+    u = r_k - r_a; column 0 is 0.  Quadratic block, id1-major (force_ij_quadratic, mld_stress.F90:
+    480-488 small cell, :584-600 large cell; placed at :233-237):
+        row[c][1 + D + D p + q] = sign_stress * 3 / V * f * sum of
+                        (g_{a,k}(x, p) B_q(k) + g_{a,k}(x, q) B_p(k)) u_k(y) / r_k,
+    with B of the neighbour k (energy(:, ik), ik = kind_neigh(ia, iw)), not of the centre a."""
     off, nbr, dxyz, rr = nl
     nat = len(off) - 1
     V = abs(float(np.linalg.det(np.asarray(cell_rows, np.float64).reshape(3, 3))))
     f = 1.0 if small else -sign_big
-    out = np.zeros((6, 1 + D))
+    ncol = 1 + D + (D * D if order == 2 else 0)
+    out = np.zeros((6, ncol))
     if grad is None or len(nbr) == 0:
         return out
+    if order == 2 and energy is None:
+        raise ValueError("stress_rows: the quadratic block needs the descriptors (energy)")
     slots = np.concatenate([np.arange(off[a] + a + 1, off[a + 1] + a + 1) for a in range(nat)]).astype(np.int64)
     g = grad[slots]                      # (P, 3, D), pair order
     unit = dxyz / rr[:, None]            # (P, 3)
     for c, (x, y) in enumerate(VOIGT):
-        out[c, 1:] = sign_stress * 3.0 / V * f * (g[:, x, :] * unit[:, y:y + 1]).sum(axis=0)
+        gu = g[:, x, :] * unit[:, y:y + 1]   # (P, D)
+        out[c, 1:1 + D] = sign_stress * 3.0 / V * f * gu.sum(axis=0)
+        if order == 2:
+            Bk = np.asarray(energy)[np.asarray(nbr)]   # (P, D): B of each pair's neighbour
+            M = gu.T @ Bk                              # M[p, q] = sum g_p u B_q
+            out[c, 1 + D:] = sign_stress * 3.0 / V * f * (M + M.T).reshape(-1)
     return out
 
 

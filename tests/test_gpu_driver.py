@@ -28,29 +28,37 @@ def test_fit17_descriptors_run(tmp_path):
 
 
 def test_fit17_linear_fit_against_oracle_rows(tmp_path):
-    """The run's normal equations (class weights, energy and force targets of the poscars) equal
-    the oracle's rows; the system is ill-conditioned (eigenvalues down to 1e-17 of the largest),
+    """The run's normal equations (class weights, energy and force targets of the poscars, and the
+    stress rows of the classes with the S flag) equal the oracle's rows; the system is ill-conditioned (eigenvalues down to 1e-17 of the largest),
     so the solution is checked through the least-squares objective it reaches, not entry by entry,
     with lambda = 1 (condition ~4e14: the SVD threshold drops nothing, both solves give the ridge
     minimiser)."""
     from mld_amd.fit import SOLVE_CHOL_IR
 
     lam = 1.0
-    ov = {**FILES, "ml_type": 0, "snap_order": 1, "lambda_krr": lam}
+    # desc_forces on (fit17 itself fits energies only): force rows for the classes with the F flag,
+    # stress rows for those with the S flag; energies are the header energy that iread_energy names
+    ov = {**FILES, "ml_type": 0, "snap_order": 1, "lambda_krr": lam, "desc_forces": True}
     w, diag = driver.fit(os.path.join(G, "fit17_vacancy.ml"), out_dir=str(tmp_path), overrides=ov, export=True)
     w2, _ = driver.fit(os.path.join(G, "fit17_vacancy.ml"), overrides=ov, method=SOLVE_CHOL_IR)
     from mld_amd import inputs
 
     nml = {**inputs.read_input_ml(os.path.join(G, "fit17_vacancy.ml")), **ov}
+    iread = int(nml["iread_energy"])
     A, y, wt = [], [], []
     for c, cl in driver.select_configs(nml, G):
         e, g, nl = O.bso4(c.pos, c.cell, c.itype, 8, 4.7)
         A.append(O.rows(e, g, nl, order=1))
-        y.append(np.concatenate([[c.energy], c.forces.reshape(-1)]))
+        y.append(np.concatenate([[c.header_energy(iread)], c.forces.reshape(-1)]))
         we = cl.weights[0] if cl.use_e else 0.0
         wf = cl.weights[1] if cl.use_f else 0.0
         wt.append(np.concatenate([[we], np.full(3 * len(c.pos), wf)]))
+        if cl.use_s and c.has_stress:
+            A.append(O.stress_rows(g, nl, c.cell, 55, O.is_small(c.cell, 4.7)))
+            y.append(np.asarray(c.stress, np.float64))
+            wt.append(np.full(6, float(cl.weights[2])))
     A, y, wt = np.concatenate(A), np.concatenate(y), np.concatenate(wt)
+    assert (wt[y != 0] > 0).any()
     Gr, hr = (A * wt[:, None]).T @ A, (A * wt[:, None]).T @ y
     assert np.abs(diag["G"] - Gr).max() / np.abs(Gr).max() < 1e-12
     assert np.abs(diag["h"] - hr).max() / np.abs(hr).max() < 1e-12

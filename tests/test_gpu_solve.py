@@ -151,3 +151,36 @@ def test_dense_fit_phi_formulation(ctx64):
     w_svd, _ = gd.solve(0.0, SOLVE_SVD, rcond=1e-12)
     assert rel(w_svd, ref) < 1e-8, rel(w_svd, ref)
     assert abs(w_svd[7] - w_svd[11]) < 1e-9 * abs(w_svd).max()
+
+
+@pytest.mark.parametrize("method", ["chol_ir", "svd"])
+def test_two_stage_qnml_solve_against_numpy(method):
+    """mld_gram_solve_qnml on a 2j_max = 4 context (D = 14, 1 + 14 + 105 columns): the linear block
+    (G_LL + lambda) w_L = h_L, then the quadratic block (G_QQ + Lambda_Q) w_Q = h_Q - G_LQ^T w_L with
+    the unique-column ridge (lambda / 2 off the diagonal pairs), against NumPy on the same G, h."""
+    import mld_amd as M
+    from mld_amd.fit import GramFit, SOLVE_CHOL_IR, SOLVE_SVD
+
+    c = M.Context(j_max=2.0, r_cut=4.7)
+    try:
+        rng = np.random.default_rng(11)
+        gf = GramFit(c, order=2)
+        n, nL = gf.ncol, 1 + c.D
+        assert (n, nL) == (120, 15)
+        A = rng.normal(size=(900, n)) * np.exp(rng.uniform(-2.0, 2.0, size=n))
+        y = rng.normal(size=900)
+        wt = rng.uniform(0.5, 2.0, size=900)
+        gf.add_rows(A, y, wt)
+        lam = 1e-3
+        w, diag = gf.solve_qnml(lam, SOLVE_CHOL_IR if method == "chol_ir" else SOLVE_SVD)
+        G, h = (A * wt[:, None]).T @ A, (A * wt[:, None]).T @ y
+        r = gf.ridge_vector(lam)
+        wL = np.linalg.solve(G[:nL, :nL] + np.diag(r[:nL]), h[:nL])
+        wQ = np.linalg.solve(G[nL:, nL:] + np.diag(r[nL:]), h[nL:] - G[:nL, nL:].T @ wL)
+        assert rel(w[:nL], wL) < 1e-10, rel(w[:nL], wL)
+        assert rel(w[nL:], wQ) < 1e-10, rel(w[nL:], wQ)
+        # not the one-stage solution of the whole system
+        wfull = np.linalg.solve(G + np.diag(r), h)
+        assert rel(w, wfull) > 1e-3
+    finally:
+        c.close()

@@ -57,3 +57,74 @@ def test_fortran_edit_descriptors():
     assert _fe(1.0) == "    0.1000000000E+01"
     assert _fe(0.0) == "    0.0000000000E+00"
     assert _fe(-0.5) == "   -0.5000000000E+00"
+
+
+def test_qnml1_coefficient_files_byte_identical_to_reference(tmp_path):
+    """fit03's bso4_qnml1_params.pot and lammps_bso4_qnml1_params.pot (W, j_max 2, r_cut 4.7, the
+    quadratic model's 1 + 14 + 14^2 coefficients in the full layout) rewritten from their own
+    coefficients; fit04's file has the same layout."""
+    gold = os.path.join(G, "fit03_bso4_qnml1_params.pot")
+    w = io.read_params_pot(gold)
+    assert w.shape == (211,)
+    pot, lmp = io.params_file_names("bso4", 2)
+    assert (pot, lmp) == ("bso4_qnml1_params.pot", "lammps_bso4_qnml1_params.pot")
+    io.write_params_pot(str(tmp_path / pot), w)
+    io.write_lammps_params_bso4(str(tmp_path / lmp), w, ["W"], 2.0, 4.7, order=2)
+    assert open(tmp_path / pot).read() == open(gold).read()
+    assert open(tmp_path / lmp).read() == open(os.path.join(G, "fit03_lammps_bso4_qnml1_params.pot")).read()
+    w4 = io.read_params_pot(os.path.join(G, "fit04_bso4_qnml1_params.pot"))
+    io.write_params_pot(str(tmp_path / "w4.pot"), w4)
+    assert open(tmp_path / "w4.pot").read() == open(os.path.join(G, "fit04_bso4_qnml1_params.pot")).read()
+
+
+def _fit03_records(prefix=""):
+    """fit03's configurations with REF's predictions (prefix of the prediction set) and targets."""
+    import fit_pins
+
+    nml, configs, z = fit_pins.load("fit03_qnml")
+    recs = []
+    for k, (c, cl) in enumerate(configs):
+        sl = slice(z["off"][k], z["off"][k + 1])
+        use_f = cl.use_f and bool(nml["desc_forces"])
+        use_s = cl.use_s and c.has_stress and bool(nml["desc_forces"])
+        recs.append(io.FitRecord(cls=cl.cls, filename=nml["db_path"] + c.name, e_target=float(z["e_ref"][k]), e_ref=0.0,
+                                 f_target=c.forces if use_f else None, s_target=np.asarray(c.stress) if use_s else None,
+                                 e_pred=float(z[prefix + "e_pred"][k]), f_pred=z[prefix + "f_pred"][sl] if use_f else None,
+                                 s_pred=z[prefix + "s_pred"][k] if use_s else None))
+    return recs
+
+
+def test_train_outputs_in_the_reference_format(tmp_path):
+    """train_{energy,force,stress}.out of fit03 (mld.F90:974, 997, 1013) from REF's own values:
+    the first lines of each REF file, byte for byte (tests/golden/fit03_REF_excerpt)."""
+    io.write_train_outputs(str(tmp_path), _fit03_records())
+    for k in ("energy", "force", "stress"):
+        ref = open(os.path.join(G, "fit03_REF_excerpt", f"train_{k}.out")).read().splitlines(True)
+        ours = open(tmp_path / f"train_{k}.out").read().splitlines(True)
+        assert ours[:len(ref)] == ref, k
+    assert io._es2(-1.5e-120) == "   -1.5000000000-120"
+
+
+def test_deviation_files_in_the_reference_format(tmp_path):
+    """energy_, force_, stress_deviation_{linear,quadratic}.milady of fit03's two-stage fit from REF's
+    linear-stage and final predictions: row indices, classes and names exactly, the three numbers to
+    about the 10 digits printed (the differences are formed here from printed values, whose
+    rounding reaches the last digits)."""
+    final = _fit03_records()
+    lin = _fit03_records("lin_")
+    for r in final + lin:  # the linear set holds the forces of every sixth configuration with forces
+        if r.f_pred is not None:
+            r.f_pred = np.nan_to_num(r.f_pred)
+    io.write_deviation_files(str(tmp_path), lin, "linear")
+    io.write_deviation_files(str(tmp_path), final, "quadratic", base=lin)
+    for stage in ("linear", "quadratic"):
+        for k in ("energy", "force", "stress"):
+            ref = open(os.path.join(G, "fit03_REF_excerpt", f"{k}_deviation_{stage}.milady")).read().splitlines()
+            if k == "force":  # the first configuration (47 atoms)
+                ref = ref[:141]
+            ours = open(tmp_path / f"{k}_deviation_{stage}.milady").read().splitlines()[:len(ref)]
+            for a, b in zip(ours, ref):
+                ta, tb = a.split(), b.split()
+                assert ta[0] == tb[0] and ta[4:] == tb[4:] and len(a) == len(b), (stage, k, a, b)
+                va, vb = np.array(ta[1:4], float), np.array(tb[1:4], float)
+                assert np.abs(va - vb).max() <= 1e-8 * max(1.0, np.abs(vb).max()), (stage, k, a, b)
