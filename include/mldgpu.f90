@@ -129,6 +129,24 @@ module mldgpu_c
       import; type(c_ptr), value :: ctx; integer(c_int), value :: order, ncol
       real(c_double), intent(in) :: w(*); type(c_ptr), value :: ref_energy
     end function
+    ! descriptor analysis (ml_type = -2) on the last batch: atoms are 0-based indices in the batch
+    integer(c_int) function mld_desc_sum(ctx, nsel, atoms, sum) bind(c)
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: nsel
+      integer(c_int), intent(in) :: atoms(*); real(c_double), intent(out) :: sum(*)
+    end function
+    integer(c_int) function mld_desc_scatter_add(ctx, nsel, atoms, mean) bind(c)
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: nsel
+      integer(c_int), intent(in) :: atoms(*); real(c_double), intent(in) :: mean(*)
+    end function
+    integer(c_int) function mld_desc_stats_dev(ctx, mean, sigma_inv, npca, pvec, norm_dev, norm_mean_dev, &
+                                               d2_dev, proj_dev) bind(c)
+      import; type(c_ptr), value :: ctx; real(c_double), intent(in) :: mean(*), sigma_inv(*), pvec(*)
+      integer(c_int), value :: npca; type(c_ptr), value :: norm_dev, norm_mean_dev, d2_dev, proj_dev
+    end function
+    integer(c_int) function mld_sym_eig(ctx, n, a, evals, evecs) bind(c)
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: n
+      real(c_double), intent(in) :: a(*); real(c_double), intent(out) :: evals(*), evecs(*)
+    end function
   end interface
 contains
   subroutine mld_check(ctx, rc, where)

@@ -181,7 +181,9 @@ int mld_gram_add_batch(mld_ctx* ctx, int order, const double* E, const double* F
  * small cell, -sign_stress_big_box in a large one (neighbours.F90:823-835); V = |det(cell)|.
  * Quadratic model (order 2), unique column (p <= q): the same sum of
  * dB_a/dr_k (x_c, p) B_q(k) + dB_a/dr_k (x_c, q) B_p(k) with B of the neighbour k
- * (force_ij_quadratic, mld_stress.F90:480-488, 584-600).
+ * (force_ij_quadratic, mld_stress.F90:480-488, 584-600).  Kernel model (order 3): columns D + 1 + c
+ * carry the same sum over the gradients of kernel feature c of the batch's last mld_kernel_po_dev
+ * (force_ij_kernel, :454-457).
  * S (6 * nstruct): the Voigt targets xx yy zz yz xz xy per structure as the poscar holds them;
  * w_s (nstruct): w_s_model per structure, 0 for none.  Needs the batch's gradients. */
 int mld_gram_add_stress(mld_ctx* ctx, int order, const double* S, const double* w_s, double sign_stress,
@@ -225,12 +227,16 @@ int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w, double* d
  * (read_ml_file.F90:353-354, passed to dgelsd by gram_reduce_and_solve,
  * module_online_fit.F90:580-583); rcond < 0 means dlamch('E') = 2^-53, as in dgelsd. */
 int mld_gram_solve_svd(mld_ctx* ctx, double lambda, double rcond, double* w, double* diag_out);
-/* The two-stage quadratic fit (mld_type_quadratic = 1, QNML, the reference's default for
- * mld_order = 2): train_snap_get_parameters_precondition_inversion_matrix (parameters.F90:832-1015)
- * fits the linear columns [N, B] first, (G_LL + lambda) w_L = h_L, then the quadratic columns to
- * the residual targets, (G_QQ + Lambda_Q) w_Q = h_Q - G_LQ^T w_L, each stage with the dense solve
- * of train_snap_get_parameters_linear (:1018-1212) on its block of the accumulated Gram, whose
- * columns are the quadratic unique layout [N, B, (B_p B_q)_{p<=q}] (gD = 1 + D + D(D+1)/2).
+/* The two-stage fit of the quadratic model (mld_type_quadratic = 1, QNML, the reference's default
+ * for mld_order = 2) and of the kernel model: train_snap_get_parameters_precondition_inversion_matrix
+ * (parameters.F90:832-1015) fits the linear columns [N, B] first, (G_LL + lambda) w_L = h_L, then the
+ * other columns to the residual targets, (G_QQ + Lambda_Q) w_Q = h_Q - G_LQ^T w_L, on the blocks of
+ * the accumulated Gram: the quadratic unique layout [N, B, (B_p B_q)_{p<=q}] (gD = 1 + D + D(D+1)/2)
+ * with the dense solve of train_snap_get_parameters_linear (:1018-1212), or the kernel rows
+ * [N, B, k] of mld_gram_add_batch order 3 (gD = 1 + D + nk) with the augmented least squares of
+ * train_snap_get_parameters_kernel (:1217-1400): ridge lambda^2 and, for MLD_SOLVE_SVD, the rows'
+ * singular-value threshold rcond as rcond^2 on the eigenvalues (rcond < 0: 2^-53 of the largest,
+ * the resolution of the normal equations).
  * method MLD_SOLVE_SVD (rcond as in mld_gram_solve_svd), MLD_SOLVE_CHOL or MLD_SOLVE_CHOL_IR (the
  * refinement then iterates on the Gram blocks).  w = [w_L, w_Q] (gD values); diag_out (optional,
  * 8 doubles): the four diagnostics of the linear stage, then those of the quadratic stage. */
@@ -269,10 +275,30 @@ int mld_kernel_po(mld_ctx* ctx, int nat, const double* energy, const double* for
 int mld_model_set(mld_ctx* ctx, int order, int ncol, const double* w, const double* ref_energy);
 int mld_mahalanobis_set(mld_ctx* ctx, const double* mean, const double* sigma_inv);
 int mld_predict_dev(mld_ctx* ctx, double* E_dev, double* F_dev, double* d2_dev);
-/* Stresses of a linear or quadratic model on the last batch (md_mld_compute_stress,
+/* Stresses of a linear, quadratic or kernel model on the last batch (md_mld_compute_stress,
  * mld.F90:1777-1867): S_dev (6 * nstruct) = w . [0, stress row] with the rows of
  * mld_gram_add_stress. */
 int mld_predict_stress_dev(mld_ctx* ctx, double sign_stress, double sign_stress_big_box, double* S_dev);
+
+/* ---- descriptor analysis (ml_type = -2, main_descriptors_analysis.F90:34-473) ----
+ * On the descriptors of the last batch (mld_batch_bso4_dev), atoms listed by their index in the batch.
+ * mld_desc_sum: sum (D doubles) of the listed atoms' descriptors -- the class mean of
+ *   get_dimension_design_Phi_by_class (module_Sigma_matrix.F90:162-202) is sum / n.
+ * mld_desc_scatter_add: the context's Gram (mld_gram_begin(ctx, D) first) += sum over the listed
+ *   atoms of (B - mean)(B - mean)^T, the Phi Phi^T of compute_Sigma_class_by_Phi (:256-308); the
+ *   covariance is that sum / (D - 1) + 1e-9 I (mld_gram_export gives the sum).
+ * mld_desc_stats_dev: per atom of the batch |B|^2, |B - mean|^2, (B - mean)^T Sigma^-1 (B - mean)
+ *   (compute_mcd_dist_for_one_config, main_descriptors_analysis.F90:1315-1405) and the projections
+ *   (B - mean) . p_k on npca vectors (get_pca_projection, :1006-1043).  mean (D), sigma_inv (D, D),
+ *   pvec (D, npca) column-major host arrays; outputs device arrays (natoms) and proj (natoms, npca)
+ *   row-major; any output may be NULL (d2 needs sigma_inv).
+ * mld_sym_eig: eigenvalues (ascending) and eigenvectors (columns of the column-major n x n evecs,
+ *   may be NULL) of a symmetric host matrix by the device's cyclic Jacobi (diago_serial_real_double). */
+int mld_desc_sum(mld_ctx* ctx, int nsel, const int* atoms, double* sum);
+int mld_desc_scatter_add(mld_ctx* ctx, int nsel, const int* atoms, const double* mean);
+int mld_desc_stats_dev(mld_ctx* ctx, const double* mean, const double* sigma_inv, int npca, const double* pvec,
+                       double* norm_dev, double* norm_mean_dev, double* d2_dev, double* proj_dev);
+int mld_sym_eig(mld_ctx* ctx, int n, const double* A, double* evals, double* evecs);
 
 /* ---- serving: continuous batching of prediction requests ----
  * No counterpart in the reference (which evaluates one configuration at a time,

@@ -29,6 +29,7 @@ EXPORTS = [
     "mld_gram_add_rows_dev", "mld_gram_add_rows_noalloc", "mld_gram_add_batch", "mld_gram_add_structures", "mld_gram_export", "mld_gram_import", "mld_gram_set_keep_rows", "mld_gram_solve", "mld_gram_solve_svd", "mld_gram_solve_qnml",
     "mld_kernel_po_set", "mld_kernel_po_dev", "mld_kernel_po", "mld_model_set", "mld_mahalanobis_set", "mld_predict_dev",
     "mld_gram_add_stress", "mld_predict_stress_dev",
+    "mld_desc_sum", "mld_desc_scatter_add", "mld_desc_stats_dev", "mld_sym_eig",
     "mld_serve_create", "mld_serve_submit", "mld_serve_poll", "mld_serve_stats", "mld_serve_destroy",
     "mld_get_stream", "mld_synchronize",
 ]
@@ -173,6 +174,10 @@ def load() -> C.CDLL:
         "mld_predict_dev": [vp, vp, vp, vp],
         "mld_gram_add_stress": [vp, i, vp, vp, d, d],
         "mld_predict_stress_dev": [vp, d, d, vp],
+        "mld_desc_sum": [vp, i, vp, vp],
+        "mld_desc_scatter_add": [vp, i, vp, vp],
+        "mld_desc_stats_dev": [vp, vp, vp, i, vp, vp, vp, vp, vp],
+        "mld_sym_eig": [vp, i, vp, vp, vp],
         "mld_serve_create": [vp, C.c_size_t, i, i, vpp],
         "mld_serve_submit": [vp, i, vp, vp, vp, i, i, i64p],
         "mld_serve_poll": [vp, C.c_longlong, i, vp, vp, vp, vp, ip],
@@ -249,6 +254,7 @@ class Context:
             raise MldError(rc, "mld_create failed: " + L.mld_last_error(None).decode())
         self.h = h
         self.L = L
+        self.device = device
         self.precision = precision
         self.nspecies = nspecies
         self.nk = 0

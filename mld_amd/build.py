@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MLD_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "neigh.hip", "bso4.hip", "gram.hip", "solve.hip", "kernel_po.hip", "predict.hip", "serve.hip", "ace.hip", "tables.cpp", "ace_basis.cpp"]
+SOURCES = ["api.hip", "neigh.hip", "bso4.hip", "gram.hip", "solve.hip", "kernel_po.hip", "predict.hip", "serve.hip", "ace.hip", "analysis.hip", "tables.cpp", "ace_basis.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", f"--offload-arch={ARCH}",
           # no packed-FP32 instructions anywhere in the device code

@@ -303,6 +303,7 @@ int mld_destroy(mld_ctx* ctx)
     gram_scratch_free(ctx);
     kept_rows_free(ctx);
     predict_free(ctx);
+    analysis_free(ctx);
     void* ptrs[] = {ctx->d_ace_tables, ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
                     ctx->d_geo, ctx->d_count, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_scan_tmp,
                     ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_pgeo, ctx->d_etgt, ctx->d_zref, ctx->d_zz, ctx->d_zref32, ctx->d_kiz};

@@ -326,8 +326,9 @@ __global__ void __launch_bounds__(256) rows_force_krr_kernel(int D, int nk, int 
 // Quadratic model (order 2), unique column (p <= q) of the block:
 //   row[c][(p, q)] = sign_stress * 3 / V * f * sum of (g_{a,k}(x, p) B_q(k) + g_{a,k}(x, q) B_p(k)) u_k(y) / r_k
 // with B of the neighbour k (force_ij_quadratic, mld_stress.F90:480-488, 584-600: energy(:, ik),
-// ik = kind_neigh(ia, iw)), the reference's full-layout entry (p, q).  Rows and targets carry
-// sqrt(w_s).  One block per stressed structure; thread t owns the columns t, t + 256, ... and
+// ik = kind_neigh(ia, iw)), the reference's full-layout entry (p, q).  Kernel model (order 3),
+// column D + 1 + c: the same sum over the gradients of kernel feature c (force_ij_kernel,
+// mld_stress.F90:454-457, placed at :246-250).  Rows and targets carry sqrt(w_s).  One block per stressed structure; thread t owns the columns t, t + 256, ... and
 // sums the structure's pairs in order.
 template <typename T>
 __global__ void __launch_bounds__(256) rows_stress_kernel(int D, int ncol, int ldA, const int* __restrict__ sidx,
@@ -336,8 +337,9 @@ __global__ void __launch_bounds__(256) rows_stress_kernel(int D, int ncol, int l
                                                           const int* __restrict__ pair_idx,
                                                           const double4* __restrict__ pair_d,
                                                           const mld::StructGeo* __restrict__ geo, const T* __restrict__ energy,
-                                                          const T* __restrict__ grad, double sign_stress, double sign_big,
-                                                          double* __restrict__ A, double* __restrict__ y)
+                                                          const T* __restrict__ grad, const T* __restrict__ fk, int nk,
+                                                          double sign_stress, double sign_big, double* __restrict__ A,
+                                                          double* __restrict__ y)
 {
     const int r = blockIdx.x;
     const int s = sidx[r];
@@ -359,6 +361,24 @@ __global__ void __launch_bounds__(256) rows_stress_kernel(int D, int ncol, int l
                     const double g0 = (double)grad[(slot * 3 + 0) * D + d];
                     const double g1 = (double)grad[(slot * 3 + 1) * D + d];
                     const double g2 = (double)grad[(slot * 3 + 2) * D + d];
+                    acc[0] += g0 * ux;
+                    acc[1] += g1 * uy;
+                    acc[2] += g2 * uz;
+                    acc[3] += g1 * uz;
+                    acc[4] += g0 * uz;
+                    acc[5] += g0 * uy;
+                }
+        } else if (fk && col > D && col < ncol) {
+            // kernel model: the kernel features' gradients of the slot (force_ij_kernel, mld_stress.F90:454-457)
+            const int c = col - 1 - D;
+            for (int a = a0; a < a1; ++a)
+                for (int p = pair_off[a]; p < pair_off[a + 1]; ++p) {
+                    const double4 pd = pair_d[p];
+                    const double ux = pd.x / pd.w, uy = pd.y / pd.w, uz = pd.z / pd.w;
+                    const size_t slot = (size_t)p + a + 1;
+                    const double g0 = (double)fk[(slot * 3 + 0) * nk + c];
+                    const double g1 = (double)fk[(slot * 3 + 1) * nk + c];
+                    const double g2 = (double)fk[(slot * 3 + 2) * nk + c];
                     acc[0] += g0 * ux;
                     acc[1] += g1 * uy;
                     acc[2] += g2 * uz;
@@ -1237,10 +1257,12 @@ int mld_gram_add_stress(mld_ctx* ctx, int order, const double* S, const double* 
 {
     if (!ctx) return MLD_E_ARG;
     if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
-    if (order != 1 && order != 2)
-        return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_gram_add_stress: stress rows of the linear and quadratic models only");
+    if (order != 1 && order != 2 && order != 3)
+        return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_gram_add_stress: order 1, 2 or 3");
     const int D = ctx->D;
-    const int ncol = 1 + D + (order == 2 ? D * (D + 1) / 2 : 0);
+    const int ncol = 1 + D + (order == 2 ? D * (D + 1) / 2 : 0) + (order == 3 ? ctx->nk : 0);
+    if (order == 3 && (!ctx->last_fk || !ctx->nk))
+        return mld_fail(ctx, MLD_E_STATE, "mld_gram_add_stress: kernel features with gradients of the batch first");
     if (ncol != ctx->gD) return mld_fail(ctx, MLD_E_ARG, "mld_gram_add_stress: the Gram is not as wide as the order's rows");
     if (!ctx->last_grad || !ctx->last_energy) return mld_fail(ctx, MLD_E_STATE, "mld_batch_bso4_dev with gradients first");
     const int nstruct = ctx->nstruct;
@@ -1274,12 +1296,14 @@ int mld_gram_add_stress(mld_ctx* ctx, int order, const double* S, const double* 
     if (ctx->p.precision == MLD_PREC_FP32)
         hipLaunchKernelGGL(rows_stress_kernel<float>, dim3(ns), dim3(256), 0, ctx->stream, D, ncol, ldA, d_idx, d_sw, d_S,
                            ctx->d_atom_off, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_geo,
-                           (const float*)ctx->last_energy, (const float*)ctx->last_grad, sign_stress,
+                           (const float*)ctx->last_energy, (const float*)ctx->last_grad,
+                           order == 3 ? (const float*)ctx->last_fk : nullptr, ctx->nk, sign_stress,
                            sign_stress_big_box, A, y);
     else
         hipLaunchKernelGGL(rows_stress_kernel<double>, dim3(ns), dim3(256), 0, ctx->stream, D, ncol, ldA, d_idx, d_sw, d_S,
                            ctx->d_atom_off, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_geo,
-                           (const double*)ctx->last_energy, (const double*)ctx->last_grad, sign_stress,
+                           (const double*)ctx->last_energy, (const double*)ctx->last_grad,
+                           order == 3 ? (const double*)ctx->last_fk : nullptr, ctx->nk, sign_stress,
                            sign_stress_big_box, A, y);
     MLD_HIP(ctx, hipGetLastError());
     int rc = gram_accumulate_rows(ctx, M, A, y, 0);

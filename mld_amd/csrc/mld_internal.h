@@ -252,6 +252,8 @@ struct mld_ctx {
     double* d_kiz = nullptr;     // [nk] (sigma^2 + |z|^2 / (2 l^2))^(-p/2)
     // fitted model and selection-score state (predict.hip)
     void* predict_state = nullptr;
+    // descriptor-analysis buffers (analysis.hip)
+    void* analysis_scratch = nullptr;
     // copies of the caller arrays named by p (species weights, inner cutoffs)
     std::vector<double> h_weight, h_weight3, h_rin;
     // ACE contexts (descriptor_type MLD_DESC_ACE)
@@ -295,6 +297,7 @@ void gram_scratch_free(mld_ctx* ctx);
 int device_ensure(mld_ctx* ctx, void** p, size_t* cap, size_t bytes);
 int pair_transpose(mld_ctx* ctx, const int** rev_off, const int** rev_pair, const int** pair_src);
 void predict_free(mld_ctx* ctx);
+void analysis_free(mld_ctx* ctx);
 int predict_clone(mld_ctx* dst, const mld_ctx* src);
 // order of the fitted model (1, 2, 3), 0 before mld_model_set
 int predict_order(const mld_ctx* ctx);

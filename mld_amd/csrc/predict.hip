@@ -230,9 +230,10 @@ __global__ void __launch_bounds__(256) pair_gdot_kernel(long long npairs, int D,
     }
 }
 
-// Stress of a linear or quadratic model (md_mld_compute_stress, mld.F90:1777-1867: w . [0, pack_stress_*]):
+// Stress of a linear, quadratic or kernel model (md_mld_compute_stress, mld.F90:1777-1867: w . [0, pack_stress_*]):
 //   S_c(s) = sign_stress * 3 / V * f * sum over the pairs p of s of t(slot(p), x_c) u_p(y_c) / r_p,
-// t = w . dB per slot and direction (linear) or the neighbour's g . dB (quadratic, pair_gdot_kernel), (x, y) of c and f as in the stress rows (mld_stress.F90:272-756);
+// t = w . dB per slot and direction (linear; + wk . dk for a kernel model) or the neighbour's g . dB
+// (quadratic, pair_gdot_kernel), (x, y) of c and f as in the stress rows (mld_stress.F90:272-756);
 // one block per structure, pairs strided over the threads, then a fixed-order tree sum.
 __global__ void __launch_bounds__(256) stress_kernel(const int* __restrict__ atom_off, const int* __restrict__ pair_off,
                                                      const int* __restrict__ pair_src, const double4* __restrict__ pair_d,
@@ -527,8 +528,8 @@ int mld_predict_stress_dev(mld_ctx* ctx, double sign_stress, double sign_stress_
     if (!ctx || !S_dev) return MLD_E_ARG;
     PredictState* ps = reinterpret_cast<PredictState*>(ctx->predict_state);
     if (!ps || !ps->d_w) return mld_fail(ctx, MLD_E_STATE, "mld_predict_stress_dev: mld_model_set first");
-    if (ps->order != 1 && ps->order != 2)
-        return mld_fail(ctx, MLD_E_UNSUPPORTED, "mld_predict_stress_dev: stresses of linear and quadratic models only");
+    if (ps->order == 3 && (ps->nk != ctx->nk || !ctx->last_fk))
+        return mld_fail(ctx, MLD_E_STATE, "mld_predict_stress_dev: kernel model needs mld_kernel_po_dev features with gradients");
     if (ps->D != ctx->D) return mld_fail(ctx, MLD_E_STATE, "mld_predict_stress_dev: model dimension changed");
     if (!ctx->last_grad || !ctx->last_energy)
         return mld_fail(ctx, MLD_E_STATE, "mld_predict_stress_dev: descriptors with gradients first");
@@ -583,6 +584,14 @@ int mld_predict_stress_dev(mld_ctx* ctx, double sign_stress, double sign_stress_
         else
             hipLaunchKernelGGL(row_dot_kernel<double>, dim3(nb), dim3(256), 0, st, nrow, D, ps->d_w,
                                (const double*)ctx->last_grad, t);
+        if (ps->order == 3) {  // + the kernel part wk . dk of every slot row
+            if (f32)
+                hipLaunchKernelGGL(krr_dot_kernel<float>, dim3(nb), dim3(256), 0, st, nrow, ps->nk, ps->d_wk,
+                                   (const float*)ctx->last_fk, t);
+            else
+                hipLaunchKernelGGL(krr_dot_kernel<double>, dim3(nb), dim3(256), 0, st, nrow, ps->nk, ps->d_wk,
+                                   (const double*)ctx->last_fk, t);
+        }
     }
     hipLaunchKernelGGL(stress_kernel, dim3(nstruct), dim3(256), 0, st, ctx->d_atom_off, ctx->d_pair_off, pair_src,
                        ctx->d_pair_d, ctx->d_geo, t, sign_stress, sign_stress_big_box, S_dev);

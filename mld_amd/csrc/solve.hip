@@ -506,6 +506,46 @@ extern "C" int mld_gram_set_keep_rows(mld_ctx* ctx, int keep)
     return MLD_OK;
 }
 
+// Cyclic Jacobi on the symmetric n2 x n2 matrix A (n2 even) with V accumulating the rotations:
+// sweeps until one applies no rotation, every |A_pq| <= eps sqrt(|A_pp A_qq|) or below the floor
+// eps * max |A_ii| (which also gives the small eigenvalues to high relative accuracy).  dscratch
+// holds n2 doubles, part >= 1 int, cs / pq n2 / 2 pairs.
+static int jacobi_sweeps(mld_ctx* ctx, int n2, double* A, double* V, double* dscratch, void* part, double2* cs,
+                         int2* pq, int* sweeps_out)
+{
+    hipStream_t st = ctx->stream;
+    const int h2 = n2 / 2;
+    const long long nthr = (long long)h2 * h2 + (long long)n2 * h2;
+    const unsigned nblk = (unsigned)((nthr + 255) / 256);
+    int* nrot = (int*)part;
+    double floor_abs = 0.0;
+    {
+        std::vector<double> d0(n2);
+        hipLaunchKernelGGL(mn_diag_kernel, dim3((n2 + 255) / 256), dim3(256), 0, st, n2, A, dscratch);
+        MLD_HIP(ctx, hipMemcpyAsync(d0.data(), dscratch, (size_t)n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        MLD_HIP(ctx, hipStreamSynchronize(st));
+        for (double v : d0) floor_abs = std::max(floor_abs, std::fabs(v));
+        floor_abs = std::max(floor_abs * DBL_EPSILON, 1e-300);
+    }
+    int sweeps = 0;
+    const int max_sweeps = 40;
+    for (; sweeps < max_sweeps && n2 >= 2; ++sweeps) {
+        MLD_HIP(ctx, hipMemsetAsync(nrot, 0, sizeof(int), st));
+        for (int step = 0; step < n2 - 1; ++step) {
+            hipLaunchKernelGGL(mn_params_kernel, dim3((h2 + 255) / 256), dim3(256), 0, st, n2, step, A, floor_abs,
+                               cs, pq, nrot);
+            hipLaunchKernelGGL(mn_rotate_kernel, dim3(nblk), dim3(256), 0, st, n2, A, V, cs, pq);
+        }
+        MLD_HIP(ctx, hipGetLastError());
+        int hrot = 0;
+        MLD_HIP(ctx, hipMemcpyAsync(&hrot, nrot, sizeof(int), hipMemcpyDeviceToHost, st));
+        MLD_HIP(ctx, hipStreamSynchronize(st));
+        if (hrot == 0) break;
+    }
+    *sweeps_out = sweeps;
+    return MLD_OK;
+}
+
 // Minimum-norm solution of (G + Lambda) w = h in the sense of the reference's SVD
 // solve (serial_lsystem_by_svd, stats_tools.F90:1153-1213, LAPACK dgelsd; rcond < 0 means LAPACK's
 // machine precision dlamch('E') = 2^-53): eigenvalues at or below rcond * max |eigenvalue| are
@@ -543,35 +583,8 @@ static int solve_min_norm(mld_ctx* ctx, const SysView& v, const std::vector<doub
     MLD_HIP(ctx, hipMemcpyAsync(dsc, sc.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(mn_build_kernel, dim3(n2), dim3(256), 0, st, n, n2, ld, v.G, dr, dsc, v.h, A, V, b);
     MLD_HIP(ctx, hipGetLastError());
-    const long long nthr = (long long)h2 * h2 + (long long)n2 * h2;
-    const unsigned nblk = (unsigned)((nthr + 255) / 256);
-    // sweeps until one applies no rotation: every |A_pq| <= eps sqrt(|A_pp A_qq|), which
-    // also gives the small eigenvalues to high relative accuracy
-    int* nrot = (int*)part;
-    double floor_abs = 0.0;
-    {
-        std::vector<double> d0(n2);
-        hipLaunchKernelGGL(mn_diag_kernel, dim3((n2 + 255) / 256), dim3(256), 0, st, n2, A, r);
-        MLD_HIP(ctx, hipMemcpyAsync(d0.data(), r, (size_t)n2 * sizeof(double), hipMemcpyDeviceToHost, st));
-        MLD_HIP(ctx, hipStreamSynchronize(st));
-        for (double v : d0) floor_abs = std::max(floor_abs, std::fabs(v));
-        floor_abs = std::max(floor_abs * DBL_EPSILON, 1e-300);
-    }
     int sweeps = 0;
-    const int max_sweeps = 40;
-    for (; sweeps < max_sweeps && n2 >= 2; ++sweeps) {
-        MLD_HIP(ctx, hipMemsetAsync(nrot, 0, sizeof(int), st));
-        for (int step = 0; step < n2 - 1; ++step) {
-            hipLaunchKernelGGL(mn_params_kernel, dim3((h2 + 255) / 256), dim3(256), 0, st, n2, step, A, floor_abs,
-                               cs, pq, nrot);
-            hipLaunchKernelGGL(mn_rotate_kernel, dim3(nblk), dim3(256), 0, st, n2, A, V, cs, pq);
-        }
-        MLD_HIP(ctx, hipGetLastError());
-        int hrot = 0;
-        MLD_HIP(ctx, hipMemcpyAsync(&hrot, nrot, sizeof(int), hipMemcpyDeviceToHost, st));
-        MLD_HIP(ctx, hipStreamSynchronize(st));
-        if (hrot == 0) break;
-    }
+    if (int rc = jacobi_sweeps(ctx, n2, A, V, r, part, cs, pq, &sweeps)) return rc;
     // eigenvalues on the diagonal; y = V^T b scaled by the pseudo-inverse; x = V y; w = S x
     std::vector<double> ev(n2), hy(n2);
     hipLaunchKernelGGL(mn_diag_kernel, dim3((n2 + 255) / 256), dim3(256), 0, st, n2, A, r);
@@ -853,14 +866,16 @@ extern "C" int mld_gram_solve_svd(mld_ctx* ctx, double lambda, double rcond, dou
     return gram_solve(ctx, full_view(ctx), lambda, MLD_SOLVE_SVD, rcond, w_out, diag_out);
 }
 
-// Two-stage quadratic fit (mld_type_quadratic = 1, QNML): train_snap_get_parameters_precondition_
-// inversion_matrix (parameters.F90:832-1015) splits the rows into the linear block L = [N, B] and the
-// quadratic block Q (split_Amat_in_Lmat_Qmat, :476-587), fits L to y (precond_linear_from_Lmat,
-// :591-674), then Q to the residual y - L w_L (precond_nonlinear_from_Qmat, :677-829), each with the
-// dense solve of train_snap_get_parameters_linear (:1018-1212): (L W L^T + lambda) w_L = L W y,
-// (Q W Q^T + lambda) w_Q = Q W (y - L^T w_L) = h_Q - G_LQ^T w_L.  Both stages read blocks of the
-// one accumulated Gram.  w = [w_L, w_Q]; diag_out (8 doubles) holds the four diagnostics of each
-// stage in order.
+// Two-stage fit of the quadratic model (mld_type_quadratic = 1, QNML) and of the kernel model:
+// train_snap_get_parameters_precondition_inversion_matrix (parameters.F90:832-1015, called for both
+// at :414-420 and :434-436) splits the rows into the linear block L = [N, B] and the rest Q
+// (split_Amat_in_Lmat_Qmat, :476-587), fits L to y (precond_linear_from_Lmat, :591-674), then Q to
+// the residual y - L w_L (precond_nonlinear_from_Qmat, :677-829): the dense solve of
+// train_snap_get_parameters_linear (:1018-1212) for L and for the quadratic Q,
+// (L W L^T + lambda) w_L = L W y, (Q W Q^T + lambda) w_Q = Q W (y - L^T w_L) = h_Q - G_LQ^T w_L,
+// and train_snap_get_parameters_kernel (:1217-1400) for the kernel Q (ridge lambda^2, see below).
+// Both stages read blocks of the one accumulated Gram.  w = [w_L, w_Q]; diag_out (8 doubles)
+// holds the four diagnostics of each stage in order.
 extern "C" int mld_gram_solve_qnml(mld_ctx* ctx, double lambda, int method, double rcond, double* w_out,
                                    double* diag_out)
 {
@@ -868,9 +883,17 @@ extern "C" int mld_gram_solve_qnml(mld_ctx* ctx, double lambda, int method, doub
     if (!ctx->d_G) return mld_fail(ctx, MLD_E_STATE, "mld_gram_begin first");
     if (!w_out) return mld_fail(ctx, MLD_E_ARG, "mld_gram_solve_qnml: null w");
     const int D = ctx->D, nL = 1 + D, n = ctx->gD;
-    if (n != nL + D * (D + 1) / 2)
-        return mld_fail(ctx, MLD_E_ARG, "mld_gram_solve_qnml: the Gram is not 1 + D + D(D+1)/2 columns wide");
-    ctx->gram_order = 2;  // the columns are the quadratic unique layout (also when packed by the caller)
+    const bool kernel = ctx->gram_order == 3 && ctx->nk > 0 && n == nL + ctx->nk;
+    if (!kernel && n != nL + D * (D + 1) / 2)
+        return mld_fail(ctx, MLD_E_ARG,
+                        "mld_gram_solve_qnml: the Gram is neither 1 + D + D(D+1)/2 nor (kernel rows) 1 + D + nk wide");
+    if (!kernel) ctx->gram_order = 2;  // the quadratic unique layout (also when packed by the caller)
+    // the kernel stage is the augmented least squares [sqrt(W) K; lambda I] of
+    // train_snap_get_parameters_kernel (parameters.F90:1297-1310): ridge lambda^2 on the normal
+    // equations, and its singular-value threshold rcond on the rows is rcond^2 on their eigenvalues
+    // (rcond < 0: every eigenvalue the normal equations resolve, above 2^-53 of the largest)
+    const double lam2 = kernel ? (lambda > 0.0 ? lambda * lambda : 0.0) : lambda;
+    const double rc2 = kernel ? (rcond < 0.0 ? -1.0 : std::max(rcond * rcond, 0.5 * DBL_EPSILON)) : rcond;
     double d1[4] = {0, 0, 0, 0}, d2[4] = {0, 0, 0, 0};
     int rc = gram_solve(ctx, SysView{ctx->d_G, ctx->d_h, nL, 0}, lambda, method, rcond, w_out, d1);
     if (rc) return rc;
@@ -887,7 +910,7 @@ extern "C" int mld_gram_solve_qnml(mld_ctx* ctx, double lambda, int method, doub
     hipLaunchKernelGGL(qnml_rhs_kernel, dim3((n - nL + 255) / 256), dim3(256), 0, st, n, nL, ctx->gld, ctx->d_G,
                        ctx->d_h, dwL, dh2);
     MLD_HIP(ctx, hipGetLastError());
-    rc = gram_solve(ctx, SysView{ctx->d_G + (size_t)nL * ctx->gld + nL, dh2, n - nL, nL}, lambda, method, rcond,
+    rc = gram_solve(ctx, SysView{ctx->d_G + (size_t)nL * ctx->gld + nL, dh2, n - nL, nL}, lam2, method, rc2,
                     w_out + nL, d2);
     MLD_HIP(ctx, hipStreamSynchronize(st));
     if (diag_out)
@@ -896,4 +919,66 @@ extern "C" int mld_gram_solve_qnml(mld_ctx* ctx, double lambda, int method, doub
             diag_out[4 + i] = d2[i];
         }
     return rc;
+}
+
+// Eigen-decomposition of a symmetric n x n matrix (diago_serial_real_double, the reference's
+// dsyev call of main_descriptors_analysis.F90:288-301) by the device's cyclic Jacobi: evals
+// ascending, evecs (n x n, column k the eigenvector of evals[k], column-major = evecs[k * n + i]).
+extern "C" int mld_sym_eig(mld_ctx* ctx, int n, const double* A, double* evals, double* evecs)
+{
+    if (!ctx || n <= 0 || !A || !evals) return MLD_E_ARG;
+    hipStream_t st = ctx->stream;
+    const int n2 = n + (n & 1), h2 = n2 / 2;
+    const size_t nn = (size_t)n2 * n2;
+    double *dA = nullptr, *dV = nullptr, *vec = nullptr;
+    MLD_HIP(ctx, hipMalloc(&dA, nn * sizeof(double)));
+    struct Free {
+        double** a;
+        double** v;
+        double** x;
+        ~Free() { (void)hipFree(*a); (void)hipFree(*v); (void)hipFree(*x); }
+    } fr{&dA, &dV, &vec};
+    MLD_HIP(ctx, hipMalloc(&dV, nn * sizeof(double)));
+    MLD_HIP(ctx, hipMalloc(&vec, (size_t)(4 * n2 + 8) * sizeof(double) + (size_t)h2 * (sizeof(double2) + sizeof(int2))));
+    double* ones = vec;            // n2: the unit scale of mn_build_kernel
+    double* zero = ones + n2;      // n2: no ridge, no right-hand side
+    double* b = zero + n2;         // n2 (unused)
+    double* d = b + n2;            // n2
+    int* part = (int*)(d + n2);    // 8 doubles
+    double2* cs = (double2*)(d + n2 + 8);
+    int2* pq = (int2*)(cs + h2);
+    std::vector<double> hA((size_t)n * n), h1(n2, 1.0), h0(n2, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) hA[(size_t)i * n + j] = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
+    double* dsrc = nullptr;
+    MLD_HIP(ctx, hipMalloc(&dsrc, hA.size() * sizeof(double)));
+    struct Free1 {
+        double** p;
+        ~Free1() { (void)hipFree(*p); }
+    } fr1{&dsrc};
+    MLD_HIP(ctx, hipMemcpyAsync(dsrc, hA.data(), hA.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    MLD_HIP(ctx, hipMemcpyAsync(ones, h1.data(), (size_t)n2 * sizeof(double), hipMemcpyHostToDevice, st));
+    MLD_HIP(ctx, hipMemcpyAsync(zero, h0.data(), (size_t)n2 * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(mn_build_kernel, dim3(n2), dim3(256), 0, st, n, n2, n, dsrc, zero, ones, zero, dA, dV, b);
+    MLD_HIP(ctx, hipGetLastError());
+    int sweeps = 0;
+    if (int rc = jacobi_sweeps(ctx, n2, dA, dV, d, part, cs, pq, &sweeps)) return rc;
+    std::vector<double> ev(n2), hV(nn);
+    hipLaunchKernelGGL(mn_diag_kernel, dim3((n2 + 255) / 256), dim3(256), 0, st, n2, dA, d);
+    MLD_HIP(ctx, hipMemcpyAsync(ev.data(), d, (size_t)n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MLD_HIP(ctx, hipMemcpyAsync(hV.data(), dV, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    MLD_HIP(ctx, hipStreamSynchronize(st));
+    // the padding index (odd n) is an exact zero row and column: drop the column whose vector
+    // lives on it, then sort ascending
+    std::vector<int> keep;
+    for (int k = 0; k < n2; ++k)
+        if (n2 == n || std::fabs(hV[(size_t)n * n2 + k]) < 0.5) keep.push_back(k);
+    if ((int)keep.size() != n) return mld_fail(ctx, MLD_E_DEVICE, "mld_sym_eig: padding column not separated");
+    std::sort(keep.begin(), keep.end(), [&](int a, int c) { return ev[a] < ev[c]; });
+    for (int k = 0; k < n; ++k) {
+        evals[k] = ev[keep[k]];
+        if (evecs)
+            for (int i = 0; i < n; ++i) evecs[(size_t)k * n + i] = hV[(size_t)i * n2 + keep[k]];
+    }
+    return MLD_OK;
 }

@@ -4,9 +4,11 @@ poscar database, through this library.
   descriptors(ml_path)  ml_type = -1: per-atom descriptors of every selected configuration
                         (written as descDB/<name>.eml when write_desc is set, the files the
                         reference writes, descriptors.F90:87-382);
-  fit(ml_path)          ml_type = 0: linear or quadratic fit of energies and forces with the
-                        per-class weights of db_model.in (module_online_fit.F90:992-1038), the
-                        coefficient files written as the reference names them.
+  fit(ml_path)          ml_type = 0: linear or quadratic fit of energies, forces and stresses
+                        with the per-class weights of db_model.in (module_online_fit.F90:992-1038);
+                        ml_type = 1: the kernel model on the reference points of kernel_matrix.dat
+                        (written by mld_amd.analysis, ml_type = -2); the coefficient files and
+                        train_*.out written as the reference names them.
 Configuration selection follows read_db_file / build_subdata (build_subdata.F90:1720-1880):
 class c of db_model.in takes the files <c>_<klm>_000001 .. <c>_<klm>_<no_total> of db_path and
 selects no_selec of them, the first ones (selection_type 1), the last ones (2) or, for the
@@ -90,17 +92,24 @@ def descriptors(ml_path: str, out_dir: Optional[str] = None, precision: int = 0,
 
 
 def fit(ml_path: str, out_dir: Optional[str] = None, precision: int = 0, method: Optional[int] = None,
-        overrides: Optional[Dict[str, Any]] = None, export: bool = False,
-        predict: bool = False) -> Tuple[np.ndarray, Dict[str, Any]]:
-    """The fit of an &input_ml file and its db_model.in selection (fit_configs)."""
+        overrides: Optional[Dict[str, Any]] = None, export: bool = False, predict: bool = False,
+        kernel_matrix: Optional[str] = None) -> Tuple[np.ndarray, Dict[str, Any]]:
+    """The fit of an &input_ml file and its db_model.in selection (fit_configs).  A kernel fit
+    (ml_type = 1) reads its reference points from kernel_matrix.dat (init_kernel,
+    compute_kernel.F90:1058-1199): kernel_matrix, or the file beside the namelist."""
+    from . import analysis
+
     nml = {**inputs.read_input_ml(ml_path), **(overrides or {})}
     base = os.path.dirname(os.path.abspath(ml_path))
-    return fit_configs(nml, select_configs(nml, base), out_dir, precision, method, export, predict)
+    z = None
+    if inputs.fit_order(nml) == 3:
+        _, z = analysis.read_kernel_matrix(kernel_matrix or os.path.join(base, "kernel_matrix.dat"))
+    return fit_configs(nml, select_configs(nml, base), out_dir, precision, method, export, predict, kernel_points=z)
 
 
 def fit_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]], out_dir: Optional[str] = None,
                 precision: int = 0, method: Optional[int] = None, export: bool = False,
-                predict: bool = False) -> Tuple[np.ndarray, Dict[str, Any]]:
+                predict: bool = False, kernel_points: Optional[np.ndarray] = None) -> Tuple[np.ndarray, Dict[str, Any]]:
     """Linear (mld_order 1) or quadratic (2) fit of the configurations (mld.F90:279-420): energy rows
     with the header energy iread_energy less ref_energy_per_element, force rows, and the stress rows
     of the classes whose S flag is on (train_fill_Amat_with_stress, mld_stress.F90:181-268), each
@@ -118,7 +127,11 @@ def fit_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]]
 
     order = inputs.fit_order(nml)
     if order == 3:
-        raise NotImplementedError("kernel fits need the kernel-point selection of the reference (MCD / CUR)")
+        if kernel_points is None:
+            raise ValueError("a kernel fit needs its standardised reference points (kernel_matrix.dat)")
+        if int(nml["kernel_type"]) != io.KERNEL_PO:
+            raise NotImplementedError(f"kernel_type {nml['kernel_type']}: the polynomial kernel_po (4) is built")
+        kernel = (float(nml["sigma_kernel"]), float(nml["length_kernel"]), float(nml["kernel_power"]))
     qtype = inputs.quadratic_type(nml) if order == 2 else None
     if order == 2 and qtype not in (inputs.QUADRATIC_QNML, inputs.QUADRATIC_QML):
         raise NotImplementedError(f"mld_type_quadratic {qtype}: the two-stage (1) and the full (2) quadratic fits "
@@ -131,7 +144,11 @@ def fit_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]]
     sgn, sgn_big = float(nml["sign_stress"]), float(nml["sign_stress_big_box"])
     ctx = Context(**inputs.context_kwargs(nml, precision))
     try:
-        gf = F.GramFit(ctx, order=order)
+        nk = 0
+        if order == 3:
+            F.kernel_po_set(ctx, kernel_points, *kernel)
+            nk = kernel_points.shape[0]
+        gf = F.GramFit(ctx, order=order, nk=nk)
         # one batch per class and energy / force weight pair (loss_per gives every atom count its own
         # weights); stress weights go per structure
         groups: Dict[Tuple[str, float, float], List[int]] = {}
@@ -146,6 +163,9 @@ def fit_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]]
             b = _batch(ctx, group, els)
             e, g = b.alloc_outputs(True)
             b.bso4(e, g)
+            if order == 3:  # compute_kernel (descriptors.F90:67-75) on the batch
+                ek, fk = F.kernel_po_alloc(ctx, b)
+                F.kernel_po(ctx, ek, fk)
             for c, _ in group:
                 if (w_e and not c.has_energy) or (w_f and (not c.has_force or c.forces is None)):
                     raise ValueError(f"{c.name}: class {cls} fits {'energies' if w_e else ''} "
@@ -159,7 +179,7 @@ def fit_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]]
                 gf.add_stress(b, S, ws, sgn, sgn_big)
             batches.append(idx)
         m = F.SOLVE_SVD if method is None else method
-        if qtype == inputs.QUADRATIC_QNML:
+        if qtype == inputs.QUADRATIC_QNML or order == 3:
             w, diag = gf.solve_qnml(inputs.ridge(nml), m, rcond=float(nml["svd_rcond"]))
         else:
             w, diag = gf.solve(inputs.ridge(nml), m, rcond=float(nml["svd_rcond"]))
@@ -169,7 +189,7 @@ def fit_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]]
             group = [configs[k] for idx in batches for k in idx]
             index = [k for idx in batches for k in idx]
             diag["predictions"] = _predict(ctx, order, w, configs, group, index, els, sgn, sgn_big)
-            if qtype == inputs.QUADRATIC_QNML:  # the linear stage alone (*_deviation_linear.milady)
+            if qtype == inputs.QUADRATIC_QNML or order == 3:  # the linear stage alone (*_deviation_linear.milady)
                 diag["predictions_linear"] = _predict(ctx, 1, w[:1 + ctx.D], configs, group, index, els, sgn,
                                                       sgn_big)
         D = ctx.D
@@ -177,13 +197,14 @@ def fit_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]]
         ctx.close()
     if out_dir:
         F.write_coefficients(out_dir, w, D, order, els, float(nml["j_max"]), float(nml["r_cut"]),
-                             lbso4_diag=bool(nml["lbso4_diag"]))
+                             lbso4_diag=bool(nml["lbso4_diag"]), kernel_points=kernel_points,
+                             kernel=kernel if order == 3 else None)
         recs = fit_records(nml, configs, weights, diag["predictions"])
         io.write_train_outputs(out_dir, recs)
         if "predictions_linear" in diag:
             lin = fit_records(nml, configs, weights, diag["predictions_linear"])
             io.write_deviation_files(out_dir, lin, "linear")
-            io.write_deviation_files(out_dir, recs, "quadratic", base=lin)
+            io.write_deviation_files(out_dir, recs, "kernel" if order == 3 else "quadratic", base=lin)
         if not predict:
             diag.pop("predictions", None)
             diag.pop("predictions_linear", None)
@@ -229,8 +250,11 @@ def _predict(ctx, order: int, w: np.ndarray, configs, group, index, els: List[st
         b = _batch(ctx, group[start:stop], els)
         e, g = b.alloc_outputs(True)
         b.bso4(e, g)
+        if order == 3:
+            ek, fk = F.kernel_po_alloc(ctx, b)
+            F.kernel_po(ctx, ek, fk)
         Ep, Fp, _ = F.predict(ctx, b)
-        Sp = F.predict_stress(ctx, b, sgn, sgn_big).cpu().numpy() if order in (1, 2) else None
+        Sp = F.predict_stress(ctx, b, sgn, sgn_big).cpu().numpy()
         Ep, Fp = Ep.cpu().numpy(), Fp.cpu().numpy()
         for j in range(stop - start):
             preds[index[start + j]] = (float(Ep[j]), Fp[b.atom_off[j]:b.atom_off[j + 1]].copy(),
@@ -240,18 +264,22 @@ def _predict(ctx, order: int, w: np.ndarray, configs, group, index, els: List[st
 
 
 def predict_configs(nml: Dict[str, Any], configs: List[Tuple[io.Config, io.DbClass]], w: np.ndarray,
-                    precision: int = 0) -> List[Tuple[float, np.ndarray, Optional[np.ndarray]]]:
-    """Energies, forces and stresses (linear and quadratic models) of the configurations with the
-    coefficients w: the evaluation the reference runs for its train / test sets
+                    precision: int = 0, kernel_points: Optional[np.ndarray] = None
+                    ) -> List[Tuple[float, np.ndarray, Optional[np.ndarray]]]:
+    """Energies, forces and stresses (linear, quadratic and kernel models) of the configurations with
+    the coefficients w: the evaluation the reference runs for its train / test sets
     (train_snap_compute_energy_force_stress, mld.F90:944-1020; main_test_mld, :2506-2743).
     One (E, F (nat, 3), S (6) or None) per configuration."""
-    from . import Context
+    from . import Context, fit as F
 
     order = inputs.fit_order(nml)
-    if order == 3:
-        raise NotImplementedError("kernel models are evaluated through the library's kernel features")
     ctx = Context(**inputs.context_kwargs(nml, precision))
     try:
+        if order == 3:
+            if kernel_points is None:
+                raise ValueError("a kernel model needs its standardised reference points (kernel_matrix.dat)")
+            F.kernel_po_set(ctx, kernel_points, float(nml["sigma_kernel"]), float(nml["length_kernel"]),
+                            float(nml["kernel_power"]))
         return _predict(ctx, order, w, configs, configs, list(range(len(configs))), inputs.elements(nml),
                         float(nml["sign_stress"]), float(nml["sign_stress_big_box"]))
     finally:

@@ -163,11 +163,12 @@ class GramFit:
                    "max_pivot": float(diag[3]), "converged": rc == 0}
 
     def solve_qnml(self, lam: float, method: int = SOLVE_SVD, rcond: float = -1.0):
-        """Two-stage quadratic fit (mld_type_quadratic = 1, QNML; mld_gram_solve_qnml): the linear
-        columns [N, B] on their own, then the quadratic columns on the residual targets.  Returns
-        w = [w_L, w_Q] and the diagnostics of both stages."""
-        if self.order != 2:
-            raise ValueError("solve_qnml: the two-stage fit needs the quadratic rows (order 2)")
+        """Two-stage fit (mld_gram_solve_qnml) of the quadratic model (mld_type_quadratic = 1, QNML)
+        or of the kernel model: the linear columns [N, B] on their own, then the quadratic or
+        kernel columns on the residual targets.  Returns w = [w_L, w_Q] and the diagnostics of
+        both stages."""
+        if self.order not in (2, 3):
+            raise ValueError("solve_qnml: the two-stage fit needs the quadratic (order 2) or kernel (3) rows")
         w = np.zeros(self.ncol)
         diag = np.zeros(8)
         self.ctx.check(self.ctx.L.mld_gram_solve_qnml(self.ctx.h, lam, method, rcond, w.ctypes.data_as(C.c_void_p),
@@ -265,9 +266,12 @@ def predict_stress(ctx, batch, sign_stress: float = 1.0, sign_stress_big_box: fl
 
 
 def write_coefficients(directory: str, w: np.ndarray, D: int, order: int, elements, j_max: float, r_cut: float,
-                       lbso4_diag: bool = False, weights=None, weights_3ch=None, ref_energy=None):
-    """Write <bso4>_<snap1|qnml1>_params.pot and lammps_... in the reference formats; a
-    quadratic solution in the unique column layout is expanded to the full D^2 block first."""
+                       lbso4_diag: bool = False, weights=None, weights_3ch=None, ref_energy=None,
+                       kernel_points=None, kernel=None):
+    """Write <bso4>_<snap1|qnml1|kernl>_params.pot and lammps_... in the reference formats; a
+    quadratic solution in the unique column layout is expanded to the full D^2 block first; a
+    kernel model (order 3) also needs its standardised reference points (nk, D) and
+    kernel = (sigma, length, power)."""
     import os
 
     from . import io
@@ -278,5 +282,6 @@ def write_coefficients(directory: str, w: np.ndarray, D: int, order: int, elemen
     pot, lmp = io.params_file_names("bso4", order)
     io.write_params_pot(os.path.join(directory, pot), w)
     io.write_lammps_params_bso4(os.path.join(directory, lmp), w, elements, j_max, r_cut, order=order,
-                                lbso4_diag=lbso4_diag, weights=weights, weights_3ch=weights_3ch, ref_energy=ref_energy)
+                                lbso4_diag=lbso4_diag, weights=weights, weights_3ch=weights_3ch, ref_energy=ref_energy,
+                                kernel_points=kernel_points, kernel=kernel)
     return os.path.join(directory, pot), os.path.join(directory, lmp)

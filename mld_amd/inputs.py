@@ -64,6 +64,17 @@ DEFAULTS: Dict[str, Any] = {
     "ace_npoints_spline": 1000,   # :290
     "ace_gencg": 2,               # :304 (1 RI, 2 RPI)
     "l_ace_set_rcut": False,      # :300
+    # descriptor analysis and kernel-point selection (ml_type = -2)
+    "classes_for_mcd": " 01 02",  # :329
+    "kernel_dump": 1,             # :341 (1 Mahalanobis normalised, 2 Mahalanobis, 3 CUR, 4 CUR-Mahalanobis)
+    "np_kernel_ref": 200,         # :342
+    "np_kernel_full": 800,        # :343
+    "power_mcd": 0.05,            # :348
+    "n_pca": 3,                   # :393
+    "seed": 14792435,             # :492
+    "write_kernel_matrix": False, # :499
+    "cur_kval": -1,               # :545
+    "cur_cval": -1,               # :547
 }
 ACE_NU_LIMIT = 6  # l_ace_order(1 .. 6), all .false. by default (:280-281)
 

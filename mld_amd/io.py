@@ -207,9 +207,9 @@ def _fe(v: float, w: int = 20, d: int = 10) -> str:
 
 
 def params_file_names(descriptor: str, order: int):
-    """<desc>_<snap1|qnml1>_params.pot and its lammps_ twin (build_name_of_parameters_file,
-    module_write_parameters.F90:764-781)."""
-    tag = {1: "snap1", 2: "qnml1"}[order]
+    """<desc>_<snap1|qnml1|kernl>_params.pot and its lammps_ twin (build_name_of_parameters_file,
+    module_write_parameters.F90:764-781); order 3 is the kernel model."""
+    tag = {1: "snap1", 2: "qnml1", 3: "kernl"}[order]
     return f"{descriptor}_{tag}_params.pot", f"lammps_{descriptor}_{tag}_params.pot"
 
 
@@ -224,11 +224,18 @@ def write_params_pot(path: str, w: np.ndarray) -> None:
             fh.write(_es(float(v)) + "\n")
 
 
+MLD_KERNEL = 7          # mld_order of a kernel model (ml_in_ndm_module.F90:1361)
+KERNEL_PO = 4           # kernel_type of the polynomial kernel (ml_in_ndm_module.F90:880)
+
+
 def write_lammps_params_bso4(path: str, w: np.ndarray, elements, j_max: float, r_cut: float, order: int = 1,
-                             lbso4_diag: bool = False, weights=None, weights_3ch=None, ref_energy=None) -> None:
-    """lammps_bso4_<snap1|qnml1>_params.pot read by milady_lammps: element line, descriptor
+                             lbso4_diag: bool = False, weights=None, weights_3ch=None, ref_energy=None,
+                             kernel_points=None, kernel=None) -> None:
+    """lammps_bso4_<snap1|qnml1|kernl>_params.pot read by milady_lammps: element line, descriptor
     line, masses, details, r_cut, then the coefficients (module_write_parameters.F90:876-937,
-    1065-1085, 1186-1188)."""
+    1065-1085, 1186-1188).  A kernel model (order 3) adds the standardised reference points
+    (global_kernel, nk x D, point by point) and (sigma, length, power) of kernel_po after the
+    coefficients, and its descriptor and detail lines count them (:809-816, 1067-1083, 1485-1512)."""
     w = np.asarray(w, dtype=np.float64).reshape(-1)
     ne = len(elements)
     nch = 1 + (weights is not None) + (weights_3ch is not None)
@@ -250,14 +257,27 @@ def write_lammps_params_bso4(path: str, w: np.ndarray, elements, j_max: float, r
         second = w2[i] if nch == 3 else 1.0
         head += el.ljust(2).rjust(4) + _fe(w1[i]) + _fe(second) + _fe(er[i])
     lines.append(head)
-    lines.append(f"{9:4d}{w.shape[0]:12d}{4:4d}")
-    lines.append("".join(f"{element_mass(el):20.4f}" for el in elements))
-    lines.append(f"{order:5d}{j_max:5.1f}{int(lbso4_diag):4d}{0:4d}")
+    if order == 3:
+        z = np.asarray(kernel_points, np.float64)
+        nk, D = z.shape
+        lines.append(f"{9:4d}{w.shape[0] + nk * D + 3:12d}{7:4d}")
+        lines.append("".join(f"{element_mass(el):20.4f}" for el in elements))
+        lines.append(f"{MLD_KERNEL:5d}{j_max:5.1f}{int(lbso4_diag):4d}{0:6d}{KERNEL_PO:6d}{D:6d}{nk:6d}")
+    else:
+        lines.append(f"{9:4d}{w.shape[0]:12d}{4:4d}")
+        lines.append("".join(f"{element_mass(el):20.4f}" for el in elements))
+        lines.append(f"{order:5d}{j_max:5.1f}{int(lbso4_diag):4d}{0:4d}")
     lines.append(f"{r_cut:20.4f}")
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
         for v in w:
             fh.write(_es(float(v)) + "\n")
+        if order == 3:
+            for row in z:
+                for v in row:
+                    fh.write(_es(float(v)) + "\n")
+            for v in kernel:   # sigma, length, power
+                fh.write(_fe(float(v), 25, 15) + "\n")
 
 
 def read_params_pot(path: str) -> np.ndarray:
