@@ -240,6 +240,9 @@ def run_fit(structs, cfg, form: str, steps: int, warmup: int, chunk=None):
         batches.append((b, E.cpu().numpy(), F.reshape(-1).contiguous()))
     natoms = sum(b.natoms for b, _, _ in batches)
     npairs = sum(b.npairs for b, _, _ in batches)
+    # the Gram accumulation's slice count for this batch shape, chosen by the library's autotuner
+    b0 = batches[0][0]
+    tuned = fit.tune_gram(ctx, b0.nstruct + 3 * b0.natoms, fit.ncols(ctx.D, cfg["order"]), i8, reps=3)
     times, kern = [], []
     w = diag = None
     for it in range(warmup + steps):
@@ -280,6 +283,7 @@ def run_fit(structs, cfg, form: str, steps: int, warmup: int, chunk=None):
         nf += 3 * b.natoms
     rows = sum(b.nstruct + 3 * b.natoms for b, _, _ in batches)
     out = dict(w=w, solve=diag, natoms=natoms, npairs=npairs, rows=rows, s_per_fit=float(np.mean(times)),
+               gram_tuning={"slices": tuned[0] or "heuristic", "ms": tuned[1], "heuristic_ms": tuned[2]},
                bso4_ms_per_fit=float(np.mean(kern)), nbatch=len(batches),
                rmse={"energy_meV_per_atom": 1e3 * math.sqrt(e2 / ne), "force_meV_per_A": 1e3 * math.sqrt(f2 / nf)})
     del batches
@@ -379,7 +383,7 @@ def main_fit(args):
         ach = flops / (r["bso4_ms_per_fit"] * 1e-3) / 1e12
         return {"value": natoms / r["s_per_fit"], "ms_per_step": 1e3 * r["s_per_fit"],
                 "bso4_ms_per_step": r["bso4_ms_per_fit"], "bso4_tflops": ach, "bso4_frac": ach / peak,
-                "solve": r["solve"], "rmse": r["rmse"]}
+                "solve": r["solve"], "rmse": r["rmse"], "gram_tuning": r["gram_tuning"]}
 
     ach = flops / (r64["bso4_ms_per_fit"] * 1e-3) / 1e12
     quad = args.workload == "fit-quadratic"

@@ -129,6 +129,11 @@ module mldgpu_c
       import; type(c_ptr), value :: ctx; integer(c_int), value :: order, ncol
       real(c_double), intent(in) :: w(*); type(c_ptr), value :: ref_energy
     end function
+    ! fastest row-slice count of the Gram accumulation for m rows of ncol columns (kept for the process)
+    integer(c_int) function mld_autotune_gram(ctx, m, ncol, mixed, reps, best_ns, best_ms, heuristic_ms) bind(c)
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: m, ncol, mixed, reps
+      integer(c_int), intent(out) :: best_ns; real(c_double), intent(out) :: best_ms, heuristic_ms
+    end function
     ! descriptor analysis (ml_type = -2) on the last batch: atoms are 0-based indices in the batch
     integer(c_int) function mld_desc_sum(ctx, nsel, atoms, sum) bind(c)
       import; type(c_ptr), value :: ctx; integer(c_int), value :: nsel

@@ -227,6 +227,15 @@ int mld_gram_solve(mld_ctx* ctx, double lambda, int method, double* w, double* d
  * (read_ml_file.F90:353-354, passed to dgelsd by gram_reduce_and_solve,
  * module_online_fit.F90:580-583); rcond < 0 means dlamch('E') = 2^-53, as in dgelsd. */
 int mld_gram_solve_svd(mld_ctx* ctx, double lambda, double rcond, double* w, double* diag_out);
+/* Autotuner of the Gram accumulation (no counterpart in the reference): times the row-slice counts
+ * the SYRK of mld_gram_add_batch / mld_gram_add_rows* can split M rows of ncol columns into (FP64
+ * MFMA form, mixed = 0, or the int8 Ozaki form, mixed = 1), on pseudo-random rows in a private
+ * Gram, reps launches each: the occupancy heuristic's own choice, then 8, 16, ... slices.  The
+ * fastest is kept for this device, padded width, form and row-count octave and used by every later
+ * accumulation of the process.  best_ns is 0 when the heuristic's choice was fastest; the times are
+ * per accumulation in ms.  Every slice count gives the same Gram up to the summation order. */
+int mld_autotune_gram(mld_ctx* ctx, int M, int ncol, int mixed, int reps, int* best_ns, double* best_ms,
+                      double* heuristic_ms);
 /* The two-stage fit of the quadratic model (mld_type_quadratic = 1, QNML, the reference's default
  * for mld_order = 2) and of the kernel model: train_snap_get_parameters_precondition_inversion_matrix
  * (parameters.F90:832-1015) fits the linear columns [N, B] first, (G_LL + lambda) w_L = h_L, then the

@@ -26,7 +26,7 @@ DESC_BSO4, DESC_ACE = 9, 300
 EXPORTS = [
     "mld_create", "mld_destroy", "mld_last_error", "mld_descriptor_dim", "mld_ace_basis_review", "mld_max_neighbours", "mld_bso4", "mld_ace",
     "mld_batch_neighbours", "mld_batch_pairs_dev", "mld_batch_pair_geometry_dev", "mld_batch_bso4_dev", "mld_gram_begin", "mld_gram_add_rows",
-    "mld_gram_add_rows_dev", "mld_gram_add_rows_noalloc", "mld_gram_add_batch", "mld_gram_add_structures", "mld_gram_export", "mld_gram_import", "mld_gram_set_keep_rows", "mld_gram_solve", "mld_gram_solve_svd", "mld_gram_solve_qnml",
+    "mld_gram_add_rows_dev", "mld_gram_add_rows_noalloc", "mld_gram_add_batch", "mld_gram_add_structures", "mld_gram_export", "mld_gram_import", "mld_gram_set_keep_rows", "mld_gram_solve", "mld_gram_solve_svd", "mld_gram_solve_qnml", "mld_autotune_gram",
     "mld_kernel_po_set", "mld_kernel_po_dev", "mld_kernel_po", "mld_model_set", "mld_mahalanobis_set", "mld_predict_dev",
     "mld_gram_add_stress", "mld_predict_stress_dev",
     "mld_desc_sum", "mld_desc_scatter_add", "mld_desc_stats_dev", "mld_sym_eig",
@@ -166,6 +166,7 @@ def load() -> C.CDLL:
         "mld_gram_solve": [vp, d, i, vp, vp],
         "mld_gram_solve_svd": [vp, d, d, vp, vp],
         "mld_gram_solve_qnml": [vp, d, i, d, vp, vp],
+        "mld_autotune_gram": [vp, i, i, i, i, vp, vp, vp],
         "mld_kernel_po_set": [vp, i, vp, d, d, d],
         "mld_kernel_po_dev": [vp, vp, vp],
         "mld_kernel_po": [vp, i, vp, vp, vp, i, vp, vp],

@@ -355,35 +355,47 @@ int mld_synchronize(mld_ctx* ctx)
     return MLD_OK;
 }
 
+__global__ void clear_count_kernel(int* p) { *p = 0; }
+
 int mld_batch_neighbours(mld_ctx* ctx, int nstruct, const int* atom_off, const double* pos_dev,
                          const double* cell_dev, const int* itype_dev, long long* npairs)
 {
     if (!ctx) return MLD_E_ARG;
     if (nstruct < 0 || !atom_off || (nstruct > 0 && (!pos_dev || !cell_dev || !itype_dev)))
         return mld_fail(ctx, MLD_E_ARG, "mld_batch_neighbours: bad arguments");
-    int natoms = atom_off[nstruct];
     if (atom_off[0] != 0) return mld_fail(ctx, MLD_E_ARG, "atom_off[0] must be 0");
     for (int s = 0; s < nstruct; ++s)
         if (atom_off[s + 1] < atom_off[s]) return mld_fail(ctx, MLD_E_ARG, "atom_off must be non-decreasing");
-    if (natoms > 0) {
-        // species indices select weight / inner-cutoff table rows: validate them once
-        // (read on the library stream, after any upload the caller queued there)
-        std::vector<int> it(natoms);
-        MLD_HIP(ctx, hipMemcpyAsync(it.data(), itype_dev, (size_t)natoms * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        MLD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        for (int a = 0; a < natoms; ++a)
-            if (it[a] < 0 || it[a] >= ctx->nsp) return mld_fail(ctx, MLD_E_ARG, "itype out of range");
-    }
-    // cells: fetch to host for the per-structure set-up (small/large box, reciprocal vectors)
+    const int natoms = atom_off[nstruct];
+    // species indices (they select weight / inner-cutoff table rows) and cells (small/large box,
+    // reciprocal vectors) are read once on the host, on the library stream after any upload the
+    // caller queued there
+    std::vector<int> it(natoms);
     std::vector<double> cells((size_t)9 * nstruct);
-    if (nstruct) {
-        MLD_HIP(ctx, hipMemcpyAsync(cells.data(), cell_dev, cells.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        MLD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    }
+    if (natoms) MLD_HIP(ctx, hipMemcpyAsync(it.data(), itype_dev, (size_t)natoms * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (nstruct) MLD_HIP(ctx, hipMemcpyAsync(cells.data(), cell_dev, cells.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (natoms || nstruct) MLD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = batch_neighbours_host(ctx, nstruct, atom_off, pos_dev, cells.data(), itype_dev, it.data(), nullptr);
+    if (rc) return rc;
+    if (npairs) *npairs = ctx->npairs;
+    return MLD_OK;
+}
+
+}  // extern "C"
+
+// mld_batch_neighbours with the cells and species indices already on the host (a caller that
+// uploaded them, such as the serving loop): no read-back before the neighbour count.  h_pair_off
+// (natoms + 1 ints, may be null) receives the pair offsets in the same read-back as the pair total.
+int batch_neighbours_host(mld_ctx* ctx, int nstruct, const int* atom_off, const double* pos_dev,
+                          const double* cells_host, const int* itype_dev, const int* itype_host, int* h_pair_off)
+{
+    const int natoms = atom_off[nstruct];
+    for (int a = 0; a < natoms; ++a)
+        if (itype_host[a] < 0 || itype_host[a] >= ctx->nsp) return mld_fail(ctx, MLD_E_ARG, "itype out of range");
     std::vector<mld::StructGeo> geo(nstruct);
     std::vector<int> astruct(natoms);
     for (int s = 0; s < nstruct; ++s) {
-        mld::struct_geometry(&cells[(size_t)9 * s], ctx->p.r_cut, geo[s]);
+        mld::struct_geometry(&cells_host[(size_t)9 * s], ctx->p.r_cut, geo[s]);
         geo[s].atom0 = atom_off[s];
         geo[s].nat = atom_off[s + 1] - atom_off[s];
         for (int a = atom_off[s]; a < atom_off[s + 1]; ++a) astruct[a] = s;
@@ -410,19 +422,20 @@ int mld_batch_neighbours(mld_ctx* ctx, int nstruct, const int* atom_off, const d
         MLD_HIP(ctx, hipMalloc(&ctx->d_atom_off, cap * sizeof(int)));
         ctx->cap_struct_off = cap;
     }
+    // the host arrays are staged in the context (pageable copies complete before the call returns)
     MLD_HIP(ctx, hipMemcpyAsync(ctx->d_atom_off, atom_off, (size_t)(nstruct + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     if (nstruct) MLD_HIP(ctx, hipMemcpyAsync(ctx->d_geo, geo.data(), geo.size() * sizeof(mld::StructGeo), hipMemcpyHostToDevice, ctx->stream));
     if (natoms) MLD_HIP(ctx, hipMemcpyAsync(ctx->d_atom_struct, astruct.data(), astruct.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    MLD_HIP(ctx, hipMemsetAsync(ctx->d_count + natoms, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(clear_count_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->d_count + natoms);
+    MLD_HIP(ctx, hipGetLastError());
     ctx->nstruct = nstruct;
     ctx->natoms = natoms;
     ctx->h_atom_off.assign(atom_off, atom_off + nstruct + 1);
     ctx->d_itype = itype_dev;
-    int rc = launch_neighbours_pos(ctx, pos_dev);
-    if (rc) return rc;
-    if (npairs) *npairs = ctx->npairs;
-    return MLD_OK;
+    return launch_neighbours_pos(ctx, pos_dev, h_pair_off);
 }
+
+extern "C" {
 
 int mld_batch_pairs_dev(mld_ctx* ctx, const int** pair_off_dev, const int** pair_idx_dev)
 {

@@ -20,7 +20,9 @@
 // Split-K over row slices writes per-slice partial tiles that a reduction kernel
 // adds in slice order: the result does not depend on scheduling.
 #include <hipcub/hipcub.hpp>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "mld_internal.h"
 
@@ -831,12 +833,43 @@ struct GramScratch {
 // the columns rounded up to 128 (reads past ldA are zero) times row slices, the slice count the
 // one closest to whole waves of 512 resident workgroups (two per CU at this kernel's register
 // use) with at least 256 rows a slice and at most ns_max slices.
+// ---------------------------------------------------------------- tuned slice counts
+// mld_autotune_gram times the row-slice counts a SYRK can use for a Gram width and a row count and
+// keeps the fastest per (device, padded width, form, row-count octave); the planners below take a
+// kept count before their occupancy heuristic.  The autotuner itself forces candidates through
+// t_force_ns.
+struct TuneKey {
+    int dev, ld, mixed, octave;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(dev, ld, mixed, octave) < std::tie(o.dev, o.ld, o.mixed, o.octave);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tuned;
+thread_local int t_force_ns = 0;
+
+int octave_of(int M)
+{
+    int o = 0;
+    while ((1 << (o + 1)) <= M) ++o;
+    return o;
+}
+
+int tuned_ns(int dev, int ld, int mixed, int M)
+{
+    if (t_force_ns > 0) return t_force_ns;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(TuneKey{dev, ld, mixed, octave_of(M)});
+    return it == g_tuned.end() ? 0 : it->second;
+}
+
 struct SyrkPlan {
     int nt2 = 0, nup2 = 0, ns = 1, rps = 0, hs = 1, hrps = 0;
     size_t part_bytes = 0, gpart_bytes = 0;
 };
 
-static SyrkPlan plan_syrk(int M, int ldA, int ns_max, int hs_max)
+static SyrkPlan plan_syrk(int M, int ldA, int ns_max, int hs_max, int ns_force = 0)
 {
     SyrkPlan pl;
     pl.nt2 = (ldA + T2 - 1) / T2;
@@ -852,6 +885,7 @@ static SyrkPlan plan_syrk(int M, int ldA, int ns_max, int hs_max)
         const double eff = (double)wg / (double)(((wg + cap - 1) / cap) * cap);
         if (eff > best + 1e-9) { best = eff; ns2 = ns; }
     }
+    if (ns_force > 0) ns2 = std::max(1, std::min(ns_force, std::max(1, M / 32)));
     int rps2 = (M + ns2 - 1) / ns2;
     rps2 = ((rps2 + 31) / 32) * 32;
     pl.ns = (M + rps2 - 1) / rps2;
@@ -910,7 +944,7 @@ int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, 
     GramScratch& gs = *reinterpret_cast<GramScratch*>(ctx->gram_scratch);
     int rc = MLD_OK;
     if (!mixed) {
-        const SyrkPlan pl = plan_syrk(M, ldA, 64, 512);
+        const SyrkPlan pl = plan_syrk(M, ldA, 64, 512, tuned_ns(ctx->p.device, ldA, 0, M));
         rc = device_ensure(ctx, &gs.part, &gs.cap_part, pl.part_bytes);
         if (rc) return rc;
         rc = device_ensure(ctx, &gs.gpart, &gs.cap_gpart, pl.gpart_bytes);
@@ -940,6 +974,7 @@ int gram_accumulate_rows(mld_ctx* ctx, int M, const double* A, const double* y, 
             if (eff > best + 1e-9) { best = eff; ns = k; }
         }
     }
+    if (const int tn = tuned_ns(ctx->p.device, ldA, 1, M)) ns = std::max(ns_lo, tn);
     int rps = (M + ns - 1) / ns;
     rps = ((rps + 1023) / 1024) * 1024;
     ns = (M + rps - 1) / rps;
@@ -1370,3 +1405,107 @@ int mld_gram_import(mld_ctx* ctx, const double* G, const double* h, int accumula
 }
 
 }  // extern "C"
+
+// pseudo-random rows in [-1, 1) with a spread of column scales (a Gram of a fit's shape)
+__global__ void tune_rows_kernel(long long n, int ld, double* A)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long x = (unsigned long long)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    x ^= x >> 31;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 29;
+    const int col = (int)(i % ld);
+    A[i] = ((double)(x >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0) * (1.0 + (col % 17));
+}
+
+extern "C" int mld_autotune_gram(mld_ctx* ctx, int M, int ncol, int mixed, int reps, int* best_ns, double* best_ms,
+                                 double* heuristic_ms)
+{
+    if (!ctx || M <= 0 || ncol <= 0 || reps <= 0) return MLD_E_ARG;
+    if (!ctx->gram_scratch) ctx->gram_scratch = new GramScratch();
+    const int ld = ((ncol + TILE - 1) / TILE) * TILE;
+    hipStream_t st = ctx->stream;
+    double *A = nullptr, *G = nullptr;
+    MLD_HIP(ctx, hipMalloc(&A, (size_t)M * ld * sizeof(double) + (size_t)M * sizeof(double)));
+    struct Free {
+        double** a;
+        double** g;
+        ~Free() { (void)hipFree(*a); (void)hipFree(*g); }
+    } fr{&A, &G};
+    MLD_HIP(ctx, hipMalloc(&G, (size_t)ld * ld * sizeof(double) + (size_t)ld * sizeof(double)));
+    const long long n = (long long)M * ld;
+    hipLaunchKernelGGL(tune_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, ld, A);
+    MLD_HIP(ctx, hipMemsetAsync(A + n, 0, (size_t)M * sizeof(double), st));
+    // the tuner runs the library's own accumulation on a private Gram of this width
+    double* sG = ctx->d_G;
+    double* sh = ctx->d_h;
+    const int sld = ctx->gld, sD = ctx->gD;
+    const long long srows = ctx->gram_rows;
+    ctx->d_G = G;
+    ctx->d_h = G + (size_t)ld * ld;
+    ctx->gld = ld;
+    ctx->gD = ncol;
+    struct Restore {
+        mld_ctx* c;
+        double *g, *h;
+        int ld, D;
+        long long rows;
+        ~Restore()
+        {
+            c->d_G = g;
+            c->d_h = h;
+            c->gld = ld;
+            c->gD = D;
+            c->gram_rows = rows;
+            t_force_ns = 0;
+        }
+    } restore{ctx, sG, sh, sld, sD, srows};
+    hipEvent_t e0, e1;
+    MLD_HIP(ctx, hipEventCreate(&e0));
+    MLD_HIP(ctx, hipEventCreate(&e1));
+    struct Ev {
+        hipEvent_t a, b;
+        ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } ev{e0, e1};
+    auto time_with = [&](int ns, double* ms) -> int {
+        t_force_ns = ns;
+        int rc = gram_accumulate_rows(ctx, M, A, A + n, mixed);  // warm-up (allocations, caches)
+        if (rc) return rc;
+        MLD_HIP(ctx, hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r)
+            if ((rc = gram_accumulate_rows(ctx, M, A, A + n, mixed))) return rc;
+        MLD_HIP(ctx, hipEventRecord(e1, st));
+        MLD_HIP(ctx, hipEventSynchronize(e1));
+        float t = 0.0f;
+        MLD_HIP(ctx, hipEventElapsedTime(&t, e0, e1));
+        *ms = (double)t / reps;
+        return MLD_OK;
+    };
+    // the heuristic's own choice first, then whole XCD groups of slices
+    double h_ms = 0.0;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned.erase(TuneKey{ctx->p.device, ld, mixed ? 1 : 0, octave_of(M)});
+    }
+    if (int rc = time_with(0, &h_ms)) return rc;
+    int bns = 0;
+    double bms = h_ms;
+    const int hi = mixed ? std::max(8, std::min(128, M / 4096)) : std::max(8, std::min(128, M / 256));
+    for (int ns = 8; ns <= hi; ns += 8) {
+        double ms = 0.0;
+        if (int rc = time_with(ns, &ms)) return rc;
+        if (ms < bms) {
+            bms = ms;
+            bns = ns;
+        }
+    }
+    if (bns > 0) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[TuneKey{ctx->p.device, ld, mixed ? 1 : 0, octave_of(M)}] = bns;
+    }
+    if (best_ns) *best_ns = bns;
+    if (best_ms) *best_ms = bms;
+    if (heuristic_ms) *heuristic_ms = h_ms;
+    return MLD_OK;
+}

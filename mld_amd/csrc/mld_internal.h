@@ -282,7 +282,10 @@ inline int set_lds_limit_once(mld_ctx* ctx, const void* fn, int bytes, std::atom
 }
 
 // launchers implemented in the .hip units
-int launch_neighbours_pos(mld_ctx* ctx, const double* pos);
+// h_pair_off (natoms + 1, may be null): the pair offsets, read back with the pair total
+int launch_neighbours_pos(mld_ctx* ctx, const double* pos, int* h_pair_off = nullptr);
+int batch_neighbours_host(mld_ctx* ctx, int nstruct, const int* atom_off, const double* pos_dev,
+                          const double* cells_host, const int* itype_dev, const int* itype_host, int* h_pair_off);
 int launch_bso4(mld_ctx* ctx, int want_grad, void* energy, void* grad);
 int launch_ace(mld_ctx* ctx, int want_grad, void* energy, void* grad);
 // ncg_level: largest CG1 range of one output level (cgbase[j+1] - cgbase[j]); ncg: all tables

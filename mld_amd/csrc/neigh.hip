@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(256) nl_fill(int natoms, const int* __restrict
 
 }  // namespace
 
-int launch_neighbours_pos(mld_ctx* ctx, const double* pos)
+int launch_neighbours_pos(mld_ctx* ctx, const double* pos, int* h_pair_off)
 {
     int natoms = ctx->natoms;
     hipStream_t s = ctx->stream;
@@ -155,8 +155,14 @@ int launch_neighbours_pos(mld_ctx* ctx, const double* pos)
     }
     MLD_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, tmp, ctx->d_count, ctx->d_pair_off, natoms + 1, s));
     int total = 0;
-    MLD_HIP(ctx, hipMemcpyAsync(&total, ctx->d_pair_off + natoms, sizeof(int), hipMemcpyDeviceToHost, s));
-    MLD_HIP(ctx, hipStreamSynchronize(s));
+    if (h_pair_off) {
+        MLD_HIP(ctx, hipMemcpyAsync(h_pair_off, ctx->d_pair_off, (size_t)(natoms + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+        MLD_HIP(ctx, hipStreamSynchronize(s));
+        total = h_pair_off[natoms];
+    } else {
+        MLD_HIP(ctx, hipMemcpyAsync(&total, ctx->d_pair_off + natoms, sizeof(int), hipMemcpyDeviceToHost, s));
+        MLD_HIP(ctx, hipStreamSynchronize(s));
+    }
     ctx->npairs = total;
     if ((size_t)total > ctx->cap_pairs) {
         if (ctx->d_pair_idx) MLD_HIP(ctx, hipFree(ctx->d_pair_idx));

@@ -154,13 +154,15 @@ int launch_neighbours(mld_server* sv, Slot& s, std::vector<size_t>& act)
     const double* d_pos = (const double*)s.d_in;
     const double* d_cell = (const double*)((char*)s.d_in + b_pos);
     const int* d_it = (const int*)((char*)s.d_in + b_pos + b_cell);
-    long long P = 0;
-    rc = mld_batch_neighbours(c, ns, s.atom_off.data(), d_pos, d_cell, d_it, &P);
-    if (rc) return rc;
-    s.npairs = P;
-    // pair counts per structure from the CSR offsets (the neighbour build has completed)
+    (void)d_cell;
+    // the cells and species are on the host already: the neighbour set-up reads nothing back
+    // before the count, and the pair offsets come back with the pair total (one wait per batch)
     std::vector<int> off(na + 1);
-    MLD_HIP(c, hipMemcpy(off.data(), c->d_pair_off, (size_t)(na + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    rc = batch_neighbours_host(c, ns, s.atom_off.data(), d_pos, (const double*)(hp + b_pos), d_it,
+                               (const int*)(hp + b_pos + b_cell), off.data());
+    if (rc) return rc;
+    s.npairs = c->npairs;
+    // pair counts per structure from the CSR offsets
     act.assign(ns, 0);
     for (int i = 0; i < ns; ++i)
         act[i] = batch_bytes(c, s.reqs[i]->nat, 1, (size_t)(off[s.atom_off[i + 1]] - off[s.atom_off[i]]),

@@ -13,6 +13,7 @@
 // straight from the rows, so the refined w solves the FP64 normal equations even
 // when the factorised Gram came from split-BF16 products.
 #include <algorithm>
+#include <cstdlib>
 #include <cfloat>
 #include <cmath>
 
@@ -506,6 +507,42 @@ extern "C" int mld_gram_set_keep_rows(mld_ctx* ctx, int keep)
     return MLD_OK;
 }
 
+// A launch sequence recorded once on the context's stream and replayed as one graph launch: the
+// launch-bound inner loops of the solves (a Jacobi sweep: 2 (n2 - 1) + 1 launches; one application of
+// the factored preconditioner: 2 nb + 2).  Nothing in a recorded sequence allocates, copies from the
+// host or clears memory with hipMemsetAsync (counters are cleared by a kernel).  MLD_SOLVE_GRAPHS=0
+// in the environment runs the same sequences launch by launch.
+struct Recorded {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~Recorded()
+    {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+static bool graphs_enabled()
+{
+    const char* e = std::getenv("MLD_SOLVE_GRAPHS");
+    return !(e && e[0] == '0');
+}
+
+template <class Body>
+static int record(mld_ctx* ctx, Recorded& r, Body&& body)
+{
+    MLD_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    const hipError_t launch = hipGetLastError();
+    const hipError_t end = hipStreamEndCapture(ctx->stream, &r.g);
+    if (launch != hipSuccess) return mld_fail(ctx, MLD_E_DEVICE, std::string("recorded launch: ") + hipGetErrorString(launch));
+    if (end != hipSuccess) return mld_fail(ctx, MLD_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(end));
+    MLD_HIP(ctx, hipGraphInstantiate(&r.x, r.g, nullptr, nullptr, 0));
+    return MLD_OK;
+}
+
+__global__ void clear_int_kernel(int* p) { *p = 0; }
+
 // Cyclic Jacobi on the symmetric n2 x n2 matrix A (n2 even) with V accumulating the rotations:
 // sweeps until one applies no rotation, every |A_pq| <= eps sqrt(|A_pp A_qq|) or below the floor
 // eps * max |A_ii| (which also gives the small eigenvalues to high relative accuracy).  dscratch
@@ -529,13 +566,23 @@ static int jacobi_sweeps(mld_ctx* ctx, int n2, double* A, double* V, double* dsc
     }
     int sweeps = 0;
     const int max_sweeps = 40;
-    for (; sweeps < max_sweeps && n2 >= 2; ++sweeps) {
-        MLD_HIP(ctx, hipMemsetAsync(nrot, 0, sizeof(int), st));
+    auto sweep = [&]() {
+        hipLaunchKernelGGL(clear_int_kernel, dim3(1), dim3(1), 0, st, nrot);
         for (int step = 0; step < n2 - 1; ++step) {
             hipLaunchKernelGGL(mn_params_kernel, dim3((h2 + 255) / 256), dim3(256), 0, st, n2, step, A, floor_abs,
                                cs, pq, nrot);
             hipLaunchKernelGGL(mn_rotate_kernel, dim3(nblk), dim3(256), 0, st, n2, A, V, cs, pq);
         }
+    };
+    Recorded rec;
+    const bool use_graph = graphs_enabled() && n2 >= 2;
+    if (use_graph)
+        if (int rc = record(ctx, rec, sweep)) return rc;
+    for (; sweeps < max_sweeps && n2 >= 2; ++sweeps) {
+        if (use_graph)
+            MLD_HIP(ctx, hipGraphLaunch(rec.x, st));
+        else
+            sweep();
         MLD_HIP(ctx, hipGetLastError());
         int hrot = 0;
         MLD_HIP(ctx, hipMemcpyAsync(&hrot, nrot, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -733,13 +780,24 @@ static int gram_solve(mld_ctx* ctx, const SysView& v, double lambda, int method,
             return mld_fail(ctx, MLD_E_SINGULAR, "Cholesky: leading minor " + std::to_string(hinfo) +
                                                      " not positive definite (increase lambda)");
     }
-    auto precond = [&](double* v) {  // v <- M^-1 v using S L L^T S
+    auto precond_launches = [&](double* v) {  // v <- M^-1 v using S L L^T S
         hipLaunchKernelGGL(scale_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, S, v);
         for (int k = 0; k < (int)nb; ++k)  // L z = v: z into tmpv, v is scratch
             hipLaunchKernelGGL(trsv_step_kernel, dim3((int)nb - k), dim3(256), 0, st, n, ld, L, Linv, v, tmpv, k, 0);
         for (int k = (int)nb - 1; k >= 0; --k)  // L^T v = z: tmpv is scratch
             hipLaunchKernelGGL(trsv_step_kernel, dim3(k + 1), dim3(256), 0, st, n, ld, L, Linv, tmpv, v, k, 1);
         hipLaunchKernelGGL(scale_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, S, v);
+    };
+    // every application is to d: recorded once, replayed per application
+    Recorded prec;
+    const bool prec_graph = graphs_enabled();
+    if (prec_graph)
+        if (int rc = record(ctx, prec, [&]() { precond_launches(d); })) return rc;
+    auto precond = [&](double* v) {
+        if (prec_graph && v == d)
+            (void)hipGraphLaunch(prec.x, st);
+        else
+            precond_launches(v);
     };
     auto to_host = [&](std::vector<double>& hv, const double* dv) -> int {
         MLD_HIP(ctx, hipMemcpyAsync(hv.data(), dv, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));

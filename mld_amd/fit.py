@@ -185,6 +185,18 @@ class GramFit:
         return r
 
 
+def tune_gram(ctx, M: int, ncol: int, mixed: bool = False, reps: int = 3):
+    """mld_autotune_gram: the fastest row-slice count of the Gram accumulation for M rows of ncol
+    columns in this form, kept for the rest of the process.  Returns (slices or 0 for the
+    heuristic's own, best ms, heuristic ms)."""
+    ns = C.c_int()
+    best = C.c_double()
+    heur = C.c_double()
+    ctx.check(ctx.L.mld_autotune_gram(ctx.h, int(M), int(ncol), int(bool(mixed)), int(reps), C.byref(ns),
+                                      C.byref(best), C.byref(heur)))
+    return ns.value, best.value, heur.value
+
+
 def kernel_po_set(ctx, zref: np.ndarray, sigma: float, length: float, power: float):
     """zref (nk, D): standardised reference points (global_kernel of init_kernel)."""
     z = np.ascontiguousarray(zref, np.float64)

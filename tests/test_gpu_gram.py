@@ -384,3 +384,45 @@ def test_gram_import_replace_and_accumulate(ctx64):
         assert rel(G2, G0) < 1e-13 and rel(h2, h0) < 1e-13
     finally:
         ctx2.close()
+
+
+def test_autotuned_slices_give_the_same_gram():
+    """mld_autotune_gram times the slice counts of the accumulation and keeps the fastest for this
+    width and row-count octave; with the kept counts the FP64 Gram equals NumPy's and the int8 Ozaki
+    Gram the FP64 one, both at the FP64 level (the slices change only the summation order)."""
+    import mld_amd as M
+    import torch
+    from mld_amd.fit import GramFit, ncols, tune_gram
+
+    c = M.Context(j_max=4.0, r_cut=4.7)
+    try:
+        rng = np.random.default_rng(41)
+        m, n = 40000, 120
+        A = rng.normal(size=(m, n)) * np.exp(rng.uniform(-3.0, 3.0, n))
+        y = rng.normal(size=m)
+        ns, best, heur = tune_gram(c, m, n, False, reps=2)
+        assert ns >= 0 and 0.0 < best <= heur
+        gf = GramFit(c, ncol=n)
+        gf.add_rows(A, y, np.ones(m))
+        G, h = gf.export()
+        Gn = A.T @ A
+        d = np.sqrt(np.diag(Gn))
+        assert np.abs((G - Gn) / np.outer(d, d)).max() < 1e-13
+        assert rel(h, A.T @ y) < 1e-12
+        # the quadratic rows of a batch through both forms, with slice counts tuned at its row count
+        b, pos, cells, types, E, F = small_batch(c, seed=6, nstruct=12, n=3)
+        rows = b.nstruct + 3 * b.natoms
+        for mixed in (False, True):
+            ns, best, heur = tune_gram(c, rows, ncols(55, 2), mixed, reps=2)
+            assert ns >= 0 and 0.0 < best <= heur
+        Fd = torch.from_numpy(F.reshape(-1)).to("cuda:0")
+        g64 = GramFit(c, order=2)
+        g64.add_batch(b, E, Fd, 1.0, 0.1)
+        G64, _ = g64.export()
+        gmx = GramFit(c, order=2, mixed=True)
+        gmx.add_batch(b, E, Fd, 1.0, 0.1)
+        Gmx, _ = gmx.export()
+        nrm = np.sqrt(np.outer(np.diag(G64), np.diag(G64)))
+        assert float((np.abs(Gmx - G64) / nrm).max()) < 3e-14
+    finally:
+        c.close()
