@@ -10,9 +10,12 @@
 //   B. Z contraction (pack_cmm_into_ZcmmABC, :1014-1143) straight into the
 //      merged gradient operand W_j (one LDS block per level j, layout [e][row]),
 //      and B_bi = Re sum conj(c_l) Z_A (:505-517) by a wavefront reduction.
-//   C. gradients: per neighbour the recursion is replayed with dU, the level-j
-//      vector V_j = conj(U) dfcut rhat + fcut conj(dU) goes to LDS and
-//      dB_bi/dr_k = Re sum_{j,e} W_j[bi][e] V_j[e]      (:505-652, sign flip of
+//   C. gradients: per neighbour the levels are rebuilt from their derivative
+//      sources: each partial derivative of U_j in z+, z-, x+, x- is one scaled
+//      level-(j-1) entry, so dU_j/dr and (by Euler's relation) U_j come from four
+//      entries of U_{j-1} with no dU recursion or dU storage.  The level-j vector
+//      X_j = conj(V_j) = U dfcut rhat + fcut dU/dr goes to LDS and
+//      dB_bi/dr_k = Re sum_{j,e} W_j[bi][e] conj(X_j[e])   (:505-652, sign flip of
 //      descriptors.F90:298-302 folded in), self slot = -sum_k (:692-695).
 // Precision T = double (plain FP64 form) or float (FP32 form; set-up geometry in FP64).
 #include <algorithm>
@@ -49,8 +52,10 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int NW = 4;
-constexpr int BS = 64 * NW;
+// wavefronts per atom workgroup (four: the ~30 neighbours of an atom split evenly, and a
+// wider FP64 workgroup measured slower)
+template <typename T>
+constexpr int nw_of() { return 4; }
 
 // upper bound of the merged W size (complex entries) over both enumerations
 constexpr int wsize_bound(int jj)
@@ -121,18 +126,19 @@ constexpr int cg_level_max(int jj)
 
 template <typename T>
 struct Geo {
-    cx<T> zp, xp;  // z- = conj(z+), x- = conj(x+) (and their derivatives) are not stored
-    cx<T> dzp[3], dxp[3];
-    T fc, dfc;
-    T drr[3];
+    cx<T> zp, xp;    // z- = conj(z+), x- = conj(x+) are not stored
+    cx<T> g[3], h[3];  // fcut dz+/dr_k and fcut dx+/dr_k
+    T fc;
+    T s[3];          // dfcut/dr rhat_k
 };
 
 // Per-neighbour scalars of spherical_4d (angular_functions.f90:79-115) and the
-// cutoff fcut = (1 - fcut_in) fcut_out (compute_bispectrum_so4.F90:349-377).
-// Evaluated in FP64 whatever T is.
+// cutoff fcut = (1 - fcut_in) fcut_out (compute_bispectrum_so4.F90:349-377), evaluated in
+// FP64 whatever T is.  The gradient terms are stored pre-multiplied by the cutoff:
+// V_k = conj(U) dfcut rhat_k + fcut conj(dU/dr_k) (angular_functions.f90:255-258) is linear
+// in (dfcut rhat_k, fcut dz+/dr_k, fcut dx+/dr_k).
 template <typename T>
-__device__ __forceinline__ Geo<T> make_geo(double4 d, double rcut, double r0, double rin, double win,
-                                           bool grad)
+__device__ __forceinline__ Geo<T> make_geo(double4 d, double rcut, double r0, double rin, double win)
 {
     const double pi = 3.14159265358979323846;
     double rr = d.w, x = d.x, y = d.y, z = d.z;
@@ -164,27 +170,25 @@ __device__ __forceinline__ Geo<T> make_geo(double4 d, double rcut, double r0, do
     g.zp = cx<T>{(T)(z0 * l0i), (T)(z * l0i)};
     g.xp = cx<T>{(T)(x * l0i), (T)(y * l0i)};
     g.fc = (T)fc;
-    g.dfc = (T)dfc;
     double drr[3] = {x / rr, y / rr, z / rr};
-    for (int k = 0; k < 3; ++k) g.drr[k] = (T)drr[k];
-    if (grad) {
-        double dz = tan_i - th0 * sin_i * sin_i;
-        double dil0 = (cs / r0 - l0i) / rr;
-        for (int k = 0; k < 3; ++k) {
-            double a = dz * l0i;
-            g.dzp[k] = cx<T>{(T)((z0 * dil0 + a) * drr[k]), (T)(z * dil0 * drr[k])};
-            g.dxp[k] = cx<T>{(T)(x * dil0 * drr[k]), (T)(y * dil0 * drr[k])};
-        }
-        g.dzp[2].i += (T)l0i;
-        g.dxp[0].r += (T)l0i;
-        g.dxp[1].i += (T)l0i;
+    double dz = tan_i - th0 * sin_i * sin_i;
+    double dil0 = (cs / r0 - l0i) / rr;
+    for (int k = 0; k < 3; ++k) {
+        g.s[k] = (T)(dfc * drr[k]);
+        double a = dz * l0i;
+        double dzr = (z0 * dil0 + a) * drr[k], dzi = z * dil0 * drr[k];
+        double dxr = x * dil0 * drr[k], dxi = y * dil0 * drr[k];
+        if (k == 2) dzi += l0i;
+        if (k == 0) dxr += l0i;
+        if (k == 1) dxi += l0i;
+        g.g[k] = cx<T>{(T)(fc * dzr), (T)(fc * dzi)};
+        g.h[k] = cx<T>{(T)(fc * dxr), (T)(fc * dxi)};
     }
     return g;
 }
 
-// Packed per-pair geometry: zp, xp, dzp[3], dxp[3] (complex), fc, dfc, rhat[3]; z-, x- and
-// their derivatives are the conjugates.  Written once per pair by pair_geo_kernel, read by
-// every wavefront that walks the pair.
+// Packed per-pair geometry: zp, xp, g[3], h[3] (complex), fc, s[3].  Written once per pair by
+// pair_geo_kernel, read by every wavefront that walks the pair.
 constexpr int GEO_STRIDE = 24;
 
 template <typename T>
@@ -194,15 +198,12 @@ __device__ __forceinline__ Geo<T> load_geo(const T* __restrict__ o, bool grad)
     g.zp = cx<T>{o[0], o[1]};
     g.xp = cx<T>{o[2], o[3]};
     g.fc = o[16];
-    g.dfc = o[17];
-    g.drr[0] = o[18];
-    g.drr[1] = o[19];
-    g.drr[2] = o[20];
     if (grad) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            g.dzp[k] = cx<T>{o[4 + 2 * k], o[5 + 2 * k]};
-            g.dxp[k] = cx<T>{o[10 + 2 * k], o[11 + 2 * k]};
+            g.g[k] = cx<T>{o[4 + 2 * k], o[5 + 2 * k]};
+            g.h[k] = cx<T>{o[10 + 2 * k], o[11 + 2 * k]};
+            g.s[k] = o[17 + k];
         }
     }
     return g;
@@ -225,7 +226,7 @@ __global__ void __launch_bounds__(256) pair_geo_kernel(int natoms, const int* __
     const int ti = itype[a];
     for (int p = pair_off[a] + lane; p < p1; p += 64) {
         const double rin = rin_tab ? rin_tab[ti * nsp + itype[pair_idx[p]]] : 0.0;
-        const Geo<T> g = make_geo<T>(pair_d[p], rcut, r0, rin, win, true);
+        const Geo<T> g = make_geo<T>(pair_d[p], rcut, r0, rin, win);
         T* o = geo + (size_t)p * GEO_STRIDE;
         o[0] = g.zp.r;
         o[1] = g.zp.i;
@@ -233,17 +234,14 @@ __global__ void __launch_bounds__(256) pair_geo_kernel(int natoms, const int* __
         o[3] = g.xp.i;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            o[4 + 2 * k] = g.dzp[k].r;
-            o[5 + 2 * k] = g.dzp[k].i;
-            o[10 + 2 * k] = g.dxp[k].r;
-            o[11 + 2 * k] = g.dxp[k].i;
+            o[4 + 2 * k] = g.g[k].r;
+            o[5 + 2 * k] = g.g[k].i;
+            o[10 + 2 * k] = g.h[k].r;
+            o[11 + 2 * k] = g.h[k].i;
+            o[17 + k] = g.s[k];
         }
         o[16] = g.fc;
-        o[17] = g.dfc;
-        o[18] = g.drr[0];
-        o[19] = g.drr[1];
-        o[20] = g.drr[2];
-        o[21] = o[22] = o[23] = (T)0;
+        o[20] = o[21] = o[22] = o[23] = (T)0;
     }
 }
 
@@ -261,14 +259,13 @@ __device__ __forceinline__ RecT<T> stage_rec(const mld::RecEntry& e)
     return RecT<T>{(T)((e.s1 & 1) ? -e.c1 : e.c1), (T)((e.s2 & 1) ? -e.c2 : e.c2), e.s1, e.s2};
 }
 
-// One recursion step for entry (a, b) of level j from level j-1 (Up, dUp):
-//   u = c1 zq U1 - i c2 xq U2, du = c1 (dzq U1 + zq dU1) - i c2 (dxq U2 + xq dU2),
-// zq = z+ and xq = x- for b = 0, z- and x+ otherwise.  z- = conj(z+), x- = conj(x+) and
-// likewise their derivatives, so the choice is a sign on the imaginary parts, folded into
-// the coefficients together with the source negations (both exact).
-template <typename T, bool GRAD>
-__device__ __forceinline__ void rec_step(const RecT<T>& R, int b, const Geo<T>& g, const cx<T>* Up,
-                                         const cx<T>* dUp, cx<T>& u, cx<T>* du)
+// One recursion step for entry (a, b) of level j from level j-1 (Up):
+//   u = c1 zq U1 - i c2 xq U2,
+// zq = z+ and xq = x- for b = 0, z- and x+ otherwise.  z- = conj(z+) and x- = conj(x+), so the
+// choice is a sign on the imaginary parts, folded into the coefficients together with the
+// source negations (both exact).
+template <typename T>
+__device__ __forceinline__ cx<T> rec_step(const RecT<T>& R, int b, const Geo<T>& g, const cx<T>* Up)
 {
     const T sg = (b == 0) ? (T)1 : (T)-1;
     const T A1 = R.c1, A2 = R.c2, A1s = sg * A1, A2s = sg * A2;
@@ -280,27 +277,41 @@ __device__ __forceinline__ void rec_step(const RecT<T>& R, int b, const Geo<T>& 
     const cx<T> t1{A1 * g.zp.r, A1s * g.zp.i};  // c1 zq
     const cx<T> t2{A2s * g.xp.i, A2 * g.xp.r};  // (i c2) xq
     // u = t1 U1 - t2 U2 as one chain of fused multiply-adds per component
+    cx<T> u;
     u.r = fma(t1.r, U1.r, fma(-t1.i, U1.i, fma(-t2.r, U2.r, t2.i * U2.i)));
     u.i = fma(t1.r, U1.i, fma(t1.i, U1.r, fma(-t2.r, U2.i, -(t2.i * U2.r))));
-    if (GRAD) {
+    return u;
+}
+
+// Level-j entry (a, b) of the gradient phase from its four level-(j-1) derivative sources
+// (DerEntry): with P_v = d_v U'(source v) for the z+, z-, x+, x- derivatives,
+//   F(g, h) = P0 g + P1 conj(g) - i P2 h - i P3 conj(h)
+//           = (P0 + P1) Re g + i (P0 - P1) Im g - i (P2 + P3) Re h + (P2 - P3) Im h,
+// U = F(z+, x+) / j (Euler's relation) and dU/dr_k = F(dz+/dr_k, dx+/dr_k).  The phase needs
+// X_k = conj(V_k) = U dfcut rhat_k + fcut dU/dr_k = U s_k + F(g_k, h_k), g and h holding the
+// cutoff-scaled derivatives; U goes to the next level, X to the contraction.
+template <typename T>
+__device__ __forceinline__ void der_step(const T* __restrict__ dv, int code, T inv_j, const Geo<T>& g,
+                                         const cx<T>* Up, cx<T>& u, cx<T> x[3])
+{
+    cx<T> P[4];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            cx<T> dU1 = dUp[3 * i1 + k], dU2 = dUp[3 * i2 + k];
-            dU1.i *= cj1;
-            dU2.i *= cj2;
-            // du = (c1 dz) U1 + t1 dU1 - (i c2 dx) U2 - t2 dU2
-            const cx<T> a1{A1 * g.dzp[k].r, A1s * g.dzp[k].i};
-            const cx<T> a2{A2s * g.dxp[k].i, A2 * g.dxp[k].r};
-            T r = fma(a1.r, U1.r, -(a1.i * U1.i));
-            r = fma(t1.r, dU1.r, fma(-t1.i, dU1.i, r));
-            r = fma(-a2.r, U2.r, fma(a2.i, U2.i, r));
-            r = fma(-t2.r, dU2.r, fma(t2.i, dU2.i, r));
-            T im = fma(a1.r, U1.i, a1.i * U1.r);
-            im = fma(t1.r, dU1.i, fma(t1.i, dU1.r, im));
-            im = fma(-a2.r, U2.i, fma(-a2.i, U2.r, im));
-            im = fma(-t2.r, dU2.i, fma(-t2.i, dU2.r, im));
-            du[k] = cx<T>{r, im};
-        }
+    for (int v = 0; v < 4; ++v) {
+        const int c = (code >> (8 * v)) & 255;
+        const cx<T> S = Up[c >> 1];
+        const T d = dv[v];
+        P[v] = cx<T>{d * S.r, ((c & 1) ? -d : d) * S.i};
+    }
+    const cx<T> s1{P[0].r + P[1].r, P[0].i + P[1].i};
+    const cx<T> d1{P[1].i - P[0].i, P[0].r - P[1].r};
+    const cx<T> s2{P[2].i + P[3].i, -(P[2].r + P[3].r)};
+    const cx<T> d2{P[2].r - P[3].r, P[2].i - P[3].i};
+    u.r = inv_j * fma(s1.r, g.zp.r, fma(d1.r, g.zp.i, fma(s2.r, g.xp.r, d2.r * g.xp.i)));
+    u.i = inv_j * fma(s1.i, g.zp.r, fma(d1.i, g.zp.i, fma(s2.i, g.xp.r, d2.i * g.xp.i)));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        x[k].r = fma(u.r, g.s[k], fma(s1.r, g.g[k].r, fma(d1.r, g.g[k].i, fma(s2.r, g.h[k].r, d2.r * g.h[k].i))));
+        x[k].i = fma(u.i, g.s[k], fma(s1.i, g.g[k].r, fma(d1.i, g.g[k].i, fma(s2.i, g.h[k].r, d2.i * g.h[k].i))));
     }
 }
 
@@ -339,6 +350,7 @@ constexpr int level_slots(int j)
 
 template <typename T, int JJ>
 struct Smem {
+    static constexpr int NW = nw_of<T>();
     static constexpr int NU = mld::ucount(JJ);
     static constexpr int NL = mld::hcount(JJ);  // canonical half of the largest level
     static constexpr int WMAX = wsize_bound(JJ);
@@ -360,11 +372,11 @@ struct Smem {
             cx<T> c[NU];
         } b;
         struct {  // C: gradients
-            RecT<T> rec[HU];  // same offset as a.rec
+            T dd[HU][4];  // derivative-source factors (DerEntry::d)
             cx<T> ub[NW][2][NL];
-            cx<T> dub[NW][2][NL][3];
-            cx<T> vb[NW][NL][3];
+            cx<T> xb[NW][NL][3];  // X_k = conj(V_k) of the current level
             T outw[NW][mld::NB_MAX][3];
+            int dcode[HU];  // derivative-source codes (DerEntry::code)
             // contraction lane map of each level: r | slice << 6 | NS << 9 | e0 << 12 | e1 << 18 | bi << 24
             // (row, slice index, slice count, entry range, output component of the row); e0 = e1 for an idle lane
             unsigned lmap[JJ + 1][64];
@@ -377,7 +389,7 @@ static_assert(sizeof(Smem<float, 8>) <= 40 * 1024, "FP32 descriptor workgroup ab
 static_assert(sizeof(Smem<double, 8>) <= 80 * 1024, "FP64 descriptor workgroup above 80 KB of LDS");
 
 template <typename T, int JJ>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) bso4_kernel(DevTables tb, int natoms, const int* __restrict__ pair_off,
+__global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) bso4_kernel(DevTables tb, int natoms, const int* __restrict__ pair_off,
                                                   const int* __restrict__ pair_idx,
                                                   const T* __restrict__ geo,
                                                   const int* __restrict__ itype, const double* __restrict__ wfac,
@@ -385,6 +397,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                                                   T* __restrict__ energy, T* __restrict__ grad)
 {
     using S = Smem<T, JJ>;
+    constexpr int NW = S::NW;
+    constexpr int BS = 64 * NW;
     constexpr int NU = S::NU;
     constexpr int NL = S::NL;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -440,8 +454,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                     if (e < K) {
                         int b = e % (j + 1);
                         const RecT<T> R = sm.u.a.rec[mld::hoff(j) + e];
-                        cx<T> u;
-                        rec_step<T, false>(R, b, g, Up, nullptr, u, nullptr);
+                        const cx<T> u = rec_step<T>(R, b, g, Up);
                         Un[e] = u;
                         dacc[j][s2] = cadd(dacc[j][s2], cscale(fw, u));
                     }
@@ -548,7 +561,12 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
 
         // ---------------- C: gradients ----------------
         T selfacc[3] = {(T)0, (T)0, (T)0};  // -sum over this wavefront's neighbours, q = lane + 64 u
-        for (int e = tid; e < S::HU; e += BS) sm.u.g.rec[e] = stage_rec<T>(tb.rec[e]);
+        for (int e = tid; e < S::HU; e += BS) {
+            const mld::DerEntry& de = tb.der[e];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sm.u.g.dd[e][v] = (T)de.d[v];
+            sm.u.g.dcode[e] = de.code;
+        }
         for (int q = tid; q < (JJ + 1) * 64; q += BS) {
             const int j = q >> 6, ln = q & 63;
             const int M = tb.M[j], K = mld::hcount(j);
@@ -576,59 +594,40 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
             const T wn = (T)wfac[ch * nsp + tj];
             T* outw = &sm.u.g.outw[w][0][0];
             for (int q = lane; q < nb * 3; q += 64) outw[q] = (T)0;
-            cx<T>* vb = &sm.u.g.vb[w][0][0];
-            // level 0: U = 1, dU = 0, V = dfcut rhat.  The level loop below is unrolled (level
-            // sizes become constants); the FP32 kernel is held to 128 VGPRs, four wavefronts
-            // per SIMD, which its LDS footprint (four workgroups per CU) allows anyway
+            cx<T>* xb = &sm.u.g.xb[w][0][0];
+            // level 0: U = 1, dU = 0, X = dfcut rhat.  The level loop below is unrolled (level
+            // sizes become constants)
             if (lane == 0) {
                 sm.u.g.ub[w][0][0] = cx<T>{(T)1, (T)0};
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    sm.u.g.dub[w][0][0][k] = cx<T>{(T)0, (T)0};
-                    vb[k] = cx<T>{g.dfc * g.drr[k], (T)0};
-                }
+                for (int k = 0; k < 3; ++k) xb[k] = cx<T>{g.s[k], (T)0};
             }
 #pragma unroll
             for (int j = 0; j <= JJ; ++j) {
                 if (j > 0) {
                     wave_sync();
                     const cx<T>* Up = sm.u.g.ub[w][(j - 1) & 1];
-                    const cx<T>* dUp = &sm.u.g.dub[w][(j - 1) & 1][0][0];
                     cx<T>* Un = sm.u.g.ub[w][j & 1];
-                    cx<T>* dUn = &sm.u.g.dub[w][j & 1][0][0];
-                    cx<T> us[level_slots<JJ>(JJ)];
-                    cx<T> dus[level_slots<JJ>(JJ)][3];
 #pragma unroll
                     for (int s = 0; s < level_slots<JJ>(JJ); ++s) {
-                        int e = lane + 64 * s;
+                        const int e = lane + 64 * s;
                         if (e < mld::hcount(j)) {
-                            int b = e % (j + 1);
-                            const RecT<T> R = sm.u.g.rec[mld::hoff(j) + e];
-                            rec_step<T, true>(R, b, g, Up, dUp, us[s], dus[s]);
-                        }
-                    }
+                            cx<T> u, x[3];
+                            der_step<T>(sm.u.g.dd[mld::hoff(j) + e], sm.u.g.dcode[mld::hoff(j) + e], (T)1 / (T)j, g,
+                                        Up, u, x);
+                            Un[e] = u;
 #pragma unroll
-                    for (int s = 0; s < level_slots<JJ>(JJ); ++s) {
-                        int e = lane + 64 * s;
-                        if (e < mld::hcount(j)) {
-                            Un[e] = us[s];
-#pragma unroll
-                            for (int k = 0; k < 3; ++k) {
-                                dUn[3 * e + k] = dus[s][k];
-                                // vecall = conj(U) dfcut rhat + fcut conj(dU)   (angular_functions.f90:255-258)
-                                T a = g.dfc * g.drr[k];
-                                vb[3 * e + k] = cx<T>{us[s].r * a + g.fc * dus[s][k].r,
-                                                      -us[s].i * a - g.fc * dus[s][k].i};
-                            }
+                            for (int k = 0; k < 3; ++k) xb[3 * e + k] = x[k];
                         }
                     }
                 }
                 wave_sync();
                 const int M = tb.M[j];
                 const cx<T>* Wj = sm.W + tb.wofs[j];
-                // contraction of level j: the K_j entries are cut into NS slices;
-                // lane NS*r + s sums slice s of row r (two interleaved partial sums
-                // per direction), then the slices are added across the lane group.
+                // contraction of level j, Re(W V_k) = Re(W conj X_k) = W.r X.r + W.i X.i: the K_j
+                // entries are cut into NS slices; lane NS*r + s sums slice s of row r (two
+                // interleaved partial sums per direction), then the slices are added across the
+                // lane group.
                 const unsigned lm = sm.u.g.lmap[j][lane];
                 const int r = lm & 63, sidx = (lm >> 6) & 7;
                 const int e0 = (lm >> 12) & 63, e1 = (lm >> 18) & 63;
@@ -637,21 +636,21 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(
                     int e = e0;
                     for (; e + 1 < e1; e += 2) {
                         cx<T> wv = Wj[e * M + r], wu = Wj[(e + 1) * M + r];
-                        cx<T> v0 = vb[3 * e], v1 = vb[3 * e + 1], v2 = vb[3 * e + 2];
-                        cx<T> u0 = vb[3 * e + 3], u1 = vb[3 * e + 4], u2 = vb[3 * e + 5];
-                        a0 = fma(wv.r, v0.r, fma(-wv.i, v0.i, a0));
-                        a1 = fma(wv.r, v1.r, fma(-wv.i, v1.i, a1));
-                        a2 = fma(wv.r, v2.r, fma(-wv.i, v2.i, a2));
-                        b0 = fma(wu.r, u0.r, fma(-wu.i, u0.i, b0));
-                        b1 = fma(wu.r, u1.r, fma(-wu.i, u1.i, b1));
-                        b2 = fma(wu.r, u2.r, fma(-wu.i, u2.i, b2));
+                        cx<T> v0 = xb[3 * e], v1 = xb[3 * e + 1], v2 = xb[3 * e + 2];
+                        cx<T> u0 = xb[3 * e + 3], u1 = xb[3 * e + 4], u2 = xb[3 * e + 5];
+                        a0 = fma(wv.r, v0.r, fma(wv.i, v0.i, a0));
+                        a1 = fma(wv.r, v1.r, fma(wv.i, v1.i, a1));
+                        a2 = fma(wv.r, v2.r, fma(wv.i, v2.i, a2));
+                        b0 = fma(wu.r, u0.r, fma(wu.i, u0.i, b0));
+                        b1 = fma(wu.r, u1.r, fma(wu.i, u1.i, b1));
+                        b2 = fma(wu.r, u2.r, fma(wu.i, u2.i, b2));
                     }
                     if (e < e1) {
                         cx<T> wv = Wj[e * M + r];
-                        cx<T> v0 = vb[3 * e], v1 = vb[3 * e + 1], v2 = vb[3 * e + 2];
-                        a0 = fma(wv.r, v0.r, fma(-wv.i, v0.i, a0));
-                        a1 = fma(wv.r, v1.r, fma(-wv.i, v1.i, a1));
-                        a2 = fma(wv.r, v2.r, fma(-wv.i, v2.i, a2));
+                        cx<T> v0 = xb[3 * e], v1 = xb[3 * e + 1], v2 = xb[3 * e + 2];
+                        a0 = fma(wv.r, v0.r, fma(wv.i, v0.i, a0));
+                        a1 = fma(wv.r, v1.r, fma(wv.i, v1.i, a1));
+                        a2 = fma(wv.r, v2.r, fma(wv.i, v2.i, a2));
                     }
                     a0 += b0;
                     a1 += b1;
@@ -729,7 +728,7 @@ int launch_t(mld_ctx* ctx, int want_grad, void* energy, void* grad)
     hipLaunchKernelGGL(pair_geo_kernel<T>, dim3((ctx->natoms + 3) / 4), dim3(256), 0, ctx->stream, ctx->natoms,
                        ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_itype, ctx->d_rin, ctx->nsp,
                        ctx->p.r_cut_width_in, ctx->p.r_cut, ctx->r0, geo);
-    hipLaunchKernelGGL(k, dim3(ctx->natoms), dim3(BS), sh, ctx->stream, ctx->dt, ctx->natoms, ctx->d_pair_off,
+    hipLaunchKernelGGL(k, dim3(ctx->natoms), dim3(64 * nw_of<T>()), sh, ctx->stream, ctx->dt, ctx->natoms, ctx->d_pair_off,
                        ctx->d_pair_idx, (const T*)geo, ctx->d_itype, ctx->d_wfac, ctx->nsp, ctx->nch, want_grad,
                        (T*)energy, (T*)grad);
     MLD_HIP(ctx, hipGetLastError());

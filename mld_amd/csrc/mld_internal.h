@@ -47,6 +47,20 @@ struct RecEntry {
     int s1, s2;
 };
 
+// Derivative sources of one canonical U^j entry (a, b).  U^j is a homogeneous polynomial of degree j
+// in z+, z-, x+, x- (taken as independent variables), and each partial derivative is a single
+// level-(j-1) entry:
+//   dU/dz+ = sqrt((j-a)(j-b)) U'(a,b)        dU/dz- = sqrt(ab) U'(a-1,b-1)
+//   dU/dx+ = -i sqrt((j-a)b) U'(a,b-1)       dU/dx- = -i sqrt(a(j-b)) U'(a-1,b)
+// so dU/dr follows from U' without a recursion of its own, and Euler's relation
+// j U = z+ dU/dz+ + z- dU/dz- + x+ dU/dx+ + x- dU/dx- gives U from the same four sources.
+// d[v] carries the square root with the source's negation folded in (0 for a source outside
+// level j-1); code holds 8 bits per source v: (canonical index << 1) | conjugate.
+struct DerEntry {
+    double d[4];
+    int code;
+};
+
 // One Z contribution to a merged W row (pack_cmm_into_ZcmmABC,
 // compute_bispectrum_so4.F90:1014-1143): Z(J1,J2 -> J)(m1,m2) =
 //   sum_{ma,mb} CG1[ia][i1] CG1[ib][i2] c_J1[ma][mb] c_J2[m1-ma][m2-mb]
@@ -66,6 +80,7 @@ struct WRow {
 struct DevTables {
     int jj, nb, nu;
     const RecEntry* rec;   // [hoff(jj + 1)] canonical entries, level j at hoff(j)
+    const DerEntry* der;   // [hoff(jj + 1)] derivative sources, same indexing
     const WRow* rows;      // [sum_j M_j], level j rows start at rowbase[j]
     const double* cg1;     // CG1 tables
     int rowbase[JJ_MAX + 2];
@@ -87,6 +102,7 @@ struct HostTables {
     int jj = 0, nb = 0, nu = 0;
     std::vector<int> l, l1, l2;
     std::vector<RecEntry> rec;
+    std::vector<DerEntry> der;
     std::vector<WRow> rows;
     std::vector<double> cg1;
     std::vector<int> tasks, tasks_all;
@@ -203,6 +219,7 @@ struct mld_ctx {
     mld::DevTables dt{};
     // device copies
     void* d_rec = nullptr;
+    void* d_der = nullptr;
     void* d_rows = nullptr;
     void* d_cg1 = nullptr;
     void* d_tasks = nullptr;

@@ -91,6 +91,25 @@ void build_tables(int jj, int diag, HostTables& t)
                 r.s2 = rec_code(r.s2, j - 1);
                 t.rec[hoff(j) + a * (j + 1) + b] = r;
             }
+    // derivative sources (the four neighbours of (a, b) in level j-1; see DerEntry)
+    t.der.assign(hoff(jj + 1), DerEntry{{0.0, 0.0, 0.0, 0.0}, 0});
+    for (int j = 1; j <= jj; ++j)
+        for (int a = 0; a <= j; ++a)
+            for (int b = 0; b <= j; ++b) {
+                if (a * (j + 1) + b >= hcount(j)) continue;
+                DerEntry d{{0.0, 0.0, 0.0, 0.0}, 0};
+                const int src[4][2] = {{a, b}, {a - 1, b - 1}, {a, b - 1}, {a - 1, b}};
+                const double fac[4] = {std::sqrt((double)((j - a) * (j - b))), std::sqrt((double)(a * b)),
+                                       std::sqrt((double)((j - a) * b)), std::sqrt((double)(a * (j - b)))};
+                for (int v = 0; v < 4; ++v) {
+                    const int sa = src[v][0], sb = src[v][1];
+                    if (sa < 0 || sb < 0 || sa >= j || sb >= j || fac[v] == 0.0) continue;
+                    const int c = rec_code(sa * j + sb, j - 1);
+                    d.d[v] = (c & 1) ? -fac[v] : fac[v];
+                    d.code |= (((c >> 2) << 1) | ((c >> 1) & 1)) << (8 * v);
+                }
+                t.der[hoff(j) + a * (j + 1) + b] = d;
+            }
     // CG1 tables keyed by (J1, J2, J)
     std::map<std::tuple<int, int, int>, int> cgmap;
     auto cgtab = [&](int J1, int J2, int J) {
