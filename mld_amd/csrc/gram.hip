@@ -239,6 +239,135 @@ __global__ void __launch_bounds__(RF_THREADS) rows_force_kernel(int order, int D
     if (tid < 3) y[row0 + tid] = F[3 * (size_t)k + tid] * swf;
 }
 
+// Quadratic force rows for descriptors of at most 64 components on the FP64 matrix cores.  The
+// quadratic block of row (k, x) is S_x(p, q) = sum_s g_s,x(p) b_s(q) + g_s,x(q) b_s(p), i.e.
+// S_x = G_x^T B + B^T G_x over the slots s of atom k (G_x: slots x D gradients, B: slots x D
+// centre descriptors), symmetric, so only the 16 x 16 blocks (ti <= tj) of the upper triangle
+// are formed: two MFMA 16x16x4 products per block and four slots.  One 256-thread workgroup
+// per atom; the 3 x nt (nt + 1) / 2 blocks (30 at D = 55) are dealt round-robin to the four
+// wavefronts.  Slots are staged QK at a time as FP64 (zero past the last slot and past D, so
+// the padded products vanish); the three rows are staged in LDS and written as one stream.
+// The columns, slot order and signs are those of rows_force_kernel.
+constexpr int QK = 16;
+constexpr int QBLK = 8;  // blocks per wavefront: ceil(3 * 10 / 4) at nt = 4
+__host__ __device__ inline size_t rows_force_q64_smem(int ldA)
+{
+    return sizeof(double) * ((size_t)QK * 4 * 64 + (size_t)ldA);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) rows_force_q64_kernel(int D, int ldA, const int* __restrict__ atom_struct,
+                                                             const int* __restrict__ atom_off, const int* __restrict__ pair_off,
+                                                             const int* __restrict__ rev_off, const int* __restrict__ rev_pair,
+                                                             const int* __restrict__ pair_src, const T* __restrict__ energy,
+                                                             const T* __restrict__ grad, const double* __restrict__ F,
+                                                             double swf, double* __restrict__ A, double* __restrict__ y)
+{
+    extern __shared__ __align__(16) double qsm[];
+    double* sg = qsm;                 // [QK][3][64]
+    double* sb = sg + QK * 3 * 64;    // [QK][64]
+    double* out = sb + QK * 64;       // [ldA]
+    const int k = blockIdx.x;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int s = atom_struct[k];
+    const size_t row0 = (size_t)s + 3 * (size_t)atom_off[s] + 1 + 3 * (size_t)(k - atom_off[s]);
+    const int r0 = rev_off[k], r1 = rev_off[k + 1];
+    const int nslot = 1 + (r1 - r0);
+    const int nt = (D + 15) >> 4;
+    const int ntri = nt * (nt + 1) / 2;
+    const int nblk = 3 * ntri;
+    // blocks of this wavefront: b = w + 4 u -> (x, ti, tj)
+    int bx[QBLK], bi[QBLK], bj[QBLK];
+#pragma unroll
+    for (int u = 0; u < QBLK; ++u) {
+        const int b = w + 4 * u;
+        int x = 0, ti = 0, tj = 0;
+        if (b < nblk) {
+            x = b / ntri;
+            int t = b - x * ntri;
+            while (t >= nt - ti) { t -= nt - ti; ++ti; }
+            tj = ti + t;
+        }
+        bx[u] = x;
+        bi[u] = ti;
+        bj[u] = tj;
+    }
+    d4 acc[QBLK];
+#pragma unroll
+    for (int u = 0; u < QBLK; ++u) acc[u] = d4{0, 0, 0, 0};
+    double lin = 0.0;  // (x, d) = (tid / D, tid % D) for tid < 3 D
+    const int lx = tid / D, ld_ = tid - lx * D;
+    const int ki = lane >> 4, ij = lane & 15;
+    for (int c0 = 0; c0 < nslot; c0 += QK) {
+        const int nc = min(QK, nslot - c0);
+        __syncthreads();
+        for (int q = tid; q < QK * 64; q += 256) {
+            const int sl = q >> 6, d = q & 63;
+            const bool in = sl < nc && d < D;
+            double g0 = 0.0, g1 = 0.0, g2 = 0.0, bb = 0.0;
+            if (in) {
+                const int idx = c0 + sl;
+                size_t slot;
+                int centre;
+                if (idx == 0) {
+                    slot = (size_t)pair_off[k] + k;  // self slot of k
+                    centre = k;
+                } else {
+                    const int p = rev_pair[r0 + idx - 1];
+                    centre = pair_src[p];
+                    slot = (size_t)p + centre + 1;
+                }
+                g0 = (double)grad[(slot * 3 + 0) * D + d];
+                g1 = (double)grad[(slot * 3 + 1) * D + d];
+                g2 = (double)grad[(slot * 3 + 2) * D + d];
+                bb = (double)energy[(size_t)centre * D + d];
+            }
+            sg[(sl * 3 + 0) * 64 + d] = g0;
+            sg[(sl * 3 + 1) * 64 + d] = g1;
+            sg[(sl * 3 + 2) * 64 + d] = g2;
+            sb[sl * 64 + d] = bb;
+        }
+        __syncthreads();
+        if (tid < 3 * D)
+            for (int sl = 0; sl < nc; ++sl) lin += sg[(sl * 3 + lx) * 64 + ld_];
+        const int kend = (nc + 3) & ~3;
+        for (int k0 = 0; k0 < kend; k0 += 4) {
+            const int kr = k0 + ki;
+#pragma unroll
+            for (int u = 0; u < QBLK; ++u) {
+                if (w + 4 * u >= nblk) break;  // uniform across the wavefront
+                const double* gx = sg + (kr * 3 + bx[u]) * 64;
+                const double* br = sb + kr * 64;
+                const double a1 = gx[bi[u] * 16 + ij], b1 = br[bj[u] * 16 + ij];
+                const double a2 = br[bi[u] * 16 + ij], b2 = gx[bj[u] * 16 + ij];
+                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[u], 0, 0, 0);
+                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2, acc[u], 0, 0, 0);
+            }
+        }
+    }
+    const int ncol = 1 + D + D * (D + 1) / 2;
+    double* Ar = A + row0 * ldA;
+    for (int x = 0; x < 3; ++x) {
+        __syncthreads();
+        if (tid == 0) out[0] = 0.0;
+        for (int q = ncol + tid; q < ldA; q += 256) out[q] = 0.0;
+        if (tid < 3 * D && lx == x) out[1 + ld_] = -lin * swf;
+        // f64 16x16x4 output: register v of lane l holds S[l/16 + 4v][l%16] of its block
+#pragma unroll
+        for (int u = 0; u < QBLK; ++u) {
+            if (w + 4 * u >= nblk || bx[u] != x) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int p = bi[u] * 16 + ki + 4 * v, q = bj[u] * 16 + ij;
+                if (p < D && q < D && p <= q) out[1 + D + p * D - p * (p - 1) / 2 - p + q] = -acc[u][v] * swf;
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < ldA; q += 256) Ar[(size_t)x * ldA + q] = out[q];
+    }
+    if (tid < 3) y[row0 + tid] = F[3 * (size_t)k + tid] * swf;
+}
+
 // Rows of a kernel model (kernel_pack_energy_descriptor / kernel_pack_force_descriptor,
 // mld_energy.F90:122-144, mld_forces.F90:268-320): columns [N, B (D), k (nk)]; energy row
 // of structure s = [N_s, sum_a B_a, sum_a k_a]; force rows of atom k = [0, -sum_slots dB,
@@ -1126,6 +1255,19 @@ int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_de
             hipLaunchKernelGGL(rows_force_krr_kernel<double>, dim3(natoms), dim3(256), 0, st, D, 0, ldA, ctx->d_atom_struct,
                                ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src, (const double*)ctx->last_grad,
                                (const double*)nullptr, F_dev, swf, A, y);
+    } else if (natoms && order == 2 && D <= 64) {
+        const size_t qsm = rows_force_q64_smem(ldA);
+        static std::atomic<unsigned long long> attr_q32{0}, attr_q64{0};
+        if (int rc0 = set_lds_limit_once(ctx, (const void*)rows_force_q64_kernel<float>, 160 * 1024, attr_q32)) return rc0;
+        if (int rc0 = set_lds_limit_once(ctx, (const void*)rows_force_q64_kernel<double>, 160 * 1024, attr_q64)) return rc0;
+        if (f32)
+            hipLaunchKernelGGL(rows_force_q64_kernel<float>, dim3(natoms), dim3(256), qsm, st, D, ldA, ctx->d_atom_struct,
+                               ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src,
+                               (const float*)ctx->last_energy, (const float*)ctx->last_grad, F_dev, swf, A, y);
+        else
+            hipLaunchKernelGGL(rows_force_q64_kernel<double>, dim3(natoms), dim3(256), qsm, st, D, ldA, ctx->d_atom_struct,
+                               ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src,
+                               (const double*)ctx->last_energy, (const double*)ctx->last_grad, F_dev, swf, A, y);
     } else if (natoms) {
         const size_t fsm = rows_force_smem(D, ldA, f32 ? sizeof(float) : sizeof(double));
         // the largest layout (D = 192 linear, D = 62 quadratic) is < 100 KB
