@@ -61,6 +61,14 @@ def test_fit19_golden_weighted_3ch_fp64(fit19):
         eo, go, nl = O.bso4(pos, cell, it, 8, 4.7, nch=3, w1=w1, w3=w3)
         assert rel(compact(f, nn), go) < 1e-10, name
     c.close()
+    # the FP32 form of the same three-channel descriptors and gradients
+    c = M.Context(j_max=4.0, r_cut=4.7, nspecies=4, weight=w1, weight_3ch=w3, precision=1)
+    for name, pos, cell, it, B in structures(fit19):
+        e, f, nn, kind = c.bso4(pos, cell, it, want_grad=True)
+        assert rel(e.T, B) < 1e-5, name
+        eo, go, nl = O.bso4(pos, cell, it, 8, 4.7, nch=3, w1=w1, w3=w3)
+        assert rel(compact(f, nn), go) < 1e-5, name
+    c.close()
 
 
 @pytest.mark.parametrize("case", ["bcc128", "small16", "liquid"])
