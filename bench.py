@@ -427,11 +427,42 @@ def main_fit(args):
                                     frac=per_launch / (SYRK_F64_MS_PER_BATCH * 1e-3) / 1e12 / PEAK_FP64,
                                     traffic=TRAFFIC_SYRK_F64_FIT_BATCH,
                                     algorithmic_flops_per_launch=per_launch)
+    tune = r64.get("gram_tuning")
+    if quad and tune:
+        # live cross-check: the autotuner's HIP-event timing of the same launch shape on the library
+        # stream (SYRK plus its slice reduction, at the slice count the fit used)
+        out["roofline_syrk"]["live_ms_per_launch_with_reduce"] = tune["ms"]
+    if quad and args.vendor_gram:
+        out["vendor_gram"] = vendor_gram(min(M, 290246), ld)   # the first 500-structure batch's rows
     if quad and not args.no_serve:
         out["serve"] = serve_measurement(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_fit(structs, cfg)
     print(json.dumps(out), flush=True)
+
+
+def vendor_gram(M: int, ld: int, reps: int = 5):
+    """The vendor library's FP64 product A^T A (torch.matmul: hipBLASLt / rocBLAS) on a random
+    matrix of one batch's row shape, HIP events on torch's stream: the full product, 2 M ld^2 flop."""
+    import torch
+
+    g = torch.Generator(device="cuda").manual_seed(SEED)
+    A = torch.randn(M, ld, dtype=torch.float64, device="cuda", generator=g)
+    for _ in range(2):
+        C = A.T @ A
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = event_pair()
+        e0.record()
+        C = A.T @ A
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    del A, C
+    ms = float(np.median(ts))
+    return {"ms": ms, "tflops": 2.0 * M * ld * ld / (ms * 1e-3) / 1e12, "rows": M, "columns": ld,
+            "note": "torch A.T @ A, FP64, full product (the library SYRK computes both triangles)"}
 
 
 # ----------------------------------------------------------------------------- kernel ridge regression
@@ -695,6 +726,8 @@ def main():
     ap.add_argument("--no-spread", dest="spread", action="store_false",
                     help="fit workloads: skip the second FP64 fit that measures FP64's own coefficient spread")
     ap.add_argument("--no-serve", action="store_true", help="fit workloads: skip the serving measurement")
+    ap.add_argument("--vendor-gram", action="store_true",
+                    help="fit-quadratic: also time the vendor library's FP64 A^T A on one batch's shape")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device 3 times (reference kernel for HBM counters)")
     args = ap.parse_args()
