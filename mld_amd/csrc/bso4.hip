@@ -365,7 +365,7 @@ struct Smem {
         struct {  // A: density
             RecT<T> rec[HU];
             cx<T> cpart[NW][NU];
-            cx<T> ub[NW][2][NL];
+            cx<T> ub[NW][2][2][NL];  // per wavefront, level parity, half (neighbour) of the wavefront
         } a;
         struct {  // B: Z -> W; c is written at the end of A, past the end of a
             T cg[NCG];
@@ -424,9 +424,11 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
 
     for (int ch = 0; ch < nch; ++ch) {
         // ---------------- A: density ----------------
-        // each lane owns fixed entries of every level and accumulates them in registers over its
-        // wavefront's neighbours; the wavefront's sums go to its cpart row at the end
-        constexpr int LS = level_slots<JJ>(JJ);
+        // a wavefront takes its neighbours two at a time, 32 lanes each; each lane owns fixed
+        // entries of every level (e = t + 32 s) and accumulates them in registers over its
+        // neighbours; the two halves are added and the wavefront's sums go to its cpart row
+        constexpr int LS = (mld::hcount(JJ) + 31) / 32;
+        const int hs = lane >> 5, t32 = lane & 31;
         for (int e = tid; e < S::HU; e += BS) sm.u.a.rec[e] = stage_rec<T>(tb.rec[e]);
         cx<T> dacc[JJ + 1][LS];
 #pragma unroll
@@ -434,23 +436,28 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
 #pragma unroll
             for (int s2 = 0; s2 < LS; ++s2) dacc[j][s2] = cx<T>{(T)0, (T)0};
         __syncthreads();
-        for (int n = w; n < nn; n += NW) {
-            const int tj = itype[pair_idx[p0 + n]];
-            const Geo<T> g = load_geo<T>(geo + (size_t)(p0 + n) * GEO_STRIDE, false);
-            T fw = g.fc * (T)wfac[ch * nsp + tj];
-            if (lane == 0) {
-                sm.u.a.ub[w][0][0] = cx<T>{(T)1, (T)0};
+        for (int n0 = 2 * w; n0 < nn; n0 += 2 * NW) {
+            const int n = n0 + hs;
+            Geo<T> g{};
+            T fw = (T)0;
+            if (n < nn) {
+                const int tj = itype[pair_idx[p0 + n]];
+                g = load_geo<T>(geo + (size_t)(p0 + n) * GEO_STRIDE, false);
+                fw = g.fc * (T)wfac[ch * nsp + tj];
+            }
+            if (t32 == 0) {
+                sm.u.a.ub[w][0][hs][0] = cx<T>{(T)1, (T)0};
                 dacc[0][0].r += fw;
             }
 #pragma unroll
             for (int j = 1; j <= JJ; ++j) {
                 wave_sync();
-                const cx<T>* Up = sm.u.a.ub[w][(j - 1) & 1];
-                cx<T>* Un = sm.u.a.ub[w][j & 1];
+                const cx<T>* Up = sm.u.a.ub[w][(j - 1) & 1][hs];
+                cx<T>* Un = sm.u.a.ub[w][j & 1][hs];
                 const int K = mld::hcount(j);
 #pragma unroll
-                for (int s2 = 0; s2 < LS; ++s2) {
-                    int e = lane + 64 * s2;
+                for (int s2 = 0; s2 < (mld::hcount(j) + 31) / 32; ++s2) {
+                    int e = t32 + 32 * s2;
                     if (e < K) {
                         int b = e % (j + 1);
                         const RecT<T> R = sm.u.a.rec[mld::hoff(j) + e];
@@ -467,9 +474,12 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
 #pragma unroll
             for (int j = 0; j <= JJ; ++j)
 #pragma unroll
-                for (int s2 = 0; s2 < LS; ++s2) {
-                    const int e = lane + 64 * s2;
-                    if (e < mld::hcount(j)) cp[mld::uoff(j) + e] = dacc[j][s2];
+                for (int s2 = 0; s2 < (mld::hcount(j) + 31) / 32; ++s2) {
+                    cx<T> v = dacc[j][s2];
+                    v.r += __shfl_xor(v.r, 32, 64);
+                    v.i += __shfl_xor(v.i, 32, 64);
+                    const int e = t32 + 32 * s2;
+                    if (hs == 0 && e < mld::hcount(j)) cp[mld::uoff(j) + e] = v;
                 }
         }
         __syncthreads();
