@@ -351,6 +351,9 @@ constexpr int level_slots(int j)
 template <typename T, int JJ>
 struct Smem {
     static constexpr int NW = nw_of<T>();
+    // neighbours a wavefront takes at once in the gradient phase: FP64 two (32 lanes each for the
+    // level recursion), FP32 one (its LDS stays under 40 KB, four workgroups per CU)
+    static constexpr int CP = sizeof(T) == 8 ? 2 : 1;
     static constexpr int NU = mld::ucount(JJ);
     static constexpr int NL = mld::hcount(JJ);  // canonical half of the largest level
     static constexpr int WMAX = wsize_bound(JJ);
@@ -373,9 +376,9 @@ struct Smem {
         } b;
         struct {  // C: gradients
             T dd[HU][4];  // derivative-source factors (DerEntry::d)
-            cx<T> ub[NW][2][NL];
-            cx<T> xb[NW][NL][3];  // X_k = conj(V_k) of the current level
-            T outw[NW][mld::NB_MAX][3];
+            cx<T> ub[NW][2][CP][NL];     // per wavefront, level parity, neighbour of the group
+            cx<T> xb[NW][CP][NL][3];     // X_k = conj(V_k) of the current level, per neighbour of the group
+            T outw[NW][CP][mld::NB_MAX][3];
             int dcode[HU];  // derivative-source codes (DerEntry::code)
             // contraction lane map of each level: r | slice << 6 | NS << 9 | e0 << 12 | e1 << 18 | bi << 24
             // (row, slice index, slice count, entry range, output component of the row); e0 = e1 for an idle lane
@@ -598,42 +601,49 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
             sm.u.g.lmap[j][ln] = v;
         }
         __syncthreads();
-        for (int n = w; n < nn; n += NW) {
-            const int tj = itype[pair_idx[p0 + n]];
-            const Geo<T> g = load_geo<T>(geo + (size_t)(p0 + n) * GEO_STRIDE, true);
-            const T wn = (T)wfac[ch * nsp + tj];
-            T* outw = &sm.u.g.outw[w][0][0];
-            for (int q = lane; q < nb * 3; q += 64) outw[q] = (T)0;
-            cx<T>* xb = &sm.u.g.xb[w][0][0];
+        // a wavefront takes its neighbours CP at a time: the level recursion runs on 64 / CP lanes per
+        // neighbour, the contraction of each level on the whole wavefront, one neighbour after the other
+        constexpr int CP = S::CP, LPN = 64 / CP;
+        const int hc = lane / LPN, tc = lane % LPN;
+        for (int n0 = CP * w; n0 < nn; n0 += CP * NW) {
+            const int n = n0 + hc;
+            const int npair = min(CP, nn - n0);  // neighbours of the group that exist (uniform)
+            Geo<T> g{};
+            if (n < nn) g = load_geo<T>(geo + (size_t)(p0 + n) * GEO_STRIDE, true);
+            for (int q = lane; q < CP * mld::NB_MAX * 3; q += 64) (&sm.u.g.outw[w][0][0][0])[q] = (T)0;
+            cx<T>* xbh = &sm.u.g.xb[w][hc][0][0];
             // level 0: U = 1, dU = 0, X = dfcut rhat.  The level loop below is unrolled (level
             // sizes become constants)
-            if (lane == 0) {
-                sm.u.g.ub[w][0][0] = cx<T>{(T)1, (T)0};
+            if (tc == 0) {
+                sm.u.g.ub[w][0][hc][0] = cx<T>{(T)1, (T)0};
 #pragma unroll
-                for (int k = 0; k < 3; ++k) xb[k] = cx<T>{g.s[k], (T)0};
+                for (int k = 0; k < 3; ++k) xbh[k] = cx<T>{g.s[k], (T)0};
             }
 #pragma unroll
             for (int j = 0; j <= JJ; ++j) {
                 if (j > 0) {
                     wave_sync();
-                    const cx<T>* Up = sm.u.g.ub[w][(j - 1) & 1];
-                    cx<T>* Un = sm.u.g.ub[w][j & 1];
+                    const cx<T>* Up = sm.u.g.ub[w][(j - 1) & 1][hc];
+                    cx<T>* Un = sm.u.g.ub[w][j & 1][hc];
 #pragma unroll
-                    for (int s = 0; s < level_slots<JJ>(JJ); ++s) {
-                        const int e = lane + 64 * s;
+                    for (int s = 0; s < (mld::hcount(j) + LPN - 1) / LPN; ++s) {
+                        const int e = tc + LPN * s;
                         if (e < mld::hcount(j)) {
                             cx<T> u, x[3];
                             der_step<T>(sm.u.g.dd[mld::hoff(j) + e], sm.u.g.dcode[mld::hoff(j) + e], (T)1 / (T)j, g,
                                         Up, u, x);
                             Un[e] = u;
 #pragma unroll
-                            for (int k = 0; k < 3; ++k) xb[3 * e + k] = x[k];
+                            for (int k = 0; k < 3; ++k) xbh[3 * e + k] = x[k];
                         }
                     }
                 }
                 wave_sync();
                 const int M = tb.M[j];
                 const cx<T>* Wj = sm.W + tb.wofs[j];
+                for (int h = 0; h < npair; ++h) {
+                const cx<T>* xb = &sm.u.g.xb[w][h][0][0];
+                T* outw = &sm.u.g.outw[w][h][0][0];
                 // contraction of level j, Re(W V_k) = Re(W conj X_k) = W.r X.r + W.i X.i: the K_j
                 // entries are cut into NS slices; lane NS*r + s sums slice s of row r (two
                 // interleaved partial sums per direction), then the slices are added across the
@@ -684,26 +694,31 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
                     outw[bi * 3 + 1] += a1;
                     outw[bi * 3 + 2] += a2;
                 }
+                }
                 wave_sync();
             }
-            // flush neighbour n: grad slot (base + 1 + n), layout [slot][x][D]
-            T* gout = grad + (base + 1 + n) * 3 * (size_t)D + ch * nb;
-            // lane + 64 u walks outw in its own (component, direction) order: q = 3 bi + x
+            // flush the group: grad slot (base + 1 + n), layout [slot][x][D]; lane + 64 u walks outw in
+            // its own (component, direction) order: q = 3 bi + x
+            for (int h = 0; h < npair; ++h) {
+                const T wn = (T)wfac[ch * nsp + itype[pair_idx[p0 + n0 + h]]];
+                T* gout = grad + (base + 1 + n0 + h) * 3 * (size_t)D + ch * nb;
+                const T* ow = &sm.u.g.outw[w][h][0][0];
 #pragma unroll
-            for (int u = 0; u < 3; ++u) {
-                const int q = lane + 64 * u;
-                if (q < nb * 3) {
-                    const int bi = q / 3, x = q - 3 * bi;
-                    const T v = wn * outw[q];
-                    gout[(size_t)x * D + bi] = v;
-                    selfacc[u] -= v;
+                for (int u = 0; u < 3; ++u) {
+                    const int q = lane + 64 * u;
+                    if (q < nb * 3) {
+                        const int bi = q / 3, x = q - 3 * bi;
+                        const T v = wn * ow[q];
+                        gout[(size_t)x * D + bi] = v;
+                        selfacc[u] -= v;
+                    }
                 }
             }
             wave_sync();
         }
         // wavefront partials of the self slot through this wavefront's own outw block
         {
-            T* outw = &sm.u.g.outw[w][0][0];
+            T* outw = &sm.u.g.outw[w][0][0][0];
 #pragma unroll
             for (int u = 0; u < 3; ++u)
                 if (lane + 64 * u < nb * 3) outw[lane + 64 * u] = selfacc[u];
@@ -714,9 +729,9 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
         T* gself = grad + base * 3 * (size_t)D + ch * nb;
         for (int q = tid; q < nb * 3; q += BS) {
             const int bi = q / 3, x = q - 3 * bi;
-            T s = sm.u.g.outw[0][0][q];
+            T s = (&sm.u.g.outw[0][0][0][0])[q];
 #pragma unroll
-            for (int qq = 1; qq < NW; ++qq) s += sm.u.g.outw[qq][0][q];
+            for (int qq = 1; qq < NW; ++qq) s += (&sm.u.g.outw[qq][0][0][0])[q];
             gself[(size_t)x * D + bi] = s;
         }
         __syncthreads();
