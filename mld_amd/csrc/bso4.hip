@@ -641,18 +641,54 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
                 wave_sync();
                 const int M = tb.M[j];
                 const cx<T>* Wj = sm.W + tb.wofs[j];
-                for (int h = 0; h < npair; ++h) {
-                const cx<T>* xb = &sm.u.g.xb[w][h][0][0];
-                T* outw = &sm.u.g.outw[w][h][0][0];
                 // contraction of level j, Re(W V_k) = Re(W conj X_k) = W.r X.r + W.i X.i: the K_j
-                // entries are cut into NS slices; lane NS*r + s sums slice s of row r (two
-                // interleaved partial sums per direction), then the slices are added across the
-                // lane group.
+                // entries are cut into NS slices; lane NS*r + s sums slice s of row r, then the slices
+                // are added across the lane group.  Both neighbours of a pair are contracted in one
+                // pass (each W value loaded once for both); a single neighbour takes two interleaved
+                // partial sums per direction instead.
                 const unsigned lm = sm.u.g.lmap[j][lane];
                 const int r = lm & 63, sidx = (lm >> 6) & 7;
                 const int e0 = (lm >> 12) & 63, e1 = (lm >> 18) & 63;
-                T a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
-                {
+                const int NSw = slice_count(M);  // uniform across the wavefront
+                auto finish = [&](T* outw, T a0, T a1, T a2) {
+                    // add the slices of row r into its slice-0 lane: (s0 + s1) + (s2 + s3)
+                    if (NSw >= 2) {
+                        a0 = quad_add<0xB1>(a0);
+                        a1 = quad_add<0xB1>(a1);
+                        a2 = quad_add<0xB1>(a2);
+                    }
+                    if (NSw == 4) {
+                        a0 = quad_add<0x4E>(a0);
+                        a1 = quad_add<0x4E>(a1);
+                        a2 = quad_add<0x4E>(a2);
+                    }
+                    if (M > 0 && sidx == 0 && r < M) {
+                        const int bi = lm >> 24;
+                        outw[bi * 3 + 0] += a0;
+                        outw[bi * 3 + 1] += a1;
+                        outw[bi * 3 + 2] += a2;
+                    }
+                };
+                if (npair == 2) {
+                    const cx<T>* xa = &sm.u.g.xb[w][0][0][0];
+                    const cx<T>* xc = &sm.u.g.xb[w][CP - 1][0][0];
+                    T a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+                    for (int e = e0; e < e1; ++e) {
+                        const cx<T> wv = Wj[e * M + r];
+                        const cx<T> v0 = xa[3 * e], v1 = xa[3 * e + 1], v2 = xa[3 * e + 2];
+                        const cx<T> u0 = xc[3 * e], u1 = xc[3 * e + 1], u2 = xc[3 * e + 2];
+                        a0 = fma(wv.r, v0.r, fma(wv.i, v0.i, a0));
+                        a1 = fma(wv.r, v1.r, fma(wv.i, v1.i, a1));
+                        a2 = fma(wv.r, v2.r, fma(wv.i, v2.i, a2));
+                        b0 = fma(wv.r, u0.r, fma(wv.i, u0.i, b0));
+                        b1 = fma(wv.r, u1.r, fma(wv.i, u1.i, b1));
+                        b2 = fma(wv.r, u2.r, fma(wv.i, u2.i, b2));
+                    }
+                    finish(&sm.u.g.outw[w][0][0][0], a0, a1, a2);
+                    finish(&sm.u.g.outw[w][CP - 1][0][0], b0, b1, b2);
+                } else {
+                    const cx<T>* xb = &sm.u.g.xb[w][0][0][0];
+                    T a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
                     int e = e0;
                     for (; e + 1 < e1; e += 2) {
                         cx<T> wv = Wj[e * M + r], wu = Wj[(e + 1) * M + r];
@@ -672,28 +708,7 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
                         a1 = fma(wv.r, v1.r, fma(wv.i, v1.i, a1));
                         a2 = fma(wv.r, v2.r, fma(wv.i, v2.i, a2));
                     }
-                    a0 += b0;
-                    a1 += b1;
-                    a2 += b2;
-                }
-                // add the slices of row r into its slice-0 lane: (s0 + s1) + (s2 + s3)
-                const int NSw = slice_count(M);  // uniform across the wavefront
-                if (NSw >= 2) {
-                    a0 = quad_add<0xB1>(a0);
-                    a1 = quad_add<0xB1>(a1);
-                    a2 = quad_add<0xB1>(a2);
-                }
-                if (NSw == 4) {
-                    a0 = quad_add<0x4E>(a0);
-                    a1 = quad_add<0x4E>(a1);
-                    a2 = quad_add<0x4E>(a2);
-                }
-                if (M > 0 && sidx == 0 && r < M) {
-                    const int bi = lm >> 24;
-                    outw[bi * 3 + 0] += a0;
-                    outw[bi * 3 + 1] += a1;
-                    outw[bi * 3 + 2] += a2;
-                }
+                    finish(&sm.u.g.outw[w][0][0][0], a0 + b0, a1 + b1, a2 + b2);
                 }
                 wave_sync();
             }
