@@ -249,14 +249,15 @@ __global__ void __launch_bounds__(RF_THREADS) rows_force_kernel(int order, int D
 // the padded products vanish); the three rows are staged in LDS and written as one stream.
 // The columns, slot order and signs are those of rows_force_kernel.
 constexpr int QK = 16;
-constexpr int QBLK = 8;  // blocks per wavefront: ceil(3 * 10 / 4) at nt = 4
+constexpr int QW = 8;    // wavefronts per workgroup
+constexpr int QBLK = 4;  // blocks per wavefront: ceil(3 * 10 / 8) at nt = 4
 __host__ __device__ inline size_t rows_force_q64_smem(int ldA)
 {
     return sizeof(double) * ((size_t)QK * 4 * 64 + (size_t)ldA);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) rows_force_q64_kernel(int D, int ldA, const int* __restrict__ atom_struct,
+__global__ void __launch_bounds__(64 * QW) rows_force_q64_kernel(int D, int ldA, const int* __restrict__ atom_struct,
                                                              const int* __restrict__ atom_off, const int* __restrict__ pair_off,
                                                              const int* __restrict__ rev_off, const int* __restrict__ rev_pair,
                                                              const int* __restrict__ pair_src, const T* __restrict__ energy,
@@ -280,7 +281,7 @@ __global__ void __launch_bounds__(256) rows_force_q64_kernel(int D, int ldA, con
     int bx[QBLK], bi[QBLK], bj[QBLK];
 #pragma unroll
     for (int u = 0; u < QBLK; ++u) {
-        const int b = w + 4 * u;
+        const int b = w + QW * u;
         int x = 0, ti = 0, tj = 0;
         if (b < nblk) {
             x = b / ntri;
@@ -301,7 +302,7 @@ __global__ void __launch_bounds__(256) rows_force_q64_kernel(int D, int ldA, con
     for (int c0 = 0; c0 < nslot; c0 += QK) {
         const int nc = min(QK, nslot - c0);
         __syncthreads();
-        for (int q = tid; q < QK * 64; q += 256) {
+        for (int q = tid; q < QK * 64; q += 64 * QW) {
             const int sl = q >> 6, d = q & 63;
             const bool in = sl < nc && d < D;
             double g0 = 0.0, g1 = 0.0, g2 = 0.0, bb = 0.0;
@@ -335,7 +336,7 @@ __global__ void __launch_bounds__(256) rows_force_q64_kernel(int D, int ldA, con
             const int kr = k0 + ki;
 #pragma unroll
             for (int u = 0; u < QBLK; ++u) {
-                if (w + 4 * u >= nblk) break;  // uniform across the wavefront
+                if (w + QW * u >= nblk) break;  // uniform across the wavefront
                 const double* gx = sg + (kr * 3 + bx[u]) * 64;
                 const double* br = sb + kr * 64;
                 const double a1 = gx[bi[u] * 16 + ij], b1 = br[bj[u] * 16 + ij];
@@ -350,12 +351,12 @@ __global__ void __launch_bounds__(256) rows_force_q64_kernel(int D, int ldA, con
     for (int x = 0; x < 3; ++x) {
         __syncthreads();
         if (tid == 0) out[0] = 0.0;
-        for (int q = ncol + tid; q < ldA; q += 256) out[q] = 0.0;
+        for (int q = ncol + tid; q < ldA; q += 64 * QW) out[q] = 0.0;
         if (tid < 3 * D && lx == x) out[1 + ld_] = -lin * swf;
         // f64 16x16x4 output: register v of lane l holds S[l/16 + 4v][l%16] of its block
 #pragma unroll
         for (int u = 0; u < QBLK; ++u) {
-            if (w + 4 * u >= nblk || bx[u] != x) continue;
+            if (w + QW * u >= nblk || bx[u] != x) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int p = bi[u] * 16 + ki + 4 * v, q = bj[u] * 16 + ij;
@@ -363,7 +364,7 @@ __global__ void __launch_bounds__(256) rows_force_q64_kernel(int D, int ldA, con
             }
         }
         __syncthreads();
-        for (int q = tid; q < ldA; q += 256) Ar[(size_t)x * ldA + q] = out[q];
+        for (int q = tid; q < ldA; q += 64 * QW) Ar[(size_t)x * ldA + q] = out[q];
     }
     if (tid < 3) y[row0 + tid] = F[3 * (size_t)k + tid] * swf;
 }
@@ -1261,11 +1262,11 @@ int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_de
         if (int rc0 = set_lds_limit_once(ctx, (const void*)rows_force_q64_kernel<float>, 160 * 1024, attr_q32)) return rc0;
         if (int rc0 = set_lds_limit_once(ctx, (const void*)rows_force_q64_kernel<double>, 160 * 1024, attr_q64)) return rc0;
         if (f32)
-            hipLaunchKernelGGL(rows_force_q64_kernel<float>, dim3(natoms), dim3(256), qsm, st, D, ldA, ctx->d_atom_struct,
+            hipLaunchKernelGGL(rows_force_q64_kernel<float>, dim3(natoms), dim3(64 * QW), qsm, st, D, ldA, ctx->d_atom_struct,
                                ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src,
                                (const float*)ctx->last_energy, (const float*)ctx->last_grad, F_dev, swf, A, y);
         else
-            hipLaunchKernelGGL(rows_force_q64_kernel<double>, dim3(natoms), dim3(256), qsm, st, D, ldA, ctx->d_atom_struct,
+            hipLaunchKernelGGL(rows_force_q64_kernel<double>, dim3(natoms), dim3(64 * QW), qsm, st, D, ldA, ctx->d_atom_struct,
                                ctx->d_atom_off, ctx->d_pair_off, rev_off, rev_pair, pair_src,
                                (const double*)ctx->last_energy, (const double*)ctx->last_grad, F_dev, swf, A, y);
     } else if (natoms) {
