@@ -96,6 +96,45 @@ def test_u_unitary_and_fd_gradient():
         assert np.abs((Up - Um) / (2 * h) - dU[:, k]).max() < 1e-8
 
 
+def test_u_derivative_sources_and_euler_relation():
+    """The gradient phase of the descriptor kernel rebuilds each level from four entries of the level
+    below: U_j(a, b) is a homogeneous polynomial of degree j in z+, z-, x+, x- whose partial derivatives
+    are single scaled U_{j-1} entries,
+      dU/dz+ = sqrt((j-a)(j-b)) U'(a,b), dU/dz- = sqrt(ab) U'(a-1,b-1),
+      dU/dx+ = -i sqrt((j-a)b) U'(a,b-1), dU/dx- = -i sqrt(a(j-b)) U'(a-1,b),
+    and j U_j = z+ dU/dz+ + z- dU/dz- + x+ dU/dx+ + x- dU/dx- (Euler).  Checked against the oracle's U and
+    dU/dr (spherical_4d), with z+-, x+- and their r-derivatives read off level 1 (U_1 = [[z+, -i x+],
+    [-i x-, z-]] in the recursion's convention)."""
+    jj = 8
+    r0 = 4.7 / (math.pi - 0.02)
+    rng = np.random.default_rng(11)
+    for _ in range(3):
+        x = rng.normal(size=3) * 1.5
+        r = float(np.linalg.norm(x))
+        U, dU, _ = O.spherical_4d(x, r, r0, 1.0, 0.0, jj)
+
+        def lev(A, j):
+            o = j * (j + 1) * (2 * j + 1) // 6
+            return A[o:o + (j + 1) ** 2].reshape((j + 1, j + 1) + A.shape[1:])
+
+        U1, dU1 = lev(U, 1), lev(dU, 1)
+        v = {"zp": U1[0, 0], "zm": U1[1, 1], "xp": 1j * U1[0, 1], "xm": 1j * U1[1, 0]}
+        dv = {"zp": dU1[0, 0], "zm": dU1[1, 1], "xp": 1j * dU1[0, 1], "xm": 1j * dU1[1, 0]}
+        assert abs(v["zm"] - np.conj(v["zp"])) < 1e-14 and abs(v["xm"] - np.conj(v["xp"])) < 1e-14
+        for j in range(2, jj + 1):
+            P, Uj, dUj = lev(U, j - 1), lev(U, j), lev(dU, j)
+            g = lambda a, b: P[a, b] if 0 <= a < j and 0 <= b < j else 0.0
+            for a in range(j + 1):
+                for b in range(j + 1):
+                    parts = {"zp": math.sqrt((j - a) * (j - b)) * g(a, b), "zm": math.sqrt(a * b) * g(a - 1, b - 1),
+                             "xp": -1j * math.sqrt((j - a) * b) * g(a, b - 1),
+                             "xm": -1j * math.sqrt(a * (j - b)) * g(a - 1, b)}
+                    euler = sum(parts[k] * v[k] for k in parts) / j
+                    assert abs(euler - Uj[a, b]) < 1e-13, (j, a, b)
+                    grad = sum(parts[k] * dv[k] for k in parts)
+                    assert np.abs(grad - dUj[a, b]).max() < 1e-12 * max(1.0, np.abs(dUj).max()), (j, a, b)
+
+
 def _slot_sum(g, nl, nat, target):
     off, nbr = nl[0], nl[1]
     out = np.zeros((nat,) + g.shape[1:])
