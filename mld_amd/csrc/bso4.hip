@@ -2,21 +2,23 @@
 //
 // One 256-thread workgroup (4 wavefronts) per atom.  Per channel:
 //   A. density c_j(a,b) = sum_nb fcut U_j(a,b) + delta_ab
-//      (compute_bispectrum_so4.F90:288-439).  A wavefront owns one neighbour at
-//      a time; lanes own the entries of a level, and the recursion of
-//      spherical_4d (angular_functions.f90:152-272) walks level by level
-//      through a wavefront-private LDS buffer.  c stays in registers per lane
-//      and is reduced over the 4 wavefronts once.
+//      (compute_bispectrum_so4.F90:288-439).  A wavefront takes two neighbours
+//      at a time, 32 lanes each; lanes own the entries of a level, and the
+//      recursion of spherical_4d (angular_functions.f90:152-272) walks level by
+//      level through wavefront-private LDS buffers.  c stays in registers per
+//      lane and is reduced over the two halves and the 4 wavefronts once.
 //   B. Z contraction (pack_cmm_into_ZcmmABC, :1014-1143) straight into the
 //      merged gradient operand W_j (one LDS block per level j, layout [e][row]),
 //      and B_bi = Re sum conj(c_l) Z_A (:505-517) by a wavefront reduction.
 //   C. gradients: per neighbour the levels are rebuilt from their derivative
 //      sources: each partial derivative of U_j in z+, z-, x+, x- is one scaled
 //      level-(j-1) entry, so dU_j/dr and (by Euler's relation) U_j come from four
-//      entries of U_{j-1} with no dU recursion or dU storage.  The level-j vector
+//      entries of U_{j-1} with no dU recursion or dU storage (FP64: two neighbours
+//      per wavefront on 32 lanes each).  The level-j vector
 //      X_j = conj(V_j) = U dfcut rhat + fcut dU/dr goes to LDS and
 //      dB_bi/dr_k = Re sum_{j,e} W_j[bi][e] conj(X_j[e])   (:505-652, sign flip of
-//      descriptors.F90:298-302 folded in), self slot = -sum_k (:692-695).
+//      descriptors.F90:298-302 folded in) over the whole wavefront (in FP64 for
+//      both neighbours of a pair in one pass), self slot = -sum_k (:692-695).
 // Precision T = double (plain FP64 form) or float (FP32 form; set-up geometry in FP64).
 #include <algorithm>
 
@@ -642,7 +644,8 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
         }
         __syncthreads();
         // a wavefront takes its neighbours CP at a time: the level recursion runs on 64 / CP lanes per
-        // neighbour, the contraction of each level on the whole wavefront, one neighbour after the other
+        // neighbour, the contraction of each level on the whole wavefront (both neighbours of a pair in
+        // one pass over W)
         constexpr int CP = S::CP, LPN = 64 / CP;
         const int hc = lane / LPN, tc = lane % LPN;
         for (int n0 = CP * w; n0 < nn; n0 += CP * NW) {
