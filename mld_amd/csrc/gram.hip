@@ -73,7 +73,8 @@ __global__ void rows_energy_kernel(int order, int D, int ldA, const int* __restr
     const size_t row = (size_t)s + 3 * (size_t)a0;
     double* Ar = A + row * ldA;
     const int ncol = 1 + D + (order == 2 ? tri_cols(D) : 0);
-    for (int c = threadIdx.x; c < ldA; c += blockDim.x) {
+    // gridDim.y blocks share a structure's row, interleaved over its columns
+    for (int c = threadIdx.x + blockDim.x * blockIdx.y; c < ldA; c += blockDim.x * gridDim.y) {
         double v = 0.0;
         if (c == 0) v = (double)(a1 - a0);
         else if (c <= D) {
@@ -86,7 +87,7 @@ __global__ void rows_energy_kernel(int order, int D, int ldA, const int* __restr
         }
         Ar[c] = v * swe;
     }
-    if (threadIdx.x == 0) y[row] = E[s] * swe;
+    if (threadIdx.x == 0 && blockIdx.y == 0) y[row] = E[s] * swe;
 }
 
 // One block per atom k: its three force rows (x = 0, 1, 2) are consecutive in A.
@@ -1213,10 +1214,10 @@ int launch_rows(mld_ctx* ctx, int order, const double* E_dev, const double* F_de
     bool f32 = ctx->p.precision == MLD_PREC_FP32;
     if (nstruct && order != 3) {
         if (f32)
-            hipLaunchKernelGGL(rows_energy_kernel<float>, dim3(nstruct), dim3(256), 0, st, order, D, ldA, ctx->d_atom_off,
+            hipLaunchKernelGGL(rows_energy_kernel<float>, dim3(nstruct, 4), dim3(256), 0, st, order, D, ldA, ctx->d_atom_off,
                                (const float*)ctx->last_energy, E_dev, swe, A, y);
         else
-            hipLaunchKernelGGL(rows_energy_kernel<double>, dim3(nstruct), dim3(256), 0, st, order, D, ldA, ctx->d_atom_off,
+            hipLaunchKernelGGL(rows_energy_kernel<double>, dim3(nstruct, 4), dim3(256), 0, st, order, D, ldA, ctx->d_atom_off,
                                (const double*)ctx->last_energy, E_dev, swe, A, y);
     }
     if (order == 3) {
