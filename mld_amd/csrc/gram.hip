@@ -922,9 +922,17 @@ __global__ void gemv_t_kernel(int M, int ldA, const double* __restrict__ A, cons
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     int r0 = blockIdx.y * rows_per_slice, r1 = min(M, r0 + rows_per_slice);
     if (c >= ldA) return;
-    double s = 0.0;
-    for (int r = r0; r < r1; ++r) s += A[(size_t)r * ldA + c] * y[r];
-    part[(size_t)blockIdx.y * ldA + c] = s;
+    // four rows per step on independent sums (four loads in flight per thread)
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int r = r0;
+    for (; r + 3 < r1; r += 4) {
+        s0 += A[(size_t)r * ldA + c] * y[r];
+        s1 += A[(size_t)(r + 1) * ldA + c] * y[r + 1];
+        s2 += A[(size_t)(r + 2) * ldA + c] * y[r + 2];
+        s3 += A[(size_t)(r + 3) * ldA + c] * y[r + 3];
+    }
+    for (; r < r1; ++r) s0 += A[(size_t)r * ldA + c] * y[r];
+    part[(size_t)blockIdx.y * ldA + c] = (s0 + s1) + (s2 + s3);
 }
 
 __global__ void gemv_reduce_kernel(int ldA, int nh, int nslice, const double* __restrict__ part, double* __restrict__ h)
