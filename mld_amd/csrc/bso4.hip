@@ -104,6 +104,24 @@ struct LevelTab {
     }
 };
 
+// bispectrum components of one channel, the larger of the two enumerations
+constexpr int nb_bound(int jj)
+{
+    int best = 0;
+    for (int diag = 0; diag < 2; ++diag) {
+        int n = 0;
+        for (int l1 = 0; l1 <= jj; ++l1)
+            for (int l2 = diag ? l1 : 0; l2 <= l1; ++l2)
+                for (int l = (l1 > l2 ? l1 - l2 : l2 - l1); l <= (jj < l1 + l2 ? jj : l1 + l2); ++l) {
+                    if ((l1 + l2 + l) & 1) continue;
+                    if (!diag && l < l1) continue;
+                    ++n;
+                }
+        if (n > best) best = n;
+    }
+    return best;
+}
+
 // exact size of the CG1 tables (distinct (J1, J2, J) triples, (J1+1)(J+1) each)
 constexpr int cg1_size(int jj)
 {
@@ -318,15 +336,15 @@ __device__ __forceinline__ cx<T> rec_step(const RecT<T>& R, int b, const Geo<T>&
 // X_k = conj(V_k) = U dfcut rhat_k + fcut dU/dr_k = U s_k + F(g_k, h_k), g and h holding the
 // cutoff-scaled derivatives; U goes to the next level, X to the contraction.
 template <typename T>
-__device__ __forceinline__ void der_step(const T* __restrict__ dv, int code, T inv_j, const Geo<T>& g,
-                                         const cx<T>* Up, cx<T>& u, cx<T> x[3])
+__device__ __forceinline__ void der_step(const T* __restrict__ dv, const double* __restrict__ dvd, int code, T inv_j,
+                                         const Geo<T>& g, const cx<T>* Up, cx<T>& u, cx<T> x[3])
 {
     cx<T> P[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int c = (code >> (8 * v)) & 255;
         const cx<T> S = Up[c >> 1];
-        const T d = dv[v];
+        const T d = dv ? dv[v] : (T)dvd[v];
         P[v] = cx<T>{d * S.r, ((c & 1) ? -d : d) * S.i};
     }
     const cx<T> s1{P[0].r + P[1].r, P[0].i + P[1].i};
@@ -378,9 +396,10 @@ constexpr int level_slots(int j)
 template <typename T, int JJ>
 struct Smem {
     static constexpr int NW = nw_of<T>();
-    // neighbours a wavefront takes at once in the gradient phase: FP64 two (32 lanes each for the
-    // level recursion), FP32 one (its LDS stays under 40 KB, four workgroups per CU)
-    static constexpr int CP = sizeof(T) == 8 ? 2 : 1;
+    // neighbours a wavefront takes at once in the gradient phase (32 lanes each for the level
+    // recursion)
+    static constexpr int CP = 2;
+    static constexpr int NBJ = nb_bound(JJ);
     static constexpr int NU = mld::ucount(JJ);
     static constexpr int NL = mld::hcount(JJ);  // canonical half of the largest level
     static constexpr int WMAX = wsize_bound(JJ);
@@ -390,6 +409,9 @@ struct Smem {
     // workgroups per CU) and the FP64 one under 80 KB (two)
     static constexpr int NCG = cg1_size(JJ);
     static constexpr int HU = mld::hoff(JJ + 1);
+    // the derivative-source factors sit in LDS in FP64; the FP32 kernel reads them from the table
+    // in global memory (L1-resident), which keeps its workgroup under 40 KB
+    static constexpr int HD = sizeof(T) == 8 ? HU : 1;
     cx<T> W[WMAX];
     union {
         struct {  // A: density
@@ -402,10 +424,10 @@ struct Smem {
             cx<T> c[NU];
         } b;
         struct {  // C: gradients
-            T dd[HU][4];  // derivative-source factors (DerEntry::d)
+            T dd[HD][4];  // derivative-source factors (DerEntry::d), FP64 kernel
             cx<T> ub[NW][2][CP][NL];     // per wavefront, level parity, neighbour of the group
             cx<T> xb[NW][CP][NL][3];     // X_k = conj(V_k) of the current level, per neighbour of the group
-            T outw[NW][CP][mld::NB_MAX][3];
+            T outw[NW][CP][NBJ][3];
             int dcode[HU];  // derivative-source codes (DerEntry::code)
             // contraction lane map of each level: r | slice << 6 | NS << 9 | e0 << 12 | e1 << 18 | bi << 24
             // (row, slice index, slice count, entry range, output component of the row); e0 = e1 for an idle lane
@@ -618,8 +640,8 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
         T selfacc[3] = {(T)0, (T)0, (T)0};  // -sum over this wavefront's neighbours, q = lane + 64 u
         for (int e = tid; e < S::HU; e += BS) {
             const mld::DerEntry& de = tb.der[e];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) sm.u.g.dd[e][v] = (T)de.d[v];
+            if constexpr (S::HD > 1)
+                for (int v = 0; v < 4; ++v) sm.u.g.dd[e][v] = (T)de.d[v];
             sm.u.g.dcode[e] = de.code;
         }
         for (int q = tid; q < (JJ + 1) * 64; q += BS) {
@@ -653,7 +675,7 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
             const int npair = min(CP, nn - n0);  // neighbours of the group that exist (uniform)
             Geo<T> g{};
             if (n < nn) g = load_geo<T>(geo + (size_t)(p0 + n) * GEO_STRIDE, true);
-            for (int q = lane; q < CP * mld::NB_MAX * 3; q += 64) (&sm.u.g.outw[w][0][0][0])[q] = (T)0;
+            for (int q = lane; q < CP * S::NBJ * 3; q += 64) (&sm.u.g.outw[w][0][0][0])[q] = (T)0;
             cx<T>* xbh = &sm.u.g.xb[w][hc][0][0];
             // level 0: U = 1, dU = 0, X = dfcut rhat.  The level loop below is unrolled (level
             // sizes become constants)
@@ -673,7 +695,11 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
                         const int e = tc + LPN * s;
                         if (e < mld::hcount(j)) {
                             cx<T> u, x[3];
-                            der_step<T>(sm.u.g.dd[mld::hoff(j) + e], sm.u.g.dcode[mld::hoff(j) + e], (T)1 / (T)j, g,
+                            const int he = mld::hoff(j) + e;
+                            const T* dvp = nullptr;
+                            const double* dvd = tb.der[he].d;
+                            if constexpr (S::HD > 1) dvp = sm.u.g.dd[he];
+                            der_step<T>(dvp, dvd, sm.u.g.dcode[he], (T)1 / (T)j, g,
                                         Up, u, x);
                             Un[e] = u;
 #pragma unroll
