@@ -76,19 +76,19 @@ PEAK_FP32, PEAK_FP64 = 157.3, 78.6   # TFLOP/s, MI355X dense
 # and WRITE_SIZE passes scaled by a 1 GiB copy (factors 2.0 and 1.0):
 # (171,748 x 2.0 + 2,037,415 x 1.0) KiB; the launch's algorithmic bytes are 2,177,392,328
 # (the excess is the per-pair geometry record, read by the density and gradient phases).
-TRAFFIC_BSO4_F32_FIT_BATCH = (167163 * 2.0 + 2036245 * 1.0) * 1024
+TRAFFIC_BSO4_F32_FIT_BATCH = (188097 * 2.0 + 2379438 * 1.0) * 1024
 ALGO_BYTES_BSO4_F32_FIT_BATCH = 2177392328.0
 # The same for bso4_kernel<double,8> (the dominant kernel of the fp64 form), and the
 # syrk128_f64_kernel launches of that workload, from the committed profile named below:
 # (310,796 x 2.0 + 4,065,094 x 1.0) KiB against 4,255,815,488 algorithmic bytes; the FP64 SYRK's
-# fit launches average 14.52 ms in the kernel trace (1,000 structures; the autotuner's trial
+# fit launches average 14.44 ms in the kernel trace (1,000 structures; the autotuner's trial
 # launches excluded) and move (6,091,137 x 2.0 + 582,400) KiB on the first batch.
 PROFILE_BSO4_F64 = "profiles/fit_quadratic_kernel_trace_and_hbm_counters.md"
 PROFILE_FIT = "profiles/fit_quadratic_kernel_trace_and_hbm_counters.md"
-TRAFFIC_BSO4_F64_FIT_BATCH = (310822 * 2.0 + 4065061 * 1.0) * 1024
+TRAFFIC_BSO4_F64_FIT_BATCH = (310880 * 2.0 + 4065086 * 1.0) * 1024
 ALGO_BYTES_BSO4_F64_FIT_BATCH = 4255815488.0
-SYRK_F64_MS_PER_BATCH = 14.52
-TRAFFIC_SYRK_F64_FIT_BATCH = (6309235 * 2.0 + 582400 * 1.0) * 1024
+SYRK_F64_MS_PER_BATCH = 14.44
+TRAFFIC_SYRK_F64_FIT_BATCH = (6200433 * 2.0 + 582400 * 1.0) * 1024
 
 
 def algorithmic_flops(natoms: int, npairs: int) -> float:

@@ -251,8 +251,12 @@ int mld_create(const mld_params* p, mld_ctx** out)
         if (bytes && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return false;
         return true;
     };
+    std::vector<float> derf(ctx->ht.der.size() * 4);
+    for (size_t e = 0; e < ctx->ht.der.size(); ++e)
+        for (int v = 0; v < 4; ++v) derf[4 * e + v] = (float)ctx->ht.der[e].d[v];
     bool ok = up(&ctx->d_rec, ctx->ht.rec.data(), ctx->ht.rec.size() * sizeof(mld::RecEntry)) &&
               up(&ctx->d_der, ctx->ht.der.data(), ctx->ht.der.size() * sizeof(mld::DerEntry)) &&
+              up(&ctx->d_derf, derf.data(), derf.size() * sizeof(float)) &&
               up(&ctx->d_rows, ctx->ht.rows.data(), ctx->ht.rows.size() * sizeof(mld::WRow)) &&
               up(&ctx->d_cg1, ctx->ht.cg1.data(), ctx->ht.cg1.size() * sizeof(double)) &&
               up(&ctx->d_tasks, ctx->ht.tasks.data(), ctx->ht.tasks.size() * sizeof(int)) &&
@@ -272,6 +276,7 @@ int mld_create(const mld_params* p, mld_ctx** out)
     dt.nu = ctx->ht.nu;
     dt.rec = (const mld::RecEntry*)ctx->d_rec;
     dt.der = (const mld::DerEntry*)ctx->d_der;
+    dt.derf = (const float*)ctx->d_derf;
     dt.rows = (const mld::WRow*)ctx->d_rows;
     dt.cg1 = (const double*)ctx->d_cg1;
     for (int j = 0; j <= mld::JJ_MAX + 1; ++j) {
@@ -306,7 +311,7 @@ int mld_destroy(mld_ctx* ctx)
     kept_rows_free(ctx);
     predict_free(ctx);
     analysis_free(ctx);
-    void* ptrs[] = {ctx->d_ace_tables, ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_der, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
+    void* ptrs[] = {ctx->d_ace_tables, ctx->d_tasks, ctx->d_tasks_all, ctx->d_atom_off, ctx->d_rec, ctx->d_der, ctx->d_derf, ctx->d_rows, ctx->d_cg1, ctx->d_wfac, ctx->d_rin, ctx->d_atom_struct,
                     ctx->d_geo, ctx->d_count, ctx->d_pair_off, ctx->d_pair_idx, ctx->d_pair_d, ctx->d_scan_tmp,
                     ctx->d_G, ctx->d_h, ctx->d_work, ctx->d_pgeo, ctx->d_etgt, ctx->d_zref, ctx->d_zz, ctx->d_zref32, ctx->d_kiz};
     for (void* q : ptrs)

@@ -336,15 +336,15 @@ __device__ __forceinline__ cx<T> rec_step(const RecT<T>& R, int b, const Geo<T>&
 // X_k = conj(V_k) = U dfcut rhat_k + fcut dU/dr_k = U s_k + F(g_k, h_k), g and h holding the
 // cutoff-scaled derivatives; U goes to the next level, X to the contraction.
 template <typename T>
-__device__ __forceinline__ void der_step(const T* __restrict__ dv, const double* __restrict__ dvd, int code, T inv_j,
-                                         const Geo<T>& g, const cx<T>* Up, cx<T>& u, cx<T> x[3])
+__device__ __forceinline__ void der_step(const T* __restrict__ dv, int code, T inv_j, const Geo<T>& g,
+                                         const cx<T>* Up, cx<T>& u, cx<T> x[3])
 {
     cx<T> P[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int c = (code >> (8 * v)) & 255;
         const cx<T> S = Up[c >> 1];
-        const T d = dv ? dv[v] : (T)dvd[v];
+        const T d = dv[v];
         P[v] = cx<T>{d * S.r, ((c & 1) ? -d : d) * S.i};
     }
     const cx<T> s1{P[0].r + P[1].r, P[0].i + P[1].i};
@@ -687,10 +687,10 @@ __global__ void __launch_bounds__(64 * nw_of<T>()) __attribute__((amdgpu_waves_p
                         if (e < mld::hcount(j)) {
                             cx<T> u, x[3];
                             const int he = mld::hoff(j) + e;
-                            const T* dvp = nullptr;
-                            const double* dvd = tb.der[he].d;
+                            const T* dvp;
                             if constexpr (S::HD > 1) dvp = sm.u.g.dd[he];
-                            der_step<T>(dvp, dvd, sm.u.g.dcode[he], (T)1 / (T)j, g,
+                            else dvp = tb.derf + 4 * he;
+                            der_step<T>(dvp, sm.u.g.dcode[he], (T)1 / (T)j, g,
                                         Up, u, x);
                             Un[e] = u;
 #pragma unroll

@@ -81,6 +81,7 @@ struct DevTables {
     int jj, nb, nu;
     const RecEntry* rec;   // [hoff(jj + 1)] canonical entries, level j at hoff(j)
     const DerEntry* der;   // [hoff(jj + 1)] derivative sources, same indexing
+    const float* derf;     // [hoff(jj + 1)][4] DerEntry::d rounded to FP32 (FP32 descriptor kernel)
     const WRow* rows;      // [sum_j M_j], level j rows start at rowbase[j]
     const double* cg1;     // CG1 tables
     int rowbase[JJ_MAX + 2];
@@ -220,6 +221,7 @@ struct mld_ctx {
     // device copies
     void* d_rec = nullptr;
     void* d_der = nullptr;
+    void* d_derf = nullptr;
     void* d_rows = nullptr;
     void* d_cg1 = nullptr;
     void* d_tasks = nullptr;
